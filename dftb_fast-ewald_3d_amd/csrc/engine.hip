@@ -1,0 +1,701 @@
+// engine.hip -- host side of the MI355X periodic Coulomb engine and its C ABI.
+//
+// ew_set_geometry is the geometry step (TCoulomb%updateLatVecs/updateCoords and
+// TShortGamma%updateCoords, src/dftbp/dftb/coulomb.F90:509-620, shortgamma.F90:177-195): it
+// orders the atoms by SPME tile, uploads them and builds the resident pair list.  ew_shift is
+// one SCC iteration (TCoulomb%updateShifts, coulomb.F90:786-880): gather charges, spread, FFT,
+// influence, inverse FFT, pair sum, gather and combine -- nine kernels on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fftewald_amd.h"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+using namespace ewamd;
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ew_handle {
+  ew_options opt{};
+  hipStream_t stream = nullptr;
+  std::string err = "no error";
+  bool have_geom = false;
+
+  int n = 0;
+  Cell cell{};
+  Params prm{};
+  bool fp32 = false;
+  int ns = 0;
+  int pair_mode = 0;
+  long long nnz = 0;
+  int T[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, ntmax = 0;
+  FftPlan px{}, py{}, pz{};
+  double self_coef = 0.0;
+
+  // device state of the current geometry
+  std::vector<DevBuf> bufs;
+  int* perm = nullptr;
+  int4* k0 = nullptr;
+  double* frac = nullptr;
+  void* theta = nullptr;
+  int* tile_of = nullptr;
+  int* tile_start = nullptr;
+  int* src_tab = nullptr;
+  int* species = nullptr;
+  double* hub = nullptr;
+  double* gcut2 = nullptr;
+  double* onsite = nullptr;
+  long long* row_ptr = nullptr;
+  int* col = nullptr;
+  void* coef = nullptr;
+  void* grid = nullptr;
+  void* cgrid = nullptr;
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  double* bsq[3] = {nullptr, nullptr, nullptr};
+  double* qs = nullptr;
+  double* phi_real = nullptr;
+  double* epart = nullptr;
+  double* energy = nullptr;
+  double* dq_io = nullptr;
+  double* shift_io = nullptr;
+  int* err_flag = nullptr;
+  size_t device_bytes = 0;
+
+  // per-phase timing (opt.profile)
+  hipEvent_t ev[EW_NUM_PHASES + 1] = {};
+  double phase_ms[EW_NUM_PHASES] = {};
+  long long phase_calls = 0;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    g_last_error = msg;
+    return code;
+  }
+  void release() {
+    for (auto& b : bufs)
+      if (b.p) (void)hipFree(b.p);
+    bufs.clear();
+    device_bytes = 0;
+    have_geom = false;
+  }
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      return fail(EW_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    bufs.push_back({p, bytes});
+    device_bytes += bytes;
+    *out = static_cast<T*>(p);
+    return EW_OK;
+  }
+};
+
+#define HIPCHK(h, expr)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return (h)->fail(EW_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                          " at " #expr);                                     \
+  } while (0)
+
+#define CHK(expr)             \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != EW_OK) return s_; \
+  } while (0)
+
+namespace {
+
+int set_global_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+// FFT workgroup shape for lines of length n: as many lines as fit the LDS budget.
+FftPlan make_plan(int n, int want_lines) {
+  FftPlan pl{};
+  pl.n = n;
+  pl.npass = fft_plan(n, pl.radix);
+  int lines = std::max(1, std::min(kFftMaxElems / n, want_lines));
+  pl.lines = lines;
+  pl.pitch = lines > 1 ? n + 1 : n;
+  return pl;
+}
+
+// Per-axis source tiles of every output tile for the spreading kernel.
+bool build_source_table(int K, int T, int nt, int p, int ntmax, int axis, std::vector<int>& tab) {
+  for (int t = 0; t < nt; ++t) {
+    const int lo = t * T, w = std::min(T, K - lo);
+    int* e = &tab[((size_t)axis * ntmax + t) * (kMaxSrcTiles + 1)];
+    e[0] = 0;
+    for (int s = 0; s < nt; ++s) {
+      const int s_lo = s * T, s_hi = std::min(K, (s + 1) * T);
+      bool hit = false;
+      for (int k0 = s_lo; k0 < s_hi && !hit; ++k0)
+        for (int o = 0; o < p && !hit; ++o) {
+          int x = (k0 + o) % K - lo;
+          if (x < 0) x += K;
+          if (x < w) hit = true;
+        }
+      if (hit) {
+        if (e[0] >= kMaxSrcTiles) return false;
+        e[1 + e[0]++] = s;
+      }
+    }
+  }
+  return true;
+}
+
+template <typename R>
+void twiddles(int n, std::vector<R>& out) {
+  out.resize(2 * (size_t)n);
+  for (int m = 0; m < n; ++m) {
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * m / n;
+    out[2 * m] = R(cosl(a));
+    out[2 * m + 1] = R(-sinl(a));
+  }
+}
+
+long long estimate_pairs(int n, const Cell& c, double r_list) {
+  const double vol_sphere = 4.0 / 3.0 * kPi * r_list * r_list * r_list;
+  double per_atom = double(n) / c.volume * vol_sphere;
+  return (long long)std::ceil(double(n) * per_atom * 1.05 + 64.0);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+
+extern "C" const char* ew_last_error(const ew_handle* h) {
+  return h ? h->err.c_str() : g_last_error.c_str();
+}
+
+extern "C" int ew_create(const ew_options* opts, ew_handle** out) {
+  if (!out) return set_global_error(EW_ERR_ARGUMENT, "ew_create: out is NULL");
+  *out = nullptr;
+  ew_options o{};
+  if (opts) o = *opts;
+  if (o.precision != EW_PRECISION_FP64 && o.precision != EW_PRECISION_MIXED)
+    return set_global_error(EW_ERR_ARGUMENT, "ew_create: unknown precision mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return set_global_error(EW_ERR_DEVICE, "ew_create: no HIP device available");
+  }
+  if (o.device < 0 || o.device >= ndev)
+    return set_global_error(EW_ERR_ARGUMENT, "ew_create: device ordinal out of range");
+  if (hipSetDevice(o.device) != hipSuccess)
+    return set_global_error(EW_ERR_DEVICE, "ew_create: hipSetDevice failed");
+  ew_handle* h = new ew_handle();
+  h->opt = o;
+  h->fp32 = o.precision == EW_PRECISION_MIXED;
+  if (hipStreamCreate(&h->stream) != hipSuccess) {
+    delete h;
+    return set_global_error(EW_ERR_DEVICE, "ew_create: hipStreamCreate failed");
+  }
+  if (o.profile)
+    for (auto& e : h->ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return set_global_error(EW_ERR_DEVICE, "ew_create: hipEventCreate failed");
+      }
+  *out = h;
+  return EW_OK;
+}
+
+extern "C" int ew_destroy(ew_handle* h) {
+  if (!h) return EW_OK;
+  (void)hipSetDevice(h->opt.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->release();
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return EW_OK;
+}
+
+static int validate_inputs(ew_handle* h, int n, const double* coords, const double* latvecs) {
+  if (n <= 0) return h->fail(EW_ERR_ARGUMENT, "nParticle must be positive");
+  if (!coords || !latvecs) return h->fail(EW_ERR_ARGUMENT, "coords and latvecs are required");
+  for (size_t i = 0; i < 3 * (size_t)n; ++i)
+    if (!std::isfinite(coords[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite coordinate");
+  return EW_OK;
+}
+
+extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const double* latvecs,
+                               const int* species, const double* hubbard_u, int n_species,
+                               const double* charges) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_set_geometry: handle is NULL");
+  CHK(validate_inputs(h, n, coords, latvecs));
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->release();
+
+  Cell c;
+  int st = cell_metrics(latvecs, &c);
+  if (st == EW_ERR_CELL) return h->fail(st, "cell volume too small");
+  if (st) return h->fail(st, "non-finite lattice vector");
+  double q2 = 0.0;
+  if (charges)
+    for (int i = 0; i < n; ++i) q2 += charges[i] * charges[i];
+  Params prm;
+  std::string perr;
+  st = choose_parameters(c, n, q2, &h->opt.tol, h->opt.alpha, h->opt.r_cut, h->opt.grid,
+                         h->opt.order, h->opt.alpha_rule, &prm, &perr);
+  if (st) return h->fail(st, perr);
+  prm.precision = h->opt.precision;
+
+  // short-range gamma (src/dftbp/dftb/shortgamma.F90:753-775: cutoffs per species pair)
+  const bool gamma = species && hubbard_u && n_species > 0;
+  std::vector<double> gcut2;
+  if (gamma) {
+    for (int i = 0; i < n; ++i)
+      if (species[i] < 0 || species[i] >= n_species)
+        return h->fail(EW_ERR_ARGUMENT, "species index out of range");
+    for (int s = 0; s < n_species; ++s)
+      if (!(hubbard_u[s] >= 1.0e-6))  // MinHubTol, src/dftbp/common/accuracy.F90:106
+        return h->fail(EW_ERR_ARGUMENT, "Hubbard U below 1e-6");
+    gcut2.assign((size_t)n_species * n_species, 0.0);
+    for (int a = 0; a < n_species; ++a)
+      for (int b = a; b < n_species; ++b) {
+        const double cut = exp_gamma_cutoff(hubbard_u[b], hubbard_u[a]);
+        const double c2 = cut * cut + 1.0e-10;  // + tolSameDist^2 of the neighbour bisection
+        gcut2[a * n_species + b] = gcut2[b * n_species + a] = c2;
+        prm.r_list = std::max(prm.r_list, cut);
+      }
+  }
+  h->n = n;
+  h->cell = c;
+  h->prm = prm;
+  h->ns = gamma ? n_species : 0;
+  h->self_coef = -2.0 * prm.alpha / kSqrtPi;
+  const int p = prm.order;
+  const int* K = prm.K;
+
+  // tiles of the SPME mesh; they also serve as the cells of the pair search
+  const int Tdef[3] = {16, 8, 8};
+  for (int a = 0; a < 3; ++a) {
+    h->T[a] = std::min(K[a], Tdef[a]);
+    h->nt[a] = (K[a] + h->T[a] - 1) / h->T[a];
+  }
+  h->ntmax = std::max(h->nt[0], std::max(h->nt[1], h->nt[2]));
+  std::vector<int> srctab((size_t)3 * h->ntmax * (kMaxSrcTiles + 1), 0);
+  for (int a = 0; a < 3; ++a)
+    if (!build_source_table(K[a], h->T[a], h->nt[a], p, h->ntmax, a, srctab))
+      return h->fail(EW_ERR_ARGUMENT, "spreading tile table overflow (order too high)");
+
+  // fractional coordinates, stencil origins, tiles; counting sort by tile
+  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  std::vector<double> fr(3 * (size_t)n);
+  std::vector<int> k0o(3 * (size_t)n), tile(n), count(ntiles + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    int tt[3];
+    for (int a = 0; a < 3; ++a) {
+      double s = c.rec[a][0] * coords[3 * i] + c.rec[a][1] * coords[3 * i + 1] +
+                 c.rec[a][2] * coords[3 * i + 2];
+      s -= std::floor(s);
+      fr[3 * (size_t)i + a] = s;
+      int fl = int(std::floor(s * K[a]));
+      if (fl >= K[a]) fl -= K[a];
+      int k0 = fl - (p - 1);
+      if (k0 < 0) k0 += K[a];
+      k0o[3 * (size_t)i + a] = k0;
+      tt[a] = k0 / h->T[a];
+    }
+    tile[i] = (tt[2] * h->nt[1] + tt[1]) * h->nt[0] + tt[0];
+    count[tile[i] + 1]++;
+  }
+  for (int t = 0; t < ntiles; ++t) count[t + 1] += count[t];
+  std::vector<int> perm(n), fill(count.begin(), count.end() - 1);
+  for (int i = 0; i < n; ++i) perm[fill[tile[i]]++] = i;
+  std::vector<double> frs(3 * (size_t)n);
+  std::vector<int4> k0s(n);
+  std::vector<int> tiles(n), sps(gamma ? n : 0);
+  std::vector<double> ons(gamma ? n : 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = perm[k];
+    for (int a = 0; a < 3; ++a) frs[3 * (size_t)k + a] = fr[3 * (size_t)i + a];
+    k0s[k] = make_int4(k0o[3 * (size_t)i], k0o[3 * (size_t)i + 1], k0o[3 * (size_t)i + 2], 0);
+    tiles[k] = tile[i];
+    if (gamma) {
+      sps[k] = species[i];
+      ons[k] = hubbard_u[species[i]];  // -expGamma(0, U, U) = U  (shortgammafuncs.F90:128-131)
+    }
+  }
+
+  // device buffers
+  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  const int Kh = K[0] / 2 + 1;
+  const size_t ngrid = (size_t)K[0] * K[1] * K[2], ncgrid = (size_t)Kh * K[1] * K[2];
+  CHK(h->alloc(&h->perm, n));
+  CHK(h->alloc(&h->k0, n));
+  CHK(h->alloc(&h->frac, 3 * (size_t)n));
+  {
+    unsigned char* th;
+    CHK(h->alloc(&th, 3 * (size_t)n * p * rs));
+    h->theta = th;
+  }
+  CHK(h->alloc(&h->tile_of, n));
+  CHK(h->alloc(&h->tile_start, ntiles + 1));
+  CHK(h->alloc(&h->src_tab, srctab.size()));
+  CHK(h->alloc(&h->row_ptr, (size_t)n + 1));
+  CHK(h->alloc(&h->qs, n));
+  CHK(h->alloc(&h->phi_real, n));
+  CHK(h->alloc(&h->epart, (size_t)(n + 255) / 256));
+  CHK(h->alloc(&h->energy, 1));
+  CHK(h->alloc(&h->dq_io, n));
+  CHK(h->alloc(&h->shift_io, n));
+  CHK(h->alloc(&h->err_flag, 1));
+  {
+    unsigned char *g, *cgp;
+    CHK(h->alloc(&g, ngrid * rs));
+    CHK(h->alloc(&cgp, ncgrid * 2 * rs));
+    h->grid = g;
+    h->cgrid = cgp;
+  }
+  if (gamma) {
+    CHK(h->alloc(&h->species, n));
+    CHK(h->alloc(&h->hub, n_species));
+    CHK(h->alloc(&h->gcut2, (size_t)n_species * n_species));
+    CHK(h->alloc(&h->onsite, n));
+  }
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->k0, k0s.data(), sizeof(int4) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->frac, frs.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->tile_of, tiles.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->tile_start, count.data(), sizeof(int) * (ntiles + 1),
+                           hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->src_tab, srctab.data(), sizeof(int) * srctab.size(),
+                           hipMemcpyHostToDevice, s));
+  if (gamma) {
+    HIPCHK(h, hipMemcpyAsync(h->species, sps.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->hub, hubbard_u, sizeof(double) * n_species,
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->gcut2, gcut2.data(), sizeof(double) * gcut2.size(),
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->onsite, ons.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  }
+  // FFT tables
+  for (int a = 0; a < 3; ++a) {
+    std::vector<double> bq(K[a]);
+    bspline_moduli(K[a], p, bq.data());
+    CHK(h->alloc(&h->bsq[a], K[a]));
+    HIPCHK(h, hipMemcpyAsync(h->bsq[a], bq.data(), sizeof(double) * K[a], hipMemcpyHostToDevice, s));
+    unsigned char* t;
+    CHK(h->alloc(&t, 2 * (size_t)K[a] * rs));
+    h->tw[a] = t;
+    if (h->fp32) {
+      std::vector<float> tv;
+      twiddles(K[a], tv);
+      HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(h, hipStreamSynchronize(s));
+    } else {
+      std::vector<double> tv;
+      twiddles(K[a], tv);
+      HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(h, hipStreamSynchronize(s));
+    }
+  }
+  for (int a = 0; a < 3; ++a)
+    if (fft_plan(K[a], h->px.radix) < 0) return h->fail(EW_ERR_ARGUMENT, "grid not 2^a 3^b 5^c");
+  h->px = make_plan(K[0], (K[1] * K[2] + 1) / 2);
+  h->py = make_plan(K[1], 16);
+  h->pz = make_plan(K[2], 16);
+
+  // B-spline weights
+  BsplineArgs ba{{K[0], K[1], K[2]}, p};
+  HIPCHK(h, launch_bspline(n, h->frac, ba, h->theta, h->fp32, s));
+
+  // pair list
+  PairArgs pa{};
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) pa.lat[3 * a + d] = c.lat[a][d];
+  pa.alpha = prm.alpha;
+  pa.rc2 = prm.r_cut * prm.r_cut;
+  pa.rl2 = prm.r_list * prm.r_list + (gamma ? 1.0e-10 : 0.0);
+  pa.ns = h->ns;
+  pa.hub = h->hub;
+  pa.gcut2 = h->gcut2;
+  bool linked = true;
+  for (int a = 0; a < 3; ++a) {
+    pa.nt[a] = h->nt[a];
+    const double wmin = double(std::min(h->T[a], K[a] - (h->nt[a] - 1) * h->T[a])) / K[a];
+    pa.D[a] = int(std::ceil(prm.r_list * c.reclen[a] / wmin));
+    if (2 * pa.D[a] + 1 > h->nt[a]) linked = false;
+  }
+  HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));
+  const bool cf = h->fp32;
+  const size_t cs = cf ? sizeof(float) : sizeof(double);
+  if (linked) {
+    h->pair_mode = 0;
+    HIPCHK(h, launch_pair_count(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
+                                h->row_ptr, h->err_flag, s));
+    std::vector<long long> rp((size_t)n + 1);
+    HIPCHK(h, hipMemcpyAsync(rp.data(), h->row_ptr, sizeof(long long) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    long long run = 0;
+    for (int i = 0; i < n; ++i) {
+      const long long c_i = rp[i];
+      rp[i] = run;
+      run += c_i;
+    }
+    rp[n] = run;
+    h->nnz = run;
+    CHK(h->alloc(&h->col, (size_t)run));
+    unsigned char* cp;
+    CHK(h->alloc(&cp, (size_t)run * cs));
+    h->coef = cp;
+    HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(h, launch_pair_fill(n, h->frac, h->species, h->tile_of, h->tile_start, pa, h->row_ptr,
+                               h->col, h->coef, cf, h->err_flag, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+  } else {
+    h->pair_mode = 1;
+    for (int a = 0; a < 3; ++a) pa.D[a] = int(std::ceil(prm.r_list * c.reclen[a])) + 1;
+    const long long nn = (long long)n * n;
+    if (nn > (1LL << 31)) return h->fail(EW_ERR_MEMORY, "cell too small for this many atoms");
+    h->nnz = nn;
+    std::vector<long long> rp((size_t)n + 1);
+    for (int i = 0; i <= n; ++i) rp[i] = (long long)i * n;
+    CHK(h->alloc(&h->col, (size_t)nn));
+    unsigned char* cp;
+    CHK(h->alloc(&cp, (size_t)nn * cs));
+    h->coef = cp;
+    HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(h, launch_pair_dense(n, h->frac, h->species, pa, h->col, h->coef, cf, h->err_flag, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+  }
+  int errflag = 0;
+  HIPCHK(h, hipMemcpy(&errflag, h->err_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (errflag) {
+    h->release();
+    return h->fail(EW_ERR_COINCIDENT, "coincident charges");
+  }
+  h->have_geom = true;
+  return EW_OK;
+}
+
+extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
+  if (!h || !out) return set_global_error(EW_ERR_ARGUMENT, "ew_get_parameters: NULL argument");
+  if (!h->have_geom) return set_global_error(EW_ERR_NO_GEOMETRY, "no geometry set");
+  std::memset(out, 0, sizeof(*out));
+  out->alpha = h->prm.alpha;
+  out->r_cut = h->prm.r_cut;
+  out->r_list = h->prm.r_list;
+  double kmin = 1e308;
+  for (int a = 0; a < 3; ++a) {
+    out->grid[a] = h->prm.K[a];
+    kmin = std::min(kmin, kPi * h->prm.K[a] / h->cell.latlen[a]);
+  }
+  out->k_cut = kmin;
+  out->order = h->prm.order;
+  out->precision = h->prm.precision;
+  out->pair_mode = h->pair_mode;
+  out->n_pairs = h->nnz;
+  out->device_bytes = (long long)h->device_bytes;
+  return EW_OK;
+}
+
+static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                    hipStream_t s) {
+  const int n = h->n;
+  const int* K = h->prm.K;
+  const bool prof = h->opt.profile != 0;
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
+  HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, s));
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
+  SpmeLaunch L{};
+  for (int a = 0; a < 3; ++a) L.K[a] = K[a];
+  L.k0 = h->k0;
+  L.theta = h->theta;
+  L.q = h->qs;
+  L.tile_start = h->tile_start;
+  for (int a = 0; a < 3; ++a) {
+    L.spread.K[a] = K[a];
+    L.spread.T[a] = h->T[a];
+    L.spread.nt[a] = h->nt[a];
+    L.tw[a] = h->tw[a];
+  }
+  L.spread.ntmax = h->ntmax;
+  L.spread.order = h->prm.order;
+  L.spread.src = h->src_tab;
+  L.px = h->px;
+  L.py = h->py;
+  L.pz = h->pz;
+  for (int a = 0; a < 3; ++a) L.inf.K[a] = K[a];
+  L.inf.Kh = K[0] / 2 + 1;
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) L.inf.rec[3 * a + d] = h->cell.rec[a][d];
+  L.inf.pref = 1.0 / (kPi * h->cell.volume);
+  L.inf.a2 = kPi * kPi / (h->prm.alpha * h->prm.alpha);
+  L.inf.bsq1 = h->bsq[0];
+  L.inf.bsq2 = h->bsq[1];
+  L.inf.bsq3 = h->bsq[2];
+  L.grid = h->grid;
+  L.cgrid = h->cgrid;
+  if (h->fp32)
+    HIPCHK(h, launch_spme<float>(L, s, prof ? &h->ev[2] : nullptr));
+  else
+    HIPCHK(h, launch_spme<double>(L, s, prof ? &h->ev[2] : nullptr));
+  HIPCHK(h, launch_pair_spmv(n, h->row_ptr, h->col, h->coef, h->fp32, h->qs, h->phi_real, s));
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
+  GatherArgs ga{{K[0], K[1], K[2]}, h->prm.order, h->self_coef};
+  if (h->fp32)
+    HIPCHK(h, launch_gather_combine<float>(n, h->k0, h->theta, h->grid, h->qs, h->phi_real,
+                                           h->onsite, h->perm, ga, d_shift, h->epart, d_energy, s));
+  else
+    HIPCHK(h, launch_gather_combine<double>(n, h->k0, h->theta, h->grid, h->qs, h->phi_real,
+                                            h->onsite, h->perm, ga, d_shift, h->epart, d_energy,
+                                            s));
+  if (prof) {
+    HIPCHK(h, hipEventRecord(h->ev[9], s));
+    HIPCHK(h, hipEventSynchronize(h->ev[9]));
+    for (int i = 0; i < EW_NUM_PHASES; ++i) {
+      float ms = 0.0f;
+      HIPCHK(h, hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+      h->phase_ms[i] += ms;
+    }
+    h->phase_calls++;
+  }
+  return EW_OK;
+}
+
+extern "C" int ew_phase_times(ew_handle* h, double* ms, long long* calls, int reset) {
+  if (!h || !ms) return set_global_error(EW_ERR_ARGUMENT, "ew_phase_times: NULL argument");
+  if (!h->opt.profile) return h->fail(EW_ERR_ARGUMENT, "ew_phase_times needs opts.profile = 1");
+  for (int i = 0; i < EW_NUM_PHASES; ++i) ms[i] = h->phase_ms[i];
+  if (calls) *calls = h->phase_calls;
+  if (reset) {
+    for (int i = 0; i < EW_NUM_PHASES; ++i) h->phase_ms[i] = 0.0;
+    h->phase_calls = 0;
+  }
+  return EW_OK;
+}
+
+extern "C" int ew_shift_device(ew_handle* h, const double* d_dq, double* d_shift,
+                               double* d_energy, void* stream) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_shift_device: handle is NULL");
+  if (!h->have_geom) return h->fail(EW_ERR_NO_GEOMETRY, "ew_shift_device before ew_set_geometry");
+  if (!d_dq || !d_shift) return h->fail(EW_ERR_ARGUMENT, "ew_shift_device: NULL array");
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  return run_eval(h, d_dq, d_shift, d_energy ? d_energy : h->energy, s);
+}
+
+extern "C" int ew_shift(ew_handle* h, const double* dq, double* shift, double* energy) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_shift: handle is NULL");
+  if (!h->have_geom) return h->fail(EW_ERR_NO_GEOMETRY, "ew_shift before ew_set_geometry");
+  if (!dq || !shift) return h->fail(EW_ERR_ARGUMENT, "ew_shift: NULL array");
+  const int n = h->n;
+  double net = 0.0, absq = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(dq[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+    net += dq[i];
+    absq += std::fabs(dq[i]);
+  }
+  if (std::fabs(net) > kNeutralityTol * absq)
+    return h->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral");
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->dq_io, dq, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  CHK(run_eval(h, h->dq_io, h->shift_io, h->energy, s));
+  HIPCHK(h, hipMemcpyAsync(shift, h->shift_io, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  double e = 0.0;
+  HIPCHK(h, hipMemcpyAsync(&e, h->energy, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  if (energy) *energy = e;
+  return EW_OK;
+}
+
+extern "C" int ew_potential_force(int n, const double* coords, const double* charges,
+                                  const double* latvecs, const ew_options* opts, double* pot,
+                                  double* force) {
+  if (!charges || !pot) return set_global_error(EW_ERR_ARGUMENT, "ew_potential_force: NULL array");
+  if (force)
+    return set_global_error(EW_ERR_ARGUMENT,
+                            "ew_potential_force: forces are not available in this build");
+  ew_handle* h = nullptr;
+  int st = ew_create(opts, &h);
+  if (st) return st;
+  st = ew_set_geometry(h, n, coords, latvecs, nullptr, nullptr, 0, charges);
+  if (!st) st = ew_shift(h, charges, pot, nullptr);
+  if (st) g_last_error = h->err;
+  ew_destroy(h);
+  return st;
+}
+
+extern "C" int ew_parameters_for(int n, const double* charges, const double* latvecs,
+                                 const ew_options* opts, ew_parameters* out) {
+  if (!latvecs || !out || n <= 0)
+    return set_global_error(EW_ERR_ARGUMENT, "ew_parameters_for: bad argument");
+  ew_options o{};
+  if (opts) o = *opts;
+  Cell c;
+  int st = cell_metrics(latvecs, &c);
+  if (st) return set_global_error(st, "invalid cell");
+  double q2 = 0.0;
+  if (charges)
+    for (int i = 0; i < n; ++i) q2 += charges[i] * charges[i];
+  Params p;
+  std::string err;
+  st = choose_parameters(c, n, q2, &o.tol, o.alpha, o.r_cut, o.grid, o.order, o.alpha_rule, &p,
+                         &err);
+  if (st) return set_global_error(st, err);
+  std::memset(out, 0, sizeof(*out));
+  out->alpha = p.alpha;
+  out->r_cut = p.r_cut;
+  out->r_list = p.r_list;
+  double kmin = 1e308;
+  for (int a = 0; a < 3; ++a) {
+    out->grid[a] = p.K[a];
+    kmin = std::min(kmin, kPi * p.K[a] / c.latlen[a]);
+  }
+  out->k_cut = kmin;
+  out->order = p.order;
+  out->precision = o.precision;
+  out->n_pairs = estimate_pairs(n, c, p.r_list);
+  return EW_OK;
+}
+
+extern "C" int ew_grid_memory(int n, const double* charges, const double* latvecs,
+                              const ew_options* opts, double* gigabytes) {
+  if (!gigabytes) return set_global_error(EW_ERR_ARGUMENT, "ew_grid_memory: NULL output");
+  ew_parameters p;
+  int st = ew_parameters_for(n, charges, latvecs, opts, &p);
+  if (st) return st;
+  const double rs = p.precision == EW_PRECISION_MIXED ? 4.0 : 8.0;
+  const double K3 = double(p.grid[0]) * p.grid[1] * p.grid[2];
+  const double Kh = double(p.grid[0] / 2 + 1) * p.grid[1] * p.grid[2];
+  const double grids = K3 * rs + Kh * 2.0 * rs;
+  const double atoms = double(n) * (4 + 16 + 24 + 3.0 * p.order * rs + 4 + 8 * 6 + 8);
+  const double pairs = double(p.n_pairs) * (4.0 + rs);
+  *gigabytes = (grids + atoms + pairs) / (1024.0 * 1024.0 * 1024.0);
+  return EW_OK;
+}

@@ -1,0 +1,93 @@
+// kernels.hpp -- kernel argument blocks and launchers (host <-> device contract).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+
+namespace ewamd {
+
+constexpr int kSpreadTileMax = 2048;  // mesh points per spreading tile (LDS: 16 KiB in FP64)
+
+struct SpreadArgs {
+  int K[3];
+  int T[3];        // tile extents
+  int nt[3];       // tiles per axis
+  int ntmax;       // max(nt) -- row length of the source-tile table
+  int order;
+  const int* src;  // [3][ntmax][1 + kMaxSrcTiles]: count, then source tiles per output tile
+};
+
+struct InfluenceArgs {
+  int K[3];
+  int Kh;
+  double rec[9];   // dual basis rows b_i (no 2 pi)
+  double pref;     // 1 / (pi V)
+  double a2;       // pi^2 / alpha^2
+  const double* bsq1;
+  const double* bsq2;
+  const double* bsq3;
+};
+
+struct GatherArgs {
+  int K[3];
+  int order;
+  double self_coef;  // -2 alpha / sqrt(pi)
+};
+
+struct BsplineArgs {
+  int K[3];
+  int order;
+};
+
+struct PairArgs {
+  double lat[9];   // rows a_i
+  double alpha;
+  double rc2;      // r_cut^2
+  double rl2;      // r_list^2
+  int D[3];        // partner tiles (linked mode) or image shells (dense mode) per side
+  int nt[3];
+  int ns;          // species count (0: no short gamma)
+  const double* hub;    // [ns]
+  const double* gcut2;  // [ns*ns] squared short-gamma cutoffs
+};
+
+struct SpmeLaunch {
+  int K[3];
+  const int4* k0;
+  const void* theta;
+  const double* q;
+  const int* tile_start;
+  SpreadArgs spread;
+  FftPlan px, py, pz;
+  const void* tw[3];
+  InfluenceArgs inf;
+  void* grid;
+  void* cgrid;
+};
+
+// ev (optional, 6 events): recorded after spread, x, y, z, inverse y, inverse x.
+template <typename R>
+hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
+
+hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
+                                 hipStream_t st);
+hipError_t launch_pair_spmv(int n, const long long* row_ptr, const int* col, const void* coef,
+                            bool fp32, const double* q, double* out, hipStream_t st);
+template <typename R>
+hipError_t launch_gather_combine(int n, const int4* k0, const void* theta, const void* grid,
+                                 const double* q, const double* phi_real, const double* onsite,
+                                 const int* perm, const GatherArgs& g, double* shift,
+                                 double* epart, double* energy, hipStream_t st);
+hipError_t launch_bspline(int n, const double* frac, const BsplineArgs& g, void* theta, bool fp32,
+                          hipStream_t st);
+hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int* tile_of,
+                             const int* tile_start, const PairArgs& g, long long* counts,
+                             int* err, hipStream_t st);
+hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
+                            const int* tile_start, const PairArgs& g, long long* row_ptr,
+                            int* col, void* coef, bool fp32, int* err, hipStream_t st);
+hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const PairArgs& g,
+                             int* col, void* coef, bool fp32, int* err, hipStream_t st);
+
+}  // namespace ewamd

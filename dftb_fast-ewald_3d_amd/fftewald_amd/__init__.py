@@ -1,0 +1,258 @@
+"""ctypes binding of the MI355X periodic Coulomb engine (``lib/libfftewald_amd.so``).
+
+Mirrors the reference library's public surface (src/dftbp/extlibs/fftewald/fft/ewald_fft_3d.f90:
+``ewald_potential_force``, ``ewald_parameters``, ``ewald_grid_memory``) and the resident
+per-SCC-iteration use TCoulomb makes of it (src/dftbp/dftb/coulomb.F90:336-357, 786-880), in the
+style of the DFTB+ ctypes wrapper (tools/pythonapi/src/dftbplus/dftbplus.py:72-75, 384-399):
+``np.ctypeslib.ndpointer`` arguments and one checking wrapper per call.
+
+There is no CPU fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libfftewald_amd.so")
+
+EW_OK = 0
+EW_ERR_ARGUMENT = 1
+EW_ERR_NOT_NEUTRAL = 2
+EW_ERR_CELL = 3
+EW_ERR_TOLERANCE = 4
+EW_ERR_COINCIDENT = 5
+EW_ERR_NO_GEOMETRY = 6
+EW_ERR_DEVICE = 7
+EW_ERR_MEMORY = 8
+EW_ERR_NOT_FINITE = 9
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+#: every symbol the C ABI (include/fftewald_amd.h) exports
+EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
+            "ew_shift", "ew_shift_device", "ew_phase_times", "ew_potential_force",
+            "ew_parameters_for", "ew_grid_memory")
+
+#: phases reported by ew_phase_times, in order
+PHASES = ("gather_charges", "spread", "fft_x_r2c", "fft_y", "fft_z_influence", "fft_y_inverse",
+          "fft_x_c2r", "pair_sum", "gather_combine")
+
+
+class EwaldError(RuntimeError):
+    """A non-zero status of the C ABI; ``code`` is the EW_ERR_* value."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("tol", C.c_double), ("alpha", C.c_double), ("r_cut", C.c_double),
+                ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
+                ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
+                ("reserved", C.c_int * 3)]
+
+
+class Parameters(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("r_cut", C.c_double), ("k_cut", C.c_double),
+                ("r_list", C.c_double), ("grid", C.c_int * 3), ("order", C.c_int),
+                ("precision", C.c_int), ("pair_mode", C.c_int), ("n_pairs", C.c_longlong),
+                ("device_bytes", C.c_longlong)]
+
+    def as_dict(self) -> dict:
+        return {"alpha": self.alpha, "r_cut": self.r_cut, "k_cut": self.k_cut,
+                "r_list": self.r_list, "grid": tuple(self.grid), "order": self.order,
+                "precision": self.precision, "pair_mode": self.pair_mode,
+                "n_pairs": self.n_pairs, "device_bytes": self.device_bytes}
+
+
+_D1 = np.ctypeslib.ndpointer(dtype=np.float64, ndim=1, flags="C_CONTIGUOUS")
+_D2 = np.ctypeslib.ndpointer(dtype=np.float64, ndim=2, flags="C_CONTIGUOUS")
+_I1 = np.ctypeslib.ndpointer(dtype=np.int32, ndim=1, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def _optional(nd):
+    """ndpointer that also accepts None (passed as NULL)."""
+
+    class _Opt(nd):
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return super().from_param(obj)
+
+    return _Opt
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"fftewald_amd: {LIB_PATH} is missing -- build it first "
+                              "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        H = C.c_void_p
+        L.ew_create.argtypes = [C.POINTER(Options), C.POINTER(H)]
+        L.ew_destroy.argtypes = [H]
+        L.ew_last_error.restype = C.c_char_p
+        L.ew_last_error.argtypes = [H]
+        L.ew_set_geometry.argtypes = [H, C.c_int, _D2, _D2, _optional(_I1), _optional(_D1),
+                                      C.c_int, _optional(_D1)]
+        L.ew_get_parameters.argtypes = [H, C.POINTER(Parameters)]
+        L.ew_shift.argtypes = [H, _D1, _D1, C.POINTER(C.c_double)]
+        L.ew_shift_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
+                                                                flags="C_CONTIGUOUS"),
+                                     C.POINTER(C.c_longlong), C.c_int]
+        L.ew_potential_force.argtypes = [C.c_int, _D2, _D1, _D2, C.POINTER(Options), _D1,
+                                         _optional(_D2)]
+        L.ew_parameters_for.argtypes = [C.c_int, _optional(_D1), _D2, C.POINTER(Options),
+                                        C.POINTER(Parameters)]
+        L.ew_grid_memory.argtypes = [C.c_int, _optional(_D1), _D2, C.POINTER(Options),
+                                     C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
+                 device=0, alpha_rule="fftewald", profile=False) -> Options:
+    o = Options()
+    o.tol = tol or 0.0
+    o.alpha = alpha or 0.0
+    o.r_cut = r_cut or 0.0
+    if grid is not None:
+        g = (grid, grid, grid) if np.isscalar(grid) else tuple(grid)
+        for a in range(3):
+            o.grid[a] = int(g[a])
+    o.order = order or 0
+    o.precision = {"fp64": PRECISION_FP64, "mixed": PRECISION_MIXED}[precision] \
+        if isinstance(precision, str) else int(precision)
+    o.device = device
+    o.alpha_rule = {"fftewald": 0, "dftb": 1}[alpha_rule] if isinstance(alpha_rule, str) \
+        else int(alpha_rule)
+    o.profile = 1 if profile else 0
+    return o
+
+
+def _check(status: int, handle=None):
+    if status != EW_OK:
+        msg = lib().ew_last_error(handle).decode()
+        raise EwaldError(status, msg)
+
+
+def _f64(a, ndim) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != ndim:
+        raise ValueError(f"expected a {ndim}-d array, got shape {a.shape}")
+    return a
+
+
+class EwaldEngine:
+    """Resident engine for one structure: TCoulomb (+ TShortGamma) state on the device.
+
+    ``set_geometry`` once per geometry step, then ``shift`` (host arrays) or ``shift_device``
+    (device pointers, e.g. ``tensor.data_ptr()``) once per SCC iteration.
+    """
+
+    def __init__(self, **opts):
+        self._h = C.c_void_p()
+        self.options = make_options(**opts)
+        _check(lib().ew_create(C.byref(self.options), C.byref(self._h)))
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().ew_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_geometry(self, coords, lattice, species=None, hubbard_u=None, charges=None):
+        coords = _f64(coords, 2)
+        lattice = _f64(lattice, 2)
+        n = coords.shape[0]
+        if coords.shape != (n, 3) or lattice.shape != (3, 3):
+            raise ValueError("coords must be (n, 3) and lattice (3, 3)")
+        sp = None if species is None else np.ascontiguousarray(species, dtype=np.int32)
+        u = None if hubbard_u is None else _f64(hubbard_u, 1)
+        q = None if charges is None else _f64(charges, 1)
+        _check(lib().ew_set_geometry(self._h, n, coords, lattice, sp, u,
+                                     0 if u is None else len(u), q), self._h)
+        self.n = n
+
+    def parameters(self) -> dict:
+        p = Parameters()
+        _check(lib().ew_get_parameters(self._h, C.byref(p)), self._h)
+        return p.as_dict()
+
+    def shift(self, dq):
+        """Returns (shift[n], energy) for charges dq (caller order)."""
+        dq = _f64(dq, 1)
+        out = np.empty(self.n)
+        e = C.c_double()
+        _check(lib().ew_shift(self._h, dq, out, C.byref(e)), self._h)
+        return out, e.value
+
+    def shift_device(self, dq_ptr: int, shift_ptr: int, energy_ptr: int = 0, stream: int = 0):
+        _check(lib().ew_shift_device(self._h, C.c_void_p(dq_ptr), C.c_void_p(shift_ptr),
+                                     C.c_void_p(energy_ptr or None), C.c_void_p(stream or None)),
+               self._h)
+
+
+    def phase_times(self, reset: bool = False):
+        """({phase: ms}, calls) accumulated since the last reset (needs profile=True)."""
+        ms = np.zeros(len(PHASES))
+        calls = C.c_longlong()
+        _check(lib().ew_phase_times(self._h, ms, C.byref(calls), 1 if reset else 0), self._h)
+        return dict(zip(PHASES, ms.tolist())), calls.value
+
+
+def ewald_potential_force(positions, charges, lattice, force=False, **opts):
+    """ewald_potential_force (fe/fft/ewald_fft_3d.f90:36-134): (pot, force or None)."""
+    positions = _f64(positions, 2)
+    charges = _f64(charges, 1)
+    lattice = _f64(lattice, 2)
+    n = len(charges)
+    pot = np.empty(n)
+    f = np.empty((n, 3)) if force else None
+    o = make_options(**opts)
+    _check(lib().ew_potential_force(n, positions, charges, lattice, C.byref(o), pot, f))
+    return pot, f
+
+
+def ewald_parameters(charges, lattice, **opts) -> dict:
+    """ewald_parameters (fe/fft/ewald_fft_3d.f90:142-191), no device needed."""
+    charges = _f64(charges, 1)
+    lattice = _f64(lattice, 2)
+    p = Parameters()
+    o = make_options(**opts)
+    _check(lib().ew_parameters_for(len(charges), charges, lattice, C.byref(o), C.byref(p)))
+    return p.as_dict()
+
+
+def ewald_grid_memory(charges, lattice, **opts) -> float:
+    """ewald_grid_memory (fe/fft/ewald_fft_3d.f90:197-244) in gigabytes, no device needed."""
+    charges = _f64(charges, 1)
+    lattice = _f64(lattice, 2)
+    g = C.c_double()
+    o = make_options(**opts)
+    _check(lib().ew_grid_memory(len(charges), charges, lattice, C.byref(o), C.byref(g)))
+    return g.value

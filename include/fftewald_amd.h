@@ -1,0 +1,153 @@
+/*
+ * fftewald_amd.h -- C ABI of the MI355X (gfx950) periodic Coulomb engine.
+ *
+ * Drop-in for the Ewald library of the DFTB+ fast-Ewald fork, whose public surface is
+ *   ewald_potential_force, ewald_parameters, ewald_grid_memory, TEwaldParameters3d
+ *   (src/dftbp/extlibs/fftewald/fft/ewald_fft_3d.f90:19-20)
+ * and for the per-SCC-iteration calls TCoulomb makes into it
+ *   (src/dftbp/dftb/coulomb.F90:336-385 fftEwaldShift / fftEwaldAddGradients,
+ *    786-880 updateShifts), composed with the short-range gamma of TShortGamma
+ *   (src/dftbp/dftb/shortgamma.F90:860-934) as TScc%updateShifts does (scc.F90:453-496).
+ *
+ * Plain C: pointers and sizes only, no HIP or torch types.  Every call returns an int status
+ * (EW_OK = 0); none aborts.  ew_last_error() gives the message of the last failure.
+ *
+ * Units are atomic (Bohr, e, Hartree), as in the reference (fft3d_readme.md:17).
+ * Array layouts are DFTB+'s native ones, so the Fortran caller passes its arrays unchanged:
+ *   coords   [n][3]  row i = Cartesian position of atom i      (DFTB+ coords(3,N))
+ *   latvecs  [3][3]  row i = lattice vector a_i                 (DFTB+ latVecs(3,3) columns)
+ *   dq, shift, pot   [n]
+ *   force, grad      [n][3]
+ * The reference's own ewald_potential_force takes positions(N,3) and latVecs with rows a_i in
+ * Fortran order; fortran/fftewald_amd.f90 provides that exact signature on top of this ABI.
+ *
+ * Threading: one handle per host thread; each handle owns a HIP stream on its device.
+ */
+#ifndef FFTEWALD_AMD_H
+#define FFTEWALD_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define EW_OK 0
+#define EW_ERR_ARGUMENT 1     /* null pointer, n <= 0, bad option value                      */
+#define EW_ERR_NOT_NEUTRAL 2  /* |sum q| > 1e-8 sum |q|  (fe/common/ewald_validation.f90:41-46) */
+#define EW_ERR_CELL 3         /* cell volume <= 1e-18    (fe/common/ewald_geometry.f90:78)      */
+#define EW_ERR_TOLERANCE 4    /* tol outside (0,1)       (fe/fft/ewald_fft_3d.f90:97-98)        */
+#define EW_ERR_COINCIDENT 5   /* two charges closer than 1e-12 Bohr (fe/fft/ewald_fft_3d_real.f90:159-160) */
+#define EW_ERR_NO_GEOMETRY 6  /* ew_shift before ew_set_geometry                                 */
+#define EW_ERR_DEVICE 7       /* HIP runtime failure                                              */
+#define EW_ERR_MEMORY 8       /* device memory allocation failed / over the admission budget     */
+#define EW_ERR_NOT_FINITE 9   /* NaN or Inf in coordinates or charges                             */
+
+/* precision modes */
+#define EW_PRECISION_FP64 0   /* every step in FP64                                               */
+#define EW_PRECISION_MIXED 1  /* FP32 spread/FFT/gather/pair coefficients, FP64 accumulation,
+                                 self term and energy                                             */
+
+typedef struct ew_handle ew_handle;
+
+/* Options; zero-initialise and set what you need.  Fields <= 0 mean "choose automatically",
+ * mirroring the optional *_in overrides of ewald_potential_force
+ * (fe/fft/ewald_fft_3d.f90:36-76). */
+typedef struct ew_options {
+  double tol;      /* requested accuracy in (0,1); 0 -> 1e-10 (fe/fft/ewald_fft_3d.f90:23)       */
+  double alpha;    /* splitting parameter override (alpha_in), Bohr^-1                           */
+  double r_cut;    /* real-space cutoff override (r_cut_in), Bohr                                */
+  int grid[3];     /* SPME grid K1,K2,K3 override (the mesh analogue of nModes_in)               */
+  int order;       /* B-spline order override (the stencil analogue of windowCutoff_in), 2..16  */
+  int precision;   /* EW_PRECISION_FP64 (default) or EW_PRECISION_MIXED                          */
+  int device;      /* HIP device ordinal                                                          */
+  int alpha_rule;  /* automatic alpha: 0 = the fast path's real-space budget rule
+                      (fe/fft/ewald_fft_3d_parameters.f90:114-121); 1 = DFTB+'s
+                      getOptimalAlphaEwald(tol) (src/dftbp/dftb/coulomb.F90:2097-2179) with
+                      r_cut = sqrt(-ln tol)/alpha (fe/common/ewald_truncation.f90:36-45)     */
+  int profile;     /* 1: record HIP events around every phase of each evaluation (ew_phase_times) */
+  int reserved[3];
+} ew_options;
+
+/* Parameters an evaluation uses; mirrors TEwaldParameters3d
+ * (fe/fft/ewald_fft_3d_parameters.f90:25-45) with the SPME mesh in place of the NFFT modes. */
+typedef struct ew_parameters {
+  double alpha;      /* splitting parameter                                                      */
+  double r_cut;      /* Ewald real-space cutoff                                                  */
+  double k_cut;      /* Fourier cutoff the mesh resolves, as a physical wavenumber              */
+  double r_list;     /* radius of the resident pair list: max(r_cut, short-gamma cutoffs)        */
+  int grid[3];       /* SPME mesh                                                                */
+  int order;         /* B-spline order                                                           */
+  int precision;
+  int pair_mode;     /* 0 = linked tiles (minimum image), 1 = dense image sum (small cells)     */
+  long long n_pairs; /* stored real-space pair coefficients                                      */
+  long long device_bytes; /* device memory held by the handle for this geometry                 */
+} ew_parameters;
+
+/* ---- resident engine (TCoulomb + TShortGamma state, kept on the device) ---- */
+
+/* Create an engine.  opts may be NULL (all defaults).  No geometry yet. */
+int ew_create(const ew_options *opts, ew_handle **out);
+int ew_destroy(ew_handle *h);
+
+/* Message of the last failure on h (or of the last failed ew_create / one-shot call when
+ * h is NULL).  Never NULL. */
+const char *ew_last_error(const ew_handle *h);
+
+/* Geometry step: TCoulomb%updateLatVecs + updateCoords and TShortGamma%updateCoords
+ * (coulomb.F90:509-620, shortgamma.F90:177-195).  Builds everything that depends only on
+ * the geometry and keeps it on the device: atom ordering, B-spline weights, the pair list
+ * with its erfc(alpha r)/r - gamma_short coefficients, FFT tables.
+ *   species   [n] in 0..n_species-1, hubbard_u [n_species] (Hartree): when both are given the
+ *             short-range gamma of shortgamma.F90 is folded into the pair coefficients and the
+ *             returned shift is the SCC shift (Coulomb + short gamma); NULL -> Coulomb only.
+ *   charges   optional [n]: only sum q^2 is read, for the automatic alpha (as
+ *             ewald_parameters does); NULL -> a unit-charge estimate. */
+int ew_set_geometry(ew_handle *h, int n, const double *coords, const double *latvecs,
+                    const int *species, const double *hubbard_u, int n_species,
+                    const double *charges);
+
+/* Parameters of the current geometry (ewald_parameters, fe/fft/ewald_fft_3d.f90:142-191). */
+int ew_get_parameters(const ew_handle *h, ew_parameters *out);
+
+/* One SCC iteration, host arrays: shift = phi(dq) and energy = 1/2 sum shift*dq
+ * (coulomb.F90:786-880, energy as printed at coulomb.F90:875).  Charges must be neutral
+ * (canUseEwaldCode, coulomb.F90:255-265).  energy may be NULL. */
+int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
+
+/* Same with device pointers (dq, shift: n doubles; energy: 1 double), enqueued on `stream`
+ * (a hipStream_t; NULL = the handle's stream).  Asynchronous: no host synchronisation, no
+ * neutrality check (validate with ew_shift or on the host). */
+int ew_shift_device(ew_handle *h, const double *d_dq, double *d_shift, double *d_energy,
+                    void *stream);
+
+/* Device time per phase of the evaluations run since the last reset, in milliseconds, with
+ * the number of evaluations; the analogue of the reference's per-phase NFFT timing
+ * [spread, transform, deconvolve] (fe/nfft/nfft.f90:228-231, 255-303).  Needs opts.profile = 1
+ * (the profiled evaluation synchronises its stream).  Phases, in order:
+ *   0 charge gather   1 spread        2 FFT x (r2c)    3 FFT y          4 FFT z + influence
+ *   5 inverse FFT y   6 FFT x (c2r)   7 pair sum       8 gather + combine + energy */
+#define EW_NUM_PHASES 9
+int ew_phase_times(ew_handle *h, double *ms, long long *calls, int reset);
+
+/* ---- one-shot calls, the reference library's own entry points ---- */
+
+/* ewald_potential_force (fe/fft/ewald_fft_3d.f90:36-134): pot[i] = phi(x_i) and, when force
+ * is not NULL, force[i][3] = -dU/dx_i (the physical force).  Ewald only (no short gamma).
+ * opts may be NULL. */
+int ew_potential_force(int n, const double *coords, const double *charges,
+                       const double *latvecs, const ew_options *opts, double *pot,
+                       double *force);
+
+/* ewald_parameters (fe/fft/ewald_fft_3d.f90:142-191) for a configuration, without a device. */
+int ew_parameters_for(int n, const double *charges, const double *latvecs,
+                      const ew_options *opts, ew_parameters *out);
+
+/* ewald_grid_memory (fe/fft/ewald_fft_3d.f90:197-244): predicted peak device memory of one
+ * engine for this configuration, in gigabytes (1024^3 bytes); the admission predictor. */
+int ew_grid_memory(int n, const double *charges, const double *latvecs, const ew_options *opts,
+                   double *gigabytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+"""Writes the golden fixtures of this directory from the reference's own Fortran modules.
+
+The reference's direct Ewald, alpha search and linked-cell real space
+(src/dftbp/extlibs/fftewald/{direct,common,fft/ewald_fft_3d_real}.f90) are compiled from
+/root/reference by ``make -C oracle ref`` into oracle/_ref/libewald_ref.so; this script calls them
+on seeded inputs and stores inputs and outputs as .npz (no pickles).  Run it in the build
+container, where /root/reference exists:
+
+    python tests/golden/make_golden.py
+
+Fixtures:
+  ref_nacl_c1.npz      the 512-atom NaCl workload: charges, alpha from DFTB+'s
+                       getOptimalAlphaEwald(1e-10) (restated; coulomb.F90 needs fypp), and the
+                       reference direct-Ewald potential and force at tol 1e-10.
+  ref_madelung.npz     ideal +-1 rocksalt, reference direct Ewald at tol 1e-12.
+  ref_triclinic.npz    300 random charges in a sheared cell, the reference's own balanced alpha,
+                       potential and force at tol 1e-12.
+  ref_small_cell.npz   40 charges in a cell too small for the linked cells (image-sum path).
+  ref_alpha.npz        balanced_alpha on a table of (minG, minR, V, tol) and its status.
+  ref_real_space.npz   the linked-cell real-space potential/force of 2000 charges.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "dftb_fast-ewald_3d_amd"))
+
+import oracle  # noqa: E402
+from fftewald_amd import workloads as W  # noqa: E402
+
+
+def main():
+    oracle.build(with_reference=True)
+    if not oracle.have_reference():
+        raise SystemExit("oracle/_ref/libewald_ref.so is missing: /root/reference not present")
+
+    # 1. the NaCl workload
+    w = W.nacl()
+    alpha = oracle.optimal_alpha_dftb(w["lattice"], 1e-10)
+    pot, force = oracle.ref_direct_ewald(w["coords"], w["charges"], w["lattice"], 1e-10,
+                                         alpha=alpha, want_force=True)
+    np.savez(os.path.join(HERE, "ref_nacl_c1.npz"), coords=w["coords"], lattice=w["lattice"],
+             charges=w["charges"], species=w["species"], hubbard_u=w["hubbard_u"],
+             alpha=np.float64(alpha), tol=np.float64(1e-10), pot=pot, force=force)
+
+    # 2. Madelung
+    w = W.nacl_ideal()
+    pot = oracle.ref_direct_ewald(w["coords"], w["charges"], w["lattice"], 1e-12)
+    np.savez(os.path.join(HERE, "ref_madelung.npz"), coords=w["coords"], lattice=w["lattice"],
+             charges=w["charges"], tol=np.float64(1e-12), pot=pot)
+
+    # 3. sheared cell, the reference's own alpha
+    rng = np.random.default_rng(20260101)
+    lat = W.lattice_from_parameters(14.0 * W.AA, 15.5 * W.AA, 13.0 * W.AA, 78.0, 101.0, 67.0)
+    n = 300
+    pos = W.poisson_disc(n, lat, 1.0 * W.AA, rng)
+    q = W.neutralise(rng.uniform(-1.0, 1.0, n))
+    pot, force = oracle.ref_direct_ewald(pos, q, lat, 1e-12, want_force=True)
+    a = oracle.ref_balanced_splitting(lat, 1e-12)
+    np.savez(os.path.join(HERE, "ref_triclinic.npz"), coords=pos, lattice=lat, charges=q,
+             tol=np.float64(1e-12), alpha=np.float64(a), pot=pot, force=force)
+
+    # 4. small cell (image sum)
+    lat = W.lattice_from_parameters(7.0 * W.AA, 7.5 * W.AA, 8.0 * W.AA, 85.0, 95.0, 100.0)
+    n = 40
+    pos = W.poisson_disc(n, lat, 1.2 * W.AA, rng)
+    q = W.neutralise(rng.uniform(-1.0, 1.0, n))
+    pot, force = oracle.ref_direct_ewald(pos, q, lat, 1e-12, want_force=True)
+    np.savez(os.path.join(HERE, "ref_small_cell.npz"), coords=pos, lattice=lat, charges=q,
+             tol=np.float64(1e-12), pot=pot, force=force)
+
+    # 5. alpha search table
+    rows = []
+    for L in (5.0, 10.0, 25.0, 42.63, 80.0, 200.0):
+        for tol in (1e-6, 1e-9, 1e-10, 1e-12):
+            for shear in (1.0, 1.3):
+                minG = 2.0 * np.pi / (L * shear)
+                minR = L
+                V = L ** 3 * shear
+                al, st = oracle.ref_balanced_alpha(minG, minR, V, tol)
+                rows.append((minG, minR, V, tol, al, st))
+    np.savez(os.path.join(HERE, "ref_alpha.npz"), table=np.array(rows, dtype=np.float64))
+
+    # 6. linked-cell real space
+    lat = W.lattice_from_parameters(28.0 * W.AA, 28.0 * W.AA, 28.0 * W.AA, 90.0, 90.0, 75.0)
+    n = 2000
+    pos = W.poisson_disc(n, lat, 1.0 * W.AA, rng)
+    q = W.neutralise(rng.uniform(-1.0, 1.0, n))
+    alpha, r_cut = 0.3, 12.0
+    pot, force = oracle.ref_short_range(pos, q, lat, alpha, r_cut)
+    np.savez(os.path.join(HERE, "ref_real_space.npz"), coords=pos, lattice=lat, charges=q,
+             alpha=np.float64(alpha), r_cut=np.float64(r_cut), pot=pot, force=force)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,85 @@
+"""C-ABI surface and host logic, without a GPU: the library loads, exports every symbol the
+header declares, and its parameter chain agrees with the reference's (through the oracle)."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+from conftest import ROOT
+
+import fftewald_amd as ew
+from fftewald_amd import workloads as W
+
+HEADER = os.path.join(ROOT, "include", "fftewald_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ew_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert set(names) == set(ew.EXPORTED)
+    L = ew.lib()
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_parameter_chain_matches_reference(oracle_mod):
+    w = W.triclinic(n=3000, side=31.0)
+    for tol in (1e-6, 1e-9, 1e-10):
+        p = ew.ewald_parameters(w["charges"], w["lattice"], tol=tol)
+        ref = oracle_mod.choose_parameters(w["lattice"], w["charges"], tol)
+        assert p["r_cut"] == pytest.approx(ref["r_cut"], rel=1e-14)
+        assert p["alpha"] == pytest.approx(ref["alpha"], rel=1e-13)
+        assert all(k % 2 == 0 or k % 3 == 0 or k % 5 == 0 for k in p["grid"])
+    # pinned cutoff: alpha follows from it (fe/fft/ewald_fft_3d_parameters.f90:114-121)
+    p = ew.ewald_parameters(w["charges"], w["lattice"], tol=1e-10, r_cut=18.0)
+    ref = oracle_mod.choose_parameters(w["lattice"], w["charges"], 1e-10, r_cut=18.0)
+    assert p["alpha"] == pytest.approx(ref["alpha"], rel=1e-13)
+    # pinned alpha: r_cut = sqrt(-ln tol)/alpha
+    p = ew.ewald_parameters(w["charges"], w["lattice"], tol=1e-10, alpha=0.3)
+    assert p["r_cut"] == pytest.approx(math.sqrt(-math.log(1e-10)) / 0.3, rel=1e-15)
+
+
+def test_dftb_alpha_rule_matches_getOptimalAlphaEwald(oracle_mod):
+    w = W.nacl()
+    p = ew.ewald_parameters(w["charges"], w["lattice"], tol=1e-10, alpha_rule="dftb",
+                            grid=16, order=8)
+    a = oracle_mod.optimal_alpha_dftb(w["lattice"], 1e-10)
+    assert p["alpha"] == pytest.approx(a, rel=1e-14)
+    assert p["r_cut"] == pytest.approx(math.sqrt(-math.log(1e-10)) / a, rel=1e-14)
+
+
+def test_grid_memory_grows_with_size_and_precision():
+    small = W.triclinic(n=2000, side=27.0)
+    big = W.triclinic(n=16000, side=54.0)
+    a = ew.ewald_grid_memory(small["charges"], small["lattice"], tol=1e-8)
+    b = ew.ewald_grid_memory(big["charges"], big["lattice"], tol=1e-8)
+    c = ew.ewald_grid_memory(big["charges"], big["lattice"], tol=1e-8, precision="mixed")
+    assert 0 < a < b and c < b
+
+
+def test_parameter_errors():
+    lat = np.eye(3) * 20.0
+    q = np.array([1.0, -1.0])
+    with pytest.raises(ew.EwaldError) as e:
+        ew.ewald_parameters(q, lat, tol=1.5)
+    assert e.value.code == ew.EW_ERR_TOLERANCE
+    with pytest.raises(ew.EwaldError) as e:
+        ew.ewald_parameters(q, np.zeros((3, 3)))
+    assert e.value.code == ew.EW_ERR_CELL
+    with pytest.raises(ew.EwaldError) as e:
+        ew.ewald_parameters(q, lat, grid=(7, 7, 7))  # 7 is not 2^a 3^b 5^c
+    assert e.value.code == ew.EW_ERR_ARGUMENT
+
+
+def test_create_without_device_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(ew.EwaldError) as e:
+        ew.EwaldEngine()
+    assert e.value.code == ew.EW_ERR_DEVICE

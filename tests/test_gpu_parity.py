@@ -1,0 +1,188 @@
+"""GPU parity of the HIP engine, through its C ABI.
+
+Tolerances (max_i |shift_i - ref_i| / max_i |ref_i| for shifts; relative for energies):
+  * against the reference's own direct Ewald (golden fixtures)           1e-10
+  * against the CPU SPME oracle at identical parameters (FP64)          1e-11
+  * mixed precision against the FP64 engine                              1e-6
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+from conftest import load_golden, rel_max
+
+import fftewald_amd as ew
+from fftewald_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def engine_shift(w, species=None, hub=None, dq=None, **opts):
+    with ew.EwaldEngine(**opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], species=species, hubbard_u=hub,
+                       charges=w["charges"])
+        s, en = e.shift(w["charges"] if dq is None else dq)
+        return s, en, e.parameters()
+
+
+def test_c1_nacl_matches_reference_direct_ewald():
+    g = load_golden("ref_nacl_c1.npz")
+    say("nacl 512 atoms, alpha=getOptimalAlphaEwald(1e-10), grid 16^3 order 8, FP64")
+    s, en, prm = engine_shift(g, tol=1e-10, alpha=float(g["alpha"]), grid=16, order=8)
+    assert prm["pair_mode"] == 1  # r_cut = 76 Bohr > cell: image-summed pairs
+    e_ref = 0.5 * float(np.dot(g["charges"], g["pot"]))
+    assert rel_max(s, g["pot"]) < 1e-10
+    assert en == pytest.approx(e_ref, rel=1e-10)
+
+
+def test_c1_alpha_rule_dftb_matches_fixture_alpha():
+    g = load_golden("ref_nacl_c1.npz")
+    say("nacl 512 atoms, alpha_rule=dftb")
+    s, en, prm = engine_shift(g, tol=1e-10, alpha_rule="dftb", grid=16, order=8)
+    assert prm["alpha"] == pytest.approx(float(g["alpha"]), rel=1e-14)
+    assert rel_max(s, g["pot"]) < 1e-10
+
+
+def test_madelung_on_gpu():
+    g = load_golden("ref_madelung.npz")
+    say("ideal rocksalt, tol 1e-12, automatic parameters")
+    s, en, prm = engine_shift(g, tol=1e-12)
+    m = -en / (len(g["charges"]) / 2) * 2.82 * W.AA
+    assert m == pytest.approx(1.747564594633182, abs=1e-10)
+    assert rel_max(s, g["pot"]) < 1e-10
+
+
+def test_triclinic_reference_own_alpha():
+    g = load_golden("ref_triclinic.npz")
+    say("300 atoms sheared cell, reference balanced alpha, tol 1e-12")
+    s, en, prm = engine_shift(g, tol=1e-12, alpha=float(g["alpha"]), order=12)
+    assert rel_max(s, g["pot"]) < 1e-10
+    assert en == pytest.approx(0.5 * float(np.dot(g["charges"], g["pot"])), rel=1e-10)
+
+
+def test_small_cell_image_sum():
+    g = load_golden("ref_small_cell.npz")
+    say("40 atoms, cell smaller than the cutoff sphere")
+    s, en, prm = engine_shift(g, tol=1e-12, order=12)
+    assert prm["pair_mode"] == 1
+    assert rel_max(s, g["pot"]) < 1e-10
+
+
+def test_c1_short_gamma_scc_shift(oracle_mod):
+    g = load_golden("ref_nacl_c1.npz")
+    say("nacl 512 atoms + short gamma (U Na 0.2, Cl 0.3)")
+    s, en, prm = engine_shift(g, species=g["species"], hub=g["hubbard_u"], tol=1e-10,
+                              alpha=float(g["alpha"]), grid=16, order=8)
+    sg = oracle_mod.short_gamma_shift(g["coords"], g["charges"], g["lattice"], g["species"],
+                                      g["hubbard_u"])
+    want = g["pot"] + sg
+    assert rel_max(s, want) < 1e-10
+    assert en == pytest.approx(0.5 * float(np.dot(g["charges"], want)), rel=1e-10)
+
+
+def _spme_reference(oracle_mod, w, prm, species=None, hub=None):
+    pot = oracle_mod.spme_ewald(w["coords"], w["charges"], w["lattice"], prm["grid"],
+                                prm["order"], prm["alpha"], prm["r_cut"])
+    if species is not None:
+        pot = pot + oracle_mod.short_gamma_shift(w["coords"], w["charges"], w["lattice"],
+                                                 species, hub)
+    return pot
+
+
+def test_c2_water_fp64_matches_cpu_spme(oracle_mod):
+    w = W.water()
+    say("water 12288 atoms, grid 64^3, order 6, r_cut 10 A, U O 0.4 H 0.3, FP64")
+    s, en, prm = engine_shift(w, species=w["species"], hub=w["hubbard_u"], tol=1e-10,
+                              r_cut=w["r_cut"], grid=w["grid"], order=w["order"])
+    assert prm["pair_mode"] == 0
+    want = _spme_reference(oracle_mod, w, prm, w["species"], w["hubbard_u"])
+    assert rel_max(s, want) < 1e-11
+    assert en == pytest.approx(0.5 * float(np.dot(w["charges"], want)), rel=1e-11)
+
+
+@pytest.mark.parametrize("K,order", [((45, 27, 50), 5), ((24, 30, 20), 4), ((36, 36, 36), 10)])
+def test_odd_grids_and_orders(oracle_mod, K, order):
+    rng = np.random.default_rng(sum(K) + order)
+    lat = W.lattice_from_parameters(22.0 * W.AA, 19.0 * W.AA, 24.0 * W.AA, 84.0, 99.0, 71.0)
+    n = 900
+    w = {"coords": W.poisson_disc(n, lat, 1.0 * W.AA, rng), "lattice": lat,
+         "charges": W.neutralise(rng.uniform(-1, 1, n))}
+    say(f"900 atoms triclinic, grid {K}, order {order}")
+    s, en, prm = engine_shift(w, tol=1e-8, grid=K, order=order)
+    want = _spme_reference(oracle_mod, w, prm)
+    assert rel_max(s, want) < 1e-11
+
+
+def test_c3_fp64_and_mixed(oracle_mod):
+    w = W.triclinic()
+    say("triclinic 98304 atoms, grid 128^3, order 6, r_cut 10 A, FP64 then mixed")
+    s64, e64, prm = engine_shift(w, tol=1e-10, r_cut=w["r_cut"], grid=w["grid"],
+                                 order=w["order"])
+    want = _spme_reference(oracle_mod, w, prm)
+    assert rel_max(s64, want) < 1e-11
+    assert e64 == pytest.approx(0.5 * float(np.dot(w["charges"], want)), rel=1e-11)
+    say("triclinic 98304 atoms, mixed precision")
+    s32, e32, prm32 = engine_shift(w, tol=1e-10, r_cut=w["r_cut"], grid=w["grid"],
+                                   order=w["order"], precision="mixed")
+    err = rel_max(s32, s64)
+    print(f"mixed vs FP64: shift {err:.3e}  energy {abs(e32 - e64) / abs(e64):.3e}")
+    assert err < 1e-6
+    assert e32 == pytest.approx(e64, rel=1e-6)
+
+
+def test_shift_device_and_repeated_calls():
+    import torch
+    w = W.water(n_side=8)
+    say("water 1536 atoms: device-pointer entry point, repeated SCC iterations")
+    rng = np.random.default_rng(7)
+    with ew.EwaldEngine(tol=1e-8) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        q = w["charges"]
+        for it in range(3):
+            q = W.scc_update(q, rng)
+            s_host, e_host = e.shift(q)
+            dq = torch.tensor(q, dtype=torch.float64, device="cuda")
+            out = torch.empty_like(dq)
+            en = torch.empty(1, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            e.shift_device(dq.data_ptr(), out.data_ptr(), en.data_ptr())
+            torch.cuda.synchronize()
+            assert rel_max(out.cpu().numpy(), s_host) < 1e-14
+            assert float(en.item()) == pytest.approx(e_host, rel=1e-14)
+
+
+def test_error_paths():
+    w = W.water(n_side=4)
+    say("error paths: no geometry, non-neutral, coincident, non-finite")
+    with ew.EwaldEngine() as e:
+        with pytest.raises(ew.EwaldError) as err:
+            e.shift(w["charges"])
+        assert err.value.code == ew.EW_ERR_NO_GEOMETRY
+        e.set_geometry(w["coords"], w["lattice"])
+        with pytest.raises(ew.EwaldError) as err:
+            e.shift(w["charges"] + 0.01)
+        assert err.value.code == ew.EW_ERR_NOT_NEUTRAL
+        q = w["charges"].copy()
+        q[3] = np.nan
+        with pytest.raises(ew.EwaldError) as err:
+            e.shift(q)
+        assert err.value.code == ew.EW_ERR_NOT_FINITE
+        bad = w["coords"].copy()
+        bad[5] = bad[9]
+        with pytest.raises(ew.EwaldError) as err:
+            e.set_geometry(bad, w["lattice"])
+        assert err.value.code == ew.EW_ERR_COINCIDENT
+
+
+def test_one_shot_potential_matches_engine():
+    g = load_golden("ref_triclinic.npz")
+    say("one-shot ewald_potential_force")
+    pot, _ = ew.ewald_potential_force(g["coords"], g["charges"], g["lattice"], tol=1e-12,
+                                      alpha=float(g["alpha"]), order=12)
+    assert rel_max(pot, g["pot"]) < 1e-10
