@@ -1,0 +1,303 @@
+!> Fortran side of the MI355X periodic Coulomb engine: iso_c_binding interfaces to the C ABI
+!> (include/fftewald_amd.h) and, on top of them, the reference library's own entry points with
+!> their exact Fortran signatures (src/dftbp/extlibs/fftewald/fft/ewald_fft_3d.f90:36-76, 142-150,
+!> 197-210), so that DFTB+'s `use ewald_fft_3d, only: ewald_potential_force` (coulomb.F90:25) can
+!> be pointed at this module instead.
+!>
+!> The resident engine (TEwaldEngine) is what TCoulomb keeps between SCC iterations: created
+!> once, given the geometry in updateCoords, evaluated in updateShifts.  It takes DFTB+'s native
+!> arrays, coords(3,N) and latVecs(3,3) with the lattice vectors as columns, without transposes.
+!>
+!> Errors: the one-shot routines keep the reference's convention (`error stop` with a message,
+!> fe/common/ewald_validation.f90:41-46); the engine routines return the C status instead.
+module fftewald_amd
+   use, intrinsic :: iso_c_binding, only: c_int, c_double, c_long_long, c_ptr, c_null_ptr, &
+                                          c_char, c_associated, c_f_pointer, c_loc
+   implicit none
+
+   private
+   public :: ew_options, ew_parameters, TEwaldEngine
+   public :: ewald_potential_force, ewald_parameters, ewald_grid_memory
+   public :: engine_create, engine_set_geometry, engine_shift, engine_destroy, engine_error
+   public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED
+
+   integer(c_int), parameter :: EW_OK = 0
+   integer(c_int), parameter :: EW_PRECISION_FP64 = 0
+   integer(c_int), parameter :: EW_PRECISION_MIXED = 1
+
+   !> Mirrors struct ew_options.
+   type, bind(C) :: ew_options
+      real(c_double) :: tol = 0.0_c_double
+      real(c_double) :: alpha = 0.0_c_double
+      real(c_double) :: r_cut = 0.0_c_double
+      integer(c_int) :: grid(3) = 0
+      integer(c_int) :: order = 0
+      integer(c_int) :: precision = 0
+      integer(c_int) :: device = 0
+      integer(c_int) :: alpha_rule = 0
+      integer(c_int) :: profile = 0
+      integer(c_int) :: reserved(3) = 0
+   end type ew_options
+
+   !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).
+   type, bind(C) :: ew_parameters
+      real(c_double) :: alpha = 0.0_c_double
+      real(c_double) :: r_cut = 0.0_c_double
+      real(c_double) :: k_cut = 0.0_c_double
+      real(c_double) :: r_list = 0.0_c_double
+      integer(c_int) :: grid(3) = 0
+      integer(c_int) :: order = 0
+      integer(c_int) :: precision = 0
+      integer(c_int) :: pair_mode = 0
+      integer(c_long_long) :: n_pairs = 0
+      integer(c_long_long) :: device_bytes = 0
+   end type ew_parameters
+
+   !> Resident engine handle.
+   type :: TEwaldEngine
+      type(c_ptr) :: handle = c_null_ptr
+      integer :: nAtom = 0
+   end type TEwaldEngine
+
+   interface
+      integer(c_int) function c_ew_create(opts, h) bind(C, name="ew_create")
+         import :: c_int, c_ptr, ew_options
+         type(ew_options), intent(in) :: opts
+         type(c_ptr), intent(out) :: h
+      end function c_ew_create
+
+      integer(c_int) function c_ew_destroy(h) bind(C, name="ew_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function c_ew_destroy
+
+      type(c_ptr) function c_ew_last_error(h) bind(C, name="ew_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: h
+      end function c_ew_last_error
+
+      integer(c_int) function c_ew_set_geometry(h, n, coords, latvecs, species, hubbard_u, &
+                                                n_species, charges) bind(C, name="ew_set_geometry")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: n, n_species
+         real(c_double), intent(in) :: coords(3, *), latvecs(3, 3)
+         type(c_ptr), value :: species, hubbard_u, charges
+      end function c_ew_set_geometry
+
+      integer(c_int) function c_ew_shift(h, dq, shift, energy) bind(C, name="ew_shift")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: dq(*)
+         real(c_double), intent(out) :: shift(*)
+         real(c_double), intent(out) :: energy
+      end function c_ew_shift
+
+      integer(c_int) function c_ew_potential_force(n, coords, charges, latvecs, opts, pot, force) &
+         bind(C, name="ew_potential_force")
+         import :: c_int, c_double, c_ptr, ew_options
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: coords(3, *), charges(*), latvecs(3, 3)
+         type(ew_options), intent(in) :: opts
+         real(c_double), intent(out) :: pot(*)
+         type(c_ptr), value :: force
+      end function c_ew_potential_force
+
+      integer(c_int) function c_ew_parameters_for(n, charges, latvecs, opts, out) &
+         bind(C, name="ew_parameters_for")
+         import :: c_int, c_double, ew_options, ew_parameters
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: charges(*), latvecs(3, 3)
+         type(ew_options), intent(in) :: opts
+         type(ew_parameters), intent(out) :: out
+      end function c_ew_parameters_for
+
+      integer(c_int) function c_ew_grid_memory(n, charges, latvecs, opts, gigabytes) &
+         bind(C, name="ew_grid_memory")
+         import :: c_int, c_double, ew_options
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: charges(*), latvecs(3, 3)
+         type(ew_options), intent(in) :: opts
+         real(c_double), intent(out) :: gigabytes
+      end function c_ew_grid_memory
+
+      integer(c_int) function c_strlen(s) bind(C, name="strlen")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+      end function c_strlen
+   end interface
+
+contains
+
+   !> Message of the last failure (of h, or of the last one-shot call when h is null).
+   function engine_error(h) result(msg)
+      type(c_ptr), intent(in) :: h
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer :: i, n
+
+      p = c_ew_last_error(h)
+      n = c_strlen(p)
+      call c_f_pointer(p, chars, [n])
+      allocate (character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = chars(i)
+      end do
+   end function engine_error
+
+   !> Options from the reference's optional overrides: nModes_in and windowCutoff_in become the
+   !> SPME mesh and B-spline order, the method this engine runs in place of the NFFT.
+   function options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in) result(o)
+      real(c_double), intent(in), optional :: tol, alpha_in, r_cut_in
+      integer, intent(in), optional :: nModes_in, windowCutoff_in
+      type(ew_options) :: o
+
+      o = ew_options()
+      if (present(tol)) o%tol = tol
+      if (present(alpha_in)) o%alpha = alpha_in
+      if (present(r_cut_in)) o%r_cut = r_cut_in
+      if (present(nModes_in)) o%grid = nModes_in
+      if (present(windowCutoff_in)) o%order = windowCutoff_in
+   end function options_from_overrides
+
+   !> fe/fft/ewald_fft_3d.f90:36-134 with the same arguments: positions(i,:) is atom i,
+   !> latVecs(i,:) is a_i; pot(i) = phi(x_i); force(:,i) = -dU/dx_i when present.
+   !> k_cut_in is accepted for signature compatibility; the mesh follows from tol or nModes_in.
+   subroutine ewald_potential_force(positions, charges, nParticle, latVecs, tol, pot, force, &
+                                    alpha_in, r_cut_in, k_cut_in, nModes_in, windowCutoff_in)
+      real(c_double), intent(in) :: positions(:, :)
+      real(c_double), intent(in) :: charges(:)
+      integer, intent(in) :: nParticle
+      real(c_double), intent(in) :: latVecs(3, 3)
+      real(c_double), intent(in), optional :: tol
+      real(c_double), intent(out) :: pot(:)
+      real(c_double), intent(out), optional, target :: force(:, :)
+      real(c_double), intent(in), optional :: alpha_in, r_cut_in, k_cut_in
+      integer, intent(in), optional :: nModes_in, windowCutoff_in
+
+      type(ew_options) :: o
+      real(c_double), allocatable :: coords(:, :)
+      real(c_double) :: lat(3, 3)
+      integer(c_int) :: st
+
+      if (nParticle <= 0) error stop "fftewald_amd: nParticle must be positive"
+      if (size(positions, 2) /= 3) error stop "fftewald_amd: positions must be (N,3)"
+      if (size(positions, 1) < nParticle .or. size(charges) < nParticle) &
+         error stop "fftewald_amd: too few positions or charges"
+      if (size(pot) < nParticle) error stop "fftewald_amd: pot must be sized (N)"
+      if (present(k_cut_in)) continue
+      o = options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in)
+      allocate (coords(3, nParticle))
+      coords = transpose(positions(1:nParticle, :))
+      lat = transpose(latVecs)
+      if (present(force)) then
+         if (size(force, 1) /= 3 .or. size(force, 2) < nParticle) &
+            error stop "fftewald_amd: force must be sized (3,N)"
+         st = c_ew_potential_force(int(nParticle, c_int), coords, charges, lat, o, pot, &
+                                   c_loc(force))
+      else
+         st = c_ew_potential_force(int(nParticle, c_int), coords, charges, lat, o, pot, c_null_ptr)
+      end if
+      if (st /= EW_OK) then
+         write (*, '(A,I0,2A)') "fftewald_amd: status ", st, ": ", engine_error(c_null_ptr)
+         error stop "fftewald_amd: ewald_potential_force failed"
+      end if
+   end subroutine ewald_potential_force
+
+   !> fe/fft/ewald_fft_3d.f90:142-191.  positions are not needed for the chain.
+   subroutine ewald_parameters(positions, charges, nParticle, latVecs, tol, params, alpha_in, &
+                               r_cut_in, k_cut_in, nModes_in, windowCutoff_in)
+      real(c_double), intent(in) :: positions(:, :)
+      real(c_double), intent(in) :: charges(:)
+      integer, intent(in) :: nParticle
+      real(c_double), intent(in) :: latVecs(3, 3)
+      real(c_double), intent(in), optional :: tol
+      type(ew_parameters), intent(out) :: params
+      real(c_double), intent(in), optional :: alpha_in, r_cut_in, k_cut_in
+      integer, intent(in), optional :: nModes_in, windowCutoff_in
+      type(ew_options) :: o
+      integer(c_int) :: st
+
+      if (size(positions, 1) < nParticle) error stop "fftewald_amd: too few positions"
+      if (present(k_cut_in)) continue
+      o = options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in)
+      st = c_ew_parameters_for(int(nParticle, c_int), charges, transpose(latVecs), o, params)
+      if (st /= EW_OK) error stop "fftewald_amd: ewald_parameters failed"
+   end subroutine ewald_parameters
+
+   !> fe/fft/ewald_fft_3d.f90:197-244: predicted device memory in gigabytes.
+   function ewald_grid_memory(charges, nParticle, latVecs, tol, alpha_in, r_cut_in, k_cut_in, &
+                              nModes_in, windowCutoff_in) result(gigabytes)
+      real(c_double), intent(in) :: charges(:)
+      integer, intent(in) :: nParticle
+      real(c_double), intent(in) :: latVecs(3, 3)
+      real(c_double), intent(in), optional :: tol, alpha_in, r_cut_in, k_cut_in
+      integer, intent(in), optional :: nModes_in, windowCutoff_in
+      real(c_double) :: gigabytes
+      type(ew_options) :: o
+      integer(c_int) :: st
+
+      if (present(k_cut_in)) continue
+      o = options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in)
+      st = c_ew_grid_memory(int(nParticle, c_int), charges, transpose(latVecs), o, gigabytes)
+      if (st /= EW_OK) error stop "fftewald_amd: ewald_grid_memory failed"
+   end function ewald_grid_memory
+
+   !> Create a resident engine (TCoulomb_init, coulomb.F90:446-506).
+   function engine_create(this, opts) result(st)
+      type(TEwaldEngine), intent(out) :: this
+      type(ew_options), intent(in) :: opts
+      integer(c_int) :: st
+
+      st = c_ew_create(opts, this%handle)
+   end function engine_create
+
+   !> Geometry step (updateCoords/updateLatVecs, coulomb.F90:509-620) with DFTB+'s native
+   !> coords(3,N) and latVecs(3,3) (columns a_i).  species/hubbardU enable the short gamma.
+   function engine_set_geometry(this, coords, latVecs, species, hubbardU, charges) result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: coords(:, :)
+      real(c_double), intent(in) :: latVecs(3, 3)
+      integer(c_int), intent(in), optional, target :: species(:)
+      real(c_double), intent(in), optional, target :: hubbardU(:)
+      real(c_double), intent(in), optional, target :: charges(:)
+      integer(c_int) :: st
+      type(c_ptr) :: ps, pu, pq
+      integer(c_int) :: ns
+
+      ps = c_null_ptr
+      pu = c_null_ptr
+      pq = c_null_ptr
+      ns = 0
+      if (present(species) .and. present(hubbardU)) then
+         ps = c_loc(species)
+         pu = c_loc(hubbardU)
+         ns = int(size(hubbardU), c_int)
+      end if
+      if (present(charges)) pq = c_loc(charges)
+      st = c_ew_set_geometry(this%handle, int(size(coords, 2), c_int), coords, latVecs, ps, pu, &
+                             ns, pq)
+      if (st == EW_OK) this%nAtom = size(coords, 2)
+   end function engine_set_geometry
+
+   !> One SCC iteration (updateShifts, coulomb.F90:786-880): shift = phi(dq), energy.
+   function engine_shift(this, deltaQAtom, shift, energy) result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: deltaQAtom(:)
+      real(c_double), intent(out) :: shift(:)
+      real(c_double), intent(out) :: energy
+      integer(c_int) :: st
+
+      st = c_ew_shift(this%handle, deltaQAtom, shift, energy)
+   end function engine_shift
+
+   subroutine engine_destroy(this)
+      type(TEwaldEngine), intent(inout) :: this
+      integer(c_int) :: st
+
+      if (c_associated(this%handle)) st = c_ew_destroy(this%handle)
+      this%handle = c_null_ptr
+   end subroutine engine_destroy
+
+end module fftewald_amd

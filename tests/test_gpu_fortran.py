@@ -1,0 +1,45 @@
+"""The Fortran host side (dftb_fast-ewald_3d_amd/fortran/fftewald_amd.f90) drives the GPU engine
+through iso_c_binding, with the reference's own ewald_potential_force signature and with
+DFTB+'s native arrays for the resident engine (tests/fortran/fortran_check.f90)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from conftest import PKG, ROOT, load_golden, rel_max
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(ROOT, "tests", "fortran", "bin", "fortran_check")
+
+
+def test_fortran_host_matches_reference(tmp_path, oracle_mod):
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} not built (run __graft_entry__.build())")
+    g = load_golden("ref_nacl_c1.npz")
+    n = len(g["charges"])
+    inp, out = tmp_path / "input.bin", tmp_path / "output.bin"
+    with open(inp, "wb") as f:
+        np.array([n], dtype=np.int32).tofile(f)
+        g["coords"].astype(np.float64).tofile(f)          # C [n][3] == Fortran coords(3,n)
+        g["charges"].astype(np.float64).tofile(f)
+        g["lattice"].astype(np.float64).tofile(f)         # C rows a_i == Fortran columns a_i
+        np.array([1e-10, float(g["alpha"])], dtype=np.float64).tofile(f)
+        np.array([16, 8, len(g["hubbard_u"])], dtype=np.int32).tofile(f)
+        g["species"].astype(np.int32).tofile(f)
+        g["hubbard_u"].astype(np.float64).tofile(f)
+    print("\n[gpu-case] fortran_check nacl 512 atoms", flush=True)
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = os.path.join(PKG, "lib") + ":" + env.get("LD_LIBRARY_PATH", "")
+    r = subprocess.run([BIN, str(inp), str(out)], env=env, capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    data = np.fromfile(out, dtype=np.float64)
+    pot, shift, energy = data[:n], data[n:2 * n], data[2 * n]
+    assert rel_max(pot, g["pot"]) < 1e-10
+    sg = oracle_mod.short_gamma_shift(g["coords"], g["charges"], g["lattice"], g["species"],
+                                      g["hubbard_u"])
+    want = g["pot"] + sg
+    assert rel_max(shift, want) < 1e-10
+    assert energy == pytest.approx(0.5 * float(np.dot(g["charges"], want)), rel=1e-10)
