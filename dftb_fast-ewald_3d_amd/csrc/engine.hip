@@ -45,6 +45,7 @@ struct ew_handle {
   int pair_mode = 0;
   long long nnz = 0;
   int T[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, ntmax = 0;
+  int spread_slots = 0;  // atoms staged per chunk by the interval spread (0: checked spread)
   FftPlan px{}, py{}, pz{};
   double self_coef = 0.0;
 
@@ -63,6 +64,7 @@ struct ew_handle {
   double* onsite = nullptr;
   long long* row_ptr = nullptr;
   int* col = nullptr;
+  short* col16 = nullptr;  // signed 16-bit partner offsets (when every row fits), else null
   void* coef = nullptr;
   void* grid = nullptr;
   void* cgrid = nullptr;
@@ -75,7 +77,22 @@ struct ew_handle {
   double* dq_io = nullptr;
   double* shift_io = nullptr;
   int* err_flag = nullptr;
+  double* force = nullptr;  // [n][3], allocated on the first force request
+  PairArgs pa{};
   size_t device_bytes = 0;
+
+  // captured evaluations, one per (dq, shift, energy) buffer triple
+  struct Captured {
+    const double* dq;
+    double* shift;
+    double* energy;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
 
   // per-phase timing (opt.profile)
   hipEvent_t ev[EW_NUM_PHASES + 1] = {};
@@ -88,11 +105,21 @@ struct ew_handle {
     return code;
   }
   void release() {
+    drop_graphs();
     for (auto& b : bufs)
       if (b.p) (void)hipFree(b.p);
     bufs.clear();
     device_bytes = 0;
     have_geom = false;
+  }
+  void free_buf(void* p) {
+    for (size_t k = 0; k < bufs.size(); ++k)
+      if (bufs[k].p == p) {
+        (void)hipFree(p);
+        device_bytes -= bufs[k].bytes;
+        bufs.erase(bufs.begin() + k);
+        return;
+      }
   }
   template <typename T>
   int alloc(T** out, size_t count) {
@@ -303,6 +330,15 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     h->nt[a] = (K[a] + h->T[a] - 1) / h->T[a];
   }
   h->ntmax = std::max(h->nt[0], std::max(h->nt[1], h->nt[2]));
+  {
+    // the interval spread needs one contiguous stencil/tile overlap per axis: K >= T + p - 1
+    bool ok = true;
+    for (int a = 0; a < 3; ++a)
+      if (K[a] < h->T[a] + p - 1) ok = false;
+    const size_t rsz = h->fp32 ? sizeof(float) : sizeof(double);
+    const size_t per = 3 * (size_t)p * rsz + 3 * sizeof(int);
+    h->spread_slots = ok ? (int)std::min<size_t>(256, kSpreadSlotBytes / per) : 0;
+  }
   std::vector<int> srctab((size_t)3 * h->ntmax * (kMaxSrcTiles + 1), 0);
   for (int a = 0; a < 3; ++a)
     if (!build_source_table(K[a], h->T[a], h->nt[a], p, h->ntmax, a, srctab))
@@ -365,7 +401,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   CHK(h->alloc(&h->row_ptr, (size_t)n + 1));
   CHK(h->alloc(&h->qs, n));
   CHK(h->alloc(&h->phi_real, n));
-  CHK(h->alloc(&h->epart, (size_t)(n + 255) / 256));
+  CHK(h->alloc(&h->epart, std::max<size_t>((size_t)(n + 255) / 256,
+                                            (size_t)h->nt[0] * h->nt[1] * h->nt[2])));
   CHK(h->alloc(&h->energy, 1));
   CHK(h->alloc(&h->dq_io, n));
   CHK(h->alloc(&h->shift_io, n));
@@ -441,13 +478,18 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   pa.ns = h->ns;
   pa.hub = h->hub;
   pa.gcut2 = h->gcut2;
+  // Linked tiles need a unique minimum image within r_list, i.e. r_list |b_a| < 1/2 on every
+  // axis (the fractional rounding of fe/fft/ewald_fft_3d_real.f90:150-157 then finds it); an
+  // axis whose partner range would wrap onto itself is swept once, tile by tile.
   bool linked = true;
   for (int a = 0; a < 3; ++a) {
     pa.nt[a] = h->nt[a];
     const double wmin = double(std::min(h->T[a], K[a] - (h->nt[a] - 1) * h->T[a])) / K[a];
     pa.D[a] = int(std::ceil(prm.r_list * c.reclen[a] / wmin));
-    if (2 * pa.D[a] + 1 > h->nt[a]) linked = false;
+    pa.full[a] = 2 * pa.D[a] + 1 > h->nt[a] ? 1 : 0;
+    if (!(prm.r_list * c.reclen[a] < 0.5)) linked = false;
   }
+  h->pa = pa;
   HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));
   const bool cf = h->fp32;
   const size_t cs = cf ? sizeof(float) : sizeof(double);
@@ -478,6 +520,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   } else {
     h->pair_mode = 1;
     for (int a = 0; a < 3; ++a) pa.D[a] = int(std::ceil(prm.r_list * c.reclen[a])) + 1;
+    h->pa = pa;
     const long long nn = (long long)n * n;
     if (nn > (1LL << 31)) return h->fail(EW_ERR_MEMORY, "cell too small for this many atoms");
     h->nnz = nn;
@@ -497,6 +540,24 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   if (errflag) {
     h->release();
     return h->fail(EW_ERR_COINCIDENT, "coincident charges");
+  }
+  // 16-bit partner offsets: rows are in tile order, so partners sit close to the row index.
+  h->col16 = nullptr;
+  if (h->pair_mode == 0 && h->nnz > 0) {
+    short* off = nullptr;
+    CHK(h->alloc(&off, (size_t)h->nnz));
+    HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));
+    HIPCHK(h, launch_compress_cols(n, h->row_ptr, h->col, off, h->err_flag, s));
+    int too_far = 0;
+    HIPCHK(h, hipMemcpyAsync(&too_far, h->err_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    if (too_far) {
+      h->free_buf(off);
+    } else {
+      h->col16 = off;
+      h->free_buf(h->col);
+      h->col = nullptr;
+    }
   }
   h->have_geom = true;
   return EW_OK;
@@ -518,13 +579,14 @@ extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
   out->order = h->prm.order;
   out->precision = h->prm.precision;
   out->pair_mode = h->pair_mode;
+  out->index_bytes = h->col16 ? 2 : 4;
   out->n_pairs = h->nnz;
   out->device_bytes = (long long)h->device_bytes;
   return EW_OK;
 }
 
-static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
-                    hipStream_t s) {
+static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                        hipStream_t s) {
   const int n = h->n;
   const int* K = h->prm.K;
   const bool prof = h->opt.profile != 0;
@@ -545,6 +607,7 @@ static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d
   }
   L.spread.ntmax = h->ntmax;
   L.spread.order = h->prm.order;
+  L.spread.slots = h->spread_slots;
   L.spread.src = h->src_tab;
   L.px = h->px;
   L.py = h->py;
@@ -564,16 +627,37 @@ static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d
     HIPCHK(h, launch_spme<float>(L, s, prof ? &h->ev[2] : nullptr));
   else
     HIPCHK(h, launch_spme<double>(L, s, prof ? &h->ev[2] : nullptr));
-  HIPCHK(h, launch_pair_spmv(n, h->row_ptr, h->col, h->coef, h->fp32, h->qs, h->phi_real, s));
+  HIPCHK(h, launch_pair_spmv(n, h->row_ptr,
+                             h->col16 ? static_cast<const void*>(h->col16) : h->col,
+                             h->col16 != nullptr, h->coef, h->fp32, h->qs, h->phi_real, s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
-  GatherArgs ga{{K[0], K[1], K[2]}, h->prm.order, h->self_coef};
-  if (h->fp32)
+  GatherArgs ga{};
+  for (int a = 0; a < 3; ++a) {
+    ga.K[a] = K[a];
+    ga.T[a] = h->T[a];
+    ga.nt[a] = h->nt[a];
+  }
+  ga.order = h->prm.order;
+  ga.self_coef = h->self_coef;
+  const int p = h->prm.order;
+  const size_t region = gather_tile_smem(h->T, p, h->fp32 ? sizeof(float) : sizeof(double));
+  if (region <= kGatherRegionMaxBytes) {
+    if (h->fp32)
+      HIPCHK(h, launch_gather_tile<float>(h->k0, h->theta, h->grid, h->qs, h->phi_real, h->onsite,
+                                          h->perm, h->tile_start, ga, d_shift, h->epart, d_energy,
+                                          s));
+    else
+      HIPCHK(h, launch_gather_tile<double>(h->k0, h->theta, h->grid, h->qs, h->phi_real,
+                                           h->onsite, h->perm, h->tile_start, ga, d_shift,
+                                           h->epart, d_energy, s));
+  } else if (h->fp32) {
     HIPCHK(h, launch_gather_combine<float>(n, h->k0, h->theta, h->grid, h->qs, h->phi_real,
                                            h->onsite, h->perm, ga, d_shift, h->epart, d_energy, s));
-  else
+  } else {
     HIPCHK(h, launch_gather_combine<double>(n, h->k0, h->theta, h->grid, h->qs, h->phi_real,
                                             h->onsite, h->perm, ga, d_shift, h->epart, d_energy,
                                             s));
+  }
   if (prof) {
     HIPCHK(h, hipEventRecord(h->ev[9], s));
     HIPCHK(h, hipEventSynchronize(h->ev[9]));
@@ -584,6 +668,38 @@ static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d
     }
     h->phase_calls++;
   }
+  return EW_OK;
+}
+
+// One evaluation on stream s.  Unless eager or profiling, the kernel sequence is captured once
+// per buffer triple on the handle's own stream and replayed as a graph on s afterwards (the
+// sequence has no memset or host work, so every replay is the same work).
+static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                    hipStream_t s) {
+  if (h->opt.eager || h->opt.profile) return enqueue_eval(h, d_dq, d_shift, d_energy, s);
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : h->graphs)
+    if (g.dq == d_dq && g.shift == d_shift && g.energy == d_energy) exec = g.exec;
+  if (!exec) {
+    if (h->graphs.size() >= 8) h->drop_graphs();
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int st = enqueue_eval(h, d_dq, d_shift, d_energy, h->stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+    if (st) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    if (ec != hipSuccess)
+      return h->fail(EW_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+      return h->fail(EW_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(ei));
+    h->graphs.push_back({d_dq, d_shift, d_energy, exec});
+  }
+  HIPCHK(h, hipGraphLaunch(exec, s));
   return EW_OK;
 }
 
@@ -634,18 +750,50 @@ extern "C" int ew_shift(ew_handle* h, const double* dq, double* shift, double* e
   return EW_OK;
 }
 
+// Physical Ewald force -dU/dx (caller order, host array) at the charges of the last evaluation,
+// which must have run on this handle's stream: the mesh still holds the convolved potential.
+static int forces_after_eval(ew_handle* h, double* force_host) {
+  const int n = h->n;
+  if (!h->force) CHK(h->alloc(&h->force, 3 * (size_t)n));
+  ForceArgs fa{};
+  for (int a = 0; a < 3; ++a) {
+    fa.K[a] = h->prm.K[a];
+    for (int d = 0; d < 3; ++d) fa.rec[3 * a + d] = h->cell.rec[a][d];
+  }
+  fa.order = h->prm.order;
+  hipStream_t s = h->stream;
+  if (h->fp32)
+    HIPCHK(h, launch_forces<float>(n, h->frac, h->k0, h->grid, h->qs, h->perm, h->tile_of,
+                                   h->tile_start, fa, h->pa, h->pair_mode == 1, h->force, s));
+  else
+    HIPCHK(h, launch_forces<double>(n, h->frac, h->k0, h->grid, h->qs, h->perm, h->tile_of,
+                                    h->tile_start, fa, h->pa, h->pair_mode == 1, h->force, s));
+  HIPCHK(h, hipMemcpyAsync(force_host, h->force, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  return EW_OK;
+}
+
+extern "C" int ew_gradients(ew_handle* h, const double* dq, double* grad) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_gradients: handle is NULL");
+  if (!grad) return h->fail(EW_ERR_ARGUMENT, "ew_gradients: NULL array");
+  if (!h->have_geom) return h->fail(EW_ERR_NO_GEOMETRY, "ew_gradients before ew_set_geometry");
+  std::vector<double> shift(h->n), f(3 * (size_t)h->n);
+  CHK(ew_shift(h, dq, shift.data(), nullptr));
+  CHK(forces_after_eval(h, f.data()));
+  for (size_t k = 0; k < f.size(); ++k) grad[k] -= f[k];
+  return EW_OK;
+}
+
 extern "C" int ew_potential_force(int n, const double* coords, const double* charges,
                                   const double* latvecs, const ew_options* opts, double* pot,
                                   double* force) {
   if (!charges || !pot) return set_global_error(EW_ERR_ARGUMENT, "ew_potential_force: NULL array");
-  if (force)
-    return set_global_error(EW_ERR_ARGUMENT,
-                            "ew_potential_force: forces are not available in this build");
   ew_handle* h = nullptr;
   int st = ew_create(opts, &h);
   if (st) return st;
   st = ew_set_geometry(h, n, coords, latvecs, nullptr, nullptr, 0, charges);
   if (!st) st = ew_shift(h, charges, pot, nullptr);
+  if (!st && force) st = forces_after_eval(h, force);
   if (st) g_last_error = h->err;
   ew_destroy(h);
   return st;

@@ -290,15 +290,18 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
 // One workgroup per output tile; atoms are listed by the tile of their stencil origin, and the
 // workgroup visits the source tiles whose stencils reach into it.  Each contribution lands in
 // exactly one output tile, so the mesh is written once, without global atomics or a clear.
+// Work items are (candidate atom, z offset) pairs, so every thread gets a similar share.
 template <typename R>
-__global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
+__global__ __launch_bounds__(256) void k_spread_checked(const int4* __restrict__ k0,
                                                 const R* __restrict__ theta,
                                                 const double* __restrict__ q,
                                                 const int* __restrict__ tile_start,
                                                 SpreadArgs g, R* __restrict__ grid) {
+  constexpr int kMaxVisit = kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles;
   __shared__ R acc[kSpreadTileMax];
-  __shared__ int src_first[kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles + 1];
-  __shared__ int src_begin[kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles];
+  __shared__ int src_count[kMaxVisit];
+  __shared__ int src_first[kMaxVisit + 1];
+  __shared__ int src_begin[kMaxVisit];
   const int p = g.order;
   int t = blockIdx.x;
   const int t1 = t % g.nt[0];
@@ -310,55 +313,55 @@ __global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
             w3 = min(g.T[2], g.K[2] - lo3);
   const int npts = w1 * w2 * w3;
   for (int i = threadIdx.x; i < npts; i += blockDim.x) acc[i] = R(0);
-  // flatten the (source tile, atom) visits
   const int* e1 = g.src + ((size_t)0 * g.ntmax + t1) * (kMaxSrcTiles + 1);
   const int* e2 = g.src + ((size_t)1 * g.ntmax + t2) * (kMaxSrcTiles + 1);
   const int* e3 = g.src + ((size_t)2 * g.ntmax + t3) * (kMaxSrcTiles + 1);
   const int n1 = e1[0], n2 = e2[0], n3 = e3[0];
   const int ns = n1 * n2 * n3;
+  for (int c = threadIdx.x; c < ns; c += blockDim.x) {
+    const int c1 = c % n1, c2 = (c / n1) % n2, c3 = c / (n1 * n2);
+    const int st = (e3[1 + c3] * g.nt[1] + e2[1 + c2]) * g.nt[0] + e1[1 + c1];
+    const int b = tile_start[st];
+    src_begin[c] = b;
+    src_count[c] = tile_start[st + 1] - b;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     int run = 0;
     for (int c = 0; c < ns; ++c) {
-      const int c1 = c % n1, c2 = (c / n1) % n2, c3 = c / (n1 * n2);
-      const int s1 = e1[1 + c1], s2 = e2[1 + c2], s3 = e3[1 + c3];
-      const int st = (s3 * g.nt[1] + s2) * g.nt[0] + s1;
       src_first[c] = run;
-      src_begin[c] = tile_start[st];
-      run += tile_start[st + 1] - tile_start[st];
+      run += src_count[c];
     }
     src_first[ns] = run;
   }
   __syncthreads();
-  const int nvisit = src_first[ns];
-  for (int v = threadIdx.x; v < nvisit; v += blockDim.x) {
+  const int nitems = src_first[ns] * p;
+  for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+    const int v = it / p, o3 = it - v * p;
     int c = 0;
     while (src_first[c + 1] <= v) ++c;
     const int a = src_begin[c] + (v - src_first[c]);
     const int4 kk = k0[a];
-    // offsets o whose mesh point (k0 + o) mod K falls inside [lo, lo + w)
-    int r1 = kk.x - lo1, r2 = kk.y - lo2, r3 = kk.z - lo3;
+    int z = kk.z - lo3 + o3;  // position relative to the tile, modulo K3
+    if (z < 0) z += g.K[2];
+    if (z >= g.K[2]) z -= g.K[2];
+    if (z >= w3) continue;
+    int r1 = kk.x - lo1, r2 = kk.y - lo2;
     if (r1 < 0) r1 += g.K[0];
     if (r2 < 0) r2 += g.K[1];
-    if (r3 < 0) r3 += g.K[2];
     const R* th = theta + (size_t)a * 3 * p;
-    const R qa = R(q[a]);
-    for (int o3 = 0; o3 < p; ++o3) {
-      int z = r3 + o3;
-      if (z >= g.K[2]) z -= g.K[2];
-      if (z >= w3) continue;
-      const R c3 = qa * th[2 * p + o3];
-      for (int o2 = 0; o2 < p; ++o2) {
-        int y = r2 + o2;
-        if (y >= g.K[1]) y -= g.K[1];
-        if (y >= w2) continue;
-        const R c23 = c3 * th[p + o2];
-        R* row = acc + (z * w2 + y) * w1;
-        for (int o1 = 0; o1 < p; ++o1) {
-          int x = r1 + o1;
-          if (x >= g.K[0]) x -= g.K[0];
-          if (x >= w1) continue;
-          atomicAdd(row + x, c23 * th[o1]);
-        }
+    const R c3 = R(q[a]) * th[2 * p + o3];
+    for (int o2 = 0; o2 < p; ++o2) {
+      int y = r2 + o2;
+      if (y >= g.K[1]) y -= g.K[1];
+      if (y >= w2) continue;
+      const R c23 = c3 * th[p + o2];
+      R* row = acc + (z * w2 + y) * w1;
+      for (int o1 = 0; o1 < p; ++o1) {
+        int x = r1 + o1;
+        if (x >= g.K[0]) x -= g.K[0];
+        if (x >= w1) continue;
+        atomicAdd(row + x, c23 * th[o1]);
       }
     }
   }
@@ -366,6 +369,144 @@ __global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
   for (int i = threadIdx.x; i < npts; i += blockDim.x) {
     const int x = i % w1, y = (i / w1) % w2, z = i / (w1 * w2);
     grid[((size_t)(lo3 + z) * g.K[1] + (lo2 + y)) * g.K[0] + lo1 + x] = acc[i];
+  }
+}
+
+// Fast spreading for meshes with K >= T + p - 1 on every axis, where the stencil of an atom
+// meets a tile in one contiguous interval per axis.  Per output tile: the candidate atoms are
+// staged in LDS in chunks -- only those that reach the tile, with their weights (q folded into
+// the z weights) and the exact valid offset interval per axis -- and the accumulation then runs
+// over those intervals without per-point tests or global loads.
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
+                                                const R* __restrict__ theta,
+                                                const double* __restrict__ q,
+                                                const int* __restrict__ tile_start,
+                                                SpreadArgs g, R* __restrict__ grid) {
+  constexpr int kMaxVisit = kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles;
+  extern __shared__ unsigned char smem_raw[];
+  const int p = P > 0 ? P : g.order;
+  const int cap = g.slots;
+  R* acc = reinterpret_cast<R*>(smem_raw);                   // [kSpreadTileMax]
+  R* w1s = acc + kSpreadTileMax;                              // [cap][p]
+  R* w2s = w1s + (size_t)cap * p;                             // [cap][p]
+  R* w3s = w2s + (size_t)cap * p;                             // [cap][p], times q
+  int* iv = reinterpret_cast<int*>(w3s + (size_t)cap * p);    // [cap][3]: a|b<<5|pos<<10
+  __shared__ int src_count[kMaxVisit];
+  __shared__ int src_first[kMaxVisit + 1];
+  __shared__ int src_begin[kMaxVisit];
+  __shared__ int nslot;
+  int t = blockIdx.x;
+  const int t1 = t % g.nt[0];
+  t /= g.nt[0];
+  const int t2 = t % g.nt[1];
+  const int t3 = t / g.nt[1];
+  const int lo[3] = {t1 * g.T[0], t2 * g.T[1], t3 * g.T[2]};
+  const int w[3] = {min(g.T[0], g.K[0] - lo[0]), min(g.T[1], g.K[1] - lo[1]),
+                    min(g.T[2], g.K[2] - lo[2])};
+  const int npts = w[0] * w[1] * w[2];
+  // odd row and plane pitches keep lanes that differ in y or z on different LDS banks
+  const int PY = w[0] | 1, PZ = (PY * w[1]) | 1;
+  for (int i = threadIdx.x; i < PZ * w[2]; i += blockDim.x) acc[i] = R(0);
+  const int* e1 = g.src + ((size_t)0 * g.ntmax + t1) * (kMaxSrcTiles + 1);
+  const int* e2 = g.src + ((size_t)1 * g.ntmax + t2) * (kMaxSrcTiles + 1);
+  const int* e3 = g.src + ((size_t)2 * g.ntmax + t3) * (kMaxSrcTiles + 1);
+  const int n1 = e1[0], n2 = e2[0], n3 = e3[0];
+  const int ns = n1 * n2 * n3;
+  for (int c = threadIdx.x; c < ns; c += blockDim.x) {
+    const int c1 = c % n1, c2 = (c / n1) % n2, c3 = c / (n1 * n2);
+    const int st = (e3[1 + c3] * g.nt[1] + e2[1 + c2]) * g.nt[0] + e1[1 + c1];
+    const int b = tile_start[st];
+    src_begin[c] = b;
+    src_count[c] = tile_start[st + 1] - b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int c = 0; c < ns; ++c) {
+      src_first[c] = run;
+      run += src_count[c];
+    }
+    src_first[ns] = run;
+  }
+  __syncthreads();
+  const int nvisit = src_first[ns];
+  for (int chunk = 0; chunk < nvisit; chunk += cap) {
+    if (threadIdx.x == 0) nslot = 0;
+    __syncthreads();
+    const int v = chunk + (int)threadIdx.x;
+    if ((int)threadIdx.x < cap && v < nvisit) {
+      int c = 0;
+      while (src_first[c + 1] <= v) ++c;
+      const int a = src_begin[c] + (v - src_first[c]);
+      const int4 kk = k0[a];
+      const int kv[3] = {kk.x, kk.y, kk.z};
+      int pk[3];
+      bool hit = true;
+      for (int ax = 0; ax < 3; ++ax) {
+        int r = kv[ax] - lo[ax];
+        if (r < 0) r += g.K[ax];
+        int ia, ib, pos;
+        if (r < w[ax]) {  // stencil starts inside the tile
+          ia = 0;
+          ib = min(p, w[ax] - r);
+          pos = r;
+        } else {          // it reaches the tile only after wrapping past K
+          ia = g.K[ax] - r;
+          ib = min(p, g.K[ax] - r + w[ax]);
+          pos = 0;
+        }
+        if (ia >= ib) hit = false;
+        pk[ax] = ia | (ib << 5) | (pos << 10);
+      }
+      if (hit) {
+        const int sl = atomicAdd(&nslot, 1);
+        const R* th = theta + (size_t)a * 3 * p;
+        const R qa = R(q[a]);
+        for (int o = 0; o < p; ++o) {
+          w1s[sl * p + o] = th[o];
+          w2s[sl * p + o] = th[p + o];
+          w3s[sl * p + o] = qa * th[2 * p + o];
+        }
+        iv[3 * sl] = pk[0];
+        iv[3 * sl + 1] = pk[1];
+        iv[3 * sl + 2] = pk[2];
+      }
+    }
+    __syncthreads();
+    const int pp = p * p;
+    const int nitems = nslot * pp;  // one (atom, z, y) row per item, an x interval each
+    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+      const int sl = it / pp, r = it - sl * pp;
+      const int o3 = r / p, o2 = r - o3 * p;
+      const int i3 = iv[3 * sl + 2], i2 = iv[3 * sl + 1];
+      const int a3 = i3 & 31, b3 = (i3 >> 5) & 31;
+      const int a2 = i2 & 31, b2 = (i2 >> 5) & 31;
+      if (o3 < a3 || o3 >= b3 || o2 < a2 || o2 >= b2) continue;
+      const int i1 = iv[3 * sl];
+      const int a1 = i1 & 31, b1 = (i1 >> 5) & 31, x0 = i1 >> 10;
+      const int z = (i3 >> 10) + o3 - a3, y = (i2 >> 10) + o2 - a2;
+      const R c23 = w3s[sl * p + o3] * w2s[sl * p + o2];
+      const R* wx = w1s + sl * p;
+      R* row = acc + z * PZ + y * PY + x0 - a1;
+      if (P > 0) {
+        // weights into registers first, then the predicated adds back to back
+        R wr[P > 0 ? P : 1];
+#pragma unroll
+        for (int o1 = 0; o1 < P; ++o1) wr[o1] = wx[o1];
+#pragma unroll
+        for (int o1 = 0; o1 < P; ++o1)
+          if (o1 >= a1 && o1 < b1) atomicAdd(row + o1, c23 * wr[o1]);
+      } else {
+        for (int o1 = a1; o1 < b1; ++o1) atomicAdd(row + o1, c23 * wx[o1]);
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+    const int x = i % w[0], y = (i / w[0]) % w[1], z = i / (w[0] * w[1]);
+    grid[((size_t)(lo[2] + z) * g.K[1] + (lo[1] + y)) * g.K[0] + lo[0] + x] =
+        acc[z * PZ + y * PY + x];
   }
 }
 
@@ -385,10 +526,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Partner index of a stored pair: absolute (int) or a signed 16-bit offset from the row
+// (cyclic in n), which halves the index stream when rows are in spatial order.
+__device__ __forceinline__ int partner(int row, int c, int) { return c; }
+__device__ __forceinline__ int partner(int row, short c, int n) {
+  int j = row + int(c);
+  j += j < 0 ? n : 0;
+  j -= j >= n ? n : 0;
+  return j;
+}
+
 // Real-space pair sum: out[i] = sum_j coef_ij q_j, one 64-lane wave per atom (row).
-template <typename CT>
+template <typename CT, typename IT>
 __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __restrict__ row_ptr,
-                                                   const int* __restrict__ col,
+                                                   const IT* __restrict__ col,
                                                    const CT* __restrict__ coef,
                                                    const double* __restrict__ q,
                                                    double* __restrict__ out) {
@@ -397,9 +548,28 @@ __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __res
   if (row >= n) return;  // uniform across the wave
   const long long a = row_ptr[row], b = row_ptr[row + 1];
   double acc = 0.0;
-  for (long long k = a + lane; k < b; k += 64) acc += double(coef[k]) * q[col[k]];
+  for (long long k = a + lane; k < b; k += 64) acc += double(coef[k]) * q[partner(row, col[k], n)];
   acc = wave_sum(acc);
   if (lane == 0) out[row] = acc;
+}
+
+// Rewrites absolute partner indices as signed 16-bit offsets from the row; flags rows whose
+// partners lie further than +-32767 atoms away (then the 32-bit list is kept).
+__global__ void k_compress_cols(int n, const long long* __restrict__ row_ptr,
+                                const int* __restrict__ col, short* __restrict__ off,
+                                int* __restrict__ too_far) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (long long k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+    int d = col[k] - i;
+    if (d >= (n + 1) / 2) d -= n;
+    if (d < -(n / 2)) d += n;
+    if (d < -32768 || d > 32767) {
+      *too_far = 1;
+      d = 0;
+    }
+    off[k] = short(d);
+  }
 }
 
 // B-spline gather of the convolved mesh, plus the real-space sum, the on-site short-gamma
@@ -456,6 +626,102 @@ __global__ __launch_bounds__(256) void k_gather_combine(
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
     epart[blockIdx.x] = s;
   }
+}
+
+// B-spline gather per mesh tile: the tile's atoms (their stencil origins lie in it) read the
+// tile plus a (p-1)-point halo, staged once in LDS; four lanes share an atom's p^2 rows and are
+// reduced by shuffles.  Then the pair sum, on-site and self terms are added and the shift is
+// written in caller order; one energy partial per tile.
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_gather_tile(
+    const int4* __restrict__ k0, const R* __restrict__ theta, const R* __restrict__ grid,
+    const double* __restrict__ q, const double* __restrict__ phi_real,
+    const double* __restrict__ onsite, const int* __restrict__ perm,
+    const int* __restrict__ tile_start, GatherArgs g, double* __restrict__ shift,
+    double* __restrict__ epart) {
+  extern __shared__ unsigned char smem_raw[];
+  R* ths = reinterpret_cast<R*>(smem_raw);  // [64][3p] weights of the 64 atoms of one pass
+  R* reg = ths + 64 * 3 * g.order;          // staged mesh region
+  __shared__ double red[4];
+  const int p = P > 0 ? P : g.order;
+  int t = blockIdx.x;
+  const int t1 = t % g.nt[0];
+  t /= g.nt[0];
+  const int t2 = t % g.nt[1];
+  const int t3 = t / g.nt[1];
+  const int lo1 = t1 * g.T[0], lo2 = t2 * g.T[1], lo3 = t3 * g.T[2];
+  const int E1 = min(g.T[0], g.K[0] - lo1) + p - 1, E2 = min(g.T[1], g.K[1] - lo2) + p - 1,
+            E3 = min(g.T[2], g.K[2] - lo3) + p - 1;
+  const int nreg = E1 * E2 * E3;
+  for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
+    const int x = i % E1, y = (i / E1) % E2, z = i / (E1 * E2);
+    int gx = lo1 + x, gy = lo2 + y, gz = lo3 + z;
+    while (gx >= g.K[0]) gx -= g.K[0];
+    while (gy >= g.K[1]) gy -= g.K[1];
+    while (gz >= g.K[2]) gz -= g.K[2];
+    reg[i] = grid[((size_t)gz * g.K[1] + gy) * g.K[0] + gx];
+  }
+  __syncthreads();
+  const int a0 = tile_start[blockIdx.x], a1 = tile_start[blockIdx.x + 1];
+  const int sub = threadIdx.x & 3, grp = threadIdx.x >> 2;
+  const int rows = p * p;
+  double e = 0.0;
+  for (int base = a0; base < a1; base += 64) {  // uniform trip count across the workgroup
+    const int a = base + grp;
+    const bool live = a < a1;
+    const int nb = min(64, a1 - base) * 3 * p;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) ths[i] = theta[(size_t)base * 3 * p + i];
+    __syncthreads();
+    double part = 0.0;
+    if (live) {
+      const int4 kk = k0[a];
+      int r1 = kk.x - lo1, r2 = kk.y - lo2, r3 = kk.z - lo3;
+      if (r1 < 0) r1 += g.K[0];
+      if (r2 < 0) r2 += g.K[1];
+      if (r3 < 0) r3 += g.K[2];
+      const R* th = ths + grp * 3 * p;
+      R acc = R(0), comp = R(0);
+      R w1r[P > 0 ? P : 1];
+      if (P > 0) {
+#pragma unroll
+        for (int o1 = 0; o1 < (P > 0 ? P : 1); ++o1) w1r[o1] = th[o1];
+      }
+      for (int r = sub; r < rows; r += 4) {
+        const int o3 = r / p, o2 = r - o3 * p;
+        const R* row = reg + ((r3 + o3) * E2 + (r2 + o2)) * E1 + r1;
+        R v = R(0);
+        if (P > 0) {
+          R rv[P > 0 ? P : 1];
+#pragma unroll
+          for (int o1 = 0; o1 < (P > 0 ? P : 1); ++o1) rv[o1] = row[o1];
+#pragma unroll
+          for (int o1 = 0; o1 < (P > 0 ? P : 1); ++o1) v += w1r[o1] * rv[o1];
+        } else {
+          for (int o1 = 0; o1 < p; ++o1) v += th[o1] * row[o1];
+        }
+        // compensated (Kahan) accumulation of the row partials
+        const R yv = v * th[p + o2] * th[2 * p + o3] - comp;
+        const R tv = acc + yv;
+        comp = (tv - acc) - yv;
+        acc = tv;
+      }
+      part = double(acc) - double(comp);
+    }
+    part += __shfl_xor(part, 1, 64);
+    part += __shfl_xor(part, 2, 64);
+    if (live && sub == 0) {
+      const double qa = q[a];
+      const double phi =
+          phi_real[a] + part + (onsite ? onsite[a] : 0.0) * qa + g.self_coef * qa;
+      shift[perm[a]] = phi;
+      e += 0.5 * phi * qa;
+    }
+  }
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) epart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 __global__ void k_energy_final(int nparts, const double* __restrict__ epart,
@@ -557,12 +823,15 @@ __global__ void k_pair_tiles(int n, const double* __restrict__ frac, const int* 
   long long pos = COUNT ? 0 : row_ptr[i];
   const long long end = COUNT ? 0 : row_ptr[i + 1];
   int cnt = 0;
-  for (int d3 = -g.D[2]; d3 <= g.D[2]; ++d3) {
-    const int u3 = ((t3 + d3) % g.nt[2] + g.nt[2]) % g.nt[2];
-    for (int d2 = -g.D[1]; d2 <= g.D[1]; ++d2) {
-      const int u2 = ((t2 + d2) % g.nt[1] + g.nt[1]) % g.nt[1];
-      for (int d1 = -g.D[0]; d1 <= g.D[0]; ++d1) {
-        const int u1 = ((t1 + d1) % g.nt[0] + g.nt[0]) % g.nt[0];
+  const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
+  const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
+  const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
+  for (int c3 = 0; c3 < n3; ++c3) {
+    const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
+    for (int c2 = 0; c2 < n2; ++c2) {
+      const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
+      for (int c1 = 0; c1 < n1; ++c1) {
+        const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
         const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
         for (int j = tile_start[tt]; j < tile_start[tt + 1]; ++j) {
           if (j == i) continue;
@@ -631,6 +900,137 @@ __global__ void k_pair_dense(int n, const double* __restrict__ frac, const int* 
 }
 
 // ----------------------------------------------------------------------------------------
+// forces (once per geometry step; fe/fft/ewald_fft_3d.f90:128-132)
+// ----------------------------------------------------------------------------------------
+
+// Reciprocal force from the convolved mesh: F_i = -q_i sum_k grad theta_i(k) Phi(k), with
+// grad theta = sum_a dtheta/du_a K_a b_a.  Weights and derivatives are rebuilt from frac.
+template <typename R>
+__global__ void k_force_recip(int n, const double* __restrict__ frac, const int4* __restrict__ k0,
+                              const R* __restrict__ grid, const double* __restrict__ q,
+                              const int* __restrict__ perm, ForceArgs g,
+                              double* __restrict__ force) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int p = g.order;
+  double th[3][kMaxOrder], dth[3][kMaxOrder];
+  for (int a = 0; a < 3; ++a) {
+    const double u = frac[3 * k + a] * g.K[a];
+    const double w = u - floor(u);
+    double c[kMaxOrder + 1], cp[kMaxOrder + 1];
+    c[0] = w;
+    c[1] = 1.0 - w;
+    cp[0] = 1.0;
+    cp[1] = 0.0;
+    for (int kk = 2; kk < p; ++kk) {
+      if (kk == p - 1)
+        for (int j = 0; j < kk; ++j) cp[j] = c[j];
+      c[kk] = (1.0 - w) * c[kk - 1] / kk;
+      for (int j = kk - 1; j >= 1; --j) c[j] = ((w + j) * c[j] + (kk + 1 - w - j) * c[j - 1]) / kk;
+      c[0] = w * c[0] / kk;
+    }
+    for (int o = 0; o < p; ++o) {
+      const int j = p - 1 - o;
+      th[a][o] = c[j];
+      dth[a][o] = (j < p - 1 ? cp[j] : 0.0) - (j > 0 ? cp[j - 1] : 0.0);
+    }
+  }
+  const int4 kk = k0[k];
+  double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+  for (int o3 = 0; o3 < p; ++o3) {
+    int z = kk.z + o3;
+    if (z >= g.K[2]) z -= g.K[2];
+    for (int o2 = 0; o2 < p; ++o2) {
+      int y = kk.y + o2;
+      if (y >= g.K[1]) y -= g.K[1];
+      const R* row = grid + ((size_t)z * g.K[1] + y) * g.K[0];
+      double v0 = 0.0, v1 = 0.0;
+      for (int o1 = 0; o1 < p; ++o1) {
+        int x = kk.x + o1;
+        if (x >= g.K[0]) x -= g.K[0];
+        const double v = double(row[x]);
+        v0 += th[0][o1] * v;
+        v1 += dth[0][o1] * v;
+      }
+      d1 += v1 * th[1][o2] * th[2][o3];
+      d2 += v0 * dth[1][o2] * th[2][o3];
+      d3 += v0 * th[1][o2] * dth[2][o3];
+    }
+  }
+  const double qk = q[k];
+  const double s1 = d1 * g.K[0], s2 = d2 * g.K[1], s3 = d3 * g.K[2];
+  const int i = perm[k];
+  for (int d = 0; d < 3; ++d)
+    force[3 * i + d] = -qk * (s1 * g.rec[d] + s2 * g.rec[3 + d] + s3 * g.rec[6 + d]);
+}
+
+// Real-space Ewald force, added to force[]: sum_j q_i q_j (2a/sqrt(pi) exp(-a^2 r^2) +
+// erfc(a r)/r) / r^2 (x_i - x_j), over linked tiles (minimum image) or, in DENSE mode, over
+// every partner and image (fe/fft/ewald_fft_3d_real.f90:166-172, direct_3d.f90:284-297).
+template <bool DENSE>
+__global__ void k_force_real(int n, const double* __restrict__ frac, const int* __restrict__ tile_of,
+                             const int* __restrict__ tile_start, const double* __restrict__ q,
+                             const int* __restrict__ perm, PairArgs g, double* __restrict__ force) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double fi0 = frac[3 * i], fi1 = frac[3 * i + 1], fi2 = frac[3 * i + 2];
+  const double two_a_sqrtpi = 2.0 * g.alpha / kSqrtPi;
+  double f[3] = {0.0, 0.0, 0.0};
+  auto add = [&](double s0, double s1, double s2, int j) {
+    const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+    const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+    const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+    const double r2 = x * x + y * y + z * z;
+    if (r2 > g.rc2 || r2 < kMinSeparationSq) return;
+    const double r = sqrt(r2);
+    const double c = q[j] * (two_a_sqrtpi * exp(-g.alpha * g.alpha * r2) + erfc(g.alpha * r) / r) / r2;
+    f[0] += c * x;
+    f[1] += c * y;
+    f[2] += c * z;
+  };
+  if (DENSE) {
+    for (int j = 0; j < n; ++j) {
+      const double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
+      for (int a = -g.D[0]; a <= g.D[0]; ++a)
+        for (int b = -g.D[1]; b <= g.D[1]; ++b)
+          for (int e = -g.D[2]; e <= g.D[2]; ++e) {
+            if (i == j && a == 0 && b == 0 && e == 0) continue;
+            add(s0 + a, s1 + b, s2 + e, j);
+          }
+    }
+  } else {
+    int t = tile_of[i];
+    const int t1 = t % g.nt[0];
+    t /= g.nt[0];
+    const int t2 = t % g.nt[1], t3 = t / g.nt[1];
+    const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
+    const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
+    const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
+    for (int c3 = 0; c3 < n3; ++c3) {
+      const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
+      for (int c2 = 0; c2 < n2; ++c2) {
+        const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
+        for (int c1 = 0; c1 < n1; ++c1) {
+          const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
+          const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
+          for (int j = tile_start[tt]; j < tile_start[tt + 1]; ++j) {
+            if (j == i) continue;
+            double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
+            s0 -= rint(s0);
+            s1 -= rint(s1);
+            s2 -= rint(s2);
+            add(s0, s1, s2, j);
+          }
+        }
+      }
+    }
+  }
+  const double qi = q[i];
+  const int o = perm[i];
+  for (int d = 0; d < 3; ++d) force[3 * o + d] += qi * f[d];
+}
+
+// ----------------------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------------------
 
@@ -650,9 +1050,29 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
   const cx<R>* tw1 = static_cast<const cx<R>*>(L.tw[0]);
   const cx<R>* tw2 = static_cast<const cx<R>*>(L.tw[1]);
   const cx<R>* tw3 = static_cast<const cx<R>*>(L.tw[2]);
-  hipLaunchKernelGGL(k_spread<R>, dim3(L.spread.nt[0] * L.spread.nt[1] * L.spread.nt[2]),
-                     dim3(256), 0, st, L.k0, static_cast<const R*>(L.theta), L.q, L.tile_start,
-                     L.spread, grid);
+  const dim3 sgrid(L.spread.nt[0] * L.spread.nt[1] * L.spread.nt[2]);
+  if (L.spread.slots > 0) {
+    const size_t smem = sizeof(R) * (kSpreadTileMax + 3 * (size_t)L.spread.slots * L.spread.order) +
+                        sizeof(int) * 3 * (size_t)L.spread.slots;
+    const R* th = static_cast<const R*>(L.theta);
+#define EW_SPREAD(PP)                                                                          \
+  hipLaunchKernelGGL((k_spread<R, PP>), sgrid, dim3(256), smem, st, L.k0, th, L.q, L.tile_start, \
+                     L.spread, grid)
+    switch (L.spread.order) {
+      case 4: EW_SPREAD(4); break;
+      case 5: EW_SPREAD(5); break;
+      case 6: EW_SPREAD(6); break;
+      case 7: EW_SPREAD(7); break;
+      case 8: EW_SPREAD(8); break;
+      case 10: EW_SPREAD(10); break;
+      case 12: EW_SPREAD(12); break;
+      default: EW_SPREAD(0); break;
+    }
+#undef EW_SPREAD
+  } else {
+    hipLaunchKernelGGL(k_spread_checked<R>, sgrid, dim3(256), 0, st, L.k0,
+                       static_cast<const R*>(L.theta), L.q, L.tile_start, L.spread, grid);
+  }
   mark(0);
   const int nrows = K2 * K3;
   const int gx = (nrows + 2 * L.px.lines - 1) / (2 * L.px.lines);
@@ -688,15 +1108,34 @@ hipError_t launch_gather_charges(int n, const int* perm, const double* dq, doubl
   return hipGetLastError();
 }
 
-hipError_t launch_pair_spmv(int n, const long long* row_ptr, const int* col, const void* coef,
-                            bool fp32, const double* q, double* out, hipStream_t st) {
+hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
+                            const void* coef, bool fp32, const double* q, double* out,
+                            hipStream_t st) {
   const dim3 grid((n + 3) / 4), block(256);
-  if (fp32)
-    hipLaunchKernelGGL(k_pair_spmv<float>, grid, block, 0, st, n, row_ptr, col,
-                       static_cast<const float*>(coef), q, out);
-  else
-    hipLaunchKernelGGL(k_pair_spmv<double>, grid, block, 0, st, n, row_ptr, col,
-                       static_cast<const double*>(coef), q, out);
+  if (col16) {
+    const short* c = static_cast<const short*>(col);
+    if (fp32)
+      hipLaunchKernelGGL((k_pair_spmv<float, short>), grid, block, 0, st, n, row_ptr, c,
+                         static_cast<const float*>(coef), q, out);
+    else
+      hipLaunchKernelGGL((k_pair_spmv<double, short>), grid, block, 0, st, n, row_ptr, c,
+                         static_cast<const double*>(coef), q, out);
+  } else {
+    const int* c = static_cast<const int*>(col);
+    if (fp32)
+      hipLaunchKernelGGL((k_pair_spmv<float, int>), grid, block, 0, st, n, row_ptr, c,
+                         static_cast<const float*>(coef), q, out);
+    else
+      hipLaunchKernelGGL((k_pair_spmv<double, int>), grid, block, 0, st, n, row_ptr, c,
+                         static_cast<const double*>(coef), q, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_compress_cols(int n, const long long* row_ptr, const int* col, short* off,
+                                int* too_far, hipStream_t st) {
+  hipLaunchKernelGGL(k_compress_cols, dim3((n + 255) / 256), dim3(256), 0, st, n, row_ptr, col,
+                     off, too_far);
   return hipGetLastError();
 }
 
@@ -712,6 +1151,40 @@ hipError_t launch_gather_combine(int n, const int4* k0, const void* theta, const
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, nb, epart, energy);
   return hipGetLastError();
 }
+template <typename R>
+hipError_t launch_gather_tile(const int4* k0, const void* theta, const void* grid,
+                              const double* q, const double* phi_real, const double* onsite,
+                              const int* perm, const int* tile_start, const GatherArgs& g,
+                              double* shift, double* epart, double* energy, hipStream_t st) {
+  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  const size_t smem = gather_tile_smem(g.T, g.order, sizeof(R));
+  const R* th = static_cast<const R*>(theta);
+  const R* gr = static_cast<const R*>(grid);
+#define EW_GATHER(PP)                                                                            \
+  hipLaunchKernelGGL((k_gather_tile<R, PP>), dim3(ntiles), dim3(256), smem, st, k0, th, gr, q,    \
+                     phi_real, onsite, perm, tile_start, g, shift, epart)
+  switch (g.order) {
+    case 4: EW_GATHER(4); break;
+    case 5: EW_GATHER(5); break;
+    case 6: EW_GATHER(6); break;
+    case 7: EW_GATHER(7); break;
+    case 8: EW_GATHER(8); break;
+    case 10: EW_GATHER(10); break;
+    default: EW_GATHER(0); break;
+  }
+#undef EW_GATHER
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, ntiles, epart, energy);
+  return hipGetLastError();
+}
+template hipError_t launch_gather_tile<double>(const int4*, const void*, const void*,
+                                               const double*, const double*, const double*,
+                                               const int*, const int*, const GatherArgs&, double*,
+                                               double*, double*, hipStream_t);
+template hipError_t launch_gather_tile<float>(const int4*, const void*, const void*,
+                                              const double*, const double*, const double*,
+                                              const int*, const int*, const GatherArgs&, double*,
+                                              double*, double*, hipStream_t);
+
 template hipError_t launch_gather_combine<double>(int, const int4*, const void*, const void*,
                                                   const double*, const double*, const double*,
                                                   const int*, const GatherArgs&, double*, double*,
@@ -753,6 +1226,31 @@ hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int*
                        static_cast<double*>(coef), err);
   return hipGetLastError();
 }
+
+template <typename R>
+hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* grid,
+                         const double* q, const int* perm, const int* tile_of,
+                         const int* tile_start, const ForceArgs& fa, const PairArgs& pa,
+                         bool dense, double* force, hipStream_t st) {
+  const dim3 grd((n + 127) / 128), blk(128);
+  hipLaunchKernelGGL(k_force_recip<R>, grd, blk, 0, st, n, frac, k0, static_cast<const R*>(grid),
+                     q, perm, fa, force);
+  if (dense)
+    hipLaunchKernelGGL(k_force_real<true>, grd, blk, 0, st, n, frac, tile_of, tile_start, q, perm,
+                       pa, force);
+  else
+    hipLaunchKernelGGL(k_force_real<false>, grd, blk, 0, st, n, frac, tile_of, tile_start, q,
+                       perm, pa, force);
+  return hipGetLastError();
+}
+template hipError_t launch_forces<double>(int, const double*, const int4*, const void*,
+                                          const double*, const int*, const int*, const int*,
+                                          const ForceArgs&, const PairArgs&, bool, double*,
+                                          hipStream_t);
+template hipError_t launch_forces<float>(int, const double*, const int4*, const void*,
+                                         const double*, const int*, const int*, const int*,
+                                         const ForceArgs&, const PairArgs&, bool, double*,
+                                         hipStream_t);
 
 hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const PairArgs& g,
                              int* col, void* coef, bool fp32, int* err, hipStream_t st) {

@@ -15,8 +15,12 @@ struct SpreadArgs {
   int nt[3];       // tiles per axis
   int ntmax;       // max(nt) -- row length of the source-tile table
   int order;
+  int slots;       // atoms staged per chunk by k_spread (0: use k_spread_checked)
   const int* src;  // [3][ntmax][1 + kMaxSrcTiles]: count, then source tiles per output tile
 };
+
+// LDS budget of k_spread's staged atoms
+constexpr size_t kSpreadSlotBytes = 40 * 1024;
 
 struct InfluenceArgs {
   int K[3];
@@ -33,7 +37,16 @@ struct GatherArgs {
   int K[3];
   int order;
   double self_coef;  // -2 alpha / sqrt(pi)
+  int T[3];          // tile extents (tile gather)
+  int nt[3];         // tiles per axis (tile gather)
 };
+
+// LDS budget of the tile gather: staged region (tile + (p-1) halo per axis) and the weights of
+// 64 atoms
+constexpr size_t kGatherRegionMaxBytes = 60 * 1024;
+inline size_t gather_tile_smem(const int* T, int p, size_t rsz) {
+  return rsz * ((size_t)(T[0] + p - 1) * (T[1] + p - 1) * (T[2] + p - 1) + 64 * 3 * (size_t)p);
+}
 
 struct BsplineArgs {
   int K[3];
@@ -46,10 +59,17 @@ struct PairArgs {
   double rc2;      // r_cut^2
   double rl2;      // r_list^2
   int D[3];        // partner tiles (linked mode) or image shells (dense mode) per side
+  int full[3];     // linked mode: 1 = sweep every tile of this axis once (2D+1 > nt)
   int nt[3];
   int ns;          // species count (0: no short gamma)
   const double* hub;    // [ns]
   const double* gcut2;  // [ns*ns] squared short-gamma cutoffs
+};
+
+struct ForceArgs {
+  int K[3];
+  int order;
+  double rec[9];  // dual basis rows b_i
 };
 
 struct SpmeLaunch {
@@ -72,13 +92,23 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
 
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
                                  hipStream_t st);
-hipError_t launch_pair_spmv(int n, const long long* row_ptr, const int* col, const void* coef,
-                            bool fp32, const double* q, double* out, hipStream_t st);
+hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
+                            const void* coef, bool fp32, const double* q, double* out,
+                            hipStream_t st);
+hipError_t launch_compress_cols(int n, const long long* row_ptr, const int* col, short* off,
+                                int* too_far, hipStream_t st);
 template <typename R>
 hipError_t launch_gather_combine(int n, const int4* k0, const void* theta, const void* grid,
                                  const double* q, const double* phi_real, const double* onsite,
                                  const int* perm, const GatherArgs& g, double* shift,
                                  double* epart, double* energy, hipStream_t st);
+// Per-tile gather with the mesh region staged in LDS; needs
+// (T1+p-1)(T2+p-1)(T3+p-1) * sizeof(R) <= kGatherRegionMaxBytes.  epart: one per tile.
+template <typename R>
+hipError_t launch_gather_tile(const int4* k0, const void* theta, const void* grid,
+                              const double* q, const double* phi_real, const double* onsite,
+                              const int* perm, const int* tile_start, const GatherArgs& g,
+                              double* shift, double* epart, double* energy, hipStream_t st);
 hipError_t launch_bspline(int n, const double* frac, const BsplineArgs& g, void* theta, bool fp32,
                           hipStream_t st);
 hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int* tile_of,
@@ -87,6 +117,12 @@ hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int
 hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
                             const int* tile_start, const PairArgs& g, long long* row_ptr,
                             int* col, void* coef, bool fp32, int* err, hipStream_t st);
+// Reciprocal force (overwrites force, caller order) then the real-space Ewald force (adds).
+template <typename R>
+hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* grid,
+                         const double* q, const int* perm, const int* tile_of,
+                         const int* tile_start, const ForceArgs& fa, const PairArgs& pa,
+                         bool dense, double* force, hipStream_t st);
 hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const PairArgs& g,
                              int* col, void* coef, bool fp32, int* err, hipStream_t st);
 

@@ -35,7 +35,7 @@ PRECISION_MIXED = 1
 
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
-            "ew_shift", "ew_shift_device", "ew_phase_times", "ew_potential_force",
+            "ew_shift", "ew_shift_device", "ew_gradients", "ew_phase_times", "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory")
 
 #: phases reported by ew_phase_times, in order
@@ -55,19 +55,21 @@ class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("alpha", C.c_double), ("r_cut", C.c_double),
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("eager", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class Parameters(C.Structure):
     _fields_ = [("alpha", C.c_double), ("r_cut", C.c_double), ("k_cut", C.c_double),
                 ("r_list", C.c_double), ("grid", C.c_int * 3), ("order", C.c_int),
-                ("precision", C.c_int), ("pair_mode", C.c_int), ("n_pairs", C.c_longlong),
+                ("precision", C.c_int), ("pair_mode", C.c_int), ("index_bytes", C.c_int),
+                ("n_pairs", C.c_longlong),
                 ("device_bytes", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {"alpha": self.alpha, "r_cut": self.r_cut, "k_cut": self.k_cut,
                 "r_list": self.r_list, "grid": tuple(self.grid), "order": self.order,
                 "precision": self.precision, "pair_mode": self.pair_mode,
+                "index_bytes": self.index_bytes,
                 "n_pairs": self.n_pairs, "device_bytes": self.device_bytes}
 
 
@@ -98,6 +100,12 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"fftewald_amd: {LIB_PATH} is missing -- build it first "
                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        try:
+            # torch ships its own libamdhip64.so.7; loading it first makes the engine bind to the
+            # same HIP runtime, so one process never holds two runtimes.
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         H = C.c_void_p
         L.ew_create.argtypes = [C.POINTER(Options), C.POINTER(H)]
@@ -109,6 +117,7 @@ def lib() -> C.CDLL:
         L.ew_get_parameters.argtypes = [H, C.POINTER(Parameters)]
         L.ew_shift.argtypes = [H, _D1, _D1, C.POINTER(C.c_double)]
         L.ew_shift_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ew_gradients.argtypes = [H, _D1, _D2]
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
@@ -123,7 +132,7 @@ def lib() -> C.CDLL:
 
 
 def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
-                 device=0, alpha_rule="fftewald", profile=False) -> Options:
+                 device=0, alpha_rule="fftewald", profile=False, eager=False) -> Options:
     o = Options()
     o.tol = tol or 0.0
     o.alpha = alpha or 0.0
@@ -139,6 +148,7 @@ def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precis
     o.alpha_rule = {"fftewald": 0, "dftb": 1}[alpha_rule] if isinstance(alpha_rule, str) \
         else int(alpha_rule)
     o.profile = 1 if profile else 0
+    o.eager = 1 if eager else 0
     return o
 
 
@@ -216,6 +226,13 @@ class EwaldEngine:
                                      C.c_void_p(energy_ptr or None), C.c_void_p(stream or None)),
                self._h)
 
+
+    def gradients(self, dq, grad=None):
+        """grad + dU/dx (Ewald part) at charges dq; grad defaults to zeros (n, 3)."""
+        dq = _f64(dq, 1)
+        g = np.zeros((self.n, 3)) if grad is None else np.array(grad, dtype=np.float64, copy=True)
+        _check(lib().ew_gradients(self._h, dq, g), self._h)
+        return g
 
     def phase_times(self, reset: bool = False):
         """({phase: ms}, calls) accumulated since the last reset (needs profile=True)."""

@@ -36,7 +36,8 @@ module fftewald_amd
       integer(c_int) :: device = 0
       integer(c_int) :: alpha_rule = 0
       integer(c_int) :: profile = 0
-      integer(c_int) :: reserved(3) = 0
+      integer(c_int) :: eager = 0
+      integer(c_int) :: reserved(2) = 0
    end type ew_options
 
    !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).
@@ -49,6 +50,7 @@ module fftewald_amd
       integer(c_int) :: order = 0
       integer(c_int) :: precision = 0
       integer(c_int) :: pair_mode = 0
+      integer(c_int) :: index_bytes = 0
       integer(c_long_long) :: n_pairs = 0
       integer(c_long_long) :: device_bytes = 0
    end type ew_parameters

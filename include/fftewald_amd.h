@@ -65,7 +65,9 @@ typedef struct ew_options {
                       getOptimalAlphaEwald(tol) (src/dftbp/dftb/coulomb.F90:2097-2179) with
                       r_cut = sqrt(-ln tol)/alpha (fe/common/ewald_truncation.f90:36-45)     */
   int profile;     /* 1: record HIP events around every phase of each evaluation (ew_phase_times) */
-  int reserved[3];
+  int eager;       /* 1: launch the evaluation's kernels one by one; 0 (default): replay a HIP
+                      graph captured once per (dq, shift, energy) buffer triple               */
+  int reserved[2];
 } ew_options;
 
 /* Parameters an evaluation uses; mirrors TEwaldParameters3d
@@ -79,6 +81,7 @@ typedef struct ew_parameters {
   int order;         /* B-spline order                                                           */
   int precision;
   int pair_mode;     /* 0 = linked tiles (minimum image), 1 = dense image sum (small cells)     */
+  int index_bytes;   /* bytes per stored partner index: 2 (offsets from the row) or 4          */
   long long n_pairs; /* stored real-space pair coefficients                                      */
   long long device_bytes; /* device memory held by the handle for this geometry                 */
 } ew_parameters;
@@ -120,6 +123,11 @@ int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
 int ew_shift_device(ew_handle *h, const double *d_dq, double *d_shift, double *d_energy,
                     void *stream);
 
+/* Energy gradient of the Ewald part at charges dq, ADDED into grad[n][3] as +dU/dx: the
+ * fftEwaldAddGradients convention (coulomb.F90:361-385, which subtracts the physical force
+ * from DFTB+'s deriv array).  The short-range gamma gradient stays with TShortGamma. */
+int ew_gradients(ew_handle *h, const double *dq, double *grad);
+
 /* Device time per phase of the evaluations run since the last reset, in milliseconds, with
  * the number of evaluations; the analogue of the reference's per-phase NFFT timing
  * [spread, transform, deconvolve] (fe/nfft/nfft.f90:228-231, 255-303).  Needs opts.profile = 1
@@ -132,7 +140,8 @@ int ew_phase_times(ew_handle *h, double *ms, long long *calls, int reset);
 /* ---- one-shot calls, the reference library's own entry points ---- */
 
 /* ewald_potential_force (fe/fft/ewald_fft_3d.f90:36-134): pot[i] = phi(x_i) and, when force
- * is not NULL, force[i][3] = -dU/dx_i (the physical force).  Ewald only (no short gamma).
+ * is not NULL, force[i][3] = -dU/dx_i (the physical force: SPME reciprocal force from the
+ * B-spline derivatives plus the real-space pair force).  Ewald only (no short gamma).
  * opts may be NULL. */
 int ew_potential_force(int n, const double *coords, const double *charges,
                        const double *latvecs, const ew_options *opts, double *pot,

@@ -133,7 +133,9 @@ def test_c3_fp64_and_mixed(oracle_mod):
     err = rel_max(s32, s64)
     print(f"mixed vs FP64: shift {err:.3e}  energy {abs(e32 - e64) / abs(e64):.3e}")
     assert err < 1e-6
-    assert e32 == pytest.approx(e64, rel=1e-6)
+    # E = -219 Ha is what is left of E_self = -4437 Ha against E_real + E_rec: the FP32 mesh
+    # error of E_rec (~1e-7 of it) shows up ~20x larger relative to the total.
+    assert e32 == pytest.approx(e64, rel=1e-5)
 
 
 def test_shift_device_and_repeated_calls():
@@ -186,3 +188,59 @@ def test_one_shot_potential_matches_engine():
     pot, _ = ew.ewald_potential_force(g["coords"], g["charges"], g["lattice"], tol=1e-12,
                                       alpha=float(g["alpha"]), order=12)
     assert rel_max(pot, g["pot"]) < 1e-10
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("ref_triclinic.npz", dict(tol=1e-12, order=12)),
+    ("ref_small_cell.npz", dict(tol=1e-12, order=12)),
+    ("ref_nacl_c1.npz", dict(tol=1e-10, grid=24, order=10)),
+])
+def test_forces_match_reference_direct(name, kw):
+    g = load_golden(name)
+    if "alpha" in g:
+        kw = dict(kw, alpha=float(g["alpha"]))
+    say(f"forces: {name} {kw}")
+    pot, force = ew.ewald_potential_force(g["coords"], g["charges"], g["lattice"], force=True, **kw)
+    assert rel_max(pot, g["pot"]) < 1e-10
+    assert rel_max(force, g["force"]) < 1e-9
+
+
+def test_engine_gradients_and_c2_forces(oracle_mod):
+    w = W.water(n_side=10)
+    say("water 3000 atoms: ew_gradients (FP64, mixed) against the CPU SPME force")
+    opts = dict(tol=1e-10, r_cut=w["r_cut"], grid=(40, 40, 40), order=6)
+    with ew.EwaldEngine(**opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        prm = e.parameters()
+        g64 = e.gradients(w["charges"], grad=np.ones((len(w["charges"]), 3)))
+    _, f = oracle_mod.spme_ewald(w["coords"], w["charges"], w["lattice"], prm["grid"],
+                                 prm["order"], prm["alpha"], prm["r_cut"], want_force=True)
+    assert rel_max(1.0 - g64, f) < 1e-10
+    with ew.EwaldEngine(precision="mixed", **opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        g32 = e.gradients(w["charges"])
+    assert rel_max(-g32, f) < 1e-5
+
+
+def test_graph_replays_match_eager():
+    import torch
+    w = W.water(n_side=8)
+    say("water 1536 atoms: captured-graph replays against eager launches")
+    rng = np.random.default_rng(11)
+    dev = torch.device("cuda:0")
+    with ew.EwaldEngine(tol=1e-8) as g, ew.EwaldEngine(tol=1e-8, eager=True) as e:
+        g.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        dq = torch.empty(len(w["charges"]), dtype=torch.float64, device=dev)
+        out = torch.empty_like(dq)
+        en = torch.empty(1, dtype=torch.float64, device=dev)
+        q = w["charges"]
+        for it in range(5):
+            q = W.scc_update(q, rng)
+            dq.copy_(torch.from_numpy(q))
+            torch.cuda.synchronize()
+            g.shift_device(dq.data_ptr(), out.data_ptr(), en.data_ptr())  # same buffers: replay
+            torch.cuda.synchronize()
+            s_eager, e_eager = e.shift(q)                                # eager work in between
+            assert rel_max(out.cpu().numpy(), s_eager) < 1e-14
+            assert float(en.item()) == pytest.approx(e_eager, rel=1e-14)
