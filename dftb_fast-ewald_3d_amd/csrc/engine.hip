@@ -46,6 +46,7 @@ struct ew_handle {
   long long nnz = 0;
   int T[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, ntmax = 0;
   int spread_slots = 0;  // atoms staged per chunk by the interval spread (0: checked spread)
+  RegionArgs region{};   // privatised spreading (scratch == null: not used)
   FftPlan px{}, py{}, pz{};
   double self_coef = 0.0;
 
@@ -336,7 +337,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     for (int a = 0; a < 3; ++a)
       if (K[a] < h->T[a] + p - 1) ok = false;
     const size_t rsz = h->fp32 ? sizeof(float) : sizeof(double);
-    const size_t per = 3 * (size_t)p * rsz + 3 * sizeof(int);
+    const size_t per = 3 * (size_t)p * rsz + sizeof(int4);
     h->spread_slots = ok ? (int)std::min<size_t>(256, kSpreadSlotBytes / per) : 0;
   }
   std::vector<int> srctab((size_t)3 * h->ntmax * (kMaxSrcTiles + 1), 0);
@@ -385,6 +386,44 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
 
   // device buffers
   const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  // privatised spreading blocks: the largest whole-sort-tile blocks whose copy fits the LDS
+  // budget, on meshes where a block copy covers no point twice (K >= S + p - 1)
+  h->region = RegionArgs{};
+  {
+    int best[3] = {0, 0, 0};
+    long long best_pts = 0;
+    for (int f3 = 1; f3 <= 4; ++f3)
+      for (int f2 = 1; f2 <= 4; ++f2)
+        for (int f1 = 1; f1 <= 2; ++f1) {
+          const int S[3] = {f1 * h->T[0], f2 * h->T[1], f3 * h->T[2]};
+          bool ok = true;
+          for (int a = 0; a < 3; ++a)
+            if (S[a] > K[a] || K[a] < S[a] + p - 1) ok = false;
+          // the copy accumulates in FP64 LDS in both precision modes
+          const size_t bytes = sizeof(double) * (size_t)(S[0] + p - 1) * (S[1] + p - 1) *
+                               (S[2] + p - 1);
+          const long long pts = (long long)S[0] * S[1] * S[2];
+          if (ok && bytes <= kRegionMaxBytes && pts > best_pts) {
+            best_pts = pts;
+            best[0] = f1;
+            best[1] = f2;
+            best[2] = f3;
+          }
+        }
+    if (best_pts > 0) {
+      RegionArgs& rg = h->region;
+      for (int a = 0; a < 3; ++a) {
+        rg.K[a] = K[a];
+        rg.f[a] = best[a];
+        rg.S[a] = best[a] * h->T[a];
+        rg.nt[a] = h->nt[a];
+        rg.nb[a] = (h->nt[a] + best[a] - 1) / best[a];
+      }
+      rg.order = p;
+      rg.region_stride =
+          (long long)(rg.S[0] + p - 1) * (rg.S[1] + p - 1) * (rg.S[2] + p - 1);
+    }
+  }
   const int Kh = K[0] / 2 + 1;
   const size_t ngrid = (size_t)K[0] * K[1] * K[2], ncgrid = (size_t)Kh * K[1] * K[2];
   CHK(h->alloc(&h->perm, n));
@@ -407,6 +446,12 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   CHK(h->alloc(&h->dq_io, n));
   CHK(h->alloc(&h->shift_io, n));
   CHK(h->alloc(&h->err_flag, 1));
+  if (h->region.region_stride > 0) {
+    const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
+    unsigned char* scr;
+    CHK(h->alloc(&scr, nblk * (size_t)h->region.region_stride * rs));
+    h->region.scratch = scr;
+  }
   {
     unsigned char *g, *cgp;
     CHK(h->alloc(&g, ngrid * rs));
@@ -608,6 +653,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   L.spread.ntmax = h->ntmax;
   L.spread.order = h->prm.order;
   L.spread.slots = h->spread_slots;
+  L.region = h->region;
   L.spread.src = h->src_tab;
   L.px = h->px;
   L.py = h->py;

@@ -373,29 +373,32 @@ __global__ __launch_bounds__(256) void k_spread_checked(const int4* __restrict__
 }
 
 // Fast spreading for meshes with K >= T + p - 1 on every axis, where the stencil of an atom
-// meets a tile in one contiguous interval per axis.  Per output tile: the candidate atoms are
-// staged in LDS in chunks -- only those that reach the tile, with their weights (q folded into
-// the z weights) and the exact valid offset interval per axis -- and the accumulation then runs
-// over those intervals without per-point tests or global loads.
+// meets a tile in one contiguous interval per axis.  Per output tile the candidate atoms are
+// staged in LDS in chunks -- only those that reach the tile, in a fixed order, with their
+// weights (q folded into the z weights) and their valid interval per axis.  Each thread owns a
+// 4-point x segment of the tile and accumulates it in registers over the staged atoms (every
+// LDS read is a broadcast), so there are no atomics and the sum order is fixed.
 template <typename R, int P>
-__global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
-                                                const R* __restrict__ theta,
-                                                const double* __restrict__ q,
-                                                const int* __restrict__ tile_start,
-                                                SpreadArgs g, R* __restrict__ grid) {
+__global__ __launch_bounds__(kSpreadThreads) void k_spread(const int4* __restrict__ k0,
+                                                           const R* __restrict__ theta,
+                                                           const double* __restrict__ q,
+                                                           const int* __restrict__ tile_start,
+                                                           SpreadArgs g, R* __restrict__ grid) {
   constexpr int kMaxVisit = kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles;
+  constexpr int kWaves = kSpreadThreads / 64;
   extern __shared__ unsigned char smem_raw[];
   const int p = P > 0 ? P : g.order;
   const int cap = g.slots;
-  R* acc = reinterpret_cast<R*>(smem_raw);                   // [kSpreadTileMax]
-  R* w1s = acc + kSpreadTileMax;                              // [cap][p]
-  R* w2s = w1s + (size_t)cap * p;                             // [cap][p]
-  R* w3s = w2s + (size_t)cap * p;                             // [cap][p], times q
-  int* iv = reinterpret_cast<int*>(w3s + (size_t)cap * p);    // [cap][3]: a|b<<5|pos<<10
+  // per-row bitmasks of the staged atoms that touch the row (rows = (z, y) of the tile)
+  unsigned long long* rmask = reinterpret_cast<unsigned long long*>(smem_raw);  // [rows][4]
+  int4* iv = reinterpret_cast<int4*>(rmask + kSpreadRows * 4);  // [cap]: a | n<<5 | pos<<10
+  R* w1s = reinterpret_cast<R*>(iv + cap);                       // [cap][p]
+  R* w2s = w1s + (size_t)cap * p;                                // [cap][p]
+  R* w3s = w2s + (size_t)cap * p;                                // [cap][p], times q
   __shared__ int src_count[kMaxVisit];
   __shared__ int src_first[kMaxVisit + 1];
   __shared__ int src_begin[kMaxVisit];
-  __shared__ int nslot;
+  __shared__ int wave_hits[kWaves];
   int t = blockIdx.x;
   const int t1 = t % g.nt[0];
   t /= g.nt[0];
@@ -404,10 +407,12 @@ __global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
   const int lo[3] = {t1 * g.T[0], t2 * g.T[1], t3 * g.T[2]};
   const int w[3] = {min(g.T[0], g.K[0] - lo[0]), min(g.T[1], g.K[1] - lo[1]),
                     min(g.T[2], g.K[2] - lo[2])};
-  const int npts = w[0] * w[1] * w[2];
-  // odd row and plane pitches keep lanes that differ in y or z on different LDS banks
-  const int PY = w[0] | 1, PZ = (PY * w[1]) | 1;
-  for (int i = threadIdx.x; i < PZ * w[2]; i += blockDim.x) acc[i] = R(0);
+  // this thread's mesh point
+  const int mx = threadIdx.x % g.T[0], my = (threadIdx.x / g.T[0]) % g.T[1],
+            mz = threadIdx.x / (g.T[0] * g.T[1]);
+  const bool owner = mx < w[0] && my < w[1] && mz < w[2];
+  R acc = R(0);
+
   const int* e1 = g.src + ((size_t)0 * g.ntmax + t1) * (kMaxSrcTiles + 1);
   const int* e2 = g.src + ((size_t)1 * g.ntmax + t2) * (kMaxSrcTiles + 1);
   const int* e3 = g.src + ((size_t)2 * g.ntmax + t3) * (kMaxSrcTiles + 1);
@@ -431,18 +436,20 @@ __global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
   }
   __syncthreads();
   const int nvisit = src_first[ns];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int chunk = 0; chunk < nvisit; chunk += cap) {
-    if (threadIdx.x == 0) nslot = 0;
-    __syncthreads();
+    for (int i = threadIdx.x; i < kSpreadRows * 4; i += blockDim.x) rmask[i] = 0ULL;
+    // stage the candidates of this chunk that reach the tile, in candidate order
     const int v = chunk + (int)threadIdx.x;
+    bool hit = false;
+    int pk[3] = {0, 0, 0}, a = 0;
     if ((int)threadIdx.x < cap && v < nvisit) {
       int c = 0;
       while (src_first[c + 1] <= v) ++c;
-      const int a = src_begin[c] + (v - src_first[c]);
+      a = src_begin[c] + (v - src_first[c]);
       const int4 kk = k0[a];
       const int kv[3] = {kk.x, kk.y, kk.z};
-      int pk[3];
-      bool hit = true;
+      hit = true;
       for (int ax = 0; ax < 3; ++ax) {
         int r = kv[ax] - lo[ax];
         if (r < 0) r += g.K[ax];
@@ -457,57 +464,170 @@ __global__ __launch_bounds__(256) void k_spread(const int4* __restrict__ k0,
           pos = 0;
         }
         if (ia >= ib) hit = false;
-        pk[ax] = ia | (ib << 5) | (pos << 10);
+        pk[ax] = ia | ((ib - ia) << 5) | (pos << 10);
       }
-      if (hit) {
-        const int sl = atomicAdd(&nslot, 1);
-        const R* th = theta + (size_t)a * 3 * p;
-        const R qa = R(q[a]);
-        for (int o = 0; o < p; ++o) {
-          w1s[sl * p + o] = th[o];
-          w2s[sl * p + o] = th[p + o];
-          w3s[sl * p + o] = qa * th[2 * p + o];
+    }
+    const unsigned long long hm = __ballot(hit);
+    if (lane == 0) wave_hits[wv] = __popcll(hm);
+    __syncthreads();
+    int sl = __popcll(hm & ((1ULL << lane) - 1ULL));
+    int nslot = 0;
+    for (int k = 0; k < kWaves; ++k) {
+      if (k < wv) sl += wave_hits[k];
+      nslot += wave_hits[k];
+    }
+    if (hit) {
+      const R* th = theta + (size_t)a * 3 * p;
+      const R qa = R(q[a]);
+      for (int o = 0; o < p; ++o) {
+        w1s[sl * p + o] = th[o];
+        w2s[sl * p + o] = th[p + o];
+        w3s[sl * p + o] = qa * th[2 * p + o];
+      }
+      iv[sl] = make_int4(pk[0], pk[1], pk[2], 0);
+      // mark the rows this atom touches
+      const int z0 = pk[2] >> 10, nz = (pk[2] >> 5) & 31, y0 = pk[1] >> 10, ny = (pk[1] >> 5) & 31;
+      const unsigned long long bit = 1ULL << (sl & 63);
+      for (int dz = 0; dz < nz; ++dz)
+        for (int dy = 0; dy < ny; ++dy)
+          atomicOr(&rmask[((z0 + dz) * w[1] + y0 + dy) * 4 + (sl >> 6)], bit);
+    }
+    __syncthreads();
+    if (owner) {
+      // only the staged atoms that touch this point's row, in slot order
+      const int row = mz * w[1] + my;
+      const int nwords = (nslot + 63) >> 6;
+      for (int wd = 0; wd < nwords; ++wd) {
+        unsigned long long bits = rmask[row * 4 + wd];
+        while (bits) {
+          const int sl2 = wd * 64 + __ffsll((long long)bits) - 1;
+          bits &= bits - 1ULL;
+          const int4 hd = iv[sl2];
+          const int dx = mx - (hd.x >> 10);
+          if ((unsigned)dx < (unsigned)((hd.x >> 5) & 31)) {
+            const int dz = mz - (hd.z >> 10), dy = my - (hd.y >> 10);
+            acc += w3s[sl2 * p + (hd.z & 31) + dz] * w2s[sl2 * p + (hd.y & 31) + dy] *
+                   w1s[sl2 * p + (hd.x & 31) + dx];
+          }
         }
-        iv[3 * sl] = pk[0];
-        iv[3 * sl + 1] = pk[1];
-        iv[3 * sl + 2] = pk[2];
-      }
-    }
-    __syncthreads();
-    const int pp = p * p;
-    const int nitems = nslot * pp;  // one (atom, z, y) row per item, an x interval each
-    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
-      const int sl = it / pp, r = it - sl * pp;
-      const int o3 = r / p, o2 = r - o3 * p;
-      const int i3 = iv[3 * sl + 2], i2 = iv[3 * sl + 1];
-      const int a3 = i3 & 31, b3 = (i3 >> 5) & 31;
-      const int a2 = i2 & 31, b2 = (i2 >> 5) & 31;
-      if (o3 < a3 || o3 >= b3 || o2 < a2 || o2 >= b2) continue;
-      const int i1 = iv[3 * sl];
-      const int a1 = i1 & 31, b1 = (i1 >> 5) & 31, x0 = i1 >> 10;
-      const int z = (i3 >> 10) + o3 - a3, y = (i2 >> 10) + o2 - a2;
-      const R c23 = w3s[sl * p + o3] * w2s[sl * p + o2];
-      const R* wx = w1s + sl * p;
-      R* row = acc + z * PZ + y * PY + x0 - a1;
-      if (P > 0) {
-        // weights into registers first, then the predicated adds back to back
-        R wr[P > 0 ? P : 1];
-#pragma unroll
-        for (int o1 = 0; o1 < P; ++o1) wr[o1] = wx[o1];
-#pragma unroll
-        for (int o1 = 0; o1 < P; ++o1)
-          if (o1 >= a1 && o1 < b1) atomicAdd(row + o1, c23 * wr[o1]);
-      } else {
-        for (int o1 = a1; o1 < b1; ++o1) atomicAdd(row + o1, c23 * wx[o1]);
       }
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < npts; i += blockDim.x) {
-    const int x = i % w[0], y = (i / w[0]) % w[1], z = i / (w[0] * w[1]);
-    grid[((size_t)(lo[2] + z) * g.K[1] + (lo[1] + y)) * g.K[0] + lo[0] + x] =
-        acc[z * PZ + y * PY + x];
+  if (owner) grid[((size_t)(lo[2] + mz) * g.K[1] + (lo[1] + my)) * g.K[0] + lo[0] + mx] = acc;
+}
+
+// Privatised spreading in two steps.
+// Step 1 (one workgroup per spread block of whole sort tiles): the atoms of the block -- those
+// whose stencil origin lies in it -- add their p^3 weights into a private LDS copy of the block
+// plus its (p-1)-point upper halo; the copy is written whole to a resident scratch buffer.
+// Step 2 (one thread per mesh point): each point is the fixed-order sum of the at most 2^3
+// block copies that cover it.  No global atomics and no mesh clear.
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ k0,
+                                                       const R* __restrict__ theta,
+                                                       const double* __restrict__ q,
+                                                       const int* __restrict__ tile_start,
+                                                       RegionArgs g, R* __restrict__ regions) {
+  // the private copy accumulates in FP64 in both precision modes (LDS FP64 adds are native
+  // and fast; the FP32 ones are not), and is stored in the mesh precision R
+  extern __shared__ unsigned char smem_raw[];
+  double* reg = reinterpret_cast<double*>(smem_raw);
+  const int p = P > 0 ? P : g.order;
+  const int RX = g.S[0] + p - 1, RY = g.S[1] + p - 1, RZ = g.S[2] + p - 1;
+  const int nreg = RX * RY * RZ;
+  for (int i = threadIdx.x; i < nreg; i += blockDim.x) reg[i] = 0.0;
+  int b = blockIdx.x;
+  const int b1 = b % g.nb[0];
+  b /= g.nb[0];
+  const int b2 = b % g.nb[1], b3 = b / g.nb[1];
+  const int lo1 = b1 * g.S[0], lo2 = b2 * g.S[1], lo3 = b3 * g.S[2];
+  __syncthreads();
+  for (int s3 = 0; s3 < g.f[2]; ++s3) {
+    const int u3 = b3 * g.f[2] + s3;
+    if (u3 >= g.nt[2]) break;
+    for (int s2 = 0; s2 < g.f[1]; ++s2) {
+      const int u2 = b2 * g.f[1] + s2;
+      if (u2 >= g.nt[1]) break;
+      // sort tiles u1 .. u1 + f[0] - 1 of one (u2, u3) row are contiguous in atom order
+      const int u1a = b1 * g.f[0], u1b = min(u1a + g.f[0], g.nt[0]);
+      const int ta = (u3 * g.nt[1] + u2) * g.nt[0];
+      const int a0 = tile_start[ta + u1a], a1 = tile_start[ta + u1b];
+      const int nitems = (a1 - a0) * p;  // one (atom, z plane) per item
+      for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+        const int a = a0 + it / p, o3 = it % p;
+        const int4 kk = k0[a];
+        const R* th = theta + (size_t)a * 3 * p;
+        const double c3 = q[a] * double(th[2 * p + o3]);
+        double* plane = reg + ((size_t)(kk.z - lo3 + o3) * RY + (kk.y - lo2)) * RX + (kk.x - lo1);
+        if (P > 0) {
+          double w1[P > 0 ? P : 1], w2[P > 0 ? P : 1];
+#pragma unroll
+          for (int o = 0; o < (P > 0 ? P : 1); ++o) {
+            w1[o] = double(th[o]);
+            w2[o] = double(th[p + o]);
+          }
+#pragma unroll
+          for (int o2 = 0; o2 < (P > 0 ? P : 1); ++o2) {
+            const double c = c3 * w2[o2];
+#pragma unroll
+            for (int o1 = 0; o1 < (P > 0 ? P : 1); ++o1)
+              atomicAdd(plane + o2 * RX + o1, c * w1[o1]);
+          }
+        } else {
+          for (int o2 = 0; o2 < p; ++o2) {
+            const double c = c3 * double(th[p + o2]);
+            for (int o1 = 0; o1 < p; ++o1) atomicAdd(plane + o2 * RX + o1, c * double(th[o1]));
+          }
+        }
+      }
+    }
   }
+  __syncthreads();
+  R* out = regions + (size_t)blockIdx.x * g.region_stride;
+  for (int i = threadIdx.x; i < nreg; i += blockDim.x) out[i] = R(reg[i]);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_spread_assemble(const R* __restrict__ regions,
+                                                         RegionArgs g, R* __restrict__ grid) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long G = (long long)g.K[0] * g.K[1] * g.K[2];
+  if (id >= G) return;
+  const int x = int(id % g.K[0]);
+  const int y = int((id / g.K[0]) % g.K[1]);
+  const int z = int(id / ((long long)g.K[0] * g.K[1]));
+  const int p = g.order;
+  const int RX = g.S[0] + p - 1, RY = g.S[1] + p - 1;
+  const int pos[3] = {x, y, z};
+  // per axis: the blocks whose copy (block + halo) covers this coordinate, and the offset in it
+  int nb[3], bi[3][4], off[3][4];
+  for (int a = 0; a < 3; ++a) {
+    const int own = pos[a] / g.S[a];
+    nb[a] = 0;
+    for (int d = 0; d < 4 && d < g.nb[a]; ++d) {
+      int t = own - d;
+      if (t < 0) t += g.nb[a];
+      const int lo = t * g.S[a];
+      const int w = min(g.S[a], g.K[a] - lo);
+      int r = pos[a] - lo;
+      if (r < 0) r += g.K[a];
+      if (r < w + p - 1) {
+        bi[a][nb[a]] = t;
+        off[a][nb[a]] = r;
+        ++nb[a];
+      }
+    }
+  }
+  R sum = R(0);
+  for (int i3 = 0; i3 < nb[2]; ++i3)
+    for (int i2 = 0; i2 < nb[1]; ++i2)
+      for (int i1 = 0; i1 < nb[0]; ++i1) {
+        const int blk = (bi[2][i3] * g.nb[1] + bi[1][i2]) * g.nb[0] + bi[0][i1];
+        sum += regions[(size_t)blk * g.region_stride +
+                       ((size_t)off[2][i3] * RY + off[1][i2]) * RX + off[0][i1]];
+      }
+  grid[id] = sum;
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1051,13 +1171,37 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
   const cx<R>* tw2 = static_cast<const cx<R>*>(L.tw[1]);
   const cx<R>* tw3 = static_cast<const cx<R>*>(L.tw[2]);
   const dim3 sgrid(L.spread.nt[0] * L.spread.nt[1] * L.spread.nt[2]);
-  if (L.spread.slots > 0) {
-    const size_t smem = sizeof(R) * (kSpreadTileMax + 3 * (size_t)L.spread.slots * L.spread.order) +
-                        sizeof(int) * 3 * (size_t)L.spread.slots;
+  if (L.region.scratch) {
+    const RegionArgs& rg = L.region;
+    const int nblk = rg.nb[0] * rg.nb[1] * rg.nb[2];
+    const size_t smem = sizeof(double) * (size_t)(rg.S[0] + rg.order - 1) *
+                        (rg.S[1] + rg.order - 1) * (rg.S[2] + rg.order - 1);
+    const R* th = static_cast<const R*>(L.theta);
+    R* scr = static_cast<R*>(rg.scratch);
+#define EW_REGION(PP)                                                                         \
+  hipLaunchKernelGGL((k_spread_region<R, PP>), dim3(nblk), dim3(256), smem, st, L.k0, th, L.q, \
+                     L.tile_start, rg, scr)
+    switch (rg.order) {
+      case 4: EW_REGION(4); break;
+      case 5: EW_REGION(5); break;
+      case 6: EW_REGION(6); break;
+      case 7: EW_REGION(7); break;
+      case 8: EW_REGION(8); break;
+      case 10: EW_REGION(10); break;
+      default: EW_REGION(0); break;
+    }
+#undef EW_REGION
+    const long long G = (long long)K1 * K2 * K3;
+    hipLaunchKernelGGL(k_spread_assemble<R>, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, st,
+                       static_cast<const R*>(scr), rg, grid);
+  } else if (L.spread.slots > 0) {
+    const size_t smem = sizeof(unsigned long long) * kSpreadRows * 4 +
+                        sizeof(int4) * (size_t)L.spread.slots +
+                        sizeof(R) * 3 * (size_t)L.spread.slots * L.spread.order;
     const R* th = static_cast<const R*>(L.theta);
 #define EW_SPREAD(PP)                                                                          \
-  hipLaunchKernelGGL((k_spread<R, PP>), sgrid, dim3(256), smem, st, L.k0, th, L.q, L.tile_start, \
-                     L.spread, grid)
+  hipLaunchKernelGGL((k_spread<R, PP>), sgrid, dim3(kSpreadThreads), smem, st, L.k0, th, L.q,    \
+                     L.tile_start, L.spread, grid)
     switch (L.spread.order) {
       case 4: EW_SPREAD(4); break;
       case 5: EW_SPREAD(5); break;

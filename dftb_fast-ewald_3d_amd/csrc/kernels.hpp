@@ -19,8 +19,10 @@ struct SpreadArgs {
   const int* src;  // [3][ntmax][1 + kMaxSrcTiles]: count, then source tiles per output tile
 };
 
-// LDS budget of k_spread's staged atoms
+// LDS budget of k_spread's staged atoms; rows (z, y) per tile for its row bitmasks
 constexpr size_t kSpreadSlotBytes = 40 * 1024;
+constexpr int kSpreadRows = 64;
+constexpr int kSpreadThreads = 1024;  // one thread per point of a 16x8x8 tile
 
 struct InfluenceArgs {
   int K[3];
@@ -72,6 +74,21 @@ struct ForceArgs {
   double rec[9];  // dual basis rows b_i
 };
 
+// Privatised spreading: blocks of f[0] x f[1] x f[2] sort tiles, S = f * T mesh points each.
+struct RegionArgs {
+  int K[3];
+  int S[3];        // block extents (mesh points)
+  int nb[3];       // blocks per axis
+  int f[3];        // sort tiles per block per axis
+  int nt[3];       // sort tiles per axis
+  int order;
+  long long region_stride;  // elements per block copy: (S0+p-1)(S1+p-1)(S2+p-1)
+  void* scratch;   // nb0*nb1*nb2 copies; null: no privatised spreading
+};
+
+// LDS budget of a block copy
+constexpr size_t kRegionMaxBytes = 60 * 1024;
+
 struct SpmeLaunch {
   int K[3];
   const int4* k0;
@@ -79,6 +96,7 @@ struct SpmeLaunch {
   const double* q;
   const int* tile_start;
   SpreadArgs spread;
+  RegionArgs region;
   FftPlan px, py, pz;
   const void* tw[3];
   InfluenceArgs inf;
