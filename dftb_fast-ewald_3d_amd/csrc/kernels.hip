@@ -96,11 +96,34 @@ struct Dft<5, R> {
   }
 };
 
+template <typename R>
+struct Dft<8, R> {
+  __device__ static void run(cx<R>* v, R s) {
+    // two radix-4 halves (even and odd inputs), then the eighth-root twiddles
+    cx<R> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<R> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, R>::run(e, s);
+    Dft<4, R>::run(o, s);
+    const R c = R(0.707106781186547524400844362104849039);
+    const cx<R> w1 = {c * (o[1].x - s * o[1].y), c * (o[1].y + s * o[1].x)};
+    const cx<R> w2 = cmuli(o[2], s);
+    const cx<R> w3 = {c * (-o[3].x - s * o[3].y), c * (-o[3].y + s * o[3].x)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
+  }
+};
+
 // One in-place Stockham pass over `nl` lines of length n (LDS pitch P).
 // tw[m] = exp(-2 pi i m / n); the inverse uses its conjugate.
 template <int RAD, typename R>
 __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, int Ns,
-                                              const cx<R>* __restrict__ tw, R dir) {
+                                              const cx<R>* tw, R dir) {
   constexpr int EPT = kFftMaxElems / kFftBlock;  // complex elements per thread and pass
   constexpr int MAXB = (EPT + RAD - 1) / RAD;
   const int nb = n / RAD;
@@ -142,13 +165,23 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
   __syncthreads();
 }
 
+// Copies the length-n twiddle table into LDS (after the line data, at s + nl * pitch).
 template <typename R>
-__device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* __restrict__ tw,
-                        R dir) {
+__device__ __forceinline__ const cx<R>* stage_twiddles(cx<R>* s, const FftPlan& pl, int nl,
+                                                       const cx<R>* __restrict__ tw) {
+  cx<R>* t = s + (size_t)nl * pl.pitch;
+  for (int i = threadIdx.x; i < pl.n; i += blockDim.x) t[i] = tw[i];
+  return t;
+}
+
+template <typename R>
+__device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R dir) {
   int Ns = 1;
   for (int i = 0; i < pl.npass; ++i) {
     const int r = pl.radix[i];
-    if (r == 4)
+    if (r == 8)
+      stockham_pass<8, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
+    else if (r == 4)
       stockham_pass<4, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
     else if (r == 2)
       stockham_pass<2, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
@@ -174,16 +207,31 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
   const int K1 = pl.n, nl = pl.lines, P = pl.pitch;
   const int row0 = blockIdx.x * nl * 2;
-  for (int idx = threadIdx.x; idx < nl * K1; idx += kFftBlock) {
-    const int l = idx / K1, e = idx - l * K1;
-    const int r0 = row0 + 2 * l, r1 = r0 + 1;
-    cx<R> z;
-    z.x = r0 < nrows ? grid[(size_t)r0 * K1 + e] : R(0);
-    z.y = r1 < nrows ? grid[(size_t)r1 * K1 + e] : R(0);
-    s[l * P + e] = z;
+  const cx<R>* tws = stage_twiddles(s, pl, nl, tw);
+  {
+    // all of a thread's loads are issued before any LDS store (one memory latency, not EPT)
+    constexpr int EPT = kFftMaxElems / kFftBlock;
+    cx<R> z[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int idx = threadIdx.x + k * kFftBlock;
+      const int l = idx / K1, e = idx - l * K1;
+      const int r0 = row0 + 2 * l, r1 = r0 + 1;
+      const bool in = idx < nl * K1;
+      z[k].x = in && r0 < nrows ? grid[(size_t)r0 * K1 + e] : R(0);
+      z[k].y = in && r1 < nrows ? grid[(size_t)r1 * K1 + e] : R(0);
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int idx = threadIdx.x + k * kFftBlock;
+      if (idx < nl * K1) {
+        const int l = idx / K1, e = idx - l * K1;
+        s[l * P + e] = z[k];
+      }
+    }
   }
   __syncthreads();
-  fft_lds<R>(s, pl, nl, tw, R(-1));
+  fft_lds<R>(s, pl, nl, tws, R(-1));
   for (int idx = threadIdx.x; idx < nl * Kh; idx += kFftBlock) {
     const int l = idx / Kh, k = idx - l * Kh;
     const int r0 = row0 + 2 * l, r1 = r0 + 1;
@@ -203,21 +251,38 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
   const int K1 = pl.n, nl = pl.lines, P = pl.pitch;
   const int row0 = blockIdx.x * nl * 2;
-  for (int idx = threadIdx.x; idx < nl * K1; idx += kFftBlock) {
-    const int l = idx / K1, k = idx - l * K1;
-    const int r0 = row0 + 2 * l, r1 = r0 + 1;
-    cx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-    const int kk = k < Kh ? k : K1 - k;
-    if (r0 < nrows) a = cg[(size_t)r0 * Kh + kk];
-    if (r1 < nrows) b = cg[(size_t)r1 * Kh + kk];
-    if (k >= Kh) {
-      a.y = -a.y;
-      b.y = -b.y;
+  const cx<R>* tws = stage_twiddles(s, pl, nl, tw);
+  {
+    constexpr int EPT = kFftMaxElems / kFftBlock;
+    cx<R> a[EPT], b[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int idx = threadIdx.x + j * kFftBlock;
+      const int l = idx / K1, k = idx - l * K1;
+      const int r0 = row0 + 2 * l, r1 = r0 + 1;
+      const bool in = idx < nl * K1;
+      const int kk = k < Kh ? k : K1 - k;
+      a[j] = {R(0), R(0)};
+      b[j] = {R(0), R(0)};
+      if (in && r0 < nrows) a[j] = cg[(size_t)r0 * Kh + kk];
+      if (in && r1 < nrows) b[j] = cg[(size_t)r1 * Kh + kk];
     }
-    s[l * P + k] = {a.x - b.y, a.y + b.x};
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int idx = threadIdx.x + j * kFftBlock;
+      if (idx < nl * K1) {
+        const int l = idx / K1, k = idx - l * K1;
+        cx<R> aa = a[j], bb = b[j];
+        if (k >= Kh) {
+          aa.y = -aa.y;
+          bb.y = -bb.y;
+        }
+        s[l * P + k] = {aa.x - bb.y, aa.y + bb.x};
+      }
+    }
   }
   __syncthreads();
-  fft_lds<R>(s, pl, nl, tw, R(1));
+  fft_lds<R>(s, pl, nl, tws, R(1));
   for (int idx = threadIdx.x; idx < nl * K1; idx += kFftBlock) {
     const int l = idx / K1, e = idx - l * K1;
     const int r0 = row0 + 2 * l, r1 = r0 + 1;
@@ -239,15 +304,29 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
   const int n = pl.n, nl = pl.lines, P = pl.pitch;
   const int L0 = blockIdx.x * nl;
-  for (int idx = threadIdx.x; idx < nl * n; idx += kFftBlock) {
-    const int e = idx / nl, l = idx - e * nl;
-    const int L = L0 + l;
-    cx<R> z = {R(0), R(0)};
-    if (L < nlines) z = cg[(long long)(L / A) * B + (L % A) + e * es];
-    s[l * P + e] = z;
+  const cx<R>* tws = stage_twiddles(s, pl, nl, tw);
+  {
+    constexpr int EPT = kFftMaxElems / kFftBlock;
+    cx<R> z[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int idx = threadIdx.x + k * kFftBlock;
+      const int e = idx / nl, l = idx - e * nl;
+      const int L = L0 + l;
+      z[k] = {R(0), R(0)};
+      if (idx < nl * n && L < nlines) z[k] = cg[(long long)(L / A) * B + (L % A) + e * es];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int idx = threadIdx.x + k * kFftBlock;
+      if (idx < nl * n) {
+        const int e = idx / nl, l = idx - e * nl;
+        s[l * P + e] = z[k];
+      }
+    }
   }
   __syncthreads();
-  fft_lds<R>(s, pl, nl, tw, MODE == 1 ? R(1) : R(-1));
+  fft_lds<R>(s, pl, nl, tws, MODE == 1 ? R(1) : R(-1));
   if (MODE == 2) {
     // influence function on the half spectrum: line L = ky*Kh + kx, element e = kz
     for (int idx = threadIdx.x; idx < nl * n; idx += kFftBlock) {
@@ -274,7 +353,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
       z.y = R(z.y * w);
     }
     __syncthreads();
-    fft_lds<R>(s, pl, nl, tw, R(1));
+    fft_lds<R>(s, pl, nl, tws, R(1));
   }
   for (int idx = threadIdx.x; idx < nl * n; idx += kFftBlock) {
     const int e = idx / nl, l = idx - e * nl;
@@ -773,13 +852,27 @@ __global__ __launch_bounds__(256) void k_gather_tile(
   const int E1 = min(g.T[0], g.K[0] - lo1) + p - 1, E2 = min(g.T[1], g.K[1] - lo2) + p - 1,
             E3 = min(g.T[2], g.K[2] - lo3) + p - 1;
   const int nreg = E1 * E2 * E3;
-  for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
-    const int x = i % E1, y = (i / E1) % E2, z = i / (E1 * E2);
-    int gx = lo1 + x, gy = lo2 + y, gz = lo3 + z;
-    while (gx >= g.K[0]) gx -= g.K[0];
-    while (gy >= g.K[1]) gy -= g.K[1];
-    while (gz >= g.K[2]) gz -= g.K[2];
-    reg[i] = grid[((size_t)gz * g.K[1] + gy) * g.K[0] + gx];
+  for (int i0 = 0; i0 < nreg; i0 += 16 * 256) {
+    // sixteen loads in flight per thread before the LDS stores
+    R v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int i = i0 + threadIdx.x + k * 256;
+      v[k] = R(0);
+      if (i < nreg) {
+        const int x = i % E1, y = (i / E1) % E2, z = i / (E1 * E2);
+        int gx = lo1 + x, gy = lo2 + y, gz = lo3 + z;
+        while (gx >= g.K[0]) gx -= g.K[0];
+        while (gy >= g.K[1]) gy -= g.K[1];
+        while (gz >= g.K[2]) gz -= g.K[2];
+        v[k] = grid[((size_t)gz * g.K[1] + gy) * g.K[0] + gx];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int i = i0 + threadIdx.x + k * 256;
+      if (i < nreg) reg[i] = v[k];
+    }
   }
   __syncthreads();
   const int a0 = tile_start[blockIdx.x], a1 = tile_start[blockIdx.x + 1];
@@ -1156,7 +1249,7 @@ __global__ void k_force_real(int n, const double* __restrict__ frac, const int* 
 
 template <typename R>
 static size_t fft_smem(const FftPlan& pl) {
-  return sizeof(cx<R>) * (size_t)pl.lines * pl.pitch;
+  return sizeof(cx<R>) * ((size_t)pl.lines * pl.pitch + pl.n);
 }
 
 template <typename R>

@@ -274,6 +274,7 @@ void bspline_moduli(int K, int p, double* bsq) {
 
 int fft_plan(int n, int* radices) {
   int np = 0, r = n;
+  while (r % 8 == 0 && np < kMaxRadixPasses) { radices[np++] = 8; r /= 8; }
   while (r % 4 == 0 && np < kMaxRadixPasses) { radices[np++] = 4; r /= 4; }
   while (r % 2 == 0 && np < kMaxRadixPasses) { radices[np++] = 2; r /= 2; }
   while (r % 3 == 0 && np < kMaxRadixPasses) { radices[np++] = 3; r /= 3; }
