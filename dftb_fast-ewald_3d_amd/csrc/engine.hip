@@ -166,7 +166,9 @@ FftPlan make_plan(int n, int want_lines) {
   pl.npass = fft_plan(n, pl.radix);
   int lines = std::max(1, std::min(kFftMaxElems / n, want_lines));
   pl.lines = lines;
-  pl.pitch = lines > 1 ? n + 1 : n;
+  // one pad slot per eight elements (line_slot) and an odd pitch between lines
+  pl.pitch = n + n / 8 + 1;
+  if (!(pl.pitch & 1)) ++pl.pitch;
   return pl;
 }
 
@@ -196,11 +198,25 @@ bool build_source_table(int K, int T, int nt, int p, int ntmax, int axis, std::v
 
 template <typename R>
 void twiddles(int n, std::vector<R>& out) {
-  out.resize(2 * (size_t)n);
-  for (int m = 0; m < n; ++m) {
-    const long double a = 2.0L * 3.14159265358979323846264338327950288L * m / n;
-    out[2 * m] = R(cosl(a));
-    out[2 * m + 1] = R(-sinl(a));
+  // Per-pass tables, concatenated in pass order: a pass of radix r after a product Ns of the
+  // earlier radices holds exp(-2 pi i t k / (Ns r)) at [(t - 1) * Ns + k], t = 1..r-1,
+  // k = 0..Ns-1, so that the lanes of a butterfly read consecutive entries.  The sizes
+  // telescope to n - 1 entries; one spare keeps the table n long.
+  int radix[kMaxRadixPasses];
+  const int np = fft_plan(n, radix);
+  out.assign(2 * (size_t)n, R(0));
+  size_t o = 0;
+  int Ns = 1;
+  for (int i = 0; i < np; ++i) {
+    const int r = radix[i];
+    for (int t = 1; t < r; ++t)
+      for (int k = 0; k < Ns; ++k, ++o) {
+        const long double a =
+            2.0L * 3.14159265358979323846264338327950288L * t * k / ((long double)Ns * r);
+        out[2 * o] = R(cosl(a));
+        out[2 * o + 1] = R(-sinl(a));
+      }
+    Ns *= r;
   }
 }
 
@@ -505,6 +521,13 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   }
   for (int a = 0; a < 3; ++a)
     if (fft_plan(K[a], h->px.radix) < 0) return h->fail(EW_ERR_ARGUMENT, "grid not 2^a 3^b 5^c");
+  for (int a = 0; a < 3; ++a) {
+    // a workgroup transforms whole lines in LDS: kFftMaxElems elements, 64 KiB
+    const size_t lds = (h->fp32 ? 8 : 16) * ((size_t)K[a] + K[a] / 8 + 2 + K[a]);
+    if (K[a] > kFftMaxElems || lds > 65536)
+      return h->fail(EW_ERR_ARGUMENT, "mesh dimension " + std::to_string(K[a]) +
+                                          " exceeds the single-workgroup FFT line limit");
+  }
   h->px = make_plan(K[0], (K[1] * K[2] + 1) / 2);
   h->py = make_plan(K[1], 16);
   h->pz = make_plan(K[2], 16);
@@ -553,9 +576,10 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     }
     rp[n] = run;
     h->nnz = run;
-    CHK(h->alloc(&h->col, (size_t)run));
+    // 8 spare entries: the pair sum reads aligned groups of eight
+    CHK(h->alloc(&h->col, (size_t)run + 8));
     unsigned char* cp;
-    CHK(h->alloc(&cp, (size_t)run * cs));
+    CHK(h->alloc(&cp, ((size_t)run + 8) * cs));
     h->coef = cp;
     HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
                              hipMemcpyHostToDevice, s));
@@ -571,9 +595,9 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     h->nnz = nn;
     std::vector<long long> rp((size_t)n + 1);
     for (int i = 0; i <= n; ++i) rp[i] = (long long)i * n;
-    CHK(h->alloc(&h->col, (size_t)nn));
+    CHK(h->alloc(&h->col, (size_t)nn + 8));
     unsigned char* cp;
-    CHK(h->alloc(&cp, (size_t)nn * cs));
+    CHK(h->alloc(&cp, ((size_t)nn + 8) * cs));
     h->coef = cp;
     HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
                              hipMemcpyHostToDevice, s));
@@ -590,7 +614,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->col16 = nullptr;
   if (h->pair_mode == 0 && h->nnz > 0) {
     short* off = nullptr;
-    CHK(h->alloc(&off, (size_t)h->nnz));
+    CHK(h->alloc(&off, (size_t)h->nnz + 8));
     HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));
     HIPCHK(h, launch_compress_cols(n, h->row_ptr, h->col, off, h->err_flag, s));
     int too_far = 0;

@@ -121,6 +121,11 @@ struct Dft<8, R> {
 
 // One in-place Stockham pass over `nl` lines of length n (LDS pitch P).
 // tw[m] = exp(-2 pi i m / n); the inverse uses its conjugate.
+// LDS slot of element i of a line: one pad slot after every eight elements, so that the
+// stride-radix stores of the first pass and the strided column reads fall in distinct banks.
+__device__ __forceinline__ int line_slot(int i) { return i + (i >> 3); }
+
+// tw: this pass's table, exp(-2 pi i t k / (Ns RAD)) at [(t - 1) * Ns + k].
 template <int RAD, typename R>
 __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, int Ns,
                                               const cx<R>* tw, R dir) {
@@ -128,7 +133,6 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
   constexpr int MAXB = (EPT + RAD - 1) / RAD;
   const int nb = n / RAD;
   const int total = nl * nb;
-  const int tscale = n / (Ns * RAD);
   cx<R> v[MAXB][RAD];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
@@ -139,9 +143,9 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
       const cx<R>* src = s + line * P;
 #pragma unroll
       for (int t = 0; t < RAD; ++t) {
-        cx<R> a = src[j + t * nb];
+        cx<R> a = src[line_slot(j + t * nb)];
         if (t > 0) {
-          cx<R> w = tw[t * k * tscale];
+          cx<R> w = tw[(t - 1) * Ns + k];
           if (dir > R(0)) w.y = -w.y;  // table holds exp(-i..); the inverse conjugates it
           a = cmul(a, w);
         }
@@ -157,9 +161,10 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
     if (idx < total) {
       const int line = idx / nb, j = idx - line * nb;
       const int k = j % Ns;
-      cx<R>* dst = s + line * P + (j / Ns) * Ns * RAD + k;
+      cx<R>* dst = s + line * P;
+      const int d0 = (j / Ns) * Ns * RAD + k;
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) dst[t * Ns] = v[b][t];
+      for (int t = 0; t < RAD; ++t) dst[line_slot(d0 + t * Ns)] = v[b][t];
     }
   }
   __syncthreads();
@@ -177,18 +182,20 @@ __device__ __forceinline__ const cx<R>* stage_twiddles(cx<R>* s, const FftPlan& 
 template <typename R>
 __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R dir) {
   int Ns = 1;
+  const cx<R>* t = tw;
   for (int i = 0; i < pl.npass; ++i) {
     const int r = pl.radix[i];
     if (r == 8)
-      stockham_pass<8, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
+      stockham_pass<8, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
     else if (r == 4)
-      stockham_pass<4, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
+      stockham_pass<4, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
     else if (r == 2)
-      stockham_pass<2, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
+      stockham_pass<2, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
     else if (r == 3)
-      stockham_pass<3, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
+      stockham_pass<3, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
     else
-      stockham_pass<5, R>(s, pl.n, nl, pl.pitch, Ns, tw, dir);
+      stockham_pass<5, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
+    t += (r - 1) * Ns;
     Ns *= r;
   }
 }
@@ -226,7 +233,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
       const int idx = threadIdx.x + k * kFftBlock;
       if (idx < nl * K1) {
         const int l = idx / K1, e = idx - l * K1;
-        s[l * P + e] = z[k];
+        s[l * P + line_slot(e)] = z[k];
       }
     }
   }
@@ -235,8 +242,8 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
   for (int idx = threadIdx.x; idx < nl * Kh; idx += kFftBlock) {
     const int l = idx / Kh, k = idx - l * Kh;
     const int r0 = row0 + 2 * l, r1 = r0 + 1;
-    const cx<R> z = s[l * P + k];
-    const cx<R> zm = s[l * P + (k == 0 ? 0 : K1 - k)];
+    const cx<R> z = s[l * P + line_slot(k)];
+    const cx<R> zm = s[l * P + line_slot(k == 0 ? 0 : K1 - k)];
     if (r0 < nrows) cg[(size_t)r0 * Kh + k] = {R(0.5) * (z.x + zm.x), R(0.5) * (z.y - zm.y)};
     if (r1 < nrows) cg[(size_t)r1 * Kh + k] = {R(0.5) * (z.y + zm.y), R(0.5) * (zm.x - z.x)};
   }
@@ -277,7 +284,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
           aa.y = -aa.y;
           bb.y = -bb.y;
         }
-        s[l * P + k] = {aa.x - bb.y, aa.y + bb.x};
+        s[l * P + line_slot(k)] = {aa.x - bb.y, aa.y + bb.x};
       }
     }
   }
@@ -286,7 +293,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
   for (int idx = threadIdx.x; idx < nl * K1; idx += kFftBlock) {
     const int l = idx / K1, e = idx - l * K1;
     const int r0 = row0 + 2 * l, r1 = r0 + 1;
-    const cx<R> z = s[l * P + e];
+    const cx<R> z = s[l * P + line_slot(e)];
     if (r0 < nrows) grid[(size_t)r0 * K1 + e] = z.x;
     if (r1 < nrows) grid[(size_t)r1 * K1 + e] = z.y;
   }
@@ -321,7 +328,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
       const int idx = threadIdx.x + k * kFftBlock;
       if (idx < nl * n) {
         const int e = idx / nl, l = idx - e * nl;
-        s[l * P + e] = z[k];
+        s[l * P + line_slot(e)] = z[k];
       }
     }
   }
@@ -348,7 +355,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
         const double msq = vx * vx + vy * vy + vz * vz;
         w = inf.pref * exp(-inf.a2 * msq) / msq * inf.bsq1[kx] * inf.bsq2[ky] * inf.bsq3[kz];
       }
-      cx<R>& z = s[l * P + kz];
+      cx<R>& z = s[l * P + line_slot(kz)];
       z.x = R(z.x * w);
       z.y = R(z.y * w);
     }
@@ -358,7 +365,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
   for (int idx = threadIdx.x; idx < nl * n; idx += kFftBlock) {
     const int e = idx / nl, l = idx - e * nl;
     const int L = L0 + l;
-    if (L < nlines) cg[(long long)(L / A) * B + (L % A) + e * es] = s[l * P + e];
+    if (L < nlines) cg[(long long)(L / A) * B + (L % A) + e * es] = s[l * P + line_slot(e)];
   }
 }
 
@@ -735,7 +742,10 @@ __device__ __forceinline__ int partner(int row, short c, int n) {
   return j;
 }
 
-// Real-space pair sum: out[i] = sum_j coef_ij q_j, one 64-lane wave per atom (row).
+// Real-space pair sum: out[i] = sum_j coef_ij q_j, one 64-lane wave per atom (row).  A sweep
+// covers 512 entries: lane l takes entries l + 64u (u < 8), so that every load instruction of
+// the wave reads consecutive entries and consecutive partners, and all eight loads of a lane
+// are issued before the first is used.
 template <typename CT, typename IT>
 __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __restrict__ row_ptr,
                                                    const IT* __restrict__ col,
@@ -747,7 +757,21 @@ __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __res
   if (row >= n) return;  // uniform across the wave
   const long long a = row_ptr[row], b = row_ptr[row + 1];
   double acc = 0.0;
-  for (long long k = a + lane; k < b; k += 64) acc += double(coef[k]) * q[partner(row, col[k], n)];
+  for (long long g = a + lane; g < b; g += 512) {
+    IT c[8];
+    CT w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long k = g + 64 * u;
+      c[u] = k < b ? col[k] : IT(0);
+      w[u] = k < b ? coef[k] : CT(0);
+    }
+    double qv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) qv[u] = g + 64 * u < b ? q[partner(row, c[u], n)] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += double(w[u]) * qv[u];
+  }
   acc = wave_sum(acc);
   if (lane == 0) out[row] = acc;
 }
