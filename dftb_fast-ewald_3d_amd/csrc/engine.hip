@@ -169,6 +169,15 @@ FftPlan make_plan(int n, int want_lines) {
   // one pad slot per eight elements (line_slot) and an odd pitch between lines
   pl.pitch = n + n / 8 + 1;
   if (!(pl.pitch & 1)) ++pl.pitch;
+  pl.nd = make_fastdiv(n);
+  pl.hd = make_fastdiv(n / 2 + 1);
+  pl.ld = make_fastdiv(lines);
+  int ns = 1;
+  for (int i = 0; i < pl.npass; ++i) {
+    pl.nbd[i] = make_fastdiv(n / pl.radix[i]);
+    pl.nsd[i] = make_fastdiv(ns);
+    ns *= pl.radix[i];
+  }
   return pl;
 }
 
@@ -684,6 +693,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   L.pz = h->pz;
   for (int a = 0; a < 3; ++a) L.inf.K[a] = K[a];
   L.inf.Kh = K[0] / 2 + 1;
+  L.inf.khd = make_fastdiv(L.inf.Kh);
   for (int a = 0; a < 3; ++a)
     for (int d = 0; d < 3; ++d) L.inf.rec[3 * a + d] = h->cell.rec[a][d];
   L.inf.pref = 1.0 / (kPi * h->cell.volume);

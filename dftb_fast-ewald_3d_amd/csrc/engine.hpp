@@ -28,7 +28,7 @@ constexpr double kMinSeparationSq = 1.0e-24;
 constexpr double kNeutralityTol = 1.0e-8;
 
 constexpr int kMaxOrder = 16;
-constexpr int kMaxRadixPasses = 16;
+constexpr int kMaxRadixPasses = 12;
 constexpr int kFftBlock = 256;      // threads per FFT workgroup
 constexpr int kFftMaxElems = 2048;  // complex elements per FFT workgroup (LDS: 32 KiB in FP64)
 constexpr int kMaxSrcTiles = 4;     // source tiles per axis in the spreading table
@@ -68,12 +68,26 @@ void bspline_moduli(int K, int order, double* bsq);
 // radix plan for a length (radices 2,3,4,5)
 int fft_plan(int n, int* radices);
 
+// Division of a non-negative 31-bit integer by a fixed divisor d >= 1 as a multiply-high and a
+// shift: q = (umulhi(x, m) + x) >> s, with s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1
+// (Granlund and Montgomery, PLDI 1994, round-up variant).
+struct FastDiv {
+  unsigned m;
+  int s;
+};
+FastDiv make_fastdiv(int d);
+
 struct FftPlan {
   int n;
   int npass;
   int radix[kMaxRadixPasses];
   int lines;   // lines per workgroup
   int pitch;   // LDS pitch per line (complex elements)
+  FastDiv nd;  // / n
+  FastDiv hd;  // / (n / 2 + 1)
+  FastDiv ld;  // / lines
+  FastDiv nbd[kMaxRadixPasses];  // / (n / radix) of each pass
+  FastDiv nsd[kMaxRadixPasses];  // / (product of the earlier radices) of each pass
 };
 
 }  // namespace ewamd

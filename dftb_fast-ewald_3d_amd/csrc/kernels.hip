@@ -125,10 +125,15 @@ struct Dft<8, R> {
 // stride-radix stores of the first pass and the strided column reads fall in distinct banks.
 __device__ __forceinline__ int line_slot(int i) { return i + (i >> 3); }
 
+__device__ __forceinline__ int fdiv(int x, FastDiv f) {
+  return int((__umulhi(unsigned(x), f.m) + unsigned(x)) >> f.s);
+}
+
 // tw: this pass's table, exp(-2 pi i t k / (Ns RAD)) at [(t - 1) * Ns + k].
 template <int RAD, typename R>
 __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, int Ns,
-                                              const cx<R>* tw, R dir) {
+                                              const cx<R>* tw, R dir, FastDiv nbd,
+                                              FastDiv nsd) {
   constexpr int EPT = kFftMaxElems / kFftBlock;  // complex elements per thread and pass
   constexpr int MAXB = (EPT + RAD - 1) / RAD;
   const int nb = n / RAD;
@@ -138,8 +143,8 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
   for (int b = 0; b < MAXB; ++b) {
     const int idx = threadIdx.x + b * kFftBlock;
     if (idx < total) {
-      const int line = idx / nb, j = idx - line * nb;
-      const int k = j % Ns;
+      const int line = fdiv(idx, nbd), j = idx - line * nb;
+      const int k = j - fdiv(j, nsd) * Ns;
       const cx<R>* src = s + line * P;
 #pragma unroll
       for (int t = 0; t < RAD; ++t) {
@@ -159,10 +164,10 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
   for (int b = 0; b < MAXB; ++b) {
     const int idx = threadIdx.x + b * kFftBlock;
     if (idx < total) {
-      const int line = idx / nb, j = idx - line * nb;
-      const int k = j % Ns;
+      const int line = fdiv(idx, nbd), j = idx - line * nb;
+      const int jq = fdiv(j, nsd), k = j - jq * Ns;
       cx<R>* dst = s + line * P;
-      const int d0 = (j / Ns) * Ns * RAD + k;
+      const int d0 = jq * Ns * RAD + k;
 #pragma unroll
       for (int t = 0; t < RAD; ++t) dst[line_slot(d0 + t * Ns)] = v[b][t];
     }
@@ -186,15 +191,15 @@ __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R 
   for (int i = 0; i < pl.npass; ++i) {
     const int r = pl.radix[i];
     if (r == 8)
-      stockham_pass<8, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
+      stockham_pass<8, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
     else if (r == 4)
-      stockham_pass<4, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
+      stockham_pass<4, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
     else if (r == 2)
-      stockham_pass<2, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
+      stockham_pass<2, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
     else if (r == 3)
-      stockham_pass<3, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
+      stockham_pass<3, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
     else
-      stockham_pass<5, R>(s, pl.n, nl, pl.pitch, Ns, t, dir);
+      stockham_pass<5, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
     t += (r - 1) * Ns;
     Ns *= r;
   }
@@ -222,7 +227,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int idx = threadIdx.x + k * kFftBlock;
-      const int l = idx / K1, e = idx - l * K1;
+      const int l = fdiv(idx, pl.nd), e = idx - l * K1;
       const int r0 = row0 + 2 * l, r1 = r0 + 1;
       const bool in = idx < nl * K1;
       z[k].x = in && r0 < nrows ? grid[(size_t)r0 * K1 + e] : R(0);
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
     for (int k = 0; k < EPT; ++k) {
       const int idx = threadIdx.x + k * kFftBlock;
       if (idx < nl * K1) {
-        const int l = idx / K1, e = idx - l * K1;
+        const int l = fdiv(idx, pl.nd), e = idx - l * K1;
         s[l * P + line_slot(e)] = z[k];
       }
     }
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
   __syncthreads();
   fft_lds<R>(s, pl, nl, tws, R(-1));
   for (int idx = threadIdx.x; idx < nl * Kh; idx += kFftBlock) {
-    const int l = idx / Kh, k = idx - l * Kh;
+    const int l = fdiv(idx, pl.hd), k = idx - l * Kh;
     const int r0 = row0 + 2 * l, r1 = r0 + 1;
     const cx<R> z = s[l * P + line_slot(k)];
     const cx<R> zm = s[l * P + line_slot(k == 0 ? 0 : K1 - k)];
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int idx = threadIdx.x + j * kFftBlock;
-      const int l = idx / K1, k = idx - l * K1;
+      const int l = fdiv(idx, pl.nd), k = idx - l * K1;
       const int r0 = row0 + 2 * l, r1 = r0 + 1;
       const bool in = idx < nl * K1;
       const int kk = k < Kh ? k : K1 - k;
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
     for (int j = 0; j < EPT; ++j) {
       const int idx = threadIdx.x + j * kFftBlock;
       if (idx < nl * K1) {
-        const int l = idx / K1, k = idx - l * K1;
+        const int l = fdiv(idx, pl.nd), k = idx - l * K1;
         cx<R> aa = a[j], bb = b[j];
         if (k >= Kh) {
           aa.y = -aa.y;
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
   __syncthreads();
   fft_lds<R>(s, pl, nl, tws, R(1));
   for (int idx = threadIdx.x; idx < nl * K1; idx += kFftBlock) {
-    const int l = idx / K1, e = idx - l * K1;
+    const int l = fdiv(idx, pl.nd), e = idx - l * K1;
     const int r0 = row0 + 2 * l, r1 = r0 + 1;
     const cx<R> z = s[l * P + line_slot(e)];
     if (r0 < nrows) grid[(size_t)r0 * K1 + e] = z.x;
@@ -303,7 +308,8 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
 // MODE 0: forward; 1: inverse; 2: forward, influence multiply, inverse (Z pass).
 template <typename R, int MODE>
 __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg, int nlines,
-                                                         int A, long long B, long long es,
+                                                         int A, FastDiv ad, long long B,
+                                                         long long es,
                                                          FftPlan pl,
                                                          const cx<R>* __restrict__ tw,
                                                          InfluenceArgs inf) {
@@ -318,16 +324,19 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int idx = threadIdx.x + k * kFftBlock;
-      const int e = idx / nl, l = idx - e * nl;
+      const int e = fdiv(idx, pl.ld), l = idx - e * nl;
       const int L = L0 + l;
       z[k] = {R(0), R(0)};
-      if (idx < nl * n && L < nlines) z[k] = cg[(long long)(L / A) * B + (L % A) + e * es];
+      if (idx < nl * n && L < nlines) {
+        const int La = fdiv(L, ad);
+        z[k] = cg[(long long)La * B + (L - La * A) + e * es];
+      }
     }
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
       const int idx = threadIdx.x + k * kFftBlock;
       if (idx < nl * n) {
-        const int e = idx / nl, l = idx - e * nl;
+        const int e = fdiv(idx, pl.ld), l = idx - e * nl;
         s[l * P + line_slot(e)] = z[k];
       }
     }
@@ -337,10 +346,10 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
   if (MODE == 2) {
     // influence function on the half spectrum: line L = ky*Kh + kx, element e = kz
     for (int idx = threadIdx.x; idx < nl * n; idx += kFftBlock) {
-      const int l = idx / n, kz = idx - l * n;
+      const int l = fdiv(idx, pl.nd), kz = idx - l * n;
       const int L = L0 + l;
       if (L >= nlines) continue;
-      const int ky = L / inf.Kh, kx = L - ky * inf.Kh;
+      const int ky = fdiv(L, inf.khd), kx = L - ky * inf.Kh;
       double w = 0.0;
       const bool nyq = (!(inf.K[0] & 1) && kx == inf.K[0] / 2) ||
                        (!(inf.K[1] & 1) && ky == inf.K[1] / 2) ||
@@ -363,9 +372,12 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
     fft_lds<R>(s, pl, nl, tws, R(1));
   }
   for (int idx = threadIdx.x; idx < nl * n; idx += kFftBlock) {
-    const int e = idx / nl, l = idx - e * nl;
+    const int e = fdiv(idx, pl.ld), l = idx - e * nl;
     const int L = L0 + l;
-    if (L < nlines) cg[(long long)(L / A) * B + (L % A) + e * es] = s[l * P + line_slot(e)];
+    if (L < nlines) {
+      const int La = fdiv(L, ad);
+      cg[(long long)La * B + (L - La * A) + e * es] = s[l * P + line_slot(e)];
+    }
   }
 }
 
@@ -1342,16 +1354,16 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
   mark(1);
   const int ny = Kh * K3;
   hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((ny + L.py.lines - 1) / L.py.lines),
-                     dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, (long long)K2 * Kh,
+                     dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh), (long long)K2 * Kh,
                      (long long)Kh, L.py, tw2, L.inf);
   mark(2);
   const int nz = Kh * K2;
   hipLaunchKernelGGL((k_fft_lines<R, 2>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
-                     dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2, 0LL,
+                     dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2, make_fastdiv(Kh * K2), 0LL,
                      (long long)Kh * K2, L.pz, tw3, L.inf);
   mark(3);
   hipLaunchKernelGGL((k_fft_lines<R, 1>), dim3((ny + L.py.lines - 1) / L.py.lines),
-                     dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, (long long)K2 * Kh,
+                     dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh), (long long)K2 * Kh,
                      (long long)Kh, L.py, tw2, L.inf);
   mark(4);
   hipLaunchKernelGGL(k_fft_x_c2r<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, cg, grid,

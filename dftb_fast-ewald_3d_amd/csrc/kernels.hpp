@@ -27,6 +27,7 @@ constexpr int kSpreadThreads = 1024;  // one thread per point of a 16x8x8 tile
 struct InfluenceArgs {
   int K[3];
   int Kh;
+  FastDiv khd;     // / Kh
   double rec[9];   // dual basis rows b_i (no 2 pi)
   double pref;     // 1 / (pi V)
   double a2;       // pi^2 / alpha^2

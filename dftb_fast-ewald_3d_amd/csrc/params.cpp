@@ -272,6 +272,13 @@ void bspline_moduli(int K, int p, double* bsq) {
   }
 }
 
+FastDiv make_fastdiv(int d) {
+  int s = 0;
+  while ((1LL << s) < d) ++s;
+  const unsigned long long m = ((1ULL << 32) * ((1ULL << s) - (unsigned long long)d)) / d + 1;
+  return FastDiv{unsigned(m), s};
+}
+
 int fft_plan(int n, int* radices) {
   int np = 0, r = n;
   while (r % 8 == 0 && np < kMaxRadixPasses) { radices[np++] = 8; r /= 8; }
