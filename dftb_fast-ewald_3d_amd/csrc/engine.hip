@@ -16,148 +16,26 @@
 
 #include "../../include/fftewald_amd.h"
 #include "engine.hpp"
+#include "handle.hpp"
 #include "kernels.hpp"
 
 using namespace ewamd;
 
-namespace {
+namespace ewamd {
 
-thread_local std::string g_last_error = "no error";
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct ew_handle {
-  ew_options opt{};
-  hipStream_t stream = nullptr;
-  std::string err = "no error";
-  bool have_geom = false;
-
-  int n = 0;
-  Cell cell{};
-  Params prm{};
-  bool fp32 = false;
-  int ns = 0;
-  int pair_mode = 0;
-  long long nnz = 0;
-  int T[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, ntmax = 0;
-  int spread_slots = 0;  // atoms staged per chunk by the interval spread (0: checked spread)
-  RegionArgs region{};   // privatised spreading (scratch == null: not used)
-  FftPlan px{}, py{}, pz{};
-  double self_coef = 0.0;
-
-  // device state of the current geometry
-  std::vector<DevBuf> bufs;
-  int* perm = nullptr;
-  int4* k0 = nullptr;
-  double* frac = nullptr;
-  void* theta = nullptr;
-  int* tile_of = nullptr;
-  int* tile_start = nullptr;
-  int* src_tab = nullptr;
-  int* species = nullptr;
-  double* hub = nullptr;
-  double* gcut2 = nullptr;
-  double* onsite = nullptr;
-  long long* row_ptr = nullptr;
-  int* col = nullptr;
-  short* col16 = nullptr;  // signed 16-bit partner offsets (when every row fits), else null
-  void* coef = nullptr;
-  void* grid = nullptr;
-  void* cgrid = nullptr;
-  void* tw[3] = {nullptr, nullptr, nullptr};
-  double* bsq[3] = {nullptr, nullptr, nullptr};
-  double* qs = nullptr;
-  double* phi_real = nullptr;
-  double* epart = nullptr;
-  double* energy = nullptr;
-  double* dq_io = nullptr;
-  double* shift_io = nullptr;
-  int* err_flag = nullptr;
-  double* force = nullptr;  // [n][3], allocated on the first force request
-  PairArgs pa{};
-  size_t device_bytes = 0;
-
-  // captured evaluations, one per (dq, shift, energy) buffer triple
-  struct Captured {
-    const double* dq;
-    double* shift;
-    double* energy;
-    hipGraphExec_t exec;
-  };
-  std::vector<Captured> graphs;
-  void drop_graphs() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
-    graphs.clear();
-  }
-
-  // per-phase timing (opt.profile)
-  hipEvent_t ev[EW_NUM_PHASES + 1] = {};
-  double phase_ms[EW_NUM_PHASES] = {};
-  long long phase_calls = 0;
-
-  int fail(int code, const std::string& msg) {
-    err = msg;
-    g_last_error = msg;
-    return code;
-  }
-  void release() {
-    drop_graphs();
-    for (auto& b : bufs)
-      if (b.p) (void)hipFree(b.p);
-    bufs.clear();
-    device_bytes = 0;
-    have_geom = false;
-  }
-  void free_buf(void* p) {
-    for (size_t k = 0; k < bufs.size(); ++k)
-      if (bufs[k].p == p) {
-        (void)hipFree(p);
-        device_bytes -= bufs[k].bytes;
-        bufs.erase(bufs.begin() + k);
-        return;
-      }
-  }
-  template <typename T>
-  int alloc(T** out, size_t count) {
-    void* p = nullptr;
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      *out = nullptr;
-      return fail(EW_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
-    }
-    bufs.push_back({p, bytes});
-    device_bytes += bytes;
-    *out = static_cast<T*>(p);
-    return EW_OK;
-  }
-};
-
-#define HIPCHK(h, expr)                                                                      \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess)                                                                    \
-      return (h)->fail(EW_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) +   \
-                                          " at " #expr);                                     \
-  } while (0)
-
-#define CHK(expr)             \
-  do {                        \
-    int s_ = (expr);          \
-    if (s_ != EW_OK) return s_; \
-  } while (0)
-
-namespace {
+std::string& global_error() {
+  thread_local std::string msg = "no error";
+  return msg;
+}
 
 int set_global_error(int code, const std::string& msg) {
-  g_last_error = msg;
+  global_error() = msg;
   return code;
 }
+
+}  // namespace ewamd
+
+namespace {
 
 // FFT workgroup shape for lines of length n: as many lines as fit the LDS budget.
 FftPlan make_plan(int n, int want_lines) {
@@ -242,7 +120,41 @@ long long estimate_pairs(int n, const Cell& c, double r_list) {
 // ------------------------------------------------------------------------------------------
 
 extern "C" const char* ew_last_error(const ew_handle* h) {
-  return h ? h->err.c_str() : g_last_error.c_str();
+  return h ? h->err.c_str() : global_error().c_str();
+}
+
+int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** out) {
+  *out = nullptr;
+  if (o.precision != EW_PRECISION_FP64 && o.precision != EW_PRECISION_MIXED)
+    return set_global_error(EW_ERR_ARGUMENT, "unknown precision mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return set_global_error(EW_ERR_DEVICE, "no HIP device available");
+  }
+  if (o.device < 0 || o.device >= ndev)
+    return set_global_error(EW_ERR_ARGUMENT, "device ordinal out of range");
+  if (hipSetDevice(o.device) != hipSuccess)
+    return set_global_error(EW_ERR_DEVICE, "hipSetDevice failed");
+  ew_handle* h = new ew_handle();
+  h->opt = o;
+  h->fp32 = o.precision == EW_PRECISION_MIXED;
+  if (borrowed) {
+    h->stream = borrowed;
+    h->own_stream = false;
+  } else if (hipStreamCreate(&h->stream) != hipSuccess) {
+    delete h;
+    return set_global_error(EW_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  if (o.profile)
+    for (auto& e : h->ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        if (h->own_stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return set_global_error(EW_ERR_DEVICE, "hipEventCreate failed");
+      }
+  *out = h;
+  return EW_OK;
 }
 
 extern "C" int ew_create(const ew_options* opts, ew_handle** out) {
@@ -250,33 +162,9 @@ extern "C" int ew_create(const ew_options* opts, ew_handle** out) {
   *out = nullptr;
   ew_options o{};
   if (opts) o = *opts;
-  if (o.precision != EW_PRECISION_FP64 && o.precision != EW_PRECISION_MIXED)
-    return set_global_error(EW_ERR_ARGUMENT, "ew_create: unknown precision mode");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-    (void)hipGetLastError();
-    return set_global_error(EW_ERR_DEVICE, "ew_create: no HIP device available");
-  }
-  if (o.device < 0 || o.device >= ndev)
-    return set_global_error(EW_ERR_ARGUMENT, "ew_create: device ordinal out of range");
-  if (hipSetDevice(o.device) != hipSuccess)
-    return set_global_error(EW_ERR_DEVICE, "ew_create: hipSetDevice failed");
-  ew_handle* h = new ew_handle();
-  h->opt = o;
-  h->fp32 = o.precision == EW_PRECISION_MIXED;
-  if (hipStreamCreate(&h->stream) != hipSuccess) {
-    delete h;
-    return set_global_error(EW_ERR_DEVICE, "ew_create: hipStreamCreate failed");
-  }
-  if (o.profile)
-    for (auto& e : h->ev)
-      if (hipEventCreate(&e) != hipSuccess) {
-        (void)hipStreamDestroy(h->stream);
-        delete h;
-        return set_global_error(EW_ERR_DEVICE, "ew_create: hipEventCreate failed");
-      }
-  *out = h;
-  return EW_OK;
+  const int st = handle_create(o, nullptr, out);
+  if (st != EW_OK) global_error() = "ew_create: " + global_error();
+  return st;
 }
 
 extern "C" int ew_destroy(ew_handle* h) {
@@ -286,7 +174,7 @@ extern "C" int ew_destroy(ew_handle* h) {
   h->release();
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return EW_OK;
 }
@@ -874,7 +762,7 @@ extern "C" int ew_potential_force(int n, const double* coords, const double* cha
   st = ew_set_geometry(h, n, coords, latvecs, nullptr, nullptr, 0, charges);
   if (!st) st = ew_shift(h, charges, pot, nullptr);
   if (!st && force) st = forces_after_eval(h, force);
-  if (st) g_last_error = h->err;
+  if (st) global_error() = h->err;
   ew_destroy(h);
   return st;
 }
@@ -926,4 +814,67 @@ extern "C" int ew_grid_memory(int n, const double* charges, const double* latvec
   const double pairs = double(p.n_pairs) * (4.0 + rs);
   *gigabytes = (grids + atoms + pairs) / (1024.0 * 1024.0 * 1024.0);
   return EW_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// grouped-kernel view of a handle
+// ------------------------------------------------------------------------------------------
+
+void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
+  *d = StructDesc{};
+  const int* K = h->prm.K;
+  const int p = h->prm.order;
+  d->n = h->n;
+  for (int a = 0; a < 3; ++a) d->K[a] = K[a];
+  d->ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  d->spread_mode = h->region.scratch ? 0 : (h->spread_slots > 0 ? 1 : 2);
+  d->gather_tile =
+      gather_tile_smem(h->T, p, h->fp32 ? sizeof(float) : sizeof(double)) <= kGatherRegionMaxBytes;
+  d->nparts = d->gather_tile ? d->ntiles : (h->n + 255) / 256;
+  d->col16 = h->col16 != nullptr;
+  d->perm = h->perm;
+  d->k0 = h->k0;
+  d->theta = h->theta;
+  d->tile_start = h->tile_start;
+  d->qs = h->qs;
+  d->phi_real = h->phi_real;
+  d->onsite = h->onsite;
+  d->epart = h->epart;
+  d->row_ptr = h->row_ptr;
+  d->col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
+  d->coef = h->coef;
+  d->grid = h->grid;
+  d->cgrid = h->cgrid;
+  for (int a = 0; a < 3; ++a) {
+    d->tw[a] = h->tw[a];
+    d->spread.K[a] = K[a];
+    d->spread.T[a] = h->T[a];
+    d->spread.nt[a] = h->nt[a];
+    d->ga.K[a] = K[a];
+    d->ga.T[a] = h->T[a];
+    d->ga.nt[a] = h->nt[a];
+    d->inf.K[a] = K[a];
+  }
+  d->spread.ntmax = h->ntmax;
+  d->spread.order = p;
+  d->spread.slots = h->spread_slots;
+  d->spread.src = h->src_tab;
+  d->region = h->region;
+  d->ga.order = p;
+  d->ga.self_coef = h->self_coef;
+  const int Kh = K[0] / 2 + 1;
+  d->inf.Kh = Kh;
+  d->inf.khd = make_fastdiv(Kh);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) d->inf.rec[3 * a + b] = h->cell.rec[a][b];
+  d->inf.pref = 1.0 / (kPi * h->cell.volume);
+  d->inf.a2 = kPi * kPi / (h->prm.alpha * h->prm.alpha);
+  d->inf.bsq1 = h->bsq[0];
+  d->inf.bsq2 = h->bsq[1];
+  d->inf.bsq3 = h->bsq[2];
+  d->px = h->px;
+  d->py = h->py;
+  d->pz = h->pz;
+  d->yd = make_fastdiv(Kh);
+  d->zd = make_fastdiv(Kh * K[1]);
 }

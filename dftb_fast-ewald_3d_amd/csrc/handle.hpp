@@ -1,0 +1,157 @@
+// handle.hpp -- the resident state of one structure (ew_handle) and host helpers shared by
+// the single-structure engine (engine.hip) and the structure pool (pool.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/fftewald_amd.h"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace ewamd {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// Message slot of ew_last_error(NULL) (per host thread).
+std::string& global_error();
+int set_global_error(int code, const std::string& msg);
+
+}  // namespace ewamd
+
+struct ew_handle {
+  ew_options opt{};
+  hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: the stream belongs to the pool that holds this handle
+  std::string err = "no error";
+  bool have_geom = false;
+
+  int n = 0;
+  ewamd::Cell cell{};
+  ewamd::Params prm{};
+  bool fp32 = false;
+  int ns = 0;
+  int pair_mode = 0;
+  long long nnz = 0;
+  int T[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, ntmax = 0;
+  int spread_slots = 0;  // atoms staged per chunk by the interval spread (0: checked spread)
+  ewamd::RegionArgs region{};   // privatised spreading (scratch == null: not used)
+  ewamd::FftPlan px{}, py{}, pz{};
+  double self_coef = 0.0;
+
+  // device state of the current geometry
+  std::vector<ewamd::DevBuf> bufs;
+  int* perm = nullptr;
+  int4* k0 = nullptr;
+  double* frac = nullptr;
+  void* theta = nullptr;
+  int* tile_of = nullptr;
+  int* tile_start = nullptr;
+  int* src_tab = nullptr;
+  int* species = nullptr;
+  double* hub = nullptr;
+  double* gcut2 = nullptr;
+  double* onsite = nullptr;
+  long long* row_ptr = nullptr;
+  int* col = nullptr;
+  short* col16 = nullptr;  // signed 16-bit partner offsets (when every row fits), else null
+  void* coef = nullptr;
+  void* grid = nullptr;
+  void* cgrid = nullptr;
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  double* bsq[3] = {nullptr, nullptr, nullptr};
+  double* qs = nullptr;
+  double* phi_real = nullptr;
+  double* epart = nullptr;
+  double* energy = nullptr;
+  double* dq_io = nullptr;
+  double* shift_io = nullptr;
+  int* err_flag = nullptr;
+  double* force = nullptr;  // [n][3], allocated on the first force request
+  ewamd::PairArgs pa{};
+  size_t device_bytes = 0;
+
+  // captured evaluations, one per (dq, shift, energy) buffer triple
+  struct Captured {
+    const double* dq;
+    double* shift;
+    double* energy;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
+  // per-phase timing (opt.profile)
+  hipEvent_t ev[EW_NUM_PHASES + 1] = {};
+  double phase_ms[EW_NUM_PHASES] = {};
+  long long phase_calls = 0;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    ewamd::global_error() = msg;
+    return code;
+  }
+  void release() {
+    drop_graphs();
+    for (auto& b : bufs)
+      if (b.p) (void)hipFree(b.p);
+    bufs.clear();
+    device_bytes = 0;
+    have_geom = false;
+  }
+  void free_buf(void* p) {
+    for (size_t k = 0; k < bufs.size(); ++k)
+      if (bufs[k].p == p) {
+        (void)hipFree(p);
+        device_bytes -= bufs[k].bytes;
+        bufs.erase(bufs.begin() + k);
+        return;
+      }
+  }
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      return fail(EW_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    bufs.push_back({p, bytes});
+    device_bytes += bytes;
+    *out = static_cast<T*>(p);
+    return EW_OK;
+  }
+};
+
+#define HIPCHK(h, expr)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return (h)->fail(EW_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                          " at " #expr);                                     \
+  } while (0)
+
+#define CHK(expr)             \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != EW_OK) return s_; \
+  } while (0)
+
+namespace ewamd {
+
+// A handle on a borrowed stream (pool members); ew_destroy leaves the stream alone.
+int handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** out);
+// The grouped kernels' view of a handle with a geometry.
+void fill_desc(const ew_handle* h, StructDesc* d);
+
+}  // namespace ewamd

@@ -211,14 +211,14 @@ __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R 
 
 // Rows of K1 reals (row = z*K2 + y); workgroup takes pl.lines complex lines = 2*lines rows.
 template <typename R>
-__global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ grid,
+__device__ __forceinline__ void k_fft_x_r2c_body(const R* __restrict__ grid,
                                                          cx<R>* __restrict__ cg, int nrows,
                                                          int Kh, FftPlan pl,
-                                                         const cx<R>* __restrict__ tw) {
+                                                         const cx<R>* __restrict__ tw, int bid) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
   const int K1 = pl.n, nl = pl.lines, P = pl.pitch;
-  const int row0 = blockIdx.x * nl * 2;
+  const int row0 = bid * nl * 2;
   const cx<R>* tws = stage_twiddles(s, pl, nl, tw);
   {
     // all of a thread's loads are issued before any LDS store (one memory latency, not EPT)
@@ -255,14 +255,22 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
 }
 
 template <typename R>
-__global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict__ cg,
+__global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ grid,
+                                                         cx<R>* __restrict__ cg, int nrows,
+                                                         int Kh, FftPlan pl,
+                                                         const cx<R>* __restrict__ tw) {
+  k_fft_x_r2c_body<R>(grid, cg, nrows, Kh, pl, tw, blockIdx.x);
+}
+
+template <typename R>
+__device__ __forceinline__ void k_fft_x_c2r_body(const cx<R>* __restrict__ cg,
                                                          R* __restrict__ grid, int nrows, int Kh,
                                                          FftPlan pl,
-                                                         const cx<R>* __restrict__ tw) {
+                                                         const cx<R>* __restrict__ tw, int bid) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
   const int K1 = pl.n, nl = pl.lines, P = pl.pitch;
-  const int row0 = blockIdx.x * nl * 2;
+  const int row0 = bid * nl * 2;
   const cx<R>* tws = stage_twiddles(s, pl, nl, tw);
   {
     constexpr int EPT = kFftMaxElems / kFftBlock;
@@ -304,19 +312,27 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict__ cg,
+                                                         R* __restrict__ grid, int nrows, int Kh,
+                                                         FftPlan pl,
+                                                         const cx<R>* __restrict__ tw) {
+  k_fft_x_c2r_body<R>(cg, grid, nrows, Kh, pl, tw, blockIdx.x);
+}
+
 // Strided complex lines: line L has base (L / A) * B + L % A and element stride `es`.
 // MODE 0: forward; 1: inverse; 2: forward, influence multiply, inverse (Z pass).
 template <typename R, int MODE>
-__global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg, int nlines,
+__device__ __forceinline__ void k_fft_lines_body(cx<R>* __restrict__ cg, int nlines,
                                                          int A, FastDiv ad, long long B,
                                                          long long es,
                                                          FftPlan pl,
                                                          const cx<R>* __restrict__ tw,
-                                                         InfluenceArgs inf) {
+                                                         InfluenceArgs inf, int bid) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
   const int n = pl.n, nl = pl.lines, P = pl.pitch;
-  const int L0 = blockIdx.x * nl;
+  const int L0 = bid * nl;
   const cx<R>* tws = stage_twiddles(s, pl, nl, tw);
   {
     constexpr int EPT = kFftMaxElems / kFftBlock;
@@ -381,6 +397,16 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
   }
 }
 
+template <typename R, int MODE>
+__global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg, int nlines,
+                                                         int A, FastDiv ad, long long B,
+                                                         long long es,
+                                                         FftPlan pl,
+                                                         const cx<R>* __restrict__ tw,
+                                                         InfluenceArgs inf) {
+  k_fft_lines_body<R, MODE>(cg, nlines, A, ad, B, es, pl, tw, inf, blockIdx.x);
+}
+
 // ----------------------------------------------------------------------------------------
 // charge spreading
 // ----------------------------------------------------------------------------------------
@@ -390,18 +416,18 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
 // exactly one output tile, so the mesh is written once, without global atomics or a clear.
 // Work items are (candidate atom, z offset) pairs, so every thread gets a similar share.
 template <typename R>
-__global__ __launch_bounds__(256) void k_spread_checked(const int4* __restrict__ k0,
+__device__ __forceinline__ void k_spread_checked_body(const int4* __restrict__ k0,
                                                 const R* __restrict__ theta,
                                                 const double* __restrict__ q,
                                                 const int* __restrict__ tile_start,
-                                                SpreadArgs g, R* __restrict__ grid) {
+                                                SpreadArgs g, R* __restrict__ grid, int bid) {
   constexpr int kMaxVisit = kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles;
   __shared__ R acc[kSpreadTileMax];
   __shared__ int src_count[kMaxVisit];
   __shared__ int src_first[kMaxVisit + 1];
   __shared__ int src_begin[kMaxVisit];
   const int p = g.order;
-  int t = blockIdx.x;
+  int t = bid;
   const int t1 = t % g.nt[0];
   t /= g.nt[0];
   const int t2 = t % g.nt[1];
@@ -470,6 +496,15 @@ __global__ __launch_bounds__(256) void k_spread_checked(const int4* __restrict__
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_spread_checked(const int4* __restrict__ k0,
+                                                const R* __restrict__ theta,
+                                                const double* __restrict__ q,
+                                                const int* __restrict__ tile_start,
+                                                SpreadArgs g, R* __restrict__ grid) {
+  k_spread_checked_body<R>(k0, theta, q, tile_start, g, grid, blockIdx.x);
+}
+
 // Fast spreading for meshes with K >= T + p - 1 on every axis, where the stencil of an atom
 // meets a tile in one contiguous interval per axis.  Per output tile the candidate atoms are
 // staged in LDS in chunks -- only those that reach the tile, in a fixed order, with their
@@ -477,11 +512,11 @@ __global__ __launch_bounds__(256) void k_spread_checked(const int4* __restrict__
 // 4-point x segment of the tile and accumulates it in registers over the staged atoms (every
 // LDS read is a broadcast), so there are no atomics and the sum order is fixed.
 template <typename R, int P>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread(const int4* __restrict__ k0,
+__device__ __forceinline__ void k_spread_body(const int4* __restrict__ k0,
                                                            const R* __restrict__ theta,
                                                            const double* __restrict__ q,
                                                            const int* __restrict__ tile_start,
-                                                           SpreadArgs g, R* __restrict__ grid) {
+                                                           SpreadArgs g, R* __restrict__ grid, int bid) {
   constexpr int kMaxVisit = kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles;
   constexpr int kWaves = kSpreadThreads / 64;
   extern __shared__ unsigned char smem_raw[];
@@ -497,7 +532,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const int4* __restric
   __shared__ int src_first[kMaxVisit + 1];
   __shared__ int src_begin[kMaxVisit];
   __shared__ int wave_hits[kWaves];
-  int t = blockIdx.x;
+  int t = bid;
   const int t1 = t % g.nt[0];
   t /= g.nt[0];
   const int t2 = t % g.nt[1];
@@ -615,6 +650,15 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const int4* __restric
   if (owner) grid[((size_t)(lo[2] + mz) * g.K[1] + (lo[1] + my)) * g.K[0] + lo[0] + mx] = acc;
 }
 
+template <typename R, int P>
+__global__ __launch_bounds__(kSpreadThreads) void k_spread(const int4* __restrict__ k0,
+                                                           const R* __restrict__ theta,
+                                                           const double* __restrict__ q,
+                                                           const int* __restrict__ tile_start,
+                                                           SpreadArgs g, R* __restrict__ grid) {
+  k_spread_body<R, P>(k0, theta, q, tile_start, g, grid, blockIdx.x);
+}
+
 // Privatised spreading in two steps.
 // Step 1 (one workgroup per spread block of whole sort tiles): the atoms of the block -- those
 // whose stencil origin lies in it -- add their p^3 weights into a private LDS copy of the block
@@ -622,11 +666,11 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(const int4* __restric
 // Step 2 (one thread per mesh point): each point is the fixed-order sum of the at most 2^3
 // block copies that cover it.  No global atomics and no mesh clear.
 template <typename R, int P>
-__global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ k0,
+__device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0,
                                                        const R* __restrict__ theta,
                                                        const double* __restrict__ q,
                                                        const int* __restrict__ tile_start,
-                                                       RegionArgs g, R* __restrict__ regions) {
+                                                       RegionArgs g, R* __restrict__ regions, int bid) {
   // the private copy accumulates in FP64 in both precision modes (LDS FP64 adds are native
   // and fast; the FP32 ones are not), and is stored in the mesh precision R
   extern __shared__ unsigned char smem_raw[];
@@ -635,7 +679,7 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
   const int RX = g.S[0] + p - 1, RY = g.S[1] + p - 1, RZ = g.S[2] + p - 1;
   const int nreg = RX * RY * RZ;
   for (int i = threadIdx.x; i < nreg; i += blockDim.x) reg[i] = 0.0;
-  int b = blockIdx.x;
+  int b = bid;
   const int b1 = b % g.nb[0];
   b /= g.nb[0];
   const int b2 = b % g.nb[1], b3 = b / g.nb[1];
@@ -682,14 +726,23 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
     }
   }
   __syncthreads();
-  R* out = regions + (size_t)blockIdx.x * g.region_stride;
+  R* out = regions + (size_t)bid * g.region_stride;
   for (int i = threadIdx.x; i < nreg; i += blockDim.x) out[i] = R(reg[i]);
 }
 
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ k0,
+                                                       const R* __restrict__ theta,
+                                                       const double* __restrict__ q,
+                                                       const int* __restrict__ tile_start,
+                                                       RegionArgs g, R* __restrict__ regions) {
+  k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x);
+}
+
 template <typename R>
-__global__ __launch_bounds__(256) void k_spread_assemble(const R* __restrict__ regions,
-                                                         RegionArgs g, R* __restrict__ grid) {
-  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ regions,
+                                                         RegionArgs g, R* __restrict__ grid, int bid) {
+  const long long id = (long long)bid * blockDim.x + threadIdx.x;
   const long long G = (long long)g.K[0] * g.K[1] * g.K[2];
   if (id >= G) return;
   const int x = int(id % g.K[0]);
@@ -728,14 +781,25 @@ __global__ __launch_bounds__(256) void k_spread_assemble(const R* __restrict__ r
   grid[id] = sum;
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_spread_assemble(const R* __restrict__ regions,
+                                                         RegionArgs g, R* __restrict__ grid) {
+  k_spread_assemble_body<R>(regions, g, grid, blockIdx.x);
+}
+
 // ----------------------------------------------------------------------------------------
 // per-atom kernels
 // ----------------------------------------------------------------------------------------
 
+__device__ __forceinline__ void k_gather_charges_body(int n, const int* __restrict__ perm,
+                                 const double* __restrict__ dq, double* __restrict__ qs, int bid) {
+  const int k = bid * blockDim.x + threadIdx.x;
+  if (k < n) qs[k] = dq[perm[k]];
+}
+
 __global__ void k_gather_charges(int n, const int* __restrict__ perm,
                                  const double* __restrict__ dq, double* __restrict__ qs) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) qs[k] = dq[perm[k]];
+  k_gather_charges_body(n, perm, dq, qs, blockIdx.x);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -759,13 +823,13 @@ __device__ __forceinline__ int partner(int row, short c, int n) {
 // the wave reads consecutive entries and consecutive partners, and all eight loads of a lane
 // are issued before the first is used.
 template <typename CT, typename IT>
-__global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __restrict__ row_ptr,
+__device__ __forceinline__ void k_pair_spmv_body(int n, const long long* __restrict__ row_ptr,
                                                    const IT* __restrict__ col,
                                                    const CT* __restrict__ coef,
                                                    const double* __restrict__ q,
-                                                   double* __restrict__ out) {
+                                                   double* __restrict__ out, int bid) {
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int row = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (row >= n) return;  // uniform across the wave
   const long long a = row_ptr[row], b = row_ptr[row + 1];
   double acc = 0.0;
@@ -786,6 +850,15 @@ __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __res
   }
   acc = wave_sum(acc);
   if (lane == 0) out[row] = acc;
+}
+
+template <typename CT, typename IT>
+__global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __restrict__ row_ptr,
+                                                   const IT* __restrict__ col,
+                                                   const CT* __restrict__ coef,
+                                                   const double* __restrict__ q,
+                                                   double* __restrict__ out) {
+  k_pair_spmv_body<CT, IT>(n, row_ptr, col, coef, q, out, blockIdx.x);
 }
 
 // Rewrites absolute partner indices as signed 16-bit offsets from the row; flags rows whose
@@ -810,13 +883,13 @@ __global__ void k_compress_cols(int n, const long long* __restrict__ row_ptr,
 // B-spline gather of the convolved mesh, plus the real-space sum, the on-site short-gamma
 // term and the Ewald self term (fe/common/ewald_self.f90:17), written in caller order.
 template <typename R>
-__global__ __launch_bounds__(256) void k_gather_combine(
+__device__ __forceinline__ void k_gather_combine_body(
     int n, const int4* __restrict__ k0, const R* __restrict__ theta,
     const R* __restrict__ grid, const double* __restrict__ q, const double* __restrict__ phi_real,
     const double* __restrict__ onsite, const int* __restrict__ perm, GatherArgs g,
-    double* __restrict__ shift, double* __restrict__ epart) {
+    double* __restrict__ shift, double* __restrict__ epart, int bid) {
   __shared__ double red[256 / 64];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = bid * blockDim.x + threadIdx.x;
   double e = 0.0;
   if (k < n) {
     const int p = g.order;
@@ -859,8 +932,17 @@ __global__ __launch_bounds__(256) void k_gather_combine(
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-    epart[blockIdx.x] = s;
+    epart[bid] = s;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_gather_combine(
+    int n, const int4* __restrict__ k0, const R* __restrict__ theta,
+    const R* __restrict__ grid, const double* __restrict__ q, const double* __restrict__ phi_real,
+    const double* __restrict__ onsite, const int* __restrict__ perm, GatherArgs g,
+    double* __restrict__ shift, double* __restrict__ epart) {
+  k_gather_combine_body<R>(n, k0, theta, grid, q, phi_real, onsite, perm, g, shift, epart, blockIdx.x);
 }
 
 // B-spline gather per mesh tile: the tile's atoms (their stencil origins lie in it) read the
@@ -868,18 +950,18 @@ __global__ __launch_bounds__(256) void k_gather_combine(
 // reduced by shuffles.  Then the pair sum, on-site and self terms are added and the shift is
 // written in caller order; one energy partial per tile.
 template <typename R, int P>
-__global__ __launch_bounds__(256) void k_gather_tile(
+__device__ __forceinline__ void k_gather_tile_body(
     const int4* __restrict__ k0, const R* __restrict__ theta, const R* __restrict__ grid,
     const double* __restrict__ q, const double* __restrict__ phi_real,
     const double* __restrict__ onsite, const int* __restrict__ perm,
     const int* __restrict__ tile_start, GatherArgs g, double* __restrict__ shift,
-    double* __restrict__ epart) {
+    double* __restrict__ epart, int bid) {
   extern __shared__ unsigned char smem_raw[];
   R* ths = reinterpret_cast<R*>(smem_raw);  // [64][3p] weights of the 64 atoms of one pass
   R* reg = ths + 64 * 3 * g.order;          // staged mesh region
   __shared__ double red[4];
   const int p = P > 0 ? P : g.order;
-  int t = blockIdx.x;
+  int t = bid;
   const int t1 = t % g.nt[0];
   t /= g.nt[0];
   const int t2 = t % g.nt[1];
@@ -911,7 +993,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     }
   }
   __syncthreads();
-  const int a0 = tile_start[blockIdx.x], a1 = tile_start[blockIdx.x + 1];
+  const int a0 = tile_start[bid], a1 = tile_start[bid + 1];
   const int sub = threadIdx.x & 3, grp = threadIdx.x >> 2;
   const int rows = p * p;
   double e = 0.0;
@@ -970,11 +1052,21 @@ __global__ __launch_bounds__(256) void k_gather_tile(
   e = wave_sum(e);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
   __syncthreads();
-  if (threadIdx.x == 0) epart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) epart[bid] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void k_energy_final(int nparts, const double* __restrict__ epart,
-                               double* __restrict__ energy) {
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_gather_tile(
+    const int4* __restrict__ k0, const R* __restrict__ theta, const R* __restrict__ grid,
+    const double* __restrict__ q, const double* __restrict__ phi_real,
+    const double* __restrict__ onsite, const int* __restrict__ perm,
+    const int* __restrict__ tile_start, GatherArgs g, double* __restrict__ shift,
+    double* __restrict__ epart) {
+  k_gather_tile_body<R, P>(k0, theta, grid, q, phi_real, onsite, perm, tile_start, g, shift, epart, blockIdx.x);
+}
+
+__device__ __forceinline__ void k_energy_final_body(int nparts, const double* __restrict__ epart,
+                               double* __restrict__ energy, int bid) {
   __shared__ double red[256];
   double s = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += epart[i];
@@ -985,6 +1077,11 @@ __global__ void k_energy_final(int nparts, const double* __restrict__ epart,
     __syncthreads();
   }
   if (threadIdx.x == 0) *energy = red[0];
+}
+
+__global__ void k_energy_final(int nparts, const double* __restrict__ epart,
+                               double* __restrict__ energy) {
+  k_energy_final_body(nparts, epart, energy, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1278,6 +1375,223 @@ __global__ void k_force_real(int n, const double* __restrict__ frac, const int* 
   const int o = perm[i];
   for (int d = 0; d < 3; ++d) force[3 * o + d] += qi * f[d];
 }
+
+// ----------------------------------------------------------------------------------------
+// grouped kernels: one launch covers the blocks of many structures
+// ----------------------------------------------------------------------------------------
+
+// Evaluation slot and local block index of this workgroup (binary search over the block
+// prefix; the prefix is uniform per workgroup, so the loads go through the scalar cache).
+__device__ __forceinline__ int group_slot(const GroupMap& m, int& local) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = m.count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (m.prefix[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  local = b - m.prefix[lo];
+  return m.slot[lo];
+}
+
+__global__ __launch_bounds__(256) void kg_gather_charges(const StructDesc* __restrict__ D,
+                                                         const IoSlot* __restrict__ io,
+                                                         GroupMap m) {
+  int b;
+  const IoSlot& o = io[group_slot(m, b)];
+  const StructDesc& d = D[o.sid];
+  k_gather_charges_body(d.n, d.perm, o.dq, d.qs, b);
+}
+
+template <typename R, int P>
+__global__ __launch_bounds__(256) void kg_spread_region(const StructDesc* __restrict__ D,
+                                                        const IoSlot* __restrict__ io,
+                                                        GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  k_spread_region_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.region,
+                             static_cast<R*>(d.region.scratch), b);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void kg_spread_assemble(const StructDesc* __restrict__ D,
+                                                          const IoSlot* __restrict__ io,
+                                                          GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  k_spread_assemble_body<R>(static_cast<const R*>(d.region.scratch), d.region,
+                            static_cast<R*>(d.grid), b);
+}
+
+template <typename R, int P>
+__global__ __launch_bounds__(kSpreadThreads) void kg_spread(const StructDesc* __restrict__ D,
+                                                            const IoSlot* __restrict__ io,
+                                                            GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  k_spread_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.spread,
+                      static_cast<R*>(d.grid), b);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void kg_spread_checked(const StructDesc* __restrict__ D,
+                                                         const IoSlot* __restrict__ io,
+                                                         GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  k_spread_checked_body<R>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.spread,
+                           static_cast<R*>(d.grid), b);
+}
+
+template <typename R, bool INVERSE>
+__global__ __launch_bounds__(kFftBlock) void kg_fft_x(const StructDesc* __restrict__ D,
+                                                      const IoSlot* __restrict__ io,
+                                                      GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  const int nrows = d.K[1] * d.K[2], Kh = d.K[0] / 2 + 1;
+  if (INVERSE)
+    k_fft_x_c2r_body<R>(static_cast<const cx<R>*>(d.cgrid), static_cast<R*>(d.grid), nrows, Kh,
+                        d.px, static_cast<const cx<R>*>(d.tw[0]), b);
+  else
+    k_fft_x_r2c_body<R>(static_cast<const R*>(d.grid), static_cast<cx<R>*>(d.cgrid), nrows, Kh,
+                        d.px, static_cast<const cx<R>*>(d.tw[0]), b);
+}
+
+// AXIS 1: y lines (MODE 0 forward, 1 inverse); AXIS 2: z lines with the influence (MODE 2).
+template <typename R, int AXIS, int MODE>
+__global__ __launch_bounds__(kFftBlock) void kg_fft_lines(const StructDesc* __restrict__ D,
+                                                          const IoSlot* __restrict__ io,
+                                                          GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  const int Kh = d.K[0] / 2 + 1;
+  cx<R>* cg = static_cast<cx<R>*>(d.cgrid);
+  if (AXIS == 1)
+    k_fft_lines_body<R, MODE>(cg, Kh * d.K[2], Kh, d.yd, (long long)d.K[1] * Kh, (long long)Kh,
+                              d.py, static_cast<const cx<R>*>(d.tw[1]), d.inf, b);
+  else
+    k_fft_lines_body<R, MODE>(cg, Kh * d.K[1], Kh * d.K[1], d.zd, 0LL, (long long)Kh * d.K[1],
+                              d.pz, static_cast<const cx<R>*>(d.tw[2]), d.inf, b);
+}
+
+template <typename CT>
+__global__ __launch_bounds__(256) void kg_pair_spmv(const StructDesc* __restrict__ D,
+                                                    const IoSlot* __restrict__ io, GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  if (d.col16)
+    k_pair_spmv_body<CT, short>(d.n, d.row_ptr, static_cast<const short*>(d.col),
+                                static_cast<const CT*>(d.coef), d.qs, d.phi_real, b);
+  else
+    k_pair_spmv_body<CT, int>(d.n, d.row_ptr, static_cast<const int*>(d.col),
+                              static_cast<const CT*>(d.coef), d.qs, d.phi_real, b);
+}
+
+template <typename R, int P>
+__global__ __launch_bounds__(256) void kg_gather_tile(const StructDesc* __restrict__ D,
+                                                      const IoSlot* __restrict__ io,
+                                                      GroupMap m) {
+  int b;
+  const IoSlot& o = io[group_slot(m, b)];
+  const StructDesc& d = D[o.sid];
+  k_gather_tile_body<R, P>(d.k0, static_cast<const R*>(d.theta), static_cast<const R*>(d.grid),
+                           d.qs, d.phi_real, d.onsite, d.perm, d.tile_start, d.ga, o.shift,
+                           d.epart, b);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void kg_gather_combine(const StructDesc* __restrict__ D,
+                                                         const IoSlot* __restrict__ io,
+                                                         GroupMap m) {
+  int b;
+  const IoSlot& o = io[group_slot(m, b)];
+  const StructDesc& d = D[o.sid];
+  k_gather_combine_body<R>(d.n, d.k0, static_cast<const R*>(d.theta),
+                           static_cast<const R*>(d.grid), d.qs, d.phi_real, d.onsite, d.perm,
+                           d.ga, o.shift, d.epart, b);
+}
+
+__global__ __launch_bounds__(256) void kg_energy_final(const StructDesc* __restrict__ D,
+                                                       const IoSlot* __restrict__ io,
+                                                       GroupMap m) {
+  int b;
+  const IoSlot& o = io[group_slot(m, b)];
+  const StructDesc& d = D[o.sid];
+  k_energy_final_body(d.nparts, d.epart, o.energy, b);
+}
+
+template <typename R>
+hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
+  const StructDesc* D = G.desc;
+  const IoSlot* io = G.io;
+  auto run = [&](int kind) { return G.blocks[kind] > 0; };
+  auto grid = [&](int kind) { return dim3(G.blocks[kind]); };
+  if (run(kGkCharges))
+    hipLaunchKernelGGL(kg_gather_charges, grid(kGkCharges), dim3(256), 0, st, D, io,
+                       G.map[kGkCharges]);
+#define EW_ORDER_SWITCH(MACRO) \
+  switch (G.order) {           \
+    case 4: MACRO(4); break;   \
+    case 5: MACRO(5); break;   \
+    case 6: MACRO(6); break;   \
+    case 7: MACRO(7); break;   \
+    case 8: MACRO(8); break;   \
+    case 10: MACRO(10); break; \
+    default: MACRO(0); break;  \
+  }
+  if (run(kGkRegion)) {
+#define EW_G_REGION(PP)                                                                  \
+  hipLaunchKernelGGL((kg_spread_region<R, PP>), grid(kGkRegion), dim3(256),             \
+                     G.smem[kGkRegion], st, D, io, G.map[kGkRegion])
+    EW_ORDER_SWITCH(EW_G_REGION)
+#undef EW_G_REGION
+    hipLaunchKernelGGL(kg_spread_assemble<R>, grid(kGkAssemble), dim3(256), 0, st, D, io,
+                       G.map[kGkAssemble]);
+  }
+  if (run(kGkSlotted)) {
+#define EW_G_SLOTTED(PP)                                                                  \
+  hipLaunchKernelGGL((kg_spread<R, PP>), grid(kGkSlotted), dim3(kSpreadThreads),         \
+                     G.smem[kGkSlotted], st, D, io, G.map[kGkSlotted])
+    EW_ORDER_SWITCH(EW_G_SLOTTED)
+#undef EW_G_SLOTTED
+  }
+  if (run(kGkChecked))
+    hipLaunchKernelGGL(kg_spread_checked<R>, grid(kGkChecked), dim3(256), 0, st, D, io,
+                       G.map[kGkChecked]);
+  hipLaunchKernelGGL((kg_fft_x<R, false>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX], st,
+                     D, io, G.map[kGkFftX]);
+  hipLaunchKernelGGL((kg_fft_lines<R, 1, 0>), grid(kGkFftY), dim3(kFftBlock), G.smem[kGkFftY],
+                     st, D, io, G.map[kGkFftY]);
+  hipLaunchKernelGGL((kg_fft_lines<R, 2, 2>), grid(kGkFftZ), dim3(kFftBlock), G.smem[kGkFftZ],
+                     st, D, io, G.map[kGkFftZ]);
+  hipLaunchKernelGGL((kg_fft_lines<R, 1, 1>), grid(kGkFftY), dim3(kFftBlock), G.smem[kGkFftY],
+                     st, D, io, G.map[kGkFftY]);
+  hipLaunchKernelGGL((kg_fft_x<R, true>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX], st,
+                     D, io, G.map[kGkFftX]);
+  if (run(kGkPairs))
+    hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, st, D, io,
+                       G.map[kGkPairs]);
+  if (run(kGkGatherTile)) {
+#define EW_G_GATHER(PP)                                                                   \
+  hipLaunchKernelGGL((kg_gather_tile<R, PP>), grid(kGkGatherTile), dim3(256),            \
+                     G.smem[kGkGatherTile], st, D, io, G.map[kGkGatherTile])
+    EW_ORDER_SWITCH(EW_G_GATHER)
+#undef EW_G_GATHER
+  }
+  if (run(kGkGatherAtom))
+    hipLaunchKernelGGL(kg_gather_combine<R>, grid(kGkGatherAtom), dim3(256), 0, st, D, io,
+                       G.map[kGkGatherAtom]);
+  hipLaunchKernelGGL(kg_energy_final, grid(kGkEnergy), dim3(256), 0, st, D, io,
+                     G.map[kGkEnergy]);
+#undef EW_ORDER_SWITCH
+  return hipGetLastError();
+}
+
+template hipError_t launch_group<double>(const GroupLaunch&, hipStream_t);
+template hipError_t launch_group<float>(const GroupLaunch&, hipStream_t);
 
 // ----------------------------------------------------------------------------------------
 // launchers

@@ -105,6 +105,87 @@ struct SpmeLaunch {
   void* cgrid;
 };
 
+// ---------------------------------------------------------------------------------------
+// Grouped evaluation: many resident structures in one launch per kernel.
+// ---------------------------------------------------------------------------------------
+
+// One resident structure as the grouped kernels see it (device copy, built per geometry).
+struct StructDesc {
+  int n;
+  int K[3];
+  int ntiles;
+  int nparts;       // energy partials written by the gather
+  int spread_mode;  // 0: privatised blocks + assembly, 1: staged (slots), 2: checked
+  int gather_tile;  // 1: tile gather, 0: per-atom gather
+  int col16;        // 1: 16-bit partner offsets
+  const int* perm;
+  const int4* k0;
+  const void* theta;
+  const int* tile_start;
+  double* qs;
+  double* phi_real;
+  const double* onsite;
+  double* epart;
+  const long long* row_ptr;
+  const void* col;
+  const void* coef;
+  void* grid;
+  void* cgrid;
+  const void* tw[3];
+  SpreadArgs spread;
+  RegionArgs region;
+  GatherArgs ga;
+  InfluenceArgs inf;
+  FftPlan px, py, pz;
+  FastDiv yd;  // / Kh            (y lines: L = kz * Kh + kx)
+  FastDiv zd;  // / (Kh * K2)     (z lines: L = ky * Kh + kx)
+};
+
+// One evaluation of a launch: structure sid, charges in, shift (caller order) and energy out.
+struct IoSlot {
+  const double* dq;
+  double* shift;
+  double* energy;
+  int sid;
+  int pad;
+};
+
+// Blocks of one grouped kernel: entry i covers blocks [prefix[i], prefix[i+1]) for the
+// evaluation slot[i] (an index into the IoSlot array).
+struct GroupMap {
+  const int* prefix;
+  const int* slot;
+  int count;
+};
+
+enum GroupKind {
+  kGkCharges = 0,
+  kGkRegion,
+  kGkAssemble,
+  kGkSlotted,
+  kGkChecked,
+  kGkFftX,
+  kGkFftY,
+  kGkFftZ,
+  kGkPairs,
+  kGkGatherTile,
+  kGkGatherAtom,
+  kGkEnergy,
+  kGroupKinds
+};
+
+struct GroupLaunch {
+  const StructDesc* desc;
+  const IoSlot* io;
+  int order;
+  GroupMap map[kGroupKinds];
+  int blocks[kGroupKinds];
+  size_t smem[kGroupKinds];  // dynamic LDS: the largest of the structures in the kind
+};
+
+template <typename R>
+hipError_t launch_group(const GroupLaunch& G, hipStream_t st);
+
 // ev (optional, 6 events): recorded after spread, x, y, z, inverse y, inverse x.
 template <typename R>
 hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);

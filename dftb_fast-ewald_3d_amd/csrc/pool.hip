@@ -1,0 +1,516 @@
+// pool.hip -- many resident structures evaluated together: the batched and serving forms.
+//
+// A pool holds the resident state of many structures (one ew_handle each, all on the pool's
+// stream) and a device table of their descriptors.  One evaluation of any subset is one launch
+// per kernel kind (gather charges, spread, five FFT passes, pair sum, gather, energy) whose
+// workgroups cover every structure of the subset; a block prefix per kind maps a workgroup to
+// its structure.  Repeated evaluations of the same subset into the same buffers are replayed
+// from a captured hipGraph.
+//
+// The per-structure work is the single-structure engine's (same kernel bodies, same variants),
+// so a structure gives the same shift and energy alone and inside any batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fftewald_amd.h"
+#include "engine.hpp"
+#include "handle.hpp"
+#include "kernels.hpp"
+
+using namespace ewamd;
+
+namespace {
+
+struct Captured {
+  std::vector<int> ids;
+  std::vector<const void*> ptrs;  // dq and shift per entry, then energies
+  hipGraphExec_t exec = nullptr;
+  void* staging = nullptr;  // map and io arrays the graph's kernels read
+};
+
+}  // namespace
+
+struct ew_pool {
+  ew_options opt{};
+  hipStream_t stream = nullptr;
+  std::string err = "no error";
+  bool fp32 = false;
+  int order = 0;  // B-spline order shared by every member (0 until the first member)
+  long long capacity = 0;
+  long long used = 0;
+  std::vector<ew_handle*> members;  // by id; null = free id
+  std::vector<StructDesc> desc;
+  StructDesc* desc_dev = nullptr;
+  size_t desc_cap = 0;
+  bool desc_dirty = true;
+  // staging of eager launches (map + io) and of the host-array entry point
+  void* staging = nullptr;
+  size_t staging_cap = 0;
+  double* io_dq = nullptr;
+  double* io_shift = nullptr;
+  double* io_energy = nullptr;
+  size_t io_atoms_cap = 0, io_struct_cap = 0;
+  std::vector<Captured> graphs;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    global_error() = msg;
+    return code;
+  }
+  void drop_graphs() {
+    for (auto& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.staging) (void)hipFree(g.staging);
+    }
+    graphs.clear();
+  }
+};
+
+#define PCHK(p, expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return (p)->fail(EW_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                          " at " #expr);                                     \
+  } while (0)
+
+namespace {
+
+size_t smem_of(const StructDesc& d, int kind, bool fp32) {
+  const size_t rs = fp32 ? sizeof(float) : sizeof(double);
+  const size_t cs = 2 * rs;
+  const int p = d.spread.order;
+  switch (kind) {
+    case kGkRegion:
+      return sizeof(double) * (size_t)(d.region.S[0] + p - 1) * (d.region.S[1] + p - 1) *
+             (d.region.S[2] + p - 1);
+    case kGkSlotted:
+      return sizeof(unsigned long long) * kSpreadRows * 4 + sizeof(int4) * (size_t)d.spread.slots +
+             rs * 3 * (size_t)d.spread.slots * p;
+    case kGkFftX:
+      return cs * ((size_t)d.px.lines * d.px.pitch + d.px.n);
+    case kGkFftY:
+      return cs * ((size_t)d.py.lines * d.py.pitch + d.py.n);
+    case kGkFftZ:
+      return cs * ((size_t)d.pz.lines * d.pz.pitch + d.pz.n);
+    case kGkGatherTile:
+      return gather_tile_smem(d.spread.T, p, rs);
+    default:
+      return 0;
+  }
+}
+
+int blocks_of(const StructDesc& d, int kind) {
+  const int Kh = d.K[0] / 2 + 1;
+  switch (kind) {
+    case kGkCharges: return (d.n + 255) / 256;
+    case kGkRegion: return d.spread_mode == 0 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] : 0;
+    case kGkAssemble:
+      return d.spread_mode == 0 ? int(((long long)d.K[0] * d.K[1] * d.K[2] + 255) / 256) : 0;
+    case kGkSlotted: return d.spread_mode == 1 ? d.ntiles : 0;
+    case kGkChecked: return d.spread_mode == 2 ? d.ntiles : 0;
+    case kGkFftX: return (d.K[1] * d.K[2] + 2 * d.px.lines - 1) / (2 * d.px.lines);
+    case kGkFftY: return (Kh * d.K[2] + d.py.lines - 1) / d.py.lines;
+    case kGkFftZ: return (Kh * d.K[1] + d.pz.lines - 1) / d.pz.lines;
+    case kGkPairs: return (d.n + 3) / 4;
+    case kGkGatherTile: return d.gather_tile ? d.ntiles : 0;
+    case kGkGatherAtom: return d.gather_tile ? 0 : (d.n + 255) / 256;
+    case kGkEnergy: return 1;
+    default: return 0;
+  }
+}
+
+int ensure(ew_pool* P, void** buf, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return EW_OK;
+  if (*buf) PCHK(P, hipFree(*buf));
+  *buf = nullptr;
+  *cap = 0;
+  const size_t want = std::max(bytes, 2 * *cap);
+  if (hipMalloc(buf, want) != hipSuccess) {
+    (void)hipGetLastError();
+    *buf = nullptr;
+    return P->fail(EW_ERR_MEMORY, "pool staging allocation failed");
+  }
+  *cap = want;
+  return EW_OK;
+}
+
+int sync_desc(ew_pool* P) {
+  if (!P->desc_dirty) return EW_OK;
+  const size_t bytes = sizeof(StructDesc) * std::max<size_t>(1, P->desc.size());
+  if (P->desc_cap < bytes) {
+    // the captured graphs read the old table
+    P->drop_graphs();
+    void* b = P->desc_dev;
+    int st = ensure(P, &b, &P->desc_cap, bytes);
+    P->desc_dev = static_cast<StructDesc*>(b);
+    if (st) return st;
+  }
+  if (!P->desc.empty())
+    PCHK(P, hipMemcpyAsync(P->desc_dev, P->desc.data(), sizeof(StructDesc) * P->desc.size(),
+                           hipMemcpyHostToDevice, P->stream));
+  PCHK(P, hipStreamSynchronize(P->stream));
+  P->desc_dirty = false;
+  return EW_OK;
+}
+
+// Host image of the map and io arrays of one launch, and the GroupLaunch that points into a
+// device copy of it at `dev`.
+struct Staged {
+  std::vector<int> words;  // per kind: prefix[count+1], slot[count]; then the IoSlots
+  size_t io_offset = 0;    // in bytes
+  int offs[kGroupKinds] = {}, counts[kGroupKinds] = {}, blocks[kGroupKinds] = {};
+  size_t smem[kGroupKinds] = {};
+};
+
+int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double* const* shift,
+          double* energies, Staged* S) {
+  std::vector<IoSlot> io((size_t)count);
+  for (int i = 0; i < count; ++i) io[i] = IoSlot{dq[i], shift[i], energies + i, ids[i], 0};
+  S->words.clear();
+  for (int k = 0; k < kGroupKinds; ++k) {
+    std::vector<int> prefix, slot;
+    int run = 0;
+    size_t sm = 0;
+    for (int i = 0; i < count; ++i) {
+      const StructDesc& d = P->desc[ids[i]];
+      const int b = blocks_of(d, k);
+      if (b <= 0) continue;
+      prefix.push_back(run);
+      slot.push_back(i);
+      run += b;
+      sm = std::max(sm, smem_of(d, k, P->fp32));
+    }
+    prefix.push_back(run);
+    S->offs[k] = (int)S->words.size();
+    S->counts[k] = (int)slot.size();
+    S->blocks[k] = run;
+    S->smem[k] = sm;
+    S->words.insert(S->words.end(), prefix.begin(), prefix.end());
+    S->words.insert(S->words.end(), slot.begin(), slot.end());
+  }
+  while ((S->words.size() * sizeof(int)) % alignof(IoSlot)) S->words.push_back(0);
+  S->io_offset = S->words.size() * sizeof(int);
+  const size_t nw = (S->io_offset + sizeof(IoSlot) * io.size() + sizeof(int) - 1) / sizeof(int);
+  S->words.resize(nw, 0);
+  std::memcpy(reinterpret_cast<char*>(S->words.data()) + S->io_offset, io.data(),
+              sizeof(IoSlot) * io.size());
+  for (int k = 0; k < kGroupKinds; ++k)
+    if (S->smem[k] > 65536) return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
+  return EW_OK;
+}
+
+GroupLaunch launch_of(const ew_pool* P, const Staged& S, const void* dev) {
+  GroupLaunch G{};
+  G.desc = P->desc_dev;
+  const int* w = static_cast<const int*>(dev);
+  G.io = reinterpret_cast<const IoSlot*>(static_cast<const char*>(dev) + S.io_offset);
+  G.order = P->order;
+  for (int k = 0; k < kGroupKinds; ++k) {
+    G.map[k] = GroupMap{w + S.offs[k], w + S.offs[k] + S.counts[k] + 1, S.counts[k]};
+    G.blocks[k] = S.blocks[k];
+    G.smem[k] = S.smem[k];
+  }
+  return G;
+}
+
+int check_ids(ew_pool* P, int count, const int* ids) {
+  if (count <= 0 || !ids) return P->fail(EW_ERR_ARGUMENT, "empty structure list");
+  std::vector<char> seen(P->members.size(), 0);
+  for (int i = 0; i < count; ++i) {
+    const int id = ids[i];
+    if (id < 0 || id >= (int)P->members.size() || !P->members[id])
+      return P->fail(EW_ERR_ARGUMENT, "unknown structure id " + std::to_string(id));
+    if (seen[id]) return P->fail(EW_ERR_ARGUMENT, "structure id listed twice in one evaluation");
+    seen[id] = 1;
+  }
+  return EW_OK;
+}
+
+int enqueue(ew_pool* P, const GroupLaunch& G, hipStream_t s) {
+  PCHK(P, P->fp32 ? launch_group<float>(G, s) : launch_group<double>(G, s));
+  return EW_OK;
+}
+
+// One grouped evaluation on stream s: graph replay when the same ids and buffers were seen.
+int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* const* shift,
+        double* energies, hipStream_t s) {
+  int st = sync_desc(P);
+  if (st) return st;
+  Staged S;
+  st = stage(P, count, ids, dq, shift, energies, &S);
+  if (st) return st;
+  const size_t bytes = S.words.size() * sizeof(int);
+  if (P->opt.eager || P->opt.profile) {
+    st = ensure(P, &P->staging, &P->staging_cap, bytes);
+    if (st) return st;
+    // a pageable source: the copy has read the host image when the call returns
+    PCHK(P, hipMemcpyAsync(P->staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
+    return enqueue(P, launch_of(P, S, P->staging), s);
+  }
+  std::vector<const void*> ptrs;
+  ptrs.reserve(2 * (size_t)count + 1);
+  for (int i = 0; i < count; ++i) {
+    ptrs.push_back(dq[i]);
+    ptrs.push_back(shift[i]);
+  }
+  ptrs.push_back(energies);
+  const std::vector<int> key(ids, ids + count);
+  for (auto& g : P->graphs)
+    if (g.ids == key && g.ptrs == ptrs) {
+      PCHK(P, hipGraphLaunch(g.exec, s));
+      return EW_OK;
+    }
+  if (P->graphs.size() >= 8) P->drop_graphs();
+  Captured c;
+  c.ids = key;
+  c.ptrs = ptrs;
+  if (hipMalloc(&c.staging, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return P->fail(EW_ERR_MEMORY, "pool staging allocation failed");
+  }
+  PCHK(P, hipMemcpy(c.staging, S.words.data(), bytes, hipMemcpyHostToDevice));
+  const GroupLaunch G = launch_of(P, S, c.staging);
+  PCHK(P, hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+  st = enqueue(P, G, P->stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(P->stream, &graph);
+  if (st != EW_OK || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(c.staging);
+    if (st != EW_OK) return st;
+    return P->fail(EW_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ec));
+  }
+  const hipError_t ei = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    (void)hipFree(c.staging);
+    return P->fail(EW_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(ei));
+  }
+  P->graphs.push_back(c);
+  PCHK(P, hipGraphLaunch(c.exec, s));
+  return EW_OK;
+}
+
+}  // namespace
+
+extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, ew_pool** out) {
+  if (!out) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_create: out is NULL");
+  *out = nullptr;
+  ew_options o{};
+  if (opts) o = *opts;
+  if (o.precision != EW_PRECISION_FP64 && o.precision != EW_PRECISION_MIXED)
+    return set_global_error(EW_ERR_ARGUMENT, "ew_pool_create: unknown precision mode");
+  if (capacity_bytes < 0)
+    return set_global_error(EW_ERR_ARGUMENT, "ew_pool_create: negative capacity");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return set_global_error(EW_ERR_DEVICE, "ew_pool_create: no HIP device available");
+  }
+  if (o.device < 0 || o.device >= ndev)
+    return set_global_error(EW_ERR_ARGUMENT, "ew_pool_create: device ordinal out of range");
+  if (hipSetDevice(o.device) != hipSuccess)
+    return set_global_error(EW_ERR_DEVICE, "ew_pool_create: hipSetDevice failed");
+  ew_pool* P = new ew_pool();
+  P->opt = o;
+  P->opt.profile = 0;  // per-phase events are a single-structure feature
+  P->fp32 = o.precision == EW_PRECISION_MIXED;
+  P->capacity = capacity_bytes;
+  if (hipStreamCreate(&P->stream) != hipSuccess) {
+    delete P;
+    return set_global_error(EW_ERR_DEVICE, "ew_pool_create: hipStreamCreate failed");
+  }
+  *out = P;
+  return EW_OK;
+}
+
+extern "C" int ew_pool_destroy(ew_pool* P) {
+  if (!P) return EW_OK;
+  (void)hipSetDevice(P->opt.device);
+  (void)hipStreamSynchronize(P->stream);
+  P->drop_graphs();
+  for (auto* h : P->members)
+    if (h) (void)ew_destroy(h);
+  if (P->desc_dev) (void)hipFree(P->desc_dev);
+  if (P->staging) (void)hipFree(P->staging);
+  if (P->io_dq) (void)hipFree(P->io_dq);
+  if (P->io_shift) (void)hipFree(P->io_shift);
+  if (P->io_energy) (void)hipFree(P->io_energy);
+  (void)hipStreamDestroy(P->stream);
+  delete P;
+  return EW_OK;
+}
+
+extern "C" const char* ew_pool_last_error(const ew_pool* P) {
+  return P ? P->err.c_str() : global_error().c_str();
+}
+
+extern "C" int ew_pool_add(ew_pool* P, int n, const double* coords, const double* latvecs,
+                           const int* species, const double* hubbard_u, int n_species,
+                           const double* charges, const int* grid, int* id) {
+  if (!P || !id) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_add: NULL argument");
+  *id = -1;
+  PCHK(P, hipSetDevice(P->opt.device));
+  ew_options o = P->opt;
+  if (grid)
+    for (int a = 0; a < 3; ++a) o.grid[a] = grid[a];
+  ew_handle* h = nullptr;
+  int st = handle_create(o, P->stream, &h);
+  if (st) return P->fail(st, global_error());
+  st = ew_set_geometry(h, n, coords, latvecs, species, hubbard_u, n_species, charges);
+  if (st) {
+    const std::string msg = h->err;
+    (void)ew_destroy(h);
+    return P->fail(st, msg);
+  }
+  if (P->order != 0 && h->prm.order != P->order) {
+    (void)ew_destroy(h);
+    return P->fail(EW_ERR_ARGUMENT, "every structure of a pool must use the same B-spline order");
+  }
+  if (P->capacity > 0 && P->used + (long long)h->device_bytes > P->capacity) {
+    const long long need = (long long)h->device_bytes;
+    (void)ew_destroy(h);
+    return P->fail(EW_ERR_MEMORY, "pool capacity exceeded: structure needs " +
+                                      std::to_string(need) + " bytes, " +
+                                      std::to_string(P->capacity - P->used) + " free");
+  }
+  P->order = h->prm.order;
+  int slot = -1;
+  for (size_t k = 0; k < P->members.size(); ++k)
+    if (!P->members[k]) {
+      slot = (int)k;
+      break;
+    }
+  if (slot < 0) {
+    slot = (int)P->members.size();
+    P->members.push_back(nullptr);
+    P->desc.emplace_back();
+  }
+  P->members[slot] = h;
+  fill_desc(h, &P->desc[slot]);
+  P->used += (long long)h->device_bytes;
+  P->desc_dirty = true;
+  *id = slot;
+  return EW_OK;
+}
+
+extern "C" int ew_pool_remove(ew_pool* P, int id) {
+  if (!P) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_remove: pool is NULL");
+  if (id < 0 || id >= (int)P->members.size() || !P->members[id])
+    return P->fail(EW_ERR_ARGUMENT, "unknown structure id " + std::to_string(id));
+  PCHK(P, hipSetDevice(P->opt.device));
+  PCHK(P, hipStreamSynchronize(P->stream));
+  // graphs may name the structure's buffers
+  P->drop_graphs();
+  P->used -= (long long)P->members[id]->device_bytes;
+  (void)ew_destroy(P->members[id]);
+  P->members[id] = nullptr;
+  P->desc[id] = StructDesc{};
+  P->desc_dirty = true;
+  if (P->used <= 0) P->order = 0;
+  return EW_OK;
+}
+
+extern "C" int ew_pool_parameters(const ew_pool* P, int id, ew_parameters* out) {
+  if (!P || !out) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_parameters: NULL argument");
+  if (id < 0 || id >= (int)P->members.size() || !P->members[id])
+    return set_global_error(EW_ERR_ARGUMENT, "unknown structure id " + std::to_string(id));
+  return ew_get_parameters(P->members[id], out);
+}
+
+extern "C" int ew_pool_usage(const ew_pool* P, long long* used, long long* capacity, int* count) {
+  if (!P) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_usage: pool is NULL");
+  if (used) *used = P->used;
+  if (capacity) *capacity = P->capacity;
+  if (count) {
+    int c = 0;
+    for (auto* h : P->members) c += h != nullptr;
+    *count = c;
+  }
+  return EW_OK;
+}
+
+extern "C" int ew_pool_shift_device(ew_pool* P, int count, const int* ids,
+                                    const double* const* d_dq, double* const* d_shift,
+                                    double* d_energies, void* stream) {
+  if (!P) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_shift_device: pool is NULL");
+  int st = check_ids(P, count, ids);
+  if (st) return st;
+  if (!d_dq || !d_shift || !d_energies)
+    return P->fail(EW_ERR_ARGUMENT, "ew_pool_shift_device: NULL array");
+  for (int i = 0; i < count; ++i)
+    if (!d_dq[i] || !d_shift[i]) return P->fail(EW_ERR_ARGUMENT, "ew_pool_shift_device: NULL buffer");
+  PCHK(P, hipSetDevice(P->opt.device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : P->stream;
+  return run(P, count, ids, d_dq, d_shift, d_energies, s);
+}
+
+extern "C" int ew_pool_shift(ew_pool* P, int count, const int* ids, const double* dq,
+                             double* shift, double* energies) {
+  if (!P) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_shift: pool is NULL");
+  int st = check_ids(P, count, ids);
+  if (st) return st;
+  if (!dq || !shift || !energies) return P->fail(EW_ERR_ARGUMENT, "ew_pool_shift: NULL array");
+  std::vector<size_t> off((size_t)count + 1, 0);
+  for (int i = 0; i < count; ++i) {
+    const int n = P->members[ids[i]]->n;
+    off[i + 1] = off[i] + n;
+    double net = 0.0, absq = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double v = dq[off[i] + k];
+      if (!std::isfinite(v)) return P->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+      net += v;
+      absq += std::fabs(v);
+    }
+    if (std::fabs(net) > kNeutralityTol * absq)
+      return P->fail(EW_ERR_NOT_NEUTRAL, "structure " + std::to_string(ids[i]) +
+                                             ": the cell must be charge neutral");
+  }
+  PCHK(P, hipSetDevice(P->opt.device));
+  const size_t atoms = off[count];
+  if (P->io_atoms_cap < atoms || P->io_struct_cap < (size_t)count) {
+    // graphs may point at the old buffers
+    PCHK(P, hipStreamSynchronize(P->stream));
+    P->drop_graphs();
+    for (double** b : {&P->io_dq, &P->io_shift, &P->io_energy})
+      if (*b) {
+        (void)hipFree(*b);
+        *b = nullptr;
+      }
+    P->io_atoms_cap = std::max(atoms, 2 * P->io_atoms_cap);
+    P->io_struct_cap = std::max((size_t)count, 2 * P->io_struct_cap);
+    if (hipMalloc(&P->io_dq, sizeof(double) * P->io_atoms_cap) != hipSuccess ||
+        hipMalloc(&P->io_shift, sizeof(double) * P->io_atoms_cap) != hipSuccess ||
+        hipMalloc(&P->io_energy, sizeof(double) * P->io_struct_cap) != hipSuccess) {
+      (void)hipGetLastError();
+      P->io_atoms_cap = P->io_struct_cap = 0;
+      return P->fail(EW_ERR_MEMORY, "pool i/o allocation failed");
+    }
+  }
+  std::vector<const double*> pin((size_t)count);
+  std::vector<double*> pout((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    pin[i] = P->io_dq + off[i];
+    pout[i] = P->io_shift + off[i];
+  }
+  hipStream_t s = P->stream;
+  PCHK(P, hipMemcpyAsync(P->io_dq, dq, sizeof(double) * atoms, hipMemcpyHostToDevice, s));
+  st = run(P, count, ids, pin.data(), pout.data(), P->io_energy, s);
+  if (st) return st;
+  PCHK(P, hipMemcpyAsync(shift, P->io_shift, sizeof(double) * atoms, hipMemcpyDeviceToHost, s));
+  PCHK(P, hipMemcpyAsync(energies, P->io_energy, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+  PCHK(P, hipStreamSynchronize(s));
+  return EW_OK;
+}
+
+extern "C" int ew_pool_sync(ew_pool* P) {
+  if (!P) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_sync: pool is NULL");
+  PCHK(P, hipStreamSynchronize(P->stream));
+  return EW_OK;
+}

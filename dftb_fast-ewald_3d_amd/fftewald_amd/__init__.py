@@ -36,7 +36,9 @@ PRECISION_MIXED = 1
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
             "ew_shift", "ew_shift_device", "ew_gradients", "ew_phase_times", "ew_potential_force",
-            "ew_parameters_for", "ew_grid_memory")
+            "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
+            "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
+            "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync")
 
 #: phases reported by ew_phase_times, in order
 PHASES = ("gather_charges", "spread", "fft_x_r2c", "fft_y", "fft_z_influence", "fft_y_inverse",
@@ -127,6 +129,20 @@ def lib() -> C.CDLL:
                                         C.POINTER(Parameters)]
         L.ew_grid_memory.argtypes = [C.c_int, _optional(_D1), _D2, C.POINTER(Options),
                                      C.POINTER(C.c_double)]
+        L.ew_pool_create.argtypes = [C.POINTER(Options), C.c_longlong, C.POINTER(H)]
+        L.ew_pool_destroy.argtypes = [H]
+        L.ew_pool_last_error.restype = C.c_char_p
+        L.ew_pool_last_error.argtypes = [H]
+        L.ew_pool_add.argtypes = [H, C.c_int, _D2, _D2, _optional(_I1), _optional(_D1), C.c_int,
+                                  _optional(_D1), _optional(_I1), C.POINTER(C.c_int)]
+        L.ew_pool_remove.argtypes = [H, C.c_int]
+        L.ew_pool_parameters.argtypes = [H, C.c_int, C.POINTER(Parameters)]
+        L.ew_pool_usage.argtypes = [H, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_int)]
+        L.ew_pool_shift.argtypes = [H, C.c_int, _I1, _D1, _D1, _D1]
+        L.ew_pool_shift_device.argtypes = [H, C.c_int, _I1, C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]
+        L.ew_pool_sync.argtypes = [H]
         _lib = L
     return _lib
 
@@ -240,6 +256,102 @@ class EwaldEngine:
         calls = C.c_longlong()
         _check(lib().ew_phase_times(self._h, ms, C.byref(calls), 1 if reset else 0), self._h)
         return dict(zip(PHASES, ms.tolist())), calls.value
+
+
+class EwaldPool:
+    """Many resident structures evaluated together (grouped launches).
+
+    ``add`` once per structure (geometry, pair list, weights and plans stay on the device),
+    then ``shift`` (host arrays) or ``shift_device`` (device pointers) for any subset of them
+    once per SCC iteration.  ``capacity_bytes`` caps the device memory of the structures; a
+    structure that does not fit is refused with EW_ERR_MEMORY.
+    """
+
+    def __init__(self, capacity_bytes: int = 0, **opts):
+        self._p = C.c_void_p()
+        self.options = make_options(**opts)
+        _check(lib().ew_pool_create(C.byref(self.options), int(capacity_bytes),
+                                    C.byref(self._p)))
+        self.sizes = {}
+
+    def _check(self, status):
+        if status != EW_OK:
+            raise EwaldError(status, lib().ew_pool_last_error(self._p).decode())
+
+    def close(self):
+        if self._p:
+            lib().ew_pool_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, coords, lattice, species=None, hubbard_u=None, charges=None, grid=None) -> int:
+        coords = _f64(coords, 2)
+        lattice = _f64(lattice, 2)
+        n = coords.shape[0]
+        if coords.shape != (n, 3) or lattice.shape != (3, 3):
+            raise ValueError("coords must be (n, 3) and lattice (3, 3)")
+        sp = None if species is None else np.ascontiguousarray(species, dtype=np.int32)
+        u = None if hubbard_u is None else _f64(hubbard_u, 1)
+        q = None if charges is None else _f64(charges, 1)
+        g = None if grid is None else np.ascontiguousarray(grid, dtype=np.int32)
+        sid = C.c_int()
+        self._check(lib().ew_pool_add(self._p, n, coords, lattice, sp, u,
+                                      0 if u is None else len(u), q, g, C.byref(sid)))
+        self.sizes[sid.value] = n
+        return sid.value
+
+    def remove(self, sid: int):
+        self._check(lib().ew_pool_remove(self._p, int(sid)))
+        self.sizes.pop(sid, None)
+
+    def parameters(self, sid: int) -> dict:
+        p = Parameters()
+        self._check(lib().ew_pool_parameters(self._p, int(sid), C.byref(p)))
+        return p.as_dict()
+
+    def usage(self):
+        """(device bytes held, capacity, structure count)."""
+        u, c, k = C.c_longlong(), C.c_longlong(), C.c_int()
+        self._check(lib().ew_pool_usage(self._p, C.byref(u), C.byref(c), C.byref(k)))
+        return u.value, c.value, k.value
+
+    def shift(self, ids, dqs):
+        """One grouped evaluation; dqs: one charge vector per id.  Returns (shifts, energies)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        dq = np.concatenate([_f64(d, 1) for d in dqs]) if len(dqs) else np.zeros(0)
+        out = np.empty_like(dq)
+        en = np.empty(len(ids))
+        self._check(lib().ew_pool_shift(self._p, len(ids), ids, dq, out, en))
+        shifts, o = [], 0
+        for i in ids:
+            n = self.sizes[int(i)]
+            shifts.append(out[o:o + n])
+            o += n
+        return shifts, en
+
+    def shift_device(self, ids, dq_ptrs, shift_ptrs, energies_ptr: int, stream: int = 0):
+        """Enqueues one grouped evaluation on device buffers (no synchronisation)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        k = len(ids)
+        dp = (C.c_void_p * k)(*[C.c_void_p(int(x)) for x in dq_ptrs])
+        sp = (C.c_void_p * k)(*[C.c_void_p(int(x)) for x in shift_ptrs])
+        self._check(lib().ew_pool_shift_device(self._p, k, ids, dp, sp,
+                                               C.c_void_p(int(energies_ptr)),
+                                               C.c_void_p(stream or None)))
+
+    def sync(self):
+        self._check(lib().ew_pool_sync(self._p))
 
 
 def ewald_potential_force(positions, charges, lattice, force=False, **opts):

@@ -156,6 +156,46 @@ int ew_parameters_for(int n, const double *charges, const double *latvecs,
 int ew_grid_memory(int n, const double *charges, const double *latvecs, const ew_options *opts,
                    double *gigabytes);
 
+/* ---- structure pools: the batched and serving forms ----
+ *
+ * The reference evaluates one structure per call (fe/fft/ewald_fft_3d.f90:36-134, called once
+ * per SCC iteration from TCoulomb%updateShifts, coulomb.F90:786-880).  A pool keeps many
+ * structures resident (geometry, pair lists, B-spline weights, FFT plans) and evaluates any
+ * subset of them in one grouped launch per kernel; DFTB+ drivers that run many independent
+ * SCC cycles (replicas, trajectories, screening) call it once per SCC iteration of the whole
+ * set.  Each structure's result is the single-structure engine's, alone or in any batch. */
+typedef struct ew_pool ew_pool;
+
+/* capacity_bytes: device memory the pool may hold (0: no limit); ew_pool_add refuses a
+ * structure that does not fit (EW_ERR_MEMORY), which is the admission rule.  Every structure
+ * of a pool shares opts (precision, tol, r_cut, order); grids may differ. */
+int ew_pool_create(const ew_options *opts, long long capacity_bytes, ew_pool **out);
+int ew_pool_destroy(ew_pool *pool);
+const char *ew_pool_last_error(const ew_pool *pool);
+
+/* Adds one structure (arguments as ew_set_geometry; grid[3] overrides opts->grid, NULL keeps
+ * it); *id receives its id, reused after ew_pool_remove. */
+int ew_pool_add(ew_pool *pool, int n, const double *coords, const double *latvecs,
+                const int *species, const double *hubbard_u, int n_species,
+                const double *charges, const int *grid, int *id);
+int ew_pool_remove(ew_pool *pool, int id);
+int ew_pool_parameters(const ew_pool *pool, int id, ew_parameters *out);
+/* Device bytes held by the pool's structures, its capacity and its structure count. */
+int ew_pool_usage(const ew_pool *pool, long long *used, long long *capacity, int *count);
+
+/* One SCC evaluation of `count` distinct structures: dq and shift are the structures' charge
+ * and shift vectors concatenated in `ids` order (caller atom order within each), energies[i]
+ * the energy of structure ids[i]. */
+int ew_pool_shift(ew_pool *pool, int count, const int *ids, const double *dq, double *shift,
+                  double *energies);
+/* The same with device buffers (host arrays of device pointers, one per structure), enqueued
+ * on `stream` (a hipStream_t; NULL = the pool's stream) without synchronising.  Evaluations of
+ * one structure must not overlap in time; a structure may appear once per call. */
+int ew_pool_shift_device(ew_pool *pool, int count, const int *ids, const double *const *d_dq,
+                         double *const *d_shift, double *d_energies, void *stream);
+/* Waits for the pool's stream. */
+int ew_pool_sync(ew_pool *pool);
+
 #ifdef __cplusplus
 }
 #endif

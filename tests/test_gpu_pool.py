@@ -1,0 +1,163 @@
+"""GPU tests of the structure pool (grouped launches over many resident structures).
+
+A pool evaluates each structure with the single-structure engine's kernel bodies and
+variants, so every structure's shift and energy must equal the single engine's alone and in
+any batch to round-off, and the CPU SPME oracle at identical parameters to 1e-11 (FP64).
+Round-off here: the spreading accumulates each mesh point with LDS atomics whose order varies
+from run to run, so two evaluations of the same charges differ by ~1e-16 relative in FP64
+(tolerance 1e-13) and by up to one FP32 ulp per mesh point in mixed precision (tolerance 1e-6,
+the mixed mode's stated accuracy).
+"""
+import sys
+
+import numpy as np
+import pytest
+from conftest import rel_max
+
+import fftewald_amd as ew
+from fftewald_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def structures():
+    # uneven sizes and triclinic cells as in the batched workload, kept small for the test
+    return W.batch(count=10, nmin=64, nmax=3000)
+
+
+TOL = {"fp64": 1e-13, "mixed": 1e-6}
+
+
+def single(w, precision, dq=None):
+    with ew.EwaldEngine(tol=1e-10, r_cut=w["r_cut"], grid=w["grid"], order=w["order"],
+                        precision=precision) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        return e.shift(w["charges"] if dq is None else dq) + (e.parameters(),)
+
+
+def make_pool(ws, precision, capacity=0):
+    p = ew.EwaldPool(capacity_bytes=capacity, tol=1e-10, r_cut=ws[0]["r_cut"],
+                     order=ws[0]["order"], precision=precision)
+    ids = [p.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"]) for w in ws]
+    return p, ids
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pool_matches_single_engine(structures, precision):
+    ws = structures
+    say(f"pool of {len(ws)} structures, N {[len(w['charges']) for w in ws]}, {precision}")
+    with make_pool(ws, precision)[0] as p:
+        ids = sorted(p.sizes)
+        shifts, energies = p.shift(ids, [w["charges"] for w in ws])
+        modes = set()
+        for i, w in zip(ids, ws):
+            s1, e1, prm = single(w, precision)
+            modes.add(prm["pair_mode"])
+            assert rel_max(shifts[i], s1) < TOL[precision]
+            assert energies[i] == pytest.approx(e1, rel=TOL[precision])
+        # both pair-list forms (linked tiles, image sums for small cells) are in the batch
+        assert modes == {0, 1}
+
+
+def test_pool_fp64_matches_cpu_spme(structures, oracle_mod):
+    ws = structures[:4]
+    say("pool FP64 against the CPU SPME oracle, 4 structures")
+    with make_pool(ws, "fp64")[0] as p:
+        ids = sorted(p.sizes)
+        shifts, energies = p.shift(ids, [w["charges"] for w in ws])
+        for i, w in zip(ids, ws):
+            prm = p.parameters(i)
+            want = oracle_mod.spme_ewald(w["coords"], w["charges"], w["lattice"], prm["grid"],
+                                         prm["order"], prm["alpha"], prm["r_cut"])
+            assert rel_max(shifts[i], want) < 1e-11
+            assert energies[i] == pytest.approx(0.5 * float(np.dot(w["charges"], want)),
+                                                rel=1e-11)
+
+
+def test_alone_and_in_the_mix(structures):
+    ws = structures
+    say("subsets of the pool give the same results as the whole batch")
+    rng = np.random.default_rng(7)
+    dqs = [W.scc_update(w["charges"], rng) for w in ws]
+    with make_pool(ws, "mixed")[0] as p:
+        ids = sorted(p.sizes)
+        full, efull = p.shift(ids, dqs)
+        for sub in ([3], [7, 1], [9, 0, 5, 2]):
+            s, e = p.shift(sub, [dqs[i] for i in sub])
+            for k, i in enumerate(sub):
+                assert rel_max(s[k], full[i]) < TOL["mixed"]
+                assert e[k] == pytest.approx(efull[i], rel=TOL["mixed"])
+
+
+def test_device_buffers_and_graph_replays(structures):
+    import torch
+    ws = structures[:6]
+    say("pool shift_device: graph replays against the host entry point")
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(11)
+    with make_pool(ws, "fp64")[0] as p:
+        ids = sorted(p.sizes)
+        dq_t = [torch.zeros(len(w["charges"]), dtype=torch.float64, device=dev) for w in ws]
+        sh_t = [torch.zeros_like(d) for d in dq_t]
+        en_t = torch.zeros(len(ws), dtype=torch.float64, device=dev)
+        for rep in range(4):
+            dqs = [W.scc_update(w["charges"], rng) for w in ws]
+            for t, d in zip(dq_t, dqs):
+                t.copy_(torch.from_numpy(d))
+            torch.cuda.synchronize()
+            p.shift_device(ids, [t.data_ptr() for t in dq_t], [t.data_ptr() for t in sh_t],
+                           en_t.data_ptr())
+            p.sync()
+            want, ewant = p.shift(ids, dqs)
+            for k in range(len(ws)):
+                assert rel_max(sh_t[k].cpu().numpy(), want[k]) < TOL["fp64"]
+            assert np.allclose(en_t.cpu().numpy(), ewant, rtol=TOL["fp64"], atol=0.0)
+
+
+def test_admission_by_footprint_and_removal(structures):
+    ws = structures[:3]
+    say("pool capacity: admission refused, then accepted after a removal")
+    with make_pool(ws[:1], "fp64")[0] as probe:
+        need = probe.usage()[0]
+    cap = int(need * 1.5)
+    with ew.EwaldPool(capacity_bytes=cap, tol=1e-10, r_cut=ws[0]["r_cut"], order=6) as p:
+        a = p.add(ws[0]["coords"], ws[0]["lattice"], charges=ws[0]["charges"], grid=ws[0]["grid"])
+        used, capacity, count = p.usage()
+        assert used == need and capacity == cap and count == 1
+        with pytest.raises(ew.EwaldError) as exc:
+            p.add(ws[0]["coords"], ws[0]["lattice"], charges=ws[0]["charges"],
+                  grid=ws[0]["grid"])
+        assert exc.value.code == ew.EW_ERR_MEMORY
+        p.remove(a)
+        b = p.add(ws[0]["coords"], ws[0]["lattice"], charges=ws[0]["charges"], grid=ws[0]["grid"])
+        assert b == a and p.usage()[2] == 1
+        s, e = p.shift([b], [ws[0]["charges"]])
+        s1, e1, _ = single(ws[0], "fp64")
+        assert rel_max(s[0], s1) < TOL["fp64"]
+
+
+def test_pool_errors(structures):
+    w = structures[0]
+    say("pool error paths")
+    with make_pool([w], "fp64")[0] as p:
+        with pytest.raises(ew.EwaldError) as exc:
+            p.shift([0, 0], [w["charges"], w["charges"]])
+        assert exc.value.code == ew.EW_ERR_ARGUMENT
+        with pytest.raises(ew.EwaldError) as exc:
+            p.shift([5], [w["charges"]])
+        assert exc.value.code == ew.EW_ERR_ARGUMENT
+        with pytest.raises(ew.EwaldError) as exc:
+            p.shift([0], [w["charges"] + 1.0])
+        assert exc.value.code == ew.EW_ERR_NOT_NEUTRAL
+        with pytest.raises(ew.EwaldError) as exc:
+            p.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"],
+                  species=np.zeros(len(w["charges"]), dtype=np.int32),
+                  hubbard_u=np.array([-1.0]))
+        assert exc.value.code == ew.EW_ERR_ARGUMENT
+        assert p.usage()[2] == 1
