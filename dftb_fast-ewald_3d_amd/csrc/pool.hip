@@ -127,10 +127,10 @@ int blocks_of(const StructDesc& d, int kind) {
 
 int ensure(ew_pool* P, void** buf, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return EW_OK;
+  const size_t want = std::max(bytes, 2 * *cap);
   if (*buf) PCHK(P, hipFree(*buf));
   *buf = nullptr;
   *cap = 0;
-  const size_t want = std::max(bytes, 2 * *cap);
   if (hipMalloc(buf, want) != hipSuccess) {
     (void)hipGetLastError();
     *buf = nullptr;
@@ -242,30 +242,33 @@ int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* 
         double* energies, hipStream_t s) {
   int st = sync_desc(P);
   if (st) return st;
+  const bool eager = P->opt.eager || P->opt.profile;
+  std::vector<const void*> ptrs;
+  const std::vector<int> key(ids, ids + count);
+  if (!eager) {
+    ptrs.reserve(2 * (size_t)count + 1);
+    for (int i = 0; i < count; ++i) {
+      ptrs.push_back(dq[i]);
+      ptrs.push_back(shift[i]);
+    }
+    ptrs.push_back(energies);
+    for (auto& g : P->graphs)
+      if (g.ids == key && g.ptrs == ptrs) {
+        PCHK(P, hipGraphLaunch(g.exec, s));
+        return EW_OK;
+      }
+  }
   Staged S;
   st = stage(P, count, ids, dq, shift, energies, &S);
   if (st) return st;
   const size_t bytes = S.words.size() * sizeof(int);
-  if (P->opt.eager || P->opt.profile) {
+  if (eager) {
     st = ensure(P, &P->staging, &P->staging_cap, bytes);
     if (st) return st;
     // a pageable source: the copy has read the host image when the call returns
     PCHK(P, hipMemcpyAsync(P->staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
     return enqueue(P, launch_of(P, S, P->staging), s);
   }
-  std::vector<const void*> ptrs;
-  ptrs.reserve(2 * (size_t)count + 1);
-  for (int i = 0; i < count; ++i) {
-    ptrs.push_back(dq[i]);
-    ptrs.push_back(shift[i]);
-  }
-  ptrs.push_back(energies);
-  const std::vector<int> key(ids, ids + count);
-  for (auto& g : P->graphs)
-    if (g.ids == key && g.ptrs == ptrs) {
-      PCHK(P, hipGraphLaunch(g.exec, s));
-      return EW_OK;
-    }
   if (P->graphs.size() >= 8) P->drop_graphs();
   Captured c;
   c.ids = key;
