@@ -21,6 +21,7 @@
 #include "engine.hpp"
 #include "handle.hpp"
 #include "kernels.hpp"
+#include "pool.hpp"
 
 using namespace ewamd;
 
@@ -301,6 +302,34 @@ int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* 
 }
 
 }  // namespace
+
+int ewamd::pool_launch_staged(ew_pool* P, int count, const int* ids, const double* const* dq,
+                              double* const* shift, double* energies, void** staging,
+                              size_t* staging_cap, hipStream_t s) {
+  int st = check_ids(P, count, ids);
+  if (st) return st;
+  st = sync_desc(P);
+  if (st) return st;
+  Staged S;
+  st = stage(P, count, ids, dq, shift, energies, &S);
+  if (st) return st;
+  const size_t bytes = S.words.size() * sizeof(int);
+  st = ensure(P, staging, staging_cap, bytes);
+  if (st) return st;
+  PCHK(P, hipMemcpyAsync(*staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
+  return enqueue(P, launch_of(P, S, *staging), s);
+}
+
+int ewamd::pool_atoms(const ew_pool* P, int id) {
+  if (id < 0 || id >= (int)P->members.size() || !P->members[id]) return -1;
+  return P->members[id]->n;
+}
+
+int ewamd::pool_slots(const ew_pool* P) { return (int)P->members.size(); }
+
+const char* ewamd::pool_error(const ew_pool* P) { return P->err.c_str(); }
+
+int ewamd::pool_device(const ew_pool* P) { return P->opt.device; }
 
 extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, ew_pool** out) {
   if (!out) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_create: out is NULL");

@@ -29,6 +29,7 @@ EW_ERR_NO_GEOMETRY = 6
 EW_ERR_DEVICE = 7
 EW_ERR_MEMORY = 8
 EW_ERR_NOT_FINITE = 9
+EW_ERR_PENDING = 10
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -38,7 +39,9 @@ EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_g
             "ew_shift", "ew_shift_device", "ew_gradients", "ew_phase_times", "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
-            "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync")
+            "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync",
+            "ew_server_create", "ew_server_destroy", "ew_server_last_error", "ew_server_submit",
+            "ew_server_wait", "ew_server_drain", "ew_server_stats")
 
 #: phases reported by ew_phase_times, in order
 PHASES = ("gather_charges", "spread", "fft_x_r2c", "fft_y", "fft_z_influence", "fft_y_inverse",
@@ -58,6 +61,19 @@ class Options(C.Structure):
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
                 ("eager", C.c_int), ("reserved", C.c_int * 2)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("max_batch", C.c_int), ("max_inflight", C.c_int), ("reserved", C.c_int * 6)]
+
+
+class ServerReport(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong),
+                ("failed", C.c_longlong), ("batches", C.c_longlong), ("mean_batch", C.c_double),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Parameters(C.Structure):
@@ -143,6 +159,15 @@ def lib() -> C.CDLL:
         L.ew_pool_shift_device.argtypes = [H, C.c_int, _I1, C.POINTER(C.c_void_p),
                                            C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]
         L.ew_pool_sync.argtypes = [H]
+        L.ew_server_create.argtypes = [H, C.POINTER(ServerOptions), C.POINTER(H)]
+        L.ew_server_destroy.argtypes = [H]
+        L.ew_server_last_error.restype = C.c_char_p
+        L.ew_server_last_error.argtypes = [H]
+        L.ew_server_submit.argtypes = [H, C.c_int, _D1, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_longlong)]
+        L.ew_server_wait.argtypes = [H, C.c_longlong, C.c_double]
+        L.ew_server_drain.argtypes = [H]
+        L.ew_server_stats.argtypes = [H, C.POINTER(ServerReport), C.c_int]
         _lib = L
     return _lib
 
@@ -352,6 +377,78 @@ class EwaldPool:
 
     def sync(self):
         self._check(lib().ew_pool_sync(self._p))
+
+
+class EwaldServer:
+    """Request loop with continuous batching over an EwaldPool.
+
+    ``submit(id, dq)`` queues one SCC evaluation of a resident structure and returns a ticket;
+    ``result(ticket)`` waits for it and returns (shift, energy).  While the server runs, only the
+    server uses the pool.
+    """
+
+    def __init__(self, pool: "EwaldPool", max_batch: int = 64, max_inflight: int = 2):
+        self._s = C.c_void_p()
+        self.pool = pool
+        o = ServerOptions()
+        o.max_batch = max_batch
+        o.max_inflight = max_inflight
+        _check(lib().ew_server_create(pool._p, C.byref(o), C.byref(self._s)))
+        self._out = {}  # ticket -> (shift array, energy array), alive until completion
+
+    def _check(self, status):
+        if status != EW_OK:
+            raise EwaldError(status, lib().ew_server_last_error(self._s).decode())
+
+    def close(self):
+        if self._s:
+            lib().ew_server_destroy(self._s)
+            self._s = C.c_void_p()
+            self._out.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, sid: int, dq) -> int:
+        dq = _f64(dq, 1)
+        n = self.pool.sizes.get(int(sid))
+        if n is not None and len(dq) != n:
+            raise ValueError(f"structure {sid} has {n} atoms, got {len(dq)} charges")
+        # an unknown id is refused by the library before anything is written
+        shift = np.empty(n if n is not None else 1)
+        energy = np.empty(1)
+        t = C.c_longlong()
+        self._check(lib().ew_server_submit(self._s, int(sid), dq,
+                                           C.c_void_p(shift.ctypes.data),
+                                           C.c_void_p(energy.ctypes.data), C.byref(t)))
+        self._out[t.value] = (shift, energy)
+        return t.value
+
+    def result(self, ticket: int, timeout: float = -1.0):
+        """Waits for a ticket; returns (shift, energy), or None at the timeout."""
+        st = lib().ew_server_wait(self._s, int(ticket), float(timeout))
+        if st == EW_ERR_PENDING:
+            return None
+        shift, energy = self._out.pop(ticket)
+        self._check(st)
+        return shift, float(energy[0])
+
+    def drain(self):
+        self._check(lib().ew_server_drain(self._s))
+
+    def stats(self, reset: bool = False) -> dict:
+        r = ServerReport()
+        self._check(lib().ew_server_stats(self._s, C.byref(r), 1 if reset else 0))
+        return r.as_dict()
 
 
 def ewald_potential_force(positions, charges, lattice, force=False, **opts):

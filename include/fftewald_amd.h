@@ -41,6 +41,7 @@ extern "C" {
 #define EW_ERR_DEVICE 7       /* HIP runtime failure                                              */
 #define EW_ERR_MEMORY 8       /* device memory allocation failed / over the admission budget     */
 #define EW_ERR_NOT_FINITE 9   /* NaN or Inf in coordinates or charges                             */
+#define EW_ERR_PENDING 10     /* ew_server_wait: the request is not finished within the timeout    */
 
 /* precision modes */
 #define EW_PRECISION_FP64 0   /* every step in FP64                                               */
@@ -195,6 +196,44 @@ int ew_pool_shift_device(ew_pool *pool, int count, const int *ids, const double 
                          double *const *d_shift, double *d_energies, void *stream);
 /* Waits for the pool's stream. */
 int ew_pool_sync(ew_pool *pool);
+
+/* ---- serving: a request loop with continuous batching over a pool ----
+ *
+ * Requests are SCC charge updates for resident structures, submitted from any thread.  A worker
+ * launches a batch whenever a slot is free (at most max_inflight batches on the GPU at once, each
+ * on its own stream): every pending request whose structure has nothing in flight, in arrival
+ * order, up to max_batch structures in one grouped evaluation.  While a server runs on a pool,
+ * only the server may evaluate or change the pool.  No reference counterpart: each request is
+ * one ewald_potential_force call (fe/fft/ewald_fft_3d.f90:36-134) on a resident structure. */
+typedef struct ew_server ew_server;
+
+typedef struct {
+  int max_batch;    /* structures per grouped launch; 0: 64                                  */
+  int max_inflight; /* batches on the GPU at once; 0: 2                                      */
+  int reserved[6];
+} ew_server_options;
+
+typedef struct {
+  long long submitted, completed, failed, batches;
+  double mean_batch;          /* requests per batch                                      */
+  double p50_ms, p99_ms, max_ms; /* latency from submission to the result in the caller's buffers */
+} ew_server_report;
+
+int ew_server_create(ew_pool *pool, const ew_server_options *opts, ew_server **out);
+/* Finishes the requests already submitted, then stops. */
+int ew_server_destroy(ew_server *server);
+const char *ew_server_last_error(const ew_server *server);
+/* Queues one evaluation of structure id: dq is copied at submission; shift[n] (and *energy
+ * when not NULL) are written when the request completes and must stay valid until then. */
+int ew_server_submit(ew_server *server, int id, const double *dq, double *shift, double *energy,
+                     long long *ticket);
+/* Waits for a ticket (timeout_s < 0: no limit); EW_OK, the request's error, or
+ * EW_ERR_PENDING at the timeout (the ticket stays valid). */
+int ew_server_wait(ew_server *server, long long ticket, double timeout_s);
+/* Waits until nothing is pending or in flight. */
+int ew_server_drain(ew_server *server);
+/* Counters and latency percentiles since the last reset. */
+int ew_server_stats(ew_server *server, ew_server_report *out, int reset);
 
 #ifdef __cplusplus
 }
