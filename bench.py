@@ -247,6 +247,123 @@ def run_batch(args, torch, ew, W):
     print(json.dumps(line))
 
 
+def poisson_load(srv, ids, dqs, qps, seconds, rng):
+    """Offers SCC requests as a Poisson stream at `qps` for `seconds` (a structure gets a new
+    request only after its previous one completed).  Returns (offered, completed, elapsed)."""
+    import queue
+    import threading
+    busy = set()
+    lock = threading.Lock()
+    q = queue.Queue()
+    done = [0]
+
+    def collector():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            sid, t = item
+            srv.result(t)
+            with lock:
+                busy.discard(sid)
+                done[0] += 1
+
+    th = threading.Thread(target=collector)
+    th.start()
+    offered = 0
+    t0 = time.perf_counter()
+    t_next = t0
+    end = t0 + seconds
+    while True:
+        t_next += rng.exponential(1.0 / qps)
+        if t_next >= end:
+            break
+        while True:
+            gap = t_next - time.perf_counter()
+            if gap <= 0:
+                break
+            if gap > 3e-4:
+                time.sleep(gap - 2e-4)
+        with lock:
+            sid = int(ids[rng.integers(len(ids))])
+            while sid in busy:
+                sid = int(ids[rng.integers(len(ids))])
+            busy.add(sid)
+        q.put((sid, srv.submit(sid, dqs[sid])))
+        offered += 1
+    q.put(None)
+    th.join()
+    srv.drain()
+    return offered, done[0], time.perf_counter() - t0
+
+
+def run_serving(args, torch, ew, W):
+    """Serving workload: 1,000 resident structures (N log-uniform in [100, 20000], built like
+    the 98,304-atom cell); SCC charge updates arrive as a Poisson stream at swept rates and are
+    served by the continuous-batching loop; throughput and p50/p99 latency, mixed and FP64."""
+    t = time.perf_counter()
+    workers = min(16, os.cpu_count() or 1)
+    ws = W.serving(workers=workers)
+    print(f"[bench] serving workload generated in {time.perf_counter() - t:.1f} s "
+          f"({workers} processes)", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(2026)
+    natoms = sum(len(w["charges"]) for w in ws)
+    sweep = [float(x) for x in args.qps.split(",")]
+    res = {}
+    for prec in ("mixed", "fp64"):
+        with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=ws[0]["order"],
+                          precision=prec) as pool:
+            t = time.perf_counter()
+            ids = [pool.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"])
+                   for w in ws]
+            setup = time.perf_counter() - t
+            used = pool.usage()[0]
+            dqs = {i: W.scc_update(w["charges"], rng) for i, w in zip(ids, ws)}
+            print(f"[bench] serving {prec}: {len(ids)} structures resident, {used / 2**30:.1f} "
+                  f"GiB, set-up {setup:.1f} s", file=sys.stderr, flush=True)
+            rows = []
+            with ew.EwaldServer(pool, max_batch=args.max_batch, max_inflight=2) as srv:
+                poisson_load(srv, ids, dqs, sweep[0], 0.5, rng)  # warm-up
+                srv.stats(reset=True)
+                for qps in sweep:
+                    offered, completed, el = poisson_load(srv, ids, dqs, qps, args.seconds, rng)
+                    st = srv.stats(reset=True)
+                    row = {"offered_qps": qps, "achieved_qps": round(completed / el, 1),
+                           "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4),
+                           "max_ms": round(st["max_ms"], 4),
+                           "mean_batch": round(st["mean_batch"], 2), "requests": completed}
+                    rows.append(row)
+                    print(f"[bench] serving {prec} {qps:.0f} qps: achieved {row['achieved_qps']}"
+                          f", p50 {row['p50_ms']} ms, p99 {row['p99_ms']} ms, batch "
+                          f"{row['mean_batch']}", file=sys.stderr, flush=True)
+            ok = [r for r in rows if r["p99_ms"] <= 2.0 and
+                  r["achieved_qps"] >= 0.97 * r["offered_qps"]]
+            best = max(ok, key=lambda r: r["achieved_qps"]) if ok else None
+            res[prec] = {"sweep": rows, "sustained": best, "setup_s": round(setup, 1),
+                         "resident_bytes": used}
+    head = res[args.precision]
+    best = head["sustained"]
+    line = {
+        "metric": "serving: sustained SCC requests/s at p99 <= 2 ms (1,000 resident structures)",
+        "value": best["achieved_qps"] if best else 0.0, "unit": "requests/s", "n_gpus": 1,
+        "steps": None, "warmup": None, "ms_per_step": best["p50_ms"] if best else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 mesh/FFT/pair coefficients + fp64 accumulation" if args.precision ==
+        "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {"workload": "1,000 resident triclinic structures (gamma 75 deg, density "
+                               "0.1036 A^-3, Poisson-disc 1 A), N log-uniform [100, 20000], "
+                               "grids ~0.777 A, order 6, r_cut 10 A, tol 1e-10; Poisson "
+                               "arrivals of dq + N(0, 0.01) updates; continuous batching, "
+                               f"max_batch {args.max_batch}, 2 batches in flight",
+                   "structures": len(ws), "atoms": natoms, "precision": args.precision,
+                   "seconds_per_rate": args.seconds},
+        "p50_ms": best["p50_ms"] if best else None, "p99_ms": best["p99_ms"] if best else None,
+        "serving": res, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,8 +375,13 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run plain 512 MiB copies first (for HBM counter scaling)")
-    ap.add_argument("--workload", default="triclinic", choices=["triclinic", "batch"],
-                    help="triclinic: the 98,304-atom cell (headline); batch: 256 structures")
+    ap.add_argument("--workload", default="triclinic", choices=["triclinic", "batch", "serving"],
+                    help="triclinic: the 98,304-atom cell (headline); batch: 256 structures; "
+                         "serving: 1,000 resident structures under Poisson load")
+    ap.add_argument("--qps", default="5000,10000,20000,40000,60000,80000",
+                    help="serving: offered request rates")
+    ap.add_argument("--seconds", type=float, default=2.0, help="serving: seconds per rate")
+    ap.add_argument("--max-batch", type=int, default=64, help="serving: structures per launch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -275,6 +397,9 @@ def main():
         copy_calibration(torch)
     if args.workload == "batch":
         run_batch(args, torch, ew, W)
+        return
+    if args.workload == "serving":
+        run_serving(args, torch, ew, W)
         return
     t = time.perf_counter()
     w = W.triclinic()

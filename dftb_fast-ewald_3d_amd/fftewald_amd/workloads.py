@@ -196,23 +196,39 @@ def batch(seed: int = SEED0 + 4, count: int = 256, nmin: int = 64, nmax: int = 8
     return out
 
 
+def _serving_structure(args):
+    n, seed, density, gamma, spacing, order, dmin = args
+    rng = np.random.default_rng(seed)
+    side = cell_for_density(n, density, (90.0, 90.0, gamma))
+    lat = lattice_from_parameters(side * AA, side * AA, side * AA, 90.0, 90.0, gamma)
+    coords = poisson_disc(n, lat, dmin * AA, rng)
+    q = neutralise(rng.uniform(-1.0, 1.0, n))
+    K = tuple(smooth235(int(math.ceil(side / spacing))) for _ in range(3))
+    return {"name": "serving", "coords": coords, "lattice": lat, "charges": q,
+            "species": None, "hubbard_u": None, "grid": K, "order": order,
+            "r_cut": 10.0 * AA}
+
+
 def serving(seed: int = SEED0 + 5, count: int = 1000, nmin: int = 100, nmax: int = 20000,
             density: float = 0.1036, gamma: float = 75.0, spacing: float = 0.777,
-            order: int = 6, dmin: float = 1.0):
-    """Resident structures built like ``triclinic`` with N log-uniform in [nmin, nmax]."""
+            order: int = 6, dmin: float = 1.0, workers: int = 0):
+    """Resident structures built like ``triclinic`` with N log-uniform in [nmin, nmax].
+
+    Sizes come from one generator seeded with ``seed``; structure k is drawn from its own
+    generator (the k-th child of ``seed``), so the set does not depend on ``workers``
+    (processes used to draw the positions; 0: serial)."""
     rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
-        side = cell_for_density(n, density, (90.0, 90.0, gamma))
-        lat = lattice_from_parameters(side * AA, side * AA, side * AA, 90.0, 90.0, gamma)
-        coords = poisson_disc(n, lat, dmin * AA, rng)
-        q = neutralise(rng.uniform(-1.0, 1.0, n))
-        K = tuple(smooth235(int(math.ceil(side / spacing))) for _ in range(3))
-        out.append({"name": "serving", "coords": coords, "lattice": lat, "charges": q,
-                    "species": None, "hubbard_u": None, "grid": K, "order": order,
-                    "r_cut": 10.0 * AA})
-    return out
+    ns = [int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+          for _ in range(count)]
+    children = np.random.SeedSequence(seed).spawn(count)
+    jobs = [(n, ch, density, gamma, spacing, order, dmin) for n, ch in zip(ns, children)]
+    if workers and workers > 1:
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        with cf.ProcessPoolExecutor(max_workers=workers,
+                                    mp_context=mp.get_context("spawn")) as ex:
+            return list(ex.map(_serving_structure, jobs, chunksize=8))
+    return [_serving_structure(j) for j in jobs]
 
 
 def scc_update(q: np.ndarray, rng: np.random.Generator, sigma: float = 0.01) -> np.ndarray:
