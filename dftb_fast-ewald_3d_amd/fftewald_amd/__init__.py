@@ -163,7 +163,7 @@ def lib() -> C.CDLL:
         L.ew_server_destroy.argtypes = [H]
         L.ew_server_last_error.restype = C.c_char_p
         L.ew_server_last_error.argtypes = [H]
-        L.ew_server_submit.argtypes = [H, C.c_int, _D1, C.c_void_p, C.c_void_p,
+        L.ew_server_submit.argtypes = [H, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.POINTER(C.c_longlong)]
         L.ew_server_wait.argtypes = [H, C.c_longlong, C.c_double]
         L.ew_server_drain.argtypes = [H]
@@ -427,11 +427,22 @@ class EwaldServer:
         shift = np.empty(n if n is not None else 1)
         energy = np.empty(1)
         t = C.c_longlong()
-        self._check(lib().ew_server_submit(self._s, int(sid), dq,
-                                           C.c_void_p(shift.ctypes.data),
-                                           C.c_void_p(energy.ctypes.data), C.byref(t)))
+        self._check(lib().ew_server_submit(self._s, int(sid), dq.ctypes.data, shift.ctypes.data,
+                                           energy.ctypes.data, C.byref(t)))
         self._out[t.value] = (shift, energy)
         return t.value
+
+    def submit_ptr(self, sid: int, dq_ptr: int, shift_ptr: int, energy_ptr: int) -> int:
+        """submit() on raw host addresses (the caller keeps the buffers alive and sized: dq is
+        copied at submission, shift and energy are written at completion); for load drivers."""
+        t = C.c_longlong()
+        self._check(lib().ew_server_submit(self._s, sid, dq_ptr, shift_ptr, energy_ptr,
+                                           C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> int:
+        """Waits for a ticket from submit_ptr; returns its status (EW_ERR_PENDING at timeout)."""
+        return lib().ew_server_wait(self._s, ticket, timeout)
 
     def result(self, ticket: int, timeout: float = -1.0):
         """Waits for a ticket; returns (shift, energy), or None at the timeout."""
