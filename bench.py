@@ -452,6 +452,11 @@ def main():
             p50, p99 = latency_run(e, torch, dqs, out, en, args.latency_steps)
             s, energy = e.shift(qs[0])
             shifts[prec] = (s, energy)
+            # the host-array entry point: charges over PCIe in, shifts back, synchronised
+            t = time.perf_counter()
+            for i in range(10):
+                e.shift(qs[i % 4])
+            host_ms = (time.perf_counter() - t) * 100.0
         with ew.EwaldEngine(precision=prec, profile=True, **opts) as e:
             e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
             sp = torch.cuda.current_stream().cuda_stream
@@ -470,7 +475,8 @@ def main():
             ph[k] = {"ms": round(avg_ms, 5), "bytes": pb[k],
                      "GBs": round(pb[k] / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
-                     "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph}
+                     "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph,
+                     "host_api_ms": host_ms}
         print(f"[bench] {prec}: {ms:.4f} ms/eval, {1e3 / ms:.1f} evals/s, geometry {geom_s:.2f} s,"
               f" p50 {p50:.3f} ms p99 {p99:.3f} ms", file=sys.stderr, flush=True)
 
@@ -521,6 +527,7 @@ def main():
         "mixed_over_fp64": round(res["mixed"]["evals_per_s"] / res["fp64"]["evals_per_s"], 3),
         "mixed_vs_fp64_rel_err": {"shift_max": err_shift, "energy": err_energy},
         "geometry_setup_s": {k: round(v["geometry_s"], 3) for k, v in res.items()},
+        "host_arrays_ms_per_eval": {k: round(v["host_api_ms"], 4) for k, v in res.items()},
         "phases": {k: v["phases"] for k, v in res.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
