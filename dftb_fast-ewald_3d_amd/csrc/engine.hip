@@ -234,6 +234,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->prm = prm;
   h->ns = gamma ? n_species : 0;
   h->self_coef = -2.0 * prm.alpha / kSqrtPi;
+  // neutralising background of a charged cell (src/dftbp/dftb/coulomb.F90:1292-1296)
+  h->background = -kPi / (c.volume * prm.alpha * prm.alpha);
   const int p = prm.order;
   const int* K = prm.K;
 
@@ -356,6 +358,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   CHK(h->alloc(&h->epart, std::max<size_t>((size_t)(n + 255) / 256,
                                             (size_t)h->nt[0] * h->nt[1] * h->nt[2])));
   CHK(h->alloc(&h->energy, 1));
+  h->qsum = nullptr;
+  if (h->opt.charged) CHK(h->alloc(&h->qsum, 1));
   CHK(h->alloc(&h->dq_io, n));
   CHK(h->alloc(&h->shift_io, n));
   CHK(h->alloc(&h->err_flag, 1));
@@ -558,6 +562,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   const bool prof = h->opt.profile != 0;
   if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
   HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, s));
+  if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
   SpmeLaunch L{};
   for (int a = 0; a < 3; ++a) L.K[a] = K[a];
@@ -607,6 +612,8 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   }
   ga.order = h->prm.order;
   ga.self_coef = h->self_coef;
+  ga.bg = h->background;
+  ga.qsum = h->qsum;
   const int p = h->prm.order;
   const size_t region = gather_tile_smem(h->T, p, h->fp32 ? sizeof(float) : sizeof(double));
   if (region <= kGatherRegionMaxBytes) {
@@ -704,8 +711,8 @@ extern "C" int ew_shift(ew_handle* h, const double* dq, double* shift, double* e
     net += dq[i];
     absq += std::fabs(dq[i]);
   }
-  if (std::fabs(net) > kNeutralityTol * absq)
-    return h->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral");
+  if (!h->opt.charged && std::fabs(net) > kNeutralityTol * absq)
+    return h->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral (or set opts.charged)");
   HIPCHK(h, hipSetDevice(h->opt.device));
   hipStream_t s = h->stream;
   HIPCHK(h, hipMemcpyAsync(h->dq_io, dq, sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -862,6 +869,8 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->region = h->region;
   d->ga.order = p;
   d->ga.self_coef = h->self_coef;
+  d->ga.bg = h->background;
+  d->ga.qsum = h->qsum;
   const int Kh = K[0] / 2 + 1;
   d->inf.Kh = Kh;
   d->inf.khd = make_fastdiv(Kh);

@@ -44,6 +44,8 @@ struct ew_handle {
   ewamd::RegionArgs region{};   // privatised spreading (scratch == null: not used)
   ewamd::FftPlan px{}, py{}, pz{};
   double self_coef = 0.0;
+  double background = 0.0;   // -pi / (V alpha^2), applied when opt.charged
+  double* qsum = nullptr;    // total charge of the last evaluation (opt.charged)
 
   // device state of the current geometry
   std::vector<ewamd::DevBuf> bufs;

@@ -923,6 +923,7 @@ __device__ __forceinline__ void k_gather_combine_body(
     }
     const double qk = q[k];
     double phi = phi_real[k] + double(acc) + (onsite ? onsite[k] : 0.0) * qk + g.self_coef * qk;
+    if (g.qsum) phi += g.bg * g.qsum[0];
     shift[perm[k]] = phi;
     e = 0.5 * phi * qk;
   }
@@ -1044,7 +1045,8 @@ __device__ __forceinline__ void k_gather_tile_body(
     if (live && sub == 0) {
       const double qa = q[a];
       const double phi =
-          phi_real[a] + part + (onsite ? onsite[a] : 0.0) * qa + g.self_coef * qa;
+          phi_real[a] + part + (onsite ? onsite[a] : 0.0) * qa + g.self_coef * qa +
+          (g.qsum ? g.bg * g.qsum[0] : 0.0);
       shift[perm[a]] = phi;
       e += 0.5 * phi * qa;
     }
@@ -1376,6 +1378,26 @@ __global__ void k_force_real(int n, const double* __restrict__ frac, const int* 
   for (int d = 0; d < 3; ++d) force[3 * o + d] += qi * f[d];
 }
 
+// Total charge in a fixed order: one workgroup, a strided partial per thread, a fixed tree.
+__device__ __forceinline__ void k_charge_sum_body(int n, const double* __restrict__ q,
+                                                  double* __restrict__ qsum, int) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += q[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) qsum[0] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_charge_sum(int n, const double* __restrict__ q,
+                                                    double* __restrict__ qsum) {
+  k_charge_sum_body(n, q, qsum, blockIdx.x);
+}
+
 // ----------------------------------------------------------------------------------------
 // grouped kernels: one launch covers the blocks of many structures
 // ----------------------------------------------------------------------------------------
@@ -1514,6 +1536,14 @@ __global__ __launch_bounds__(256) void kg_gather_combine(const StructDesc* __res
                            d.ga, o.shift, d.epart, b);
 }
 
+__global__ __launch_bounds__(256) void kg_charge_sum(const StructDesc* __restrict__ D,
+                                                     const IoSlot* __restrict__ io,
+                                                     GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  k_charge_sum_body(d.n, d.qs, const_cast<double*>(d.ga.qsum), b);
+}
+
 __global__ __launch_bounds__(256) void kg_energy_final(const StructDesc* __restrict__ D,
                                                        const IoSlot* __restrict__ io,
                                                        GroupMap m) {
@@ -1532,6 +1562,9 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
   if (run(kGkCharges))
     hipLaunchKernelGGL(kg_gather_charges, grid(kGkCharges), dim3(256), 0, st, D, io,
                        G.map[kGkCharges]);
+  if (run(kGkChargeSum))
+    hipLaunchKernelGGL(kg_charge_sum, grid(kGkChargeSum), dim3(256), 0, st, D, io,
+                       G.map[kGkChargeSum]);
 #define EW_ORDER_SWITCH(MACRO) \
   switch (G.order) {           \
     case 4: MACRO(4); break;   \
@@ -1692,6 +1725,11 @@ template hipError_t launch_spme<float>(const SpmeLaunch&, hipStream_t, hipEvent_
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
                                  hipStream_t st) {
   hipLaunchKernelGGL(k_gather_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, perm, dq, qs);
+  return hipGetLastError();
+}
+
+hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t st) {
+  hipLaunchKernelGGL(k_charge_sum, dim3(1), dim3(256), 0, st, n, qs, qsum);
   return hipGetLastError();
 }
 

@@ -40,6 +40,8 @@ struct GatherArgs {
   int K[3];
   int order;
   double self_coef;  // -2 alpha / sqrt(pi)
+  double bg;            // neutralising background -pi / (V alpha^2) (charged cells)
+  const double* qsum;   // total charge (charged cells), null: neutral, no background term
   int T[3];          // tile extents (tile gather)
   int nt[3];         // tiles per axis (tile gather)
 };
@@ -171,6 +173,7 @@ enum GroupKind {
   kGkGatherTile,
   kGkGatherAtom,
   kGkEnergy,
+  kGkChargeSum,
   kGroupKinds
 };
 
@@ -192,6 +195,8 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
 
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
                                  hipStream_t st);
+// Total charge of qs[0..n) in a fixed order (one workgroup) into *qsum.
+hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t st);
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
                             const void* coef, bool fp32, const double* q, double* out,
                             hipStream_t st);

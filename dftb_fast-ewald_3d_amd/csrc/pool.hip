@@ -122,6 +122,7 @@ int blocks_of(const StructDesc& d, int kind) {
     case kGkGatherTile: return d.gather_tile ? d.ntiles : 0;
     case kGkGatherAtom: return d.gather_tile ? 0 : (d.n + 255) / 256;
     case kGkEnergy: return 1;
+    case kGkChargeSum: return d.ga.qsum ? 1 : 0;
     default: return 0;
   }
 }
@@ -331,6 +332,8 @@ const char* ewamd::pool_error(const ew_pool* P) { return P->err.c_str(); }
 
 int ewamd::pool_device(const ew_pool* P) { return P->opt.device; }
 
+bool ewamd::pool_charged(const ew_pool* P) { return P->opt.charged != 0; }
+
 extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, ew_pool** out) {
   if (!out) return set_global_error(EW_ERR_ARGUMENT, "ew_pool_create: out is NULL");
   *out = nullptr;
@@ -500,7 +503,7 @@ extern "C" int ew_pool_shift(ew_pool* P, int count, const int* ids, const double
       net += v;
       absq += std::fabs(v);
     }
-    if (std::fabs(net) > kNeutralityTol * absq)
+    if (!P->opt.charged && std::fabs(net) > kNeutralityTol * absq)
       return P->fail(EW_ERR_NOT_NEUTRAL, "structure " + std::to_string(ids[i]) +
                                              ": the cell must be charge neutral");
   }

@@ -20,5 +20,7 @@ int pool_atoms(const ew_pool* P, int id);
 int pool_slots(const ew_pool* P);
 const char* pool_error(const ew_pool* P);
 int pool_device(const ew_pool* P);
+// Whether the pool accepts non-neutral charges (options.charged).
+bool pool_charged(const ew_pool* P);
 
 }  // namespace ewamd

@@ -325,7 +325,7 @@ extern "C" int ew_server_submit(ew_server* S, int id, const double* dq, double* 
     net += dq[i];
     absq += std::fabs(dq[i]);
   }
-  if (std::fabs(net) > kNeutralityTol * absq)
+  if (!pool_charged(S->pool) && std::fabs(net) > kNeutralityTol * absq)
     return S->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral");
   Request r;
   r.id = id;

@@ -60,7 +60,7 @@ class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("alpha", C.c_double), ("r_cut", C.c_double),
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
-                ("eager", C.c_int), ("reserved", C.c_int * 2)]
+                ("eager", C.c_int), ("charged", C.c_int), ("reserved", C.c_int)]
 
 
 class ServerOptions(C.Structure):
@@ -173,7 +173,8 @@ def lib() -> C.CDLL:
 
 
 def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
-                 device=0, alpha_rule="fftewald", profile=False, eager=False) -> Options:
+                 device=0, alpha_rule="fftewald", profile=False, eager=False,
+                 charged=False) -> Options:
     o = Options()
     o.tol = tol or 0.0
     o.alpha = alpha or 0.0
@@ -190,6 +191,7 @@ def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precis
         else int(alpha_rule)
     o.profile = 1 if profile else 0
     o.eager = 1 if eager else 0
+    o.charged = 1 if charged else 0
     return o
 
 

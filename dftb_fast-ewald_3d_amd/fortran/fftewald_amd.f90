@@ -37,7 +37,8 @@ module fftewald_amd
       integer(c_int) :: alpha_rule = 0
       integer(c_int) :: profile = 0
       integer(c_int) :: eager = 0
-      integer(c_int) :: reserved(2) = 0
+      integer(c_int) :: charged = 0
+      integer(c_int) :: reserved = 0
    end type ew_options
 
    !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).

@@ -68,7 +68,11 @@ typedef struct ew_options {
   int profile;     /* 1: record HIP events around every phase of each evaluation (ew_phase_times) */
   int eager;       /* 1: launch the evaluation's kernels one by one; 0 (default): replay a HIP
                       graph captured once per (dq, shift, energy) buffer triple               */
-  int reserved[2];
+  int charged;     /* 1: accept non-neutral charges and add the neutralising background
+                      -pi/(V alpha^2) * sum(q) to every shift, as DFTB+'s stock Ewald does
+                      (src/dftbp/dftb/coulomb.F90:1292-1296); 0 (default): reject them, as the
+                      fast path does (coulomb.F90:255-265, fe/common/ewald_validation.f90:26-48) */
+  int reserved;
 } ew_options;
 
 /* Parameters an evaluation uses; mirrors TEwaldParameters3d

@@ -244,3 +244,62 @@ def test_graph_replays_match_eager():
             s_eager, e_eager = e.shift(q)                                # eager work in between
             assert rel_max(out.cpu().numpy(), s_eager) < 1e-14
             assert float(en.item()) == pytest.approx(e_eager, rel=1e-14)
+
+
+# ---- charged cells: the neutralising background of DFTB+'s stock Ewald ----
+# (src/dftbp/dftb/coulomb.F90:1292-1296).  Pins: the simple-cubic Wigner constant
+# xi = -2.837297479480620 (potential at a point charge from its images and the uniform
+# background, in units of q/L; literature value, independent of the engine's formulas) and the
+# independence of the potential from the splitting parameter alpha.
+
+XI_SC = -2.837297479480620
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_charged_point_charge_simple_cubic(precision, tol):
+    L, q = 20.0, 0.7
+    say(f"charged cell: one charge in a cubic box, {precision}")
+    with ew.EwaldEngine(tol=1e-12, charged=True, precision=precision) as e:
+        e.set_geometry(np.array([[1.3, 2.1, 0.4]]), np.eye(3) * L, charges=np.array([q]))
+        s, en = e.shift(np.array([q]))
+    assert s[0] == pytest.approx(q * XI_SC / L, rel=tol)
+    assert en == pytest.approx(0.5 * q * q * XI_SC / L, rel=tol)
+
+
+def test_charged_cell_alpha_independent_and_rejected_by_default():
+    w = W.batch(seed=W.SEED0 + 21, count=1, nmin=300, nmax=300)[0]
+    rng = np.random.default_rng(4)
+    q = w["charges"] + 0.05 + 0.01 * rng.random(len(w["charges"]))  # net charge ~ +16
+    say("charged cell: alpha independence, 300 atoms, net charge %.2f" % q.sum())
+    out = []
+    for scale in (1.0, 1.4):
+        with ew.EwaldEngine(tol=1e-12, charged=True) as e:
+            e.set_geometry(w["coords"], w["lattice"], charges=q)
+            a = e.parameters()["alpha"]
+        with ew.EwaldEngine(tol=1e-12, charged=True, alpha=a * scale) as e:
+            e.set_geometry(w["coords"], w["lattice"], charges=q)
+            out.append(e.shift(q))
+    assert rel_max(out[1][0], out[0][0]) < 1e-9
+    assert out[1][1] == pytest.approx(out[0][1], rel=1e-9)
+    with ew.EwaldEngine(tol=1e-12) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=q)
+        with pytest.raises(ew.EwaldError) as exc:
+            e.shift(q)
+        assert exc.value.code == ew.EW_ERR_NOT_NEUTRAL
+
+
+def test_charged_cell_matches_cpu_spme_plus_background(oracle_mod):
+    w = W.water(n_side=6)
+    q = w["charges"].copy()
+    q[::7] += 0.3
+    say("charged water box against CPU SPME + background, FP64")
+    with ew.EwaldEngine(tol=1e-10, charged=True, r_cut=w["r_cut"], grid=24, order=6) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=q)
+        s, en = e.shift(q)
+        prm = e.parameters()
+    pot = oracle_mod.spme_ewald(w["coords"], q, w["lattice"], prm["grid"], prm["order"],
+                                prm["alpha"], prm["r_cut"])
+    vol = abs(np.linalg.det(w["lattice"]))
+    want = pot - math.pi / (vol * prm["alpha"] ** 2) * q.sum()
+    assert rel_max(s, want) < 1e-11
+    assert en == pytest.approx(0.5 * float(np.dot(q, want)), rel=1e-11)
