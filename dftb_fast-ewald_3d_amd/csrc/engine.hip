@@ -142,7 +142,15 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
   if (borrowed) {
     h->stream = borrowed;
     h->own_stream = false;
-  } else if (hipStreamCreate(&h->stream) != hipSuccess) {
+  } else if (hipStreamCreate(&h->stream) != hipSuccess ||
+             hipStreamCreate(&h->aux) != hipSuccess ||
+             hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+    if (h->aux) (void)hipStreamDestroy(h->aux);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return set_global_error(EW_ERR_DEVICE, "hipStreamCreate failed");
   }
@@ -170,10 +178,12 @@ extern "C" int ew_create(const ew_options* opts, ew_handle** out) {
 extern "C" int ew_destroy(ew_handle* h) {
   if (!h) return EW_OK;
   (void)hipSetDevice(h->opt.device);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->release();
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+  if (h->aux) (void)hipStreamDestroy(h->aux);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return EW_OK;
@@ -432,6 +442,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->px = make_plan(K[0], (K[1] * K[2] + 1) / 2);
   h->py = make_plan(K[1], 16);
   h->pz = make_plan(K[2], 16);
+  h->plane = h->opt.fft_mode == 0 && plane_fits(h->px, h->py, 2 * rs);
+  if (h->plane) HIPCHK(h, prepare_plane_kernels());
 
   // B-spline weights
   BsplineArgs ba{{K[0], K[1], K[2]}, p};
@@ -564,8 +576,20 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, s));
   if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
+  // the real-space pair sum only needs the sorted charges: it runs on the auxiliary stream,
+  // beside the spreading and FFTs (which leave CUs idle), and joins before the gather
+  const bool fork = !prof && h->aux;
+  const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
+  if (fork) {
+    HIPCHK(h, hipEventRecord(h->fork_ev, s));
+    HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
+    HIPCHK(h, launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32, h->qs,
+                               h->phi_real, h->aux));
+    HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+  }
   SpmeLaunch L{};
   for (int a = 0; a < 3; ++a) L.K[a] = K[a];
+  L.plane = h->plane;
   L.k0 = h->k0;
   L.theta = h->theta;
   L.q = h->qs;
@@ -600,9 +624,11 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     HIPCHK(h, launch_spme<float>(L, s, prof ? &h->ev[2] : nullptr));
   else
     HIPCHK(h, launch_spme<double>(L, s, prof ? &h->ev[2] : nullptr));
-  HIPCHK(h, launch_pair_spmv(n, h->row_ptr,
-                             h->col16 ? static_cast<const void*>(h->col16) : h->col,
-                             h->col16 != nullptr, h->coef, h->fp32, h->qs, h->phi_real, s));
+  if (fork)
+    HIPCHK(h, hipStreamWaitEvent(s, h->join_ev, 0));
+  else
+    HIPCHK(h, launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32, h->qs,
+                               h->phi_real, s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
   GatherArgs ga{};
   for (int a = 0; a < 3; ++a) {
@@ -884,6 +910,7 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->px = h->px;
   d->py = h->py;
   d->pz = h->pz;
+  d->plane = h->plane;
   d->yd = make_fastdiv(Kh);
   d->zd = make_fastdiv(Kh * K[1]);
 }

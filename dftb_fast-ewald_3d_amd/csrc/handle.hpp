@@ -29,6 +29,8 @@ struct ew_handle {
   ew_options opt{};
   hipStream_t stream = nullptr;
   bool own_stream = true;  // false: the stream belongs to the pool that holds this handle
+  hipStream_t aux = nullptr;      // runs the pair sum beside the SPME chain (own-stream handles)
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err = "no error";
   bool have_geom = false;
 
@@ -43,6 +45,7 @@ struct ew_handle {
   int spread_slots = 0;  // atoms staged per chunk by the interval spread (0: checked spread)
   ewamd::RegionArgs region{};   // privatised spreading (scratch == null: not used)
   ewamd::FftPlan px{}, py{}, pz{};
+  bool plane = false;        // fused x-y plane transforms
   double self_coef = 0.0;
   double background = 0.0;   // -pi / (V alpha^2), applied when opt.charged
   double* qsum = nullptr;    // total charge of the last evaluation (opt.charged)
@@ -103,6 +106,9 @@ struct ew_handle {
     return code;
   }
   void release() {
+    // work queued on the handle's streams may still read the buffers
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (aux) (void)hipStreamSynchronize(aux);
     drop_graphs();
     for (auto& b : bufs)
       if (b.p) (void)hipFree(b.p);

@@ -129,19 +129,19 @@ __device__ __forceinline__ int fdiv(int x, FastDiv f) {
   return int((__umulhi(unsigned(x), f.m) + unsigned(x)) >> f.s);
 }
 
-// tw: this pass's table, exp(-2 pi i t k / (Ns RAD)) at [(t - 1) * Ns + k].
-template <int RAD, typename R>
+// tw: this pass's table, exp(-2 pi i t k / (Ns RAD)) at [(t - 1) * Ns + k].  BLOCK threads
+// cover nl lines of n elements with at most EPT elements per thread (nl * n <= BLOCK * EPT).
+template <int RAD, typename R, int BLOCK, int EPT>
 __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, int Ns,
                                               const cx<R>* tw, R dir, FastDiv nbd,
                                               FastDiv nsd) {
-  constexpr int EPT = kFftMaxElems / kFftBlock;  // complex elements per thread and pass
   constexpr int MAXB = (EPT + RAD - 1) / RAD;
   const int nb = n / RAD;
   const int total = nl * nb;
   cx<R> v[MAXB][RAD];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
-    const int idx = threadIdx.x + b * kFftBlock;
+    const int idx = threadIdx.x + b * BLOCK;
     if (idx < total) {
       const int line = fdiv(idx, nbd), j = idx - line * nb;
       const int k = j - fdiv(j, nsd) * Ns;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void stockham_pass(cx<R>* s, int n, int nl, int P, in
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
-    const int idx = threadIdx.x + b * kFftBlock;
+    const int idx = threadIdx.x + b * BLOCK;
     if (idx < total) {
       const int line = fdiv(idx, nbd), j = idx - line * nb;
       const int jq = fdiv(j, nsd), k = j - jq * Ns;
@@ -184,22 +184,27 @@ __device__ __forceinline__ const cx<R>* stage_twiddles(cx<R>* s, const FftPlan& 
   return t;
 }
 
-template <typename R>
+template <typename R, int BLOCK = kFftBlock, int EPT = kFftMaxElems / kFftBlock>
 __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R dir) {
   int Ns = 1;
   const cx<R>* t = tw;
   for (int i = 0; i < pl.npass; ++i) {
     const int r = pl.radix[i];
     if (r == 8)
-      stockham_pass<8, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
+      stockham_pass<8, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
+                                          pl.nsd[i]);
     else if (r == 4)
-      stockham_pass<4, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
+      stockham_pass<4, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
+                                          pl.nsd[i]);
     else if (r == 2)
-      stockham_pass<2, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
+      stockham_pass<2, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
+                                          pl.nsd[i]);
     else if (r == 3)
-      stockham_pass<3, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
+      stockham_pass<3, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
+                                          pl.nsd[i]);
     else
-      stockham_pass<5, R>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i], pl.nsd[i]);
+      stockham_pass<5, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
+                                          pl.nsd[i]);
     t += (r - 1) * Ns;
     Ns *= r;
   }
@@ -213,7 +218,7 @@ __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R 
 template <typename R>
 __device__ __forceinline__ void k_fft_x_r2c_body(const R* __restrict__ grid,
                                                          cx<R>* __restrict__ cg, int nrows,
-                                                         int Kh, FftPlan pl,
+                                                         int Kh, const FftPlan& pl,
                                                          const cx<R>* __restrict__ tw, int bid) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ g
 template <typename R>
 __device__ __forceinline__ void k_fft_x_c2r_body(const cx<R>* __restrict__ cg,
                                                          R* __restrict__ grid, int nrows, int Kh,
-                                                         FftPlan pl,
+                                                         const FftPlan& pl,
                                                          const cx<R>* __restrict__ tw, int bid) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
@@ -326,7 +331,7 @@ template <typename R, int MODE>
 __device__ __forceinline__ void k_fft_lines_body(cx<R>* __restrict__ cg, int nlines,
                                                          int A, FastDiv ad, long long B,
                                                          long long es,
-                                                         FftPlan pl,
+                                                         const FftPlan& pl,
                                                          const cx<R>* __restrict__ tw,
                                                          InfluenceArgs inf, int bid) {
   extern __shared__ unsigned char smem_raw[];
@@ -405,6 +410,172 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
                                                          const cx<R>* __restrict__ tw,
                                                          InfluenceArgs inf) {
   k_fft_lines_body<R, MODE>(cg, nlines, A, ad, B, es, pl, tw, inf, blockIdx.x);
+}
+
+// One x-y plane per workgroup (z = bid): the K1 real rows are packed two per complex line
+// (row 2l real, row 2l+1 imaginary) and transformed along x, split into the half spectra of
+// the two rows, re-laid out as Kh columns (one per kx) and transformed along y; the plane of the
+// half spectrum is written as cg[(z * K1 + ky) * Kh + kx].  LDS: the x layout (line l at
+// l * px.pitch) and the y layout (column kx at kx * py.pitch) share one buffer, followed by the
+// x and y twiddle tables.
+template <typename R>
+__device__ __forceinline__ void k_fft_xy_r2c_body(const R* __restrict__ grid,
+                                                  cx<R>* __restrict__ cg, const FftPlan& px,
+                                                  const FftPlan& py, const cx<R>* __restrict__ twx,
+                                                  const cx<R>* __restrict__ twy, int bid) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
+  const int K0 = px.n, K1 = py.n, Kh = K0 / 2 + 1, nl = (K1 + 1) / 2;
+  const int Px = px.pitch, Py = py.pitch;
+  const int span = nl * Px > Kh * Py ? nl * Px : Kh * Py;
+  cx<R>* tx = s + span;
+  cx<R>* ty = tx + K0;
+  for (int i = threadIdx.x; i < K0; i += kPlaneBlock) tx[i] = twx[i];
+  for (int i = threadIdx.x; i < K1; i += kPlaneBlock) ty[i] = twy[i];
+  const R* plane = grid + (size_t)bid * K1 * K0;
+  {
+    cx<R> z[kPlaneEpt];
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      const int l = fdiv(idx, px.nd), x = idx - l * K0;
+      const bool in = idx < nl * K0;
+      z[k].x = in ? plane[(size_t)(2 * l) * K0 + x] : R(0);
+      z[k].y = in && 2 * l + 1 < K1 ? plane[(size_t)(2 * l + 1) * K0 + x] : R(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      if (idx < nl * K0) {
+        const int l = fdiv(idx, px.nd), x = idx - l * K0;
+        s[l * Px + line_slot(x)] = z[k];
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<R, kPlaneBlock, kPlaneEpt>(s, px, nl, tx, R(-1));
+  {
+    // split the packed lines: row r, mode kx  ->  column kx, element r
+    cx<R> o[kPlaneEpt];
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      if (idx < K1 * Kh) {
+        const int r = fdiv(idx, px.hd), kx = idx - r * Kh;
+        const cx<R>* line = s + (r >> 1) * Px;
+        const cx<R> z = line[line_slot(kx)];
+        const cx<R> zm = line[line_slot(kx == 0 ? 0 : K0 - kx)];
+        o[k] = (r & 1) ? cx<R>{R(0.5) * (z.y + zm.y), R(0.5) * (zm.x - z.x)}
+                       : cx<R>{R(0.5) * (z.x + zm.x), R(0.5) * (z.y - zm.y)};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      if (idx < K1 * Kh) {
+        const int r = fdiv(idx, px.hd), kx = idx - r * Kh;
+        s[kx * Py + line_slot(r)] = o[k];
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<R, kPlaneBlock, kPlaneEpt>(s, py, Kh, ty, R(-1));
+  cx<R>* out = cg + (size_t)bid * K1 * Kh;
+  for (int idx = threadIdx.x; idx < K1 * Kh; idx += kPlaneBlock) {
+    const int ky = fdiv(idx, px.hd), kx = idx - ky * Kh;
+    out[idx] = s[kx * Py + line_slot(ky)];
+  }
+}
+
+// Inverse of k_fft_xy_r2c_body: plane z = bid of the half spectrum -> K1 real rows.
+template <typename R>
+__device__ __forceinline__ void k_fft_xy_c2r_body(const cx<R>* __restrict__ cg,
+                                                  R* __restrict__ grid, const FftPlan& px, const FftPlan& py,
+                                                  const cx<R>* __restrict__ twx,
+                                                  const cx<R>* __restrict__ twy, int bid) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<R>* s = reinterpret_cast<cx<R>*>(smem_raw);
+  const int K0 = px.n, K1 = py.n, Kh = K0 / 2 + 1, nl = (K1 + 1) / 2;
+  const int Px = px.pitch, Py = py.pitch;
+  const int span = nl * Px > Kh * Py ? nl * Px : Kh * Py;
+  cx<R>* tx = s + span;
+  cx<R>* ty = tx + K0;
+  for (int i = threadIdx.x; i < K0; i += kPlaneBlock) tx[i] = twx[i];
+  for (int i = threadIdx.x; i < K1; i += kPlaneBlock) ty[i] = twy[i];
+  const cx<R>* in = cg + (size_t)bid * K1 * Kh;
+  {
+    cx<R> z[kPlaneEpt];
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      z[k] = idx < K1 * Kh ? in[idx] : cx<R>{R(0), R(0)};
+    }
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      if (idx < K1 * Kh) {
+        const int ky = fdiv(idx, px.hd), kx = idx - ky * Kh;
+        s[kx * Py + line_slot(ky)] = z[k];
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<R, kPlaneBlock, kPlaneEpt>(s, py, Kh, ty, R(1));
+  {
+    // pack rows 2l and 2l+1 of each mode into complex line l (Hermitian extension for kx >= Kh)
+    cx<R> o[kPlaneEpt];
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      if (idx < nl * K0) {
+        const int l = fdiv(idx, px.nd), x = idx - l * K0;
+        const int kk = x < Kh ? x : K0 - x;
+        cx<R> a = s[kk * Py + line_slot(2 * l)];
+        cx<R> b = 2 * l + 1 < K1 ? s[kk * Py + line_slot(2 * l + 1)] : cx<R>{R(0), R(0)};
+        if (x >= Kh) {
+          a.y = -a.y;
+          b.y = -b.y;
+        }
+        o[k] = {a.x - b.y, a.y + b.x};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPlaneEpt; ++k) {
+      const int idx = threadIdx.x + k * kPlaneBlock;
+      if (idx < nl * K0) {
+        const int l = fdiv(idx, px.nd), x = idx - l * K0;
+        s[l * Px + line_slot(x)] = o[k];
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<R, kPlaneBlock, kPlaneEpt>(s, px, nl, tx, R(1));
+  R* plane = grid + (size_t)bid * K1 * K0;
+  for (int idx = threadIdx.x; idx < K1 * K0; idx += kPlaneBlock) {
+    const int r = fdiv(idx, px.nd), x = idx - r * K0;
+    const cx<R> z = s[(r >> 1) * Px + line_slot(x)];
+    plane[idx] = (r & 1) ? z.y : z.x;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_r2c(const R* __restrict__ grid,
+                                                            cx<R>* __restrict__ cg, FftPlan px,
+                                                            FftPlan py,
+                                                            const cx<R>* __restrict__ twx,
+                                                            const cx<R>* __restrict__ twy) {
+  k_fft_xy_r2c_body<R>(grid, cg, px, py, twx, twy, blockIdx.x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_c2r(const cx<R>* __restrict__ cg,
+                                                            R* __restrict__ grid, FftPlan px,
+                                                            FftPlan py,
+                                                            const cx<R>* __restrict__ twx,
+                                                            const cx<R>* __restrict__ twy) {
+  k_fft_xy_c2r_body<R>(cg, grid, px, py, twx, twy, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1499,6 +1670,22 @@ __global__ __launch_bounds__(kFftBlock) void kg_fft_lines(const StructDesc* __re
                               d.pz, static_cast<const cx<R>*>(d.tw[2]), d.inf, b);
 }
 
+template <typename R, bool INVERSE>
+__global__ __launch_bounds__(kPlaneBlock) void kg_fft_xy(const StructDesc* __restrict__ D,
+                                                         const IoSlot* __restrict__ io,
+                                                         GroupMap m) {
+  int b;
+  const StructDesc& d = D[io[group_slot(m, b)].sid];
+  if (INVERSE)
+    k_fft_xy_c2r_body<R>(static_cast<const cx<R>*>(d.cgrid), static_cast<R*>(d.grid), d.px,
+                         d.py, static_cast<const cx<R>*>(d.tw[0]),
+                         static_cast<const cx<R>*>(d.tw[1]), b);
+  else
+    k_fft_xy_r2c_body<R>(static_cast<const R*>(d.grid), static_cast<cx<R>*>(d.cgrid), d.px,
+                         d.py, static_cast<const cx<R>*>(d.tw[0]),
+                         static_cast<const cx<R>*>(d.tw[1]), b);
+}
+
 template <typename CT>
 __global__ __launch_bounds__(256) void kg_pair_spmv(const StructDesc* __restrict__ D,
                                                     const IoSlot* __restrict__ io, GroupMap m) {
@@ -1565,6 +1752,16 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
   if (run(kGkChargeSum))
     hipLaunchKernelGGL(kg_charge_sum, grid(kGkChargeSum), dim3(256), 0, st, D, io,
                        G.map[kGkChargeSum]);
+  const bool fork = G.aux && run(kGkPairs);
+  if (fork) {
+    hipError_t e = hipEventRecord(G.fork_ev, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(G.aux, G.fork_ev, 0);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, G.aux, D, io,
+                       G.map[kGkPairs]);
+    e = hipEventRecord(G.join_ev, G.aux);
+    if (e != hipSuccess) return e;
+  }
 #define EW_ORDER_SWITCH(MACRO) \
   switch (G.order) {           \
     case 4: MACRO(4); break;   \
@@ -1594,19 +1791,33 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
   if (run(kGkChecked))
     hipLaunchKernelGGL(kg_spread_checked<R>, grid(kGkChecked), dim3(256), 0, st, D, io,
                        G.map[kGkChecked]);
-  hipLaunchKernelGGL((kg_fft_x<R, false>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX], st,
-                     D, io, G.map[kGkFftX]);
-  hipLaunchKernelGGL((kg_fft_lines<R, 1, 0>), grid(kGkFftY), dim3(kFftBlock), G.smem[kGkFftY],
-                     st, D, io, G.map[kGkFftY]);
+  if (run(kGkPlane))
+    hipLaunchKernelGGL((kg_fft_xy<R, false>), grid(kGkPlane), dim3(kPlaneBlock),
+                       G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
+  if (run(kGkFftX)) {
+    hipLaunchKernelGGL((kg_fft_x<R, false>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
+                       st, D, io, G.map[kGkFftX]);
+    hipLaunchKernelGGL((kg_fft_lines<R, 1, 0>), grid(kGkFftY), dim3(kFftBlock),
+                       G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
+  }
   hipLaunchKernelGGL((kg_fft_lines<R, 2, 2>), grid(kGkFftZ), dim3(kFftBlock), G.smem[kGkFftZ],
                      st, D, io, G.map[kGkFftZ]);
-  hipLaunchKernelGGL((kg_fft_lines<R, 1, 1>), grid(kGkFftY), dim3(kFftBlock), G.smem[kGkFftY],
-                     st, D, io, G.map[kGkFftY]);
-  hipLaunchKernelGGL((kg_fft_x<R, true>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX], st,
-                     D, io, G.map[kGkFftX]);
-  if (run(kGkPairs))
+  if (run(kGkFftX)) {
+    hipLaunchKernelGGL((kg_fft_lines<R, 1, 1>), grid(kGkFftY), dim3(kFftBlock),
+                       G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
+    hipLaunchKernelGGL((kg_fft_x<R, true>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
+                       st, D, io, G.map[kGkFftX]);
+  }
+  if (run(kGkPlane))
+    hipLaunchKernelGGL((kg_fft_xy<R, true>), grid(kGkPlane), dim3(kPlaneBlock),
+                       G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
+  if (fork) {
+    const hipError_t e = hipStreamWaitEvent(st, G.join_ev, 0);
+    if (e != hipSuccess) return e;
+  } else if (run(kGkPairs)) {
     hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, st, D, io,
                        G.map[kGkPairs]);
+  }
   if (run(kGkGatherTile)) {
 #define EW_G_GATHER(PP)                                                                   \
   hipLaunchKernelGGL((kg_gather_tile<R, PP>), grid(kGkGatherTile), dim3(256),            \
@@ -1696,27 +1907,62 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
   mark(0);
   const int nrows = K2 * K3;
   const int gx = (nrows + 2 * L.px.lines - 1) / (2 * L.px.lines);
-  hipLaunchKernelGGL(k_fft_x_r2c<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, grid, cg,
-                     nrows, Kh, L.px, tw1);
-  mark(1);
   const int ny = Kh * K3;
-  hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((ny + L.py.lines - 1) / L.py.lines),
-                     dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh), (long long)K2 * Kh,
-                     (long long)Kh, L.py, tw2, L.inf);
+  const size_t psmem = plane_smem(L.px, L.py, sizeof(cx<R>));
+  if (L.plane) {
+    // x and y in one pass per plane (the y phase event follows immediately)
+    hipLaunchKernelGGL(k_fft_xy_r2c<R>, dim3(K3), dim3(kPlaneBlock), psmem, st, grid, cg, L.px,
+                       L.py, tw1, tw2);
+    mark(1);
+  } else {
+    hipLaunchKernelGGL(k_fft_x_r2c<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, grid,
+                       cg, nrows, Kh, L.px, tw1);
+    mark(1);
+    hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((ny + L.py.lines - 1) / L.py.lines),
+                       dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh),
+                       (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf);
+  }
   mark(2);
   const int nz = Kh * K2;
   hipLaunchKernelGGL((k_fft_lines<R, 2>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
                      dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2, make_fastdiv(Kh * K2), 0LL,
                      (long long)Kh * K2, L.pz, tw3, L.inf);
   mark(3);
-  hipLaunchKernelGGL((k_fft_lines<R, 1>), dim3((ny + L.py.lines - 1) / L.py.lines),
-                     dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh), (long long)K2 * Kh,
-                     (long long)Kh, L.py, tw2, L.inf);
-  mark(4);
-  hipLaunchKernelGGL(k_fft_x_c2r<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, cg, grid,
-                     nrows, Kh, L.px, tw1);
+  if (L.plane) {
+    mark(4);
+    hipLaunchKernelGGL(k_fft_xy_c2r<R>, dim3(K3), dim3(kPlaneBlock), psmem, st, cg, grid, L.px,
+                       L.py, tw1, tw2);
+  } else {
+    hipLaunchKernelGGL((k_fft_lines<R, 1>), dim3((ny + L.py.lines - 1) / L.py.lines),
+                       dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh),
+                       (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf);
+    mark(4);
+    hipLaunchKernelGGL(k_fft_x_c2r<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, cg,
+                       grid, nrows, Kh, L.px, tw1);
+  }
   mark(5);
   return hipGetLastError();
+}
+
+hipError_t prepare_plane_kernels() {
+  static hipError_t done = [] {
+    const int b = (int)kPlaneMaxLds;
+    const void* fns[] = {
+        reinterpret_cast<const void*>(&k_fft_xy_r2c<double>),
+        reinterpret_cast<const void*>(&k_fft_xy_r2c<float>),
+        reinterpret_cast<const void*>(&k_fft_xy_c2r<double>),
+        reinterpret_cast<const void*>(&k_fft_xy_c2r<float>),
+        reinterpret_cast<const void*>(&kg_fft_xy<double, false>),
+        reinterpret_cast<const void*>(&kg_fft_xy<float, false>),
+        reinterpret_cast<const void*>(&kg_fft_xy<double, true>),
+        reinterpret_cast<const void*>(&kg_fft_xy<float, true>)};
+    for (const void* f : fns) {
+      const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }();
+  return done;
 }
 
 template hipError_t launch_spme<double>(const SpmeLaunch&, hipStream_t, hipEvent_t*);

@@ -92,8 +92,28 @@ struct RegionArgs {
 // LDS budget of a block copy
 constexpr size_t kRegionMaxBytes = 60 * 1024;
 
+// Fused plane transforms: one workgroup of kPlaneBlock threads holds a whole x-y plane of the
+// half spectrum in LDS (up to kPlaneMaxLds, the gfx950 workgroup limit) and runs the x R2C and
+// y passes of that plane (and the inverse y and x C2R passes) in one launch.
+constexpr int kPlaneBlock = 1024;
+constexpr int kPlaneEpt = 10;  // complex elements per thread: planes up to 10240 elements
+constexpr size_t kPlaneMaxLds = 160 * 1024;
+inline size_t plane_smem(const FftPlan& px, const FftPlan& py, size_t cs) {
+  const size_t nl = (size_t)(py.n + 1) / 2, Kh = (size_t)px.n / 2 + 1;
+  const size_t lines = nl * px.pitch > Kh * py.pitch ? nl * px.pitch : Kh * py.pitch;
+  return cs * (lines + px.n + py.n);
+}
+inline bool plane_fits(const FftPlan& px, const FftPlan& py, size_t cs) {
+  const long long nl = (py.n + 1) / 2, Kh = px.n / 2 + 1;
+  const long long cap = (long long)kPlaneBlock * kPlaneEpt;
+  return nl * px.n <= cap && Kh * py.n <= cap && plane_smem(px, py, cs) <= kPlaneMaxLds;
+}
+// Raises the dynamic LDS limit of the plane kernels to kPlaneMaxLds (once per process).
+hipError_t prepare_plane_kernels();
+
 struct SpmeLaunch {
   int K[3];
+  int plane;  // 1: fused x-y plane transforms (plane_fits), 0: separate x and y passes
   const int4* k0;
   const void* theta;
   const double* q;
@@ -139,6 +159,7 @@ struct StructDesc {
   GatherArgs ga;
   InfluenceArgs inf;
   FftPlan px, py, pz;
+  int plane;   // 1: fused x-y plane transforms
   FastDiv yd;  // / Kh            (y lines: L = kz * Kh + kx)
   FastDiv zd;  // / (Kh * K2)     (z lines: L = ky * Kh + kx)
 };
@@ -174,6 +195,7 @@ enum GroupKind {
   kGkGatherAtom,
   kGkEnergy,
   kGkChargeSum,
+  kGkPlane,
   kGroupKinds
 };
 
@@ -181,6 +203,10 @@ struct GroupLaunch {
   const StructDesc* desc;
   const IoSlot* io;
   int order;
+  // optional: the pair sums run on `aux` beside the SPME chain (fork after the charges are
+  // gathered, join before the gather)
+  hipStream_t aux;
+  hipEvent_t fork_ev, join_ev;
   GroupMap map[kGroupKinds];
   int blocks[kGroupKinds];
   size_t smem[kGroupKinds];  // dynamic LDS: the largest of the structures in the kind

@@ -39,6 +39,8 @@ struct Captured {
 struct ew_pool {
   ew_options opt{};
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;  // pair sums beside the SPME chain
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err = "no error";
   bool fp32 = false;
   int order = 0;  // B-spline order shared by every member (0 until the first member)
@@ -101,6 +103,8 @@ size_t smem_of(const StructDesc& d, int kind, bool fp32) {
       return cs * ((size_t)d.pz.lines * d.pz.pitch + d.pz.n);
     case kGkGatherTile:
       return gather_tile_smem(d.spread.T, p, rs);
+    case kGkPlane:
+      return d.plane ? plane_smem(d.px, d.py, cs) : 0;
     default:
       return 0;
   }
@@ -115,8 +119,9 @@ int blocks_of(const StructDesc& d, int kind) {
       return d.spread_mode == 0 ? int(((long long)d.K[0] * d.K[1] * d.K[2] + 255) / 256) : 0;
     case kGkSlotted: return d.spread_mode == 1 ? d.ntiles : 0;
     case kGkChecked: return d.spread_mode == 2 ? d.ntiles : 0;
-    case kGkFftX: return (d.K[1] * d.K[2] + 2 * d.px.lines - 1) / (2 * d.px.lines);
-    case kGkFftY: return (Kh * d.K[2] + d.py.lines - 1) / d.py.lines;
+    case kGkFftX: return d.plane ? 0 : (d.K[1] * d.K[2] + 2 * d.px.lines - 1) / (2 * d.px.lines);
+    case kGkFftY: return d.plane ? 0 : (Kh * d.K[2] + d.py.lines - 1) / d.py.lines;
+    case kGkPlane: return d.plane ? d.K[2] : 0;
     case kGkFftZ: return (Kh * d.K[1] + d.pz.lines - 1) / d.pz.lines;
     case kGkPairs: return (d.n + 3) / 4;
     case kGkGatherTile: return d.gather_tile ? d.ntiles : 0;
@@ -203,7 +208,8 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
   std::memcpy(reinterpret_cast<char*>(S->words.data()) + S->io_offset, io.data(),
               sizeof(IoSlot) * io.size());
   for (int k = 0; k < kGroupKinds; ++k)
-    if (S->smem[k] > 65536) return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
+    if (S->smem[k] > (k == kGkPlane ? kPlaneMaxLds : 65536))
+      return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
   return EW_OK;
 }
 
@@ -234,7 +240,11 @@ int check_ids(ew_pool* P, int count, const int* ids) {
   return EW_OK;
 }
 
-int enqueue(ew_pool* P, const GroupLaunch& G, hipStream_t s) {
+int enqueue(ew_pool* P, GroupLaunch G, hipStream_t s, hipStream_t aux, hipEvent_t fork_ev,
+            hipEvent_t join_ev) {
+  G.aux = aux;
+  G.fork_ev = fork_ev;
+  G.join_ev = join_ev;
   PCHK(P, P->fp32 ? launch_group<float>(G, s) : launch_group<double>(G, s));
   return EW_OK;
 }
@@ -269,7 +279,7 @@ int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* 
     if (st) return st;
     // a pageable source: the copy has read the host image when the call returns
     PCHK(P, hipMemcpyAsync(P->staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
-    return enqueue(P, launch_of(P, S, P->staging), s);
+    return enqueue(P, launch_of(P, S, P->staging), s, P->aux, P->fork_ev, P->join_ev);
   }
   if (P->graphs.size() >= 8) P->drop_graphs();
   Captured c;
@@ -282,7 +292,7 @@ int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* 
   PCHK(P, hipMemcpy(c.staging, S.words.data(), bytes, hipMemcpyHostToDevice));
   const GroupLaunch G = launch_of(P, S, c.staging);
   PCHK(P, hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-  st = enqueue(P, G, P->stream);
+  st = enqueue(P, G, P->stream, P->aux, P->fork_ev, P->join_ev);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(P->stream, &graph);
   if (st != EW_OK || ec != hipSuccess) {
@@ -306,7 +316,8 @@ int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* 
 
 int ewamd::pool_launch_staged(ew_pool* P, int count, const int* ids, const double* const* dq,
                               double* const* shift, double* energies, void** staging,
-                              size_t* staging_cap, hipStream_t s) {
+                              size_t* staging_cap, hipStream_t s, hipStream_t aux,
+                              hipEvent_t fork_ev, hipEvent_t join_ev) {
   int st = check_ids(P, count, ids);
   if (st) return st;
   st = sync_desc(P);
@@ -318,7 +329,7 @@ int ewamd::pool_launch_staged(ew_pool* P, int count, const int* ids, const doubl
   st = ensure(P, staging, staging_cap, bytes);
   if (st) return st;
   PCHK(P, hipMemcpyAsync(*staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
-  return enqueue(P, launch_of(P, S, *staging), s);
+  return enqueue(P, launch_of(P, S, *staging), s, aux, fork_ev, join_ev);
 }
 
 int ewamd::pool_atoms(const ew_pool* P, int id) {
@@ -357,7 +368,14 @@ extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, 
   P->opt.profile = 0;  // per-phase events are a single-structure feature
   P->fp32 = o.precision == EW_PRECISION_MIXED;
   P->capacity = capacity_bytes;
-  if (hipStreamCreate(&P->stream) != hipSuccess) {
+  if (hipStreamCreate(&P->stream) != hipSuccess || hipStreamCreate(&P->aux) != hipSuccess ||
+      hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (P->join_ev) (void)hipEventDestroy(P->join_ev);
+    if (P->fork_ev) (void)hipEventDestroy(P->fork_ev);
+    if (P->aux) (void)hipStreamDestroy(P->aux);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
     return set_global_error(EW_ERR_DEVICE, "ew_pool_create: hipStreamCreate failed");
   }
@@ -369,6 +387,7 @@ extern "C" int ew_pool_destroy(ew_pool* P) {
   if (!P) return EW_OK;
   (void)hipSetDevice(P->opt.device);
   (void)hipStreamSynchronize(P->stream);
+  (void)hipStreamSynchronize(P->aux);
   P->drop_graphs();
   for (auto* h : P->members)
     if (h) (void)ew_destroy(h);
@@ -377,6 +396,9 @@ extern "C" int ew_pool_destroy(ew_pool* P) {
   if (P->io_dq) (void)hipFree(P->io_dq);
   if (P->io_shift) (void)hipFree(P->io_shift);
   if (P->io_energy) (void)hipFree(P->io_energy);
+  (void)hipEventDestroy(P->fork_ev);
+  (void)hipEventDestroy(P->join_ev);
+  (void)hipStreamDestroy(P->aux);
   (void)hipStreamDestroy(P->stream);
   delete P;
   return EW_OK;

@@ -49,6 +49,8 @@ struct Request {
 struct Batch {
   ew_server* server = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;  // the batch's pair sums, beside its SPME chain
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   double* h_dq = nullptr;  // pinned
   double* h_shift = nullptr;
   double* h_energy = nullptr;
@@ -150,7 +152,8 @@ void launch(ew_server* S, Batch* b) {
   }
   if (st == EW_OK) {
     st = pool_launch_staged(S->pool, count, ids.data(), dq.data(), sh.data(), b->d_energy,
-                            &b->staging, &b->staging_cap, b->stream);
+                            &b->staging, &b->staging_cap, b->stream, b->aux, b->fork_ev,
+                            b->join_ev);
     if (st) msg = pool_error(S->pool);
   }
   if (st == EW_OK &&
@@ -226,6 +229,9 @@ double percentile(std::vector<double> v, double p) {
 
 void free_batch(Batch& b) {
   if (b.stream) (void)hipStreamDestroy(b.stream);
+  if (b.aux) (void)hipStreamDestroy(b.aux);
+  if (b.fork_ev) (void)hipEventDestroy(b.fork_ev);
+  if (b.join_ev) (void)hipEventDestroy(b.join_ev);
   if (b.h_dq) (void)hipHostFree(b.h_dq);
   if (b.h_shift) (void)hipHostFree(b.h_shift);
   if (b.h_energy) (void)hipHostFree(b.h_energy);
@@ -270,6 +276,9 @@ extern "C" int ew_server_create(ew_pool* pool, const ew_server_options* opts, ew
   for (Batch& b : S->batches) {
     b.server = S;
     const bool ok = hipStreamCreate(&b.stream) == hipSuccess &&
+                    hipStreamCreate(&b.aux) == hipSuccess &&
+                    hipEventCreateWithFlags(&b.fork_ev, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&b.join_ev, hipEventDisableTiming) == hipSuccess &&
                     hipHostMalloc(&b.h_dq, sizeof(double) * big) == hipSuccess &&
                     hipHostMalloc(&b.h_shift, sizeof(double) * big) == hipSuccess &&
                     hipHostMalloc(&b.h_energy, sizeof(double) * S->max_batch) == hipSuccess &&
@@ -297,8 +306,10 @@ extern "C" int ew_server_destroy(ew_server* S) {
   }
   S->cv_work.notify_all();
   if (S->worker.joinable()) S->worker.join();
-  for (Batch& b : S->batches)
+  for (Batch& b : S->batches) {
     if (b.stream) (void)hipStreamSynchronize(b.stream);
+    if (b.aux) (void)hipStreamSynchronize(b.aux);
+  }
   {
     // host functions of the last batches may still be returning their slots
     std::unique_lock<std::mutex> lk(S->mu);

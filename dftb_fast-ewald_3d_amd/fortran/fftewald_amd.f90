@@ -38,7 +38,7 @@ module fftewald_amd
       integer(c_int) :: profile = 0
       integer(c_int) :: eager = 0
       integer(c_int) :: charged = 0
-      integer(c_int) :: reserved = 0
+      integer(c_int) :: fft_mode = 0
    end type ew_options
 
    !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).

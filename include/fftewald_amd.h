@@ -72,7 +72,8 @@ typedef struct ew_options {
                       -pi/(V alpha^2) * sum(q) to every shift, as DFTB+'s stock Ewald does
                       (src/dftbp/dftb/coulomb.F90:1292-1296); 0 (default): reject them, as the
                       fast path does (coulomb.F90:255-265, fe/common/ewald_validation.f90:26-48) */
-  int reserved;
+  int fft_mode;    /* 0 (default): fused x-y plane transforms where a plane fits one workgroup's
+                      LDS, separate x and y passes elsewhere; 1: always separate passes        */
 } ew_options;
 
 /* Parameters an evaluation uses; mirrors TEwaldParameters3d
