@@ -910,46 +910,67 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
   k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x);
 }
 
+// Assembly of the privatised copies, one workgroup per block of the mesh: a point at offset l
+// (per axis) inside its block receives its own copy at l and, for small l, the halo of the
+// copies of the preceding blocks at l + (widths of the blocks in between); they are summed in a
+// fixed order.  Everything but the offsets and the loads is uniform across the workgroup.
 template <typename R>
 __device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ regions,
                                                          RegionArgs g, R* __restrict__ grid, int bid) {
-  const long long id = (long long)bid * blockDim.x + threadIdx.x;
-  const long long G = (long long)g.K[0] * g.K[1] * g.K[2];
-  if (id >= G) return;
-  const int x = int(id % g.K[0]);
-  const int y = int((id / g.K[0]) % g.K[1]);
-  const int z = int(id / ((long long)g.K[0] * g.K[1]));
   const int p = g.order;
   const int RX = g.S[0] + p - 1, RY = g.S[1] + p - 1;
-  const int pos[3] = {x, y, z};
-  // per axis: the blocks whose copy (block + halo) covers this coordinate, and the offset in it
-  int nb[3], bi[3][4], off[3][4];
+  int b[3];
+  {
+    int t = bid;
+    b[0] = t % g.nb[0];
+    t /= g.nb[0];
+    b[1] = t % g.nb[1];
+    b[2] = t / g.nb[1];
+  }
+  // per axis: candidate blocks d = 0..3 back (wrapping), the offset shift of each and its width
+  int cb[3][4], sh[3][4], lim[3][4], nc[3], w[3], lo[3];
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int own = pos[a] / g.S[a];
-    nb[a] = 0;
-    for (int d = 0; d < 4 && d < g.nb[a]; ++d) {
-      int t = own - d;
+    lo[a] = b[a] * g.S[a];
+    w[a] = min(g.S[a], g.K[a] - lo[a]);
+    nc[a] = 0;
+    int shift = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      if (d >= g.nb[a] && d > 0) break;
+      int t = b[a] - d;
       if (t < 0) t += g.nb[a];
-      const int lo = t * g.S[a];
-      const int w = min(g.S[a], g.K[a] - lo);
-      int r = pos[a] - lo;
-      if (r < 0) r += g.K[a];
-      if (r < w + p - 1) {
-        bi[a][nb[a]] = t;
-        off[a][nb[a]] = r;
-        ++nb[a];
-      }
+      const int wt = min(g.S[a], g.K[a] - t * g.S[a]);
+      if (d > 0) shift += wt;
+      // offsets l + shift inside copy t (width wt, halo p-1) are valid for l < wt + p - 1 - shift
+      const int l_max = wt + p - 1 - shift;
+      if (l_max <= 0) break;
+      cb[a][d] = t;
+      sh[a][d] = shift;
+      lim[a][d] = l_max;
+      nc[a] = d + 1;
     }
   }
-  R sum = R(0);
-  for (int i3 = 0; i3 < nb[2]; ++i3)
-    for (int i2 = 0; i2 < nb[1]; ++i2)
-      for (int i1 = 0; i1 < nb[0]; ++i1) {
-        const int blk = (bi[2][i3] * g.nb[1] + bi[1][i2]) * g.nb[0] + bi[0][i1];
-        sum += regions[(size_t)blk * g.region_stride +
-                       ((size_t)off[2][i3] * RY + off[1][i2]) * RX + off[0][i1]];
+  const int npl = w[0] * w[1];
+  for (int xy = threadIdx.x; xy < npl; xy += blockDim.x) {
+    const int ly = xy / w[0], lx = xy - ly * w[0];
+    for (int lz = 0; lz < w[2]; ++lz) {
+      R sum = R(0);
+      for (int d3 = 0; d3 < nc[2]; ++d3) {
+        if (lz >= lim[2][d3]) continue;
+        for (int d2 = 0; d2 < nc[1]; ++d2) {
+          if (ly >= lim[1][d2]) continue;
+          const size_t base = (size_t)(cb[2][d3] * g.nb[1] + cb[1][d2]) * g.nb[0];
+          const size_t roff = ((size_t)(lz + sh[2][d3]) * RY + (ly + sh[1][d2])) * RX;
+          for (int d1 = 0; d1 < nc[0]; ++d1) {
+            if (lx >= lim[0][d1]) continue;
+            sum += regions[(base + cb[0][d1]) * g.region_stride + roff + lx + sh[0][d1]];
+          }
+        }
       }
-  grid[id] = sum;
+      grid[((size_t)(lo[2] + lz) * g.K[1] + (lo[1] + ly)) * g.K[0] + lo[0] + lx] = sum;
+    }
+  }
 }
 
 template <typename R>
@@ -1878,8 +1899,7 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
       default: EW_REGION(0); break;
     }
 #undef EW_REGION
-    const long long G = (long long)K1 * K2 * K3;
-    hipLaunchKernelGGL(k_spread_assemble<R>, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk), dim3(256), 0, st,
                        static_cast<const R*>(scr), rg, grid);
   } else if (L.spread.slots > 0) {
     const size_t smem = sizeof(unsigned long long) * kSpreadRows * 4 +
