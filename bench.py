@@ -52,11 +52,12 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spme_triclinic98k_hbm_traffic.json")
 
 
-def kernel_symbol(phase, prec, index_bytes):
+def kernel_symbol(phase, prec, index_bytes, pair_mode=0):
     """The kernel that carries a phase (the single-kernel phases only), as rocprofv3 names it."""
     r = "float" if prec == "mixed" else "double"
     i = "short" if index_bytes == 2 else "int"
-    return {"pair_sum": f"void ewamd::k_pair_spmv<{r}, {i}>",
+    return {"pair_sum": f"void ewamd::k_pair_spmv<{r}, {i}>" if pair_mode != 2 else
+            f"void ewamd::k_pair_half<{r}>",
             "fft_x_r2c": f"void ewamd::k_fft_x_r2c<{r}>",
             "fft_y": f"void ewamd::k_fft_lines<{r}, 0>",
             "fft_z_influence": f"void ewamd::k_fft_lines<{r}, 2>",
@@ -488,7 +489,8 @@ def main():
     dom = max(head["phases"].items(), key=lambda kv: kv[1]["ms"])
     name, d = dom
     achieved = d["GBs"]
-    sym = kernel_symbol(name, args.precision, head["params"]["index_bytes"])
+    sym = kernel_symbol(name, args.precision, head["params"]["index_bytes"],
+                        head["params"]["pair_mode"])
     roofline = {"bound": "hbm", "kernel": sym or name, "phase": name, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(sym) if sym else None,

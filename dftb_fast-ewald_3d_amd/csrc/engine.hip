@@ -197,6 +197,147 @@ static int validate_inputs(ew_handle* h, int n, const double* coords, const doub
   return EW_OK;
 }
 
+// Symmetric storage of the linked-tile pair list (each unordered pair once, grouped by the tile
+// of its lower atom and by chunks of <= 256 atoms of that tile's partner tiles), built from the
+// full row list; on success the row list is freed.  Skipped (h->half stays false) when a tile
+// holds more than 255 atoms or a tile has more than kHalfMaxChunks chunks.
+static int build_half_pairs(ew_handle* h, const std::vector<int>& ts, hipStream_t s) {
+  const int nt0 = h->nt[0], nt1 = h->nt[1], nt2 = h->nt[2];
+  const int ntiles = nt0 * nt1 * nt2;
+  for (int t = 0; t < ntiles; ++t)
+    if (ts[t + 1] - ts[t] > 255) return EW_OK;
+  const PairArgs& pa = h->pa;
+  std::vector<int> tile_chunk(ntiles + 1, 0), chunk_tile(1, 0), chunk_atoms, ptile, ptile_off,
+      ptile_chunk, rc_base(ntiles + 1, 0);
+  std::vector<int> cand;
+  for (int A = 0; A < ntiles; ++A) {
+    const int t1 = A % nt0, t2 = (A / nt0) % nt1, t3 = A / (nt0 * nt1);
+    const int nt[3] = {nt0, nt1, nt2}, tc[3] = {t1, t2, t3};
+    int b[3], m[3];
+    for (int a = 0; a < 3; ++a) {
+      b[a] = pa.full[a] ? 0 : tc[a] - pa.D[a];
+      m[a] = pa.full[a] ? nt[a] : 2 * pa.D[a] + 1;
+    }
+    cand.clear();
+    for (int c3 = 0; c3 < m[2]; ++c3)
+      for (int c2 = 0; c2 < m[1]; ++c2)
+        for (int c1 = 0; c1 < m[0]; ++c1) {
+          const int u1 = ((b[0] + c1) % nt0 + nt0) % nt0;
+          const int u2 = ((b[1] + c2) % nt1 + nt1) % nt1;
+          const int u3 = ((b[2] + c3) % nt2 + nt2) % nt2;
+          const int B = (u3 * nt1 + u2) * nt0 + u1;
+          if (B >= A) cand.push_back(B);
+        }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    if (ts[A + 1] == ts[A]) cand.clear();  // an empty tile has no pairs
+    int rel = -1, fill = 0;
+    for (int B : cand) {
+      const int nb = ts[B + 1] - ts[B];
+      if (nb == 0) continue;  // empty partner tiles hold no atoms
+      if (rel < 0 || fill + nb > 256) {  // open a chunk
+        ++rel;
+        if (rel >= kHalfMaxChunks) return EW_OK;
+        chunk_atoms.push_back(0);
+        chunk_tile.push_back(chunk_tile.back());
+        fill = 0;
+      }
+      ptile.push_back(B);
+      ptile_off.push_back(fill);
+      ptile_chunk.push_back(rel);
+      fill += nb;
+      chunk_atoms.back() = fill;
+      chunk_tile.back() = (int)ptile.size();
+    }
+    tile_chunk[A + 1] = (int)chunk_atoms.size();
+    rc_base[A + 1] = rc_base[A] + (rel + 1) * (ts[A + 1] - ts[A]);
+  }
+  const int nchunks = (int)chunk_atoms.size();
+  // invariants the kernels rely on: each chunk stages exactly the atoms of its partner tiles
+  // (at most 256), and every partner tile belongs to a chunk of its tile
+  for (int c = 0; c < nchunks; ++c) {
+    int atoms = 0;
+    for (int k = chunk_tile[c]; k < chunk_tile[c + 1]; ++k) {
+      if (ptile_off[k] != atoms || ptile_chunk[k] < 0) return EW_OK;
+      atoms += ts[ptile[k] + 1] - ts[ptile[k]];
+    }
+    if (atoms != chunk_atoms[c] || atoms > 256) return EW_OK;
+  }
+  int *d_tc, *d_ct, *d_ce, *d_ca, *d_pt, *d_po, *d_pc, *d_rb, *d_rc;
+  CHK(h->alloc(&d_tc, ntiles + 1));
+  CHK(h->alloc(&d_ct, nchunks + 1));
+  CHK(h->alloc(&d_ce, nchunks + 1));
+  CHK(h->alloc(&d_ca, std::max(1, nchunks)));
+  CHK(h->alloc(&d_pt, std::max<size_t>(1, ptile.size())));
+  CHK(h->alloc(&d_po, std::max<size_t>(1, ptile.size())));
+  CHK(h->alloc(&d_pc, std::max<size_t>(1, ptile.size())));
+  CHK(h->alloc(&d_rb, ntiles + 1));
+  CHK(h->alloc(&d_rc, std::max(1, rc_base[ntiles])));
+  auto up = [&](int* d, const std::vector<int>& v) {
+    return v.empty() ? hipSuccess
+                     : hipMemcpyAsync(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, s);
+  };
+  HIPCHK(h, up(d_tc, tile_chunk));
+  HIPCHK(h, up(d_ct, chunk_tile));
+  HIPCHK(h, up(d_ca, chunk_atoms));
+  HIPCHK(h, up(d_pt, ptile));
+  HIPCHK(h, up(d_po, ptile_off));
+  HIPCHK(h, up(d_pc, ptile_chunk));
+  HIPCHK(h, up(d_rb, rc_base));
+  HalfPairs hp{};
+  hp.tile_chunk = d_tc;
+  hp.chunk_tile = d_ct;
+  hp.chunk_entry = d_ce;
+  hp.chunk_atoms = d_ca;
+  hp.ptile = d_pt;
+  hp.ptile_off = d_po;
+  hp.ptile_chunk = d_pc;
+  hp.rc_base = d_rb;
+  HIPCHK(h, launch_half_count(h->n, h->row_ptr, h->col, h->tile_of, h->tile_start, hp, d_rc, s));
+  std::vector<int> rc(rc_base[ntiles]);
+  if (!rc.empty())
+    HIPCHK(h, hipMemcpyAsync(rc.data(), d_rc, sizeof(int) * rc.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  // starting positions: chunk by chunk, rows in order within a chunk
+  std::vector<int> chunk_entry(nchunks + 1, 0);
+  long long run = 0;
+  for (int A = 0; A < ntiles; ++A) {
+    const int nA = ts[A + 1] - ts[A];
+    for (int c = tile_chunk[A]; c < tile_chunk[A + 1]; ++c) {
+      const int rel = c - tile_chunk[A];
+      chunk_entry[c] = (int)run;
+      for (int i = 0; i < nA; ++i) {
+        int& v = rc[rc_base[A] + rel * nA + i];
+        const int cnt = v;
+        v = (int)run;
+        run += cnt;
+      }
+    }
+  }
+  if (run >= (1LL << 31)) return EW_OK;  // stays on the row list
+  chunk_entry[nchunks] = (int)run;
+  HIPCHK(h, up(d_ce, chunk_entry));
+  HIPCHK(h, up(d_rc, rc));
+  unsigned short* ent;
+  unsigned char* cf;
+  CHK(h->alloc(&ent, (size_t)run + 8));
+  CHK(h->alloc(&cf, ((size_t)run + 8) * (h->fp32 ? sizeof(float) : sizeof(double))));
+  HIPCHK(h, launch_half_fill(h->n, h->row_ptr, h->col, h->coef, h->fp32, h->tile_of,
+                             h->tile_start, hp, d_rc, ent, cf, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  hp.entry = ent;
+  hp.coef = cf;
+  h->free_buf(d_rc);
+  h->free_buf(h->col);
+  h->free_buf(h->coef);
+  h->col = nullptr;
+  h->coef = nullptr;
+  h->hp = hp;
+  h->half = true;
+  h->half_entries = run;
+  return EW_OK;
+}
+
 extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const double* latvecs,
                                const int* species, const double* hubbard_u, int n_species,
                                const double* charges) {
@@ -523,9 +664,15 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     h->release();
     return h->fail(EW_ERR_COINCIDENT, "coincident charges");
   }
-  // 16-bit partner offsets: rows are in tile order, so partners sit close to the row index.
+  // symmetric storage of the linked list (half the bytes per evaluation), else 16-bit partner
+  // offsets: rows are in tile order, so partners sit close to the row index.
   h->col16 = nullptr;
-  if (h->pair_mode == 0 && h->nnz > 0) {
+  h->half = false;
+  h->hp = HalfPairs{};
+  h->half_entries = 0;
+  if (h->pair_mode == 0 && h->nnz > 0 && h->opt.pair_storage == 1)
+    CHK(build_half_pairs(h, count, s));
+  if (h->pair_mode == 0 && h->nnz > 0 && !h->half) {
     short* off = nullptr;
     CHK(h->alloc(&off, (size_t)h->nnz + 8));
     HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));
@@ -560,9 +707,9 @@ extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
   out->k_cut = kmin;
   out->order = h->prm.order;
   out->precision = h->prm.precision;
-  out->pair_mode = h->pair_mode;
-  out->index_bytes = h->col16 ? 2 : 4;
-  out->n_pairs = h->nnz;
+  out->pair_mode = h->half ? 2 : h->pair_mode;
+  out->index_bytes = (h->col16 || h->half) ? 2 : 4;
+  out->n_pairs = h->half ? h->half_entries : h->nnz;
   out->device_bytes = (long long)h->device_bytes;
   return EW_OK;
 }
@@ -573,18 +720,24 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   const int* K = h->prm.K;
   const bool prof = h->opt.profile != 0;
   if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
-  HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, s));
+  HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, h->half ? h->phi_real : nullptr, s));
   if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
   // the real-space pair sum only needs the sorted charges: it runs on the auxiliary stream,
   // beside the spreading and FFTs (which leave CUs idle), and joins before the gather
   const bool fork = !prof && h->aux;
   const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
+  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  auto pairs = [&](hipStream_t st) {
+    return h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real,
+                                      st)
+                   : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
+                                      h->qs, h->phi_real, st);
+  };
   if (fork) {
     HIPCHK(h, hipEventRecord(h->fork_ev, s));
     HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
-    HIPCHK(h, launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32, h->qs,
-                               h->phi_real, h->aux));
+    HIPCHK(h, pairs(h->aux));
     HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
   }
   SpmeLaunch L{};
@@ -627,8 +780,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   if (fork)
     HIPCHK(h, hipStreamWaitEvent(s, h->join_ev, 0));
   else
-    HIPCHK(h, launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32, h->qs,
-                               h->phi_real, s));
+    HIPCHK(h, pairs(s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
   GatherArgs ga{};
   for (int a = 0; a < 3; ++a) {
@@ -875,6 +1027,8 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->epart = h->epart;
   d->row_ptr = h->row_ptr;
   d->col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
+  d->half = h->half;
+  d->hp = h->hp;
   d->coef = h->coef;
   d->grid = h->grid;
   d->cgrid = h->cgrid;

@@ -66,6 +66,9 @@ struct ew_handle {
   long long* row_ptr = nullptr;
   int* col = nullptr;
   short* col16 = nullptr;  // signed 16-bit partner offsets (when every row fits), else null
+  bool half = false;       // symmetric pair storage (hp) in place of the row list
+  ewamd::HalfPairs hp{};
+  long long half_entries = 0;
   void* coef = nullptr;
   void* grid = nullptr;
   void* cgrid = nullptr;

@@ -984,14 +984,19 @@ __global__ __launch_bounds__(256) void k_spread_assemble(const R* __restrict__ r
 // ----------------------------------------------------------------------------------------
 
 __device__ __forceinline__ void k_gather_charges_body(int n, const int* __restrict__ perm,
-                                 const double* __restrict__ dq, double* __restrict__ qs, int bid) {
+                                 const double* __restrict__ dq, double* __restrict__ qs,
+                                 double* __restrict__ zero, int bid) {
   const int k = bid * blockDim.x + threadIdx.x;
-  if (k < n) qs[k] = dq[perm[k]];
+  if (k < n) {
+    qs[k] = dq[perm[k]];
+    if (zero) zero[k] = 0.0;
+  }
 }
 
 __global__ void k_gather_charges(int n, const int* __restrict__ perm,
-                                 const double* __restrict__ dq, double* __restrict__ qs) {
-  k_gather_charges_body(n, perm, dq, qs, blockIdx.x);
+                                 const double* __restrict__ dq, double* __restrict__ qs,
+                                 double* __restrict__ zero) {
+  k_gather_charges_body(n, perm, dq, qs, zero, blockIdx.x);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1051,6 +1056,121 @@ __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __res
                                                    const double* __restrict__ q,
                                                    double* __restrict__ out) {
   k_pair_spmv_body<CT, IT>(n, row_ptr, col, coef, q, out, blockIdx.x);
+}
+
+// ---- symmetric pair storage ----
+
+// Partner slot of tile B in tile A's ascending partner list [k0, k1), or -1.
+__device__ __forceinline__ int half_find(const int* __restrict__ ptile, int k0, int k1, int B) {
+  while (k0 < k1) {
+    const int m = (k0 + k1) >> 1;
+    const int v = ptile[m];
+    if (v == B) return m;
+    if (v < B)
+      k0 = m + 1;
+    else
+      k1 = m;
+  }
+  return -1;
+}
+
+// One thread per atom i (tile A): its entries j of the full list with (tile(j), j) after
+// (A, i), per chunk of A.  COUNT: rc[base + chunk * nA + i_local] = count; else: written at
+// rc[...] onwards, in the order of the full row.
+template <bool COUNT, typename CT>
+__global__ void k_half_pairs(int n, const long long* __restrict__ row_ptr,
+                             const int* __restrict__ col, const CT* __restrict__ coef,
+                             const int* __restrict__ tile_of, const int* __restrict__ tile_start,
+                             HalfPairs hp, int* __restrict__ rc, unsigned short* __restrict__ out,
+                             CT* __restrict__ coef_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int A = tile_of[i];
+  const int a0 = tile_start[A], nA = tile_start[A + 1] - a0, il = i - a0;
+  const int c0 = hp.tile_chunk[A], nch = hp.tile_chunk[A + 1] - c0;
+  const int k0 = hp.chunk_tile[c0], k1 = hp.chunk_tile[c0 + nch];
+  const int base = hp.rc_base[A] + il;
+  int cnt[kHalfMaxChunks];
+#pragma unroll
+  for (int c = 0; c < kHalfMaxChunks; ++c) cnt[c] = 0;
+  for (long long k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+    const int j = col[k];
+    const int B = tile_of[j];
+    if (B < A || (B == A && j <= i)) continue;
+    const int slot = half_find(hp.ptile, k0, k1, B);
+    if (slot < 0) continue;  // cannot happen: every partner tile is in A's list
+    const int rel = hp.ptile_chunk[slot];
+    if (COUNT) {
+      for (int c = 0; c < kHalfMaxChunks; ++c)
+        if (c == rel) ++cnt[c];
+    } else {
+      int pos = rc[base + rel * nA];
+      for (int c = 0; c < kHalfMaxChunks; ++c)
+        if (c == rel) pos += cnt[c]++;
+      out[pos] = (unsigned short)((il << 8) | (hp.ptile_off[slot] + j - tile_start[B]));
+      coef_out[pos] = coef[k];
+    }
+  }
+  if (COUNT)
+    for (int c = 0; c < nch; ++c) {
+      int v = 0;
+#pragma unroll
+      for (int cc = 0; cc < kHalfMaxChunks; ++cc)
+        if (cc == c) v = cnt[cc];
+      rc[base + c * nA] = v;
+    }
+}
+
+// out += sum over the stored pairs of tile A (= bid): c_ij q_j into i and c_ij q_i into j, in LDS
+// per chunk, then one global atomic add per atom and chunk.
+template <typename CT>
+__device__ __forceinline__ void k_pair_half_body(const int* __restrict__ tile_start,
+                                                 const HalfPairs& hp,
+                                                 const double* __restrict__ q,
+                                                 double* __restrict__ out, int bid) {
+  __shared__ double qa[256], acc_a[256], qc[256], acc_c[256];
+  const int A = bid;
+  const int a0 = tile_start[A], nA = tile_start[A + 1] - a0;
+  const int tid = threadIdx.x;
+  if (tid < nA) {
+    qa[tid] = q[a0 + tid];
+    acc_a[tid] = 0.0;
+  }
+  const unsigned short* __restrict__ ent = hp.entry;
+  const CT* __restrict__ cf = static_cast<const CT*>(hp.coef);
+  for (int c = hp.tile_chunk[A]; c < hp.tile_chunk[A + 1]; ++c) {
+    const int k0 = hp.chunk_tile[c], k1 = hp.chunk_tile[c + 1];
+    const int na = hp.chunk_atoms[c];
+    int gj = -1;  // this thread's staged atom (global index)
+    if (tid < na) {
+      int k = k0;
+      while (k + 1 < k1 && hp.ptile_off[k + 1] <= tid) ++k;
+      const int B = hp.ptile[k], t0 = tile_start[B], r = tid - hp.ptile_off[k];
+      if (r < tile_start[B + 1] - t0) gj = t0 + r;  // inside the partner tile
+      qc[tid] = gj >= 0 ? q[gj] : 0.0;
+      acc_c[tid] = 0.0;
+    }
+    __syncthreads();
+    const int e1 = hp.chunk_entry[c + 1];
+    for (int e = hp.chunk_entry[c] + tid; e < e1; e += blockDim.x) {
+      const unsigned v = ent[e];
+      const int i = v >> 8, j = v & 255;
+      const double w = double(cf[e]);
+      atomicAdd(&acc_a[i], w * qc[j]);
+      atomicAdd(&acc_c[j], w * qa[i]);
+    }
+    __syncthreads();
+    if (gj >= 0) unsafeAtomicAdd(out + gj, acc_c[tid]);
+    __syncthreads();
+  }
+  if (tid < nA) unsafeAtomicAdd(out + a0 + tid, acc_a[tid]);
+}
+
+template <typename CT>
+__global__ __launch_bounds__(256) void k_pair_half(const int* __restrict__ tile_start,
+                                                   HalfPairs hp, const double* __restrict__ q,
+                                                   double* __restrict__ out) {
+  k_pair_half_body<CT>(tile_start, hp, q, out, blockIdx.x);
 }
 
 // Rewrites absolute partner indices as signed 16-bit offsets from the row; flags rows whose
@@ -1616,7 +1736,7 @@ __global__ __launch_bounds__(256) void kg_gather_charges(const StructDesc* __res
   int b;
   const IoSlot& o = io[group_slot(m, b)];
   const StructDesc& d = D[o.sid];
-  k_gather_charges_body(d.n, d.perm, o.dq, d.qs, b);
+  k_gather_charges_body(d.n, d.perm, o.dq, d.qs, d.half ? d.phi_real : nullptr, b);
 }
 
 template <typename R, int P>
@@ -1712,7 +1832,9 @@ __global__ __launch_bounds__(256) void kg_pair_spmv(const StructDesc* __restrict
                                                     const IoSlot* __restrict__ io, GroupMap m) {
   int b;
   const StructDesc& d = D[io[group_slot(m, b)].sid];
-  if (d.col16)
+  if (d.half)
+    k_pair_half_body<CT>(d.tile_start, d.hp, d.qs, d.phi_real, b);
+  else if (d.col16)
     k_pair_spmv_body<CT, short>(d.n, d.row_ptr, static_cast<const short*>(d.col),
                                 static_cast<const CT*>(d.coef), d.qs, d.phi_real, b);
   else
@@ -1989,8 +2111,9 @@ template hipError_t launch_spme<double>(const SpmeLaunch&, hipStream_t, hipEvent
 template hipError_t launch_spme<float>(const SpmeLaunch&, hipStream_t, hipEvent_t*);
 
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, perm, dq, qs);
+                                 double* zero, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, perm, dq, qs,
+                     zero);
   return hipGetLastError();
 }
 
@@ -2020,6 +2143,42 @@ hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bo
       hipLaunchKernelGGL((k_pair_spmv<double, int>), grid, block, 0, st, n, row_ptr, c,
                          static_cast<const double*>(coef), q, out);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_half_count(int n, const long long* row_ptr, const int* col, const int* tile_of,
+                             const int* tile_start, const HalfPairs& hp, int* rc,
+                             hipStream_t st) {
+  hipLaunchKernelGGL((k_half_pairs<true, double>), dim3((n + 127) / 128), dim3(128), 0, st, n,
+                     row_ptr, col, static_cast<const double*>(nullptr), tile_of, tile_start, hp,
+                     rc, static_cast<unsigned short*>(nullptr), static_cast<double*>(nullptr));
+  return hipGetLastError();
+}
+
+hipError_t launch_half_fill(int n, const long long* row_ptr, const int* col, const void* coef,
+                            bool fp32, const int* tile_of, const int* tile_start,
+                            const HalfPairs& hp, const int* rc, unsigned short* entry,
+                            void* coef_out, hipStream_t st) {
+  int* rcw = const_cast<int*>(rc);
+  if (fp32)
+    hipLaunchKernelGGL((k_half_pairs<false, float>), dim3((n + 127) / 128), dim3(128), 0, st, n,
+                       row_ptr, col, static_cast<const float*>(coef), tile_of, tile_start, hp,
+                       rcw, entry, static_cast<float*>(coef_out));
+  else
+    hipLaunchKernelGGL((k_half_pairs<false, double>), dim3((n + 127) / 128), dim3(128), 0, st,
+                       n, row_ptr, col, static_cast<const double*>(coef), tile_of, tile_start,
+                       hp, rcw, entry, static_cast<double*>(coef_out));
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_half(int ntiles, const int* tile_start, const HalfPairs& hp, bool fp32,
+                            const double* q, double* out, hipStream_t st) {
+  if (fp32)
+    hipLaunchKernelGGL(k_pair_half<float>, dim3(ntiles), dim3(256), 0, st, tile_start, hp, q,
+                       out);
+  else
+    hipLaunchKernelGGL(k_pair_half<double>, dim3(ntiles), dim3(256), 0, st, tile_start, hp, q,
+                       out);
   return hipGetLastError();
 }
 

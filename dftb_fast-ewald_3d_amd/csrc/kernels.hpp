@@ -24,6 +24,23 @@ constexpr size_t kSpreadSlotBytes = 40 * 1024;
 constexpr int kSpreadRows = 64;
 constexpr int kSpreadThreads = 1024;  // one thread per point of a 16x8x8 tile
 
+// Symmetric ("half") pair storage of the linked-tile list: every unordered pair once, grouped
+// by the tile A of its lower atom and by chunks of <= 256 atoms of A's partner tiles B >= A.
+// An entry is (i in A) << 8 | (j in chunk), with the coefficient beside it.
+constexpr int kHalfMaxChunks = 64;  // chunks per tile
+struct HalfPairs {
+  const int* tile_chunk;   // [ntiles + 1]: chunks of tile A
+  const int* chunk_tile;   // [nchunks + 1]: partner tiles of a chunk (index into ptile)
+  const int* chunk_entry;  // [nchunks + 1]: entries of a chunk
+  const int* chunk_atoms;  // [nchunks]: atoms staged by a chunk (<= 256)
+  const int* ptile;        // partner tile ids, ascending within a tile's list
+  const int* ptile_off;    // atom offset of each partner tile inside its chunk
+  const int* ptile_chunk;  // chunk of each partner tile, relative to its tile's first chunk
+  const int* rc_base;      // [ntiles + 1]: per-(chunk, row) table base of a tile
+  const unsigned short* entry;
+  const void* coef;
+};
+
 struct InfluenceArgs {
   int K[3];
   int Kh;
@@ -151,6 +168,8 @@ struct StructDesc {
   const long long* row_ptr;
   const void* col;
   const void* coef;
+  int half;        // 1: symmetric pair storage (hp), one pair-sum block per tile
+  HalfPairs hp;
   void* grid;
   void* cgrid;
   const void* tw[3];
@@ -219,8 +238,9 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st);
 template <typename R>
 hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
 
+// qs[k] = dq[perm[k]]; zero (when not null) is cleared for the symmetric pair sum.
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
-                                 hipStream_t st);
+                                 double* zero, hipStream_t st);
 // Total charge of qs[0..n) in a fixed order (one workgroup) into *qsum.
 hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t st);
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
@@ -228,6 +248,19 @@ hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bo
                             hipStream_t st);
 hipError_t launch_compress_cols(int n, const long long* row_ptr, const int* col, short* off,
                                 int* too_far, hipStream_t st);
+
+// Geometry time: count the entries per (chunk, row) of the full list (row_ptr, col) into rc,
+// then write them (rc holding the starting positions) with their coefficients.
+hipError_t launch_half_count(int n, const long long* row_ptr, const int* col, const int* tile_of,
+                             const int* tile_start, const HalfPairs& hp, int* rc,
+                             hipStream_t st);
+hipError_t launch_half_fill(int n, const long long* row_ptr, const int* col, const void* coef,
+                            bool fp32, const int* tile_of, const int* tile_start,
+                            const HalfPairs& hp, const int* rc, unsigned short* entry,
+                            void* coef_out, hipStream_t st);
+// Per evaluation: out (zeroed beforehand) += the pair sum, one workgroup per tile.
+hipError_t launch_pair_half(int ntiles, const int* tile_start, const HalfPairs& hp, bool fp32,
+                            const double* q, double* out, hipStream_t st);
 template <typename R>
 hipError_t launch_gather_combine(int n, const int4* k0, const void* theta, const void* grid,
                                  const double* q, const double* phi_real, const double* onsite,

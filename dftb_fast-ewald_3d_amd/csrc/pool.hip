@@ -123,7 +123,7 @@ int blocks_of(const StructDesc& d, int kind) {
     case kGkFftY: return d.plane ? 0 : (Kh * d.K[2] + d.py.lines - 1) / d.py.lines;
     case kGkPlane: return d.plane ? d.K[2] : 0;
     case kGkFftZ: return (Kh * d.K[1] + d.pz.lines - 1) / d.pz.lines;
-    case kGkPairs: return (d.n + 3) / 4;
+    case kGkPairs: return d.half ? d.ntiles : (d.n + 3) / 4;
     case kGkGatherTile: return d.gather_tile ? d.ntiles : 0;
     case kGkGatherAtom: return d.gather_tile ? 0 : (d.n + 255) / 256;
     case kGkEnergy: return 1;

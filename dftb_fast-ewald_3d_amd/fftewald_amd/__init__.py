@@ -60,7 +60,8 @@ class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("alpha", C.c_double), ("r_cut", C.c_double),
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
-                ("eager", C.c_int), ("charged", C.c_int), ("fft_mode", C.c_int)]
+                ("eager", C.c_int), ("charged", C.c_int), ("fft_mode", C.c_int),
+                ("pair_storage", C.c_int), ("reserved", C.c_int)]
 
 
 class ServerOptions(C.Structure):
@@ -174,7 +175,7 @@ def lib() -> C.CDLL:
 
 def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
                  device=0, alpha_rule="fftewald", profile=False, eager=False,
-                 charged=False, fft_mode=0) -> Options:
+                 charged=False, fft_mode=0, pair_storage=0) -> Options:
     o = Options()
     o.tol = tol or 0.0
     o.alpha = alpha or 0.0
@@ -193,6 +194,7 @@ def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precis
     o.eager = 1 if eager else 0
     o.charged = 1 if charged else 0
     o.fft_mode = int(fft_mode)
+    o.pair_storage = int(pair_storage)
     return o
 
 

@@ -39,6 +39,8 @@ module fftewald_amd
       integer(c_int) :: eager = 0
       integer(c_int) :: charged = 0
       integer(c_int) :: fft_mode = 0
+      integer(c_int) :: pair_storage = 0
+      integer(c_int) :: reserved = 0
    end type ew_options
 
    !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).

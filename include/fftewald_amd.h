@@ -74,6 +74,10 @@ typedef struct ew_options {
                       fast path does (coulomb.F90:255-265, fe/common/ewald_validation.f90:26-48) */
   int fft_mode;    /* 0 (default): fused x-y plane transforms where a plane fits one workgroup's
                       LDS, separate x and y passes elsewhere; 1: always separate passes        */
+  int pair_storage; /* 0 (default): the row list (every linked pair in both rows, the fastest
+                       pair sum); 1: each linked pair stored once (symmetric, tile-blocked):
+                       half the pair memory, about twice the pair-sum time                    */
+  int reserved;
 } ew_options;
 
 /* Parameters an evaluation uses; mirrors TEwaldParameters3d
@@ -86,7 +90,9 @@ typedef struct ew_parameters {
   int grid[3];       /* SPME mesh                                                                */
   int order;         /* B-spline order                                                           */
   int precision;
-  int pair_mode;     /* 0 = linked tiles (minimum image), 1 = dense image sum (small cells)     */
+  int pair_mode;     /* 0 = linked tiles (minimum image), each pair in both rows;
+                        1 = dense image sum (small cells); 2 = linked tiles, each pair stored
+                        once (symmetric, tile-blocked)                                         */
   int index_bytes;   /* bytes per stored partner index: 2 (offsets from the row) or 4          */
   long long n_pairs; /* stored real-space pair coefficients                                      */
   long long device_bytes; /* device memory held by the handle for this geometry                 */

@@ -303,3 +303,23 @@ def test_charged_cell_matches_cpu_spme_plus_background(oracle_mod):
     want = pot - math.pi / (vol * prm["alpha"] ** 2) * q.sum()
     assert rel_max(s, want) < 1e-11
     assert en == pytest.approx(0.5 * float(np.dot(q, want)), rel=1e-11)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_symmetric_pair_storage_matches_row_list(precision, tol):
+    w = W.water()
+    say(f"water: symmetric pair storage against the full row list, {precision}")
+    out = {}
+    for storage in (1, 0):
+        with ew.EwaldEngine(tol=1e-10, r_cut=w["r_cut"], grid=w["grid"], order=w["order"],
+                            precision=precision, pair_storage=storage) as e:
+            e.set_geometry(w["coords"], w["lattice"], species=w["species"],
+                           hubbard_u=w["hubbard_u"], charges=w["charges"])
+            prm = e.parameters()
+            out[storage] = e.shift(w["charges"]) + (prm,)
+    assert out[1][2]["pair_mode"] == 2 and out[0][2]["pair_mode"] == 0
+    # each unordered pair once: half the entries of the row list
+    assert 2 * out[1][2]["n_pairs"] == out[0][2]["n_pairs"]
+    assert out[1][2]["device_bytes"] < out[0][2]["device_bytes"]
+    assert rel_max(out[1][0], out[0][0]) < tol
+    assert out[1][1] == pytest.approx(out[0][1], rel=tol)

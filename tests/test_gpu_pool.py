@@ -65,6 +65,20 @@ def test_pool_matches_single_engine(structures, precision):
         assert modes == {0, 1}
 
 
+def test_pool_symmetric_pair_storage(structures):
+    ws = structures[:5]
+    say("pool with symmetric pair storage against single engines with the row list")
+    with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=6, pair_storage=1) as p:
+        ids = [p.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"])
+               for w in ws]
+        assert {p.parameters(i)["pair_mode"] for i in ids} <= {1, 2}
+        shifts, energies = p.shift(ids, [w["charges"] for w in ws])
+        for i, w in zip(ids, ws):
+            s1, e1, _ = single(w, "fp64")
+            assert rel_max(shifts[i], s1) < TOL["fp64"]
+            assert energies[i] == pytest.approx(e1, rel=TOL["fp64"])
+
+
 def test_pool_fp64_matches_cpu_spme(structures, oracle_mod):
     ws = structures[:4]
     say("pool FP64 against the CPU SPME oracle, 4 structures")
