@@ -714,32 +714,9 @@ extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
   return EW_OK;
 }
 
-static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
-                        hipStream_t s) {
-  const int n = h->n;
+// Launch description of the reciprocal chain of a handle with a geometry.
+static SpmeLaunch make_spme_launch(const ew_handle* h) {
   const int* K = h->prm.K;
-  const bool prof = h->opt.profile != 0;
-  if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
-  HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, h->half ? h->phi_real : nullptr, s));
-  if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, s));
-  if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
-  // the real-space pair sum only needs the sorted charges: it runs on the auxiliary stream,
-  // beside the spreading and FFTs (which leave CUs idle), and joins before the gather
-  const bool fork = !prof && h->aux;
-  const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
-  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
-  auto pairs = [&](hipStream_t st) {
-    return h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real,
-                                      st)
-                   : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
-                                      h->qs, h->phi_real, st);
-  };
-  if (fork) {
-    HIPCHK(h, hipEventRecord(h->fork_ev, s));
-    HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
-    HIPCHK(h, pairs(h->aux));
-    HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
-  }
   SpmeLaunch L{};
   for (int a = 0; a < 3; ++a) L.K[a] = K[a];
   L.plane = h->plane;
@@ -773,6 +750,36 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   L.inf.bsq3 = h->bsq[2];
   L.grid = h->grid;
   L.cgrid = h->cgrid;
+  return L;
+}
+
+static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                        hipStream_t s) {
+  const int n = h->n;
+  const int* K = h->prm.K;
+  const bool prof = h->opt.profile != 0;
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
+  HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, h->half ? h->phi_real : nullptr, s));
+  if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, s));
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
+  // the real-space pair sum only needs the sorted charges: it runs on the auxiliary stream,
+  // beside the spreading and FFTs (which leave CUs idle), and joins before the gather
+  const bool fork = !prof && h->aux;
+  const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
+  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  auto pairs = [&](hipStream_t st) {
+    return h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real,
+                                      st)
+                   : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
+                                      h->qs, h->phi_real, st);
+  };
+  if (fork) {
+    HIPCHK(h, hipEventRecord(h->fork_ev, s));
+    HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
+    HIPCHK(h, pairs(h->aux));
+    HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+  }
+  SpmeLaunch L = make_spme_launch(h);
   if (h->fp32)
     HIPCHK(h, launch_spme<float>(L, s, prof ? &h->ev[2] : nullptr));
   else
@@ -934,6 +941,73 @@ extern "C" int ew_gradients(ew_handle* h, const double* dq, double* grad) {
   CHK(ew_shift(h, dq, shift.data(), nullptr));
   CHK(forces_after_eval(h, f.data()));
   for (size_t k = 0; k < f.size(); ++k) grad[k] -= f[k];
+  return EW_OK;
+}
+
+// Stress of the Ewald energy at charges dq, added into stress[3][3] (row-major, symmetric,
+// Hartree / Bohr^3): sigma = W / V with the virial W = -dE/d(strain), the convention of
+// TCoulomb%addStress (src/dftbp/dftb/coulomb.F90:721-757 with invRStress, 2777-2863):
+// reciprocal part from the SPME spectrum, real-space part over the pairs within r_cut, and for
+// charged cells the neutralising background (E_bg delta_ab).  Short gamma is not included
+// (TShortGamma has its own stress).
+extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_stress: handle is NULL");
+  if (!dq || !stress) return h->fail(EW_ERR_ARGUMENT, "ew_stress: NULL array");
+  if (!h->have_geom) return h->fail(EW_ERR_NO_GEOMETRY, "ew_stress before ew_set_geometry");
+  const int n = h->n;
+  double net = 0.0, absq = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(dq[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+    net += dq[i];
+    absq += std::fabs(dq[i]);
+  }
+  if (!h->opt.charged && std::fabs(net) > kNeutralityTol * absq)
+    return h->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral (or set opts.charged)");
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->dq_io, dq, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(h, launch_gather_charges(n, h->perm, h->dq_io, h->qs, nullptr, s));
+  SpmeLaunch L = make_spme_launch(h);
+  L.forward_only = 1;
+  const long long gh = (long long)(h->prm.K[0] / 2 + 1) * h->prm.K[1] * h->prm.K[2];
+  const size_t nrec = (size_t)((gh + 255) / 256), nreal = (size_t)((n + 255) / 256);
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * 6 * (nrec + nreal)) != hipSuccess) {
+    (void)hipGetLastError();
+    return h->fail(EW_ERR_MEMORY, "ew_stress: allocation failed");
+  }
+  int nb1 = 0, nb2 = 0;
+  hipError_t e = h->fp32 ? launch_spme<float>(L, s, nullptr) : launch_spme<double>(L, s, nullptr);
+  if (e == hipSuccess)
+    e = h->fp32 ? launch_recip_virial<float>(L, part, &nb1, s)
+                : launch_recip_virial<double>(L, part, &nb1, s);
+  if (e == hipSuccess)
+    e = launch_real_virial(n, h->frac, h->tile_of, h->tile_start, h->qs, h->pa, h->pair_mode == 1,
+                           part + 6 * nrec, &nb2, s);
+  std::vector<double> hp(6 * (nrec + nreal), 0.0);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(part);
+  if (e != hipSuccess)
+    return h->fail(EW_ERR_DEVICE, std::string("ew_stress: ") + hipGetErrorString(e));
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb1; ++b)
+    for (int c = 0; c < 6; ++c) w[c] += hp[6 * (size_t)b + c];
+  for (int b = 0; b < nb2; ++b)
+    for (int c = 0; c < 6; ++c) w[c] += hp[6 * (nrec + (size_t)b) + c];
+  if (h->opt.charged) {
+    const double ebg = 0.5 * h->background * net * net;
+    w[0] += ebg;
+    w[1] += ebg;
+    w[2] += ebg;
+  }
+  const double iv = 1.0 / h->cell.volume;
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  for (int c = 0; c < 6; ++c) {
+    stress[3 * ia[c] + ib[c]] += w[c] * iv;
+    if (ia[c] != ib[c]) stress[3 * ib[c] + ia[c]] += w[c] * iv;
+  }
   return EW_OK;
 }
 

@@ -1981,6 +1981,168 @@ template hipError_t launch_group<double>(const GroupLaunch&, hipStream_t);
 template hipError_t launch_group<float>(const GroupLaunch&, hipStream_t);
 
 // ----------------------------------------------------------------------------------------
+// virial (stress)
+// ----------------------------------------------------------------------------------------
+
+// Sums 6 per-thread components over the workgroup (256 threads) into part[blockIdx.x * 6 ..].
+__device__ __forceinline__ void block_sum6(double (&v)[6], double* __restrict__ part) {
+  __shared__ double red[6][4];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double w = wave_sum(v[c]);
+    if ((threadIdx.x & 63) == 0) red[c][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    part[(size_t)blockIdx.x * 6 + c] = red[c][0] + red[c][1] + red[c][2] + red[c][3];
+  }
+}
+
+// Reciprocal virial of SPME (Essmann et al. 1995, eq. 2.8): over the full spectrum,
+// W_ab = sum_m E_m (delta_ab - 2 (1 + pi^2 m^2 / alpha^2) m_a m_b / m^2), E_m = w(m) |F(Q)(m)|^2 / 2,
+// with the influence function w of the evaluation; the half spectrum counts kx > 0 twice.  The
+// modes the evaluation drops (m = 0, Nyquist planes of even meshes) are dropped here too.
+template <typename R>
+__global__ __launch_bounds__(256) void k_recip_virial(const cx<R>* __restrict__ cg,
+                                                      InfluenceArgs inf,
+                                                      double* __restrict__ part) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)inf.Kh * inf.K[1] * inf.K[2];
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (id < total) {
+    const int kx = int(id % inf.Kh);
+    const int ky = int((id / inf.Kh) % inf.K[1]);
+    const int kz = int(id / ((long long)inf.Kh * inf.K[1]));
+    const bool nyq = (!(inf.K[0] & 1) && kx == inf.K[0] / 2) ||
+                     (!(inf.K[1] & 1) && ky == inf.K[1] / 2) ||
+                     (!(inf.K[2] & 1) && kz == inf.K[2] / 2);
+    if (!nyq && (kx | ky | kz)) {
+      const int m1 = kx;
+      const int m2 = ky <= (inf.K[1] - 1) / 2 ? ky : ky - inf.K[1];
+      const int m3 = kz <= (inf.K[2] - 1) / 2 ? kz : kz - inf.K[2];
+      const double mx = m1 * inf.rec[0] + m2 * inf.rec[3] + m3 * inf.rec[6];
+      const double my = m1 * inf.rec[1] + m2 * inf.rec[4] + m3 * inf.rec[7];
+      const double mz = m1 * inf.rec[2] + m2 * inf.rec[5] + m3 * inf.rec[8];
+      const double msq = mx * mx + my * my + mz * mz;
+      const double w = inf.pref * exp(-inf.a2 * msq) / msq * inf.bsq1[kx] * inf.bsq2[ky] *
+                       inf.bsq3[kz];
+      const cx<R> z = cg[id];
+      const double e = (kx == 0 ? 0.5 : 1.0) * w * (double(z.x) * z.x + double(z.y) * z.y);
+      const double f = 2.0 * (1.0 + inf.a2 * msq) / msq;
+      v[0] = e * (1.0 - f * mx * mx);
+      v[1] = e * (1.0 - f * my * my);
+      v[2] = e * (1.0 - f * mz * mz);
+      v[3] = -e * f * mx * my;
+      v[4] = -e * f * mx * mz;
+      v[5] = -e * f * my * mz;
+    }
+  }
+  block_sum6(v, part);
+}
+
+// Real-space virial W_ab = 1/2 sum over ordered pairs (i, j + images) within r_cut of
+// -q_i q_j phi'(r) x_a x_b / r, phi(r) = erfc(alpha r) / r, x = r_i - r_j (the Ewald real-space
+// energy only, as the Coulomb stress of the reference: coulomb.F90:2861-2900).
+template <bool DENSE>
+__global__ __launch_bounds__(256) void k_real_virial(int n, const double* __restrict__ frac,
+                                                     const int* __restrict__ tile_of,
+                                                     const int* __restrict__ tile_start,
+                                                     const double* __restrict__ q, PairArgs g,
+                                                     double* __restrict__ part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const double fi0 = frac[3 * i], fi1 = frac[3 * i + 1], fi2 = frac[3 * i + 2];
+    const double two_a_sqrtpi = 2.0 * g.alpha / kSqrtPi;
+    auto add = [&](double s0, double s1, double s2, int j) {
+      const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+      const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+      const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+      const double r2 = x * x + y * y + z * z;
+      if (r2 > g.rc2 || r2 < kMinSeparationSq) return;
+      const double r = sqrt(r2);
+      // -phi'(r) / r
+      const double c = q[j] * (two_a_sqrtpi * exp(-g.alpha * g.alpha * r2) + erfc(g.alpha * r) / r) / r2;
+      v[0] += c * x * x;
+      v[1] += c * y * y;
+      v[2] += c * z * z;
+      v[3] += c * x * y;
+      v[4] += c * x * z;
+      v[5] += c * y * z;
+    };
+    if (DENSE) {
+      for (int j = 0; j < n; ++j) {
+        const double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
+        for (int a = -g.D[0]; a <= g.D[0]; ++a)
+          for (int b = -g.D[1]; b <= g.D[1]; ++b)
+            for (int e = -g.D[2]; e <= g.D[2]; ++e) {
+              if (i == j && a == 0 && b == 0 && e == 0) continue;
+              add(s0 + a, s1 + b, s2 + e, j);
+            }
+      }
+    } else {
+      int t = tile_of[i];
+      const int t1 = t % g.nt[0];
+      t /= g.nt[0];
+      const int t2 = t % g.nt[1], t3 = t / g.nt[1];
+      const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
+      const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
+      const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
+      for (int c3 = 0; c3 < n3; ++c3) {
+        const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
+        for (int c2 = 0; c2 < n2; ++c2) {
+          const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
+          for (int c1 = 0; c1 < n1; ++c1) {
+            const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
+            const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
+            for (int j = tile_start[tt]; j < tile_start[tt + 1]; ++j) {
+              if (j == i) continue;
+              double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
+              s0 -= rint(s0);
+              s1 -= rint(s1);
+              s2 -= rint(s2);
+              add(s0, s1, s2, j);
+            }
+          }
+        }
+      }
+    }
+    const double h = 0.5 * q[i];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] *= h;
+  }
+  block_sum6(v, part);
+}
+
+template <typename R>
+hipError_t launch_recip_virial(const SpmeLaunch& L, double* partials, int* nparts,
+                               hipStream_t st) {
+  const long long total = (long long)(L.K[0] / 2 + 1) * L.K[1] * L.K[2];
+  const int nb = int((total + 255) / 256);
+  hipLaunchKernelGGL(k_recip_virial<R>, dim3(nb), dim3(256), 0, st,
+                     static_cast<const cx<R>*>(L.cgrid), L.inf, partials);
+  *nparts = nb;
+  return hipGetLastError();
+}
+template hipError_t launch_recip_virial<double>(const SpmeLaunch&, double*, int*, hipStream_t);
+template hipError_t launch_recip_virial<float>(const SpmeLaunch&, double*, int*, hipStream_t);
+
+hipError_t launch_real_virial(int n, const double* frac, const int* tile_of,
+                              const int* tile_start, const double* q, const PairArgs& pa,
+                              bool dense, double* partials, int* nparts, hipStream_t st) {
+  const int nb = (n + 255) / 256;
+  if (dense)
+    hipLaunchKernelGGL(k_real_virial<true>, dim3(nb), dim3(256), 0, st, n, frac, tile_of,
+                       tile_start, q, pa, partials);
+  else
+    hipLaunchKernelGGL(k_real_virial<false>, dim3(nb), dim3(256), 0, st, n, frac, tile_of,
+                       tile_start, q, pa, partials);
+  *nparts = nb;
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------------------
 
@@ -2066,6 +2228,12 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
   }
   mark(2);
   const int nz = Kh * K2;
+  if (L.forward_only) {
+    hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
+                       dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2,
+                       make_fastdiv(Kh * K2), 0LL, (long long)Kh * K2, L.pz, tw3, L.inf);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL((k_fft_lines<R, 2>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
                      dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2, make_fastdiv(Kh * K2), 0LL,
                      (long long)Kh * K2, L.pz, tw3, L.inf);

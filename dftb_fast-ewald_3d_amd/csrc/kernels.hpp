@@ -131,6 +131,7 @@ hipError_t prepare_plane_kernels();
 struct SpmeLaunch {
   int K[3];
   int plane;  // 1: fused x-y plane transforms (plane_fits), 0: separate x and y passes
+  int forward_only;  // 1: stop after the forward transform (cgrid = spectrum of the charges)
   const int4* k0;
   const void* theta;
   const double* q;
@@ -281,6 +282,17 @@ hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int
 hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
                             const int* tile_start, const PairArgs& g, long long* row_ptr,
                             int* col, void* coef, bool fp32, int* err, hipStream_t st);
+// Virial W_ab = -dE/d(strain_ab) of the Ewald energy, as per-block partials of 6 components
+// (xx, yy, zz, xy, xz, yz) that the caller sums: the reciprocal part from the forward spectrum
+// left in L.cgrid by a forward_only launch_spme (nparts = number of blocks written), the
+// real-space part over the pairs within r_cut.
+template <typename R>
+hipError_t launch_recip_virial(const SpmeLaunch& L, double* partials, int* nparts,
+                               hipStream_t st);
+hipError_t launch_real_virial(int n, const double* frac, const int* tile_of,
+                              const int* tile_start, const double* q, const PairArgs& pa,
+                              bool dense, double* partials, int* nparts, hipStream_t st);
+
 // Reciprocal force (overwrites force, caller order) then the real-space Ewald force (adds).
 template <typename R>
 hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* grid,

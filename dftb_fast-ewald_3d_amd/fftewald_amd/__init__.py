@@ -36,7 +36,7 @@ PRECISION_MIXED = 1
 
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
-            "ew_shift", "ew_shift_device", "ew_gradients", "ew_phase_times", "ew_potential_force",
+            "ew_shift", "ew_shift_device", "ew_gradients", "ew_stress", "ew_phase_times", "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
             "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync",
@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
         L.ew_shift.argtypes = [H, _D1, _D1, C.POINTER(C.c_double)]
         L.ew_shift_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ew_gradients.argtypes = [H, _D1, _D2]
+        L.ew_stress.argtypes = [H, _D1, _D2]
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
@@ -279,6 +280,13 @@ class EwaldEngine:
         g = np.zeros((self.n, 3)) if grad is None else np.array(grad, dtype=np.float64, copy=True)
         _check(lib().ew_gradients(self._h, dq, g), self._h)
         return g
+
+    def stress(self, dq, stress=None):
+        """stress + the Ewald stress (3, 3) at charges dq (TCoulomb%addStress convention)."""
+        dq = _f64(dq, 1)
+        st = np.zeros((3, 3)) if stress is None else np.array(stress, dtype=np.float64, copy=True)
+        _check(lib().ew_stress(self._h, dq, st), self._h)
+        return st
 
     def phase_times(self, reset: bool = False):
         """({phase: ms}, calls) accumulated since the last reset (needs profile=True)."""

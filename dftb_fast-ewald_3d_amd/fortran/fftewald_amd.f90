@@ -19,6 +19,7 @@ module fftewald_amd
    public :: ew_options, ew_parameters, TEwaldEngine
    public :: ewald_potential_force, ewald_parameters, ewald_grid_memory
    public :: engine_create, engine_set_geometry, engine_shift, engine_destroy, engine_error
+   public :: engine_add_gradients, engine_add_stress
    public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED
 
    integer(c_int), parameter :: EW_OK = 0
@@ -97,6 +98,20 @@ module fftewald_amd
          real(c_double), intent(out) :: shift(*)
          real(c_double), intent(out) :: energy
       end function c_ew_shift
+
+      integer(c_int) function c_ew_gradients(h, dq, grad) bind(C, name="ew_gradients")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: dq(*)
+         real(c_double), intent(inout) :: grad(3, *)
+      end function c_ew_gradients
+
+      integer(c_int) function c_ew_stress(h, dq, stress) bind(C, name="ew_stress")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: dq(*)
+         real(c_double), intent(inout) :: stress(3, 3)
+      end function c_ew_stress
 
       integer(c_int) function c_ew_potential_force(n, coords, charges, latvecs, opts, pot, force) &
          bind(C, name="ew_potential_force")
@@ -287,6 +302,27 @@ contains
    end function engine_set_geometry
 
    !> One SCC iteration (updateShifts, coulomb.F90:786-880): shift = phi(dq), energy.
+   !> TCoulomb%addGradients (coulomb.F90:705-708): gradient(3, N) += dU/dx of the Ewald energy
+   function engine_add_gradients(this, deltaQAtom, gradient) result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: deltaQAtom(:)
+      real(c_double), intent(inout) :: gradient(:, :)
+      integer(c_int) :: st
+
+      st = c_ew_gradients(this%handle, deltaQAtom, gradient)
+   end function engine_add_gradients
+
+   !> TCoulomb%addStress (coulomb.F90:721-757): stress(3, 3) += the Ewald stress (symmetric, so
+   !> the row-major C layout and Fortran's column-major one agree)
+   function engine_add_stress(this, deltaQAtom, stress) result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: deltaQAtom(:)
+      real(c_double), intent(inout) :: stress(3, 3)
+      integer(c_int) :: st
+
+      st = c_ew_stress(this%handle, deltaQAtom, stress)
+   end function engine_add_stress
+
    function engine_shift(this, deltaQAtom, shift, energy) result(st)
       type(TEwaldEngine), intent(inout) :: this
       real(c_double), intent(in) :: deltaQAtom(:)

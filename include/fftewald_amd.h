@@ -149,6 +149,12 @@ int ew_gradients(ew_handle *h, const double *dq, double *grad);
 #define EW_NUM_PHASES 9
 int ew_phase_times(ew_handle *h, double *ms, long long *calls, int reset);
 
+/* TCoulomb%addStress (src/dftbp/dftb/coulomb.F90:721-757, invRStress 2777-2863): adds the Ewald
+ * stress at charges dq into stress[3][3] (row-major, Hartree/Bohr^3): sigma = W/V with the
+ * virial W = -dE/d(strain) -- SPME reciprocal virial, real-space pair virial within r_cut and,
+ * for charged cells (opts.charged), the neutralising background.  Short gamma not included. */
+int ew_stress(ew_handle *h, const double *dq, double *stress);
+
 /* ---- one-shot calls, the reference library's own entry points ---- */
 
 /* ewald_potential_force (fe/fft/ewald_fft_3d.f90:36-134): pot[i] = phi(x_i) and, when force

@@ -323,3 +323,75 @@ def test_symmetric_pair_storage_matches_row_list(precision, tol):
     assert out[1][2]["device_bytes"] < out[0][2]["device_bytes"]
     assert rel_max(out[1][0], out[0][0]) < tol
     assert out[1][1] == pytest.approx(out[0][1], rel=tol)
+
+
+# ---- stress (TCoulomb%addStress convention: sigma = W / V, W = -dE/d(strain)) ----
+# Pins: central differences of the reference's own direct Ewald energy (the oracle's restatement,
+# an independent method) under symmetric strains, and for a charged cell the closed form
+# trace(sigma) = E / V of a single charge in a cubic cell (E = q^2 xi / 2L scales as 1/L).
+
+def _strained(coords, lat, a, b, h):
+    eps = np.zeros((3, 3))
+    eps[a, b] += 0.5 * h
+    eps[b, a] += 0.5 * h
+    m = np.eye(3) + eps
+    return coords @ m.T, lat @ m.T
+
+
+def test_stress_matches_strain_derivative_of_direct_energy(oracle_mod):
+    g = load_golden("ref_triclinic.npz")
+    q = g["charges"]
+    say("stress: 300 charges, sheared cell, against strain derivatives of the direct energy")
+    with ew.EwaldEngine(tol=1e-12) as e:
+        e.set_geometry(g["coords"], g["lattice"], charges=q)
+        st = e.stress(q)
+    vol = abs(np.linalg.det(g["lattice"]))
+    h = 1e-5
+    want = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(a, 3):
+            en = []
+            for sgn in (1, -1):
+                c, l = _strained(g["coords"], g["lattice"], a, b, sgn * h)
+                pot, _, _ = oracle_mod.direct_ewald(c, q, l, tol=1e-13)
+                en.append(0.5 * float(np.dot(q, pot)))
+            want[a, b] = want[b, a] = -(en[0] - en[1]) / (2 * h) / vol
+    assert np.allclose(st, st.T, rtol=0, atol=1e-14)
+    assert np.abs(st - want).max() / np.abs(want).max() < 1e-6
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-7), ("mixed", 1e-4)])
+def test_stress_is_strain_derivative_of_engine_energy_linked(precision, tol):
+    w = W.water(n_side=6)
+    say(f"stress: water 648 atoms (linked pairs), strain derivatives of the SPME energy, {precision}")
+    opts = dict(tol=1e-12, alpha=0.3, r_cut=w["r_cut"], grid=(36, 36, 36), order=8,
+                precision=precision)
+    with ew.EwaldEngine(**opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        assert e.parameters()["pair_mode"] == 0
+        st = e.stress(w["charges"])
+    vol = abs(np.linalg.det(w["lattice"]))
+    h = 1e-5 if precision == "fp64" else 1e-3
+    want = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(a, 3):
+            en = []
+            for sgn in (1, -1):
+                c, l = _strained(w["coords"], w["lattice"], a, b, sgn * h)
+                with ew.EwaldEngine(**opts) as e:
+                    e.set_geometry(c, l, charges=w["charges"])
+                    en.append(e.shift(w["charges"])[1])
+            want[a, b] = want[b, a] = -(en[0] - en[1]) / (2 * h) / vol
+    assert np.abs(st - want).max() / np.abs(want).max() < tol
+
+
+def test_stress_charged_cell_closed_form():
+    L, q = 20.0, 0.7
+    say("stress: one charge in a cubic cell with the neutralising background")
+    with ew.EwaldEngine(tol=1e-12, charged=True) as e:
+        e.set_geometry(np.array([[1.3, 2.1, 0.4]]), np.eye(3) * L, charges=np.array([q]))
+        st = e.stress(np.array([q]))
+    energy = 0.5 * q * q * XI_SC / L
+    assert np.trace(st) == pytest.approx(energy / L ** 3, rel=1e-9)
+    assert np.allclose(np.diag(st), np.trace(st) / 3, rtol=1e-9, atol=0)
+    assert np.abs(st - np.diag(np.diag(st))).max() < 1e-12 * abs(np.trace(st))
