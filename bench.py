@@ -92,6 +92,23 @@ def timed_run(e, torch, dqs, out, en, steps, warmup):
     return t0.elapsed_time(t1) / steps
 
 
+def timed_multi(e, torch, nvec, dq, out, en, steps, warmup):
+    """ms per ew_shift_multi_device call of nvec vectors (device-resident [nvec][n] arrays)."""
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        e.shift_multi_device(nvec, dq.data_ptr(), out.data_ptr(), en.data_ptr(), sp)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        e.shift_multi_device(nvec, dq.data_ptr(), out.data_ptr(), en.data_ptr(), sp)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps
+
+
 def latency_run(e, torch, dqs, out, en, steps):
     """Per-evaluation latency with a synchronisation after each (p50/p99)."""
     stream = torch.cuda.current_stream()
@@ -406,7 +423,10 @@ def main():
                     help="serving: offered request rates")
     ap.add_argument("--seconds", type=float, default=2.0, help="serving: seconds per rate")
     ap.add_argument("--max-batch", type=int, default=64, help="serving: structures per launch")
+    ap.add_argument("--vectors", default="2,4,8",
+                    help="triclinic: charge vectors per multi-vector call (empty: skip)")
     args = ap.parse_args()
+    args.vectors = [int(v) for v in args.vectors.split(",") if v.strip()]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
               "(structures shard across GPUs by footprint, see DESIGN.md)", file=sys.stderr)
@@ -432,7 +452,7 @@ def main():
     n = len(w["charges"])
     rng = np.random.default_rng(12345)
     qs = [w["charges"]]
-    for _ in range(3):
+    for _ in range(max(3, max(args.vectors, default=0) - 1)):
         qs.append(W.scc_update(qs[-1], rng))
     dev = torch.device("cuda:0")
     dqs = [torch.tensor(q, dtype=torch.float64, device=dev) for q in qs]
@@ -458,6 +478,22 @@ def main():
             for i in range(10):
                 e.shift(qs[i % 4])
             host_ms = (time.perf_counter() - t) * 100.0
+            # several charge vectors through the same geometry per call
+            multi = {}
+            for nv in args.vectors:
+                dqm = torch.stack(dqs[:nv]).contiguous()
+                outm = torch.empty_like(dqm)
+                enm = torch.empty(nv, dtype=torch.float64, device=dev)
+                msm = timed_multi(e, torch, nv, dqm, outm, enm, max(1, args.steps // 2),
+                                  args.warmup)
+                multi[nv] = {"ms_per_call": round(msm, 5), "evals_per_s": round(nv * 1e3 / msm, 1)}
+                if prec == "mixed" and "fp64" in shifts:
+                    s0 = shifts["fp64"][0]
+                    multi[nv]["rel_err_vs_fp64"] = float(
+                        np.abs(outm[0].cpu().numpy() - s0).max() / np.abs(s0).max())
+                print(f"[bench] {prec} multi-vector x{nv}: {msm:.4f} ms/call, "
+                      f"{nv * 1e3 / msm:.1f} evals/s", file=sys.stderr, flush=True)
+                del dqm, outm, enm
         with ew.EwaldEngine(precision=prec, profile=True, **opts) as e:
             e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
             sp = torch.cuda.current_stream().cuda_stream
@@ -476,6 +512,7 @@ def main():
             ph[k] = {"ms": round(avg_ms, 5), "bytes": pb[k],
                      "GBs": round(pb[k] / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
+                     "multi": multi,
                      "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph,
                      "host_api_ms": host_ms}
         print(f"[bench] {prec}: {ms:.4f} ms/eval, {1e3 / ms:.1f} evals/s, geometry {geom_s:.2f} s,"
@@ -528,6 +565,10 @@ def main():
         "mixed_evals_per_s": round(res["mixed"]["evals_per_s"], 2),
         "mixed_over_fp64": round(res["mixed"]["evals_per_s"] / res["fp64"]["evals_per_s"], 3),
         "mixed_vs_fp64_rel_err": {"shift_max": err_shift, "energy": err_energy},
+        "multi_vector": {k: v["multi"] for k, v in res.items()},
+        "best_mixed_multi_over_plain_fp64": round(
+            max(m["evals_per_s"] for m in res["mixed"]["multi"].values()) /
+            res["fp64"]["evals_per_s"], 3) if res["mixed"]["multi"] else None,
         "geometry_setup_s": {k: round(v["geometry_s"], 3) for k, v in res.items()},
         "host_arrays_ms_per_eval": {k: round(v["host_api_ms"], 4) for k, v in res.items()},
         "phases": {k: v["phases"] for k, v in res.items()},

@@ -17,6 +17,7 @@
 #include "../../include/fftewald_amd.h"
 #include "engine.hpp"
 #include "handle.hpp"
+#include "group.hpp"
 #include "kernels.hpp"
 
 using namespace ewamd;
@@ -907,6 +908,230 @@ extern "C" int ew_shift(ew_handle* h, const double* dq, double* shift, double* e
   HIPCHK(h, hipMemcpyAsync(&e, h->energy, sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(h, hipStreamSynchronize(s));
   if (energy) *energy = e;
+  return EW_OK;
+}
+
+// ---- multi-vector evaluation: several charge vectors through one geometry ----
+
+// Per-vector buffers and descriptors for at least nvec vectors; they stay until the geometry
+// changes.  Every vector has its own charges, potentials, meshes and partials and shares the
+// geometry's sort, weights, pair list and plans.
+static int ensure_views(ew_handle* h, int nvec) {
+  auto& V = h->views;
+  if (V.cap >= nvec) return EW_OK;
+  // the smaller buffers go: nothing queued may still read them
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->aux) HIPCHK(h, hipStreamSynchronize(h->aux));
+  for (auto& g : h->mgraphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipFree(g.staging);
+  }
+  h->mgraphs.clear();
+  for (void* p : {(void*)V.qs, (void*)V.phi, (void*)V.epart, (void*)V.qsum, V.grid, V.cgrid,
+                  V.region, (void*)V.dq, (void*)V.shift, (void*)V.energy, (void*)V.desc})
+    if (p) h->free_buf(p);
+  const void* staging = V.staging;
+  V = ew_handle::Views{};
+  V.staging = const_cast<void*>(staging);
+  h->views_host.clear();
+  const int n = h->n;
+  const int* K = h->prm.K;
+  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  const size_t ngrid = (size_t)K[0] * K[1] * K[2];
+  const size_t ncgrid = (size_t)(K[0] / 2 + 1) * K[1] * K[2];
+  const size_t nparts = std::max<size_t>((size_t)(n + 255) / 256,
+                                         (size_t)h->nt[0] * h->nt[1] * h->nt[2]);
+  const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
+  const size_t rstride = h->region.scratch ? nblk * (size_t)h->region.region_stride * rs : 0;
+  const size_t cap = (size_t)nvec;
+  unsigned char *g, *cg, *rg = nullptr;
+  CHK(h->alloc(&V.qs, cap * n));
+  CHK(h->alloc(&V.phi, cap * n));
+  CHK(h->alloc(&V.epart, cap * nparts));
+  if (h->qsum) CHK(h->alloc(&V.qsum, cap));
+  CHK(h->alloc(&g, cap * ngrid * rs));
+  CHK(h->alloc(&cg, cap * ncgrid * 2 * rs));
+  if (rstride) CHK(h->alloc(&rg, cap * rstride));
+  V.grid = g;
+  V.cgrid = cg;
+  V.region = rg;
+  CHK(h->alloc(&V.dq, cap * n));
+  CHK(h->alloc(&V.shift, cap * n));
+  CHK(h->alloc(&V.energy, cap));
+  CHK(h->alloc(&V.desc, cap));
+  std::vector<StructDesc> d(cap);
+  for (size_t v = 0; v < cap; ++v) {
+    fill_desc(h, &d[v]);
+    d[v].qs = V.qs + v * n;
+    d[v].phi_real = V.phi + v * n;
+    d[v].epart = V.epart + v * nparts;
+    d[v].ga.qsum = V.qsum ? V.qsum + v : nullptr;
+    d[v].grid = g + v * ngrid * rs;
+    d[v].cgrid = cg + v * ncgrid * 2 * rs;
+    if (rg) d[v].region.scratch = rg + v * rstride;
+  }
+  HIPCHK(h, hipMemcpyAsync(V.desc, d.data(), sizeof(StructDesc) * cap, hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->views_host = std::move(d);
+  V.cap = nvec;
+  return EW_OK;
+}
+
+// The grouped launch of nvec vectors over a device copy (at dev) of S's words.  With the row
+// list the pair sums of all vectors are one multi-vector sum; the symmetric storage keeps one
+// pair-sum block set per vector.
+static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, const void* dev) {
+  GroupLaunch G = group_launch_of(S, h->views.desc, dev, h->prm.order);
+  if (!h->half) {
+    G.has_pm = 1;
+    G.pm = PairMulti{h->n,
+                     nvec,
+                     h->row_ptr,
+                     h->col16 ? static_cast<const void*>(h->col16) : h->col,
+                     h->col16 != nullptr,
+                     h->coef,
+                     h->views.qs,
+                     h->views.phi};
+  }
+  G.aux = h->aux;
+  G.fork_ev = h->fork_ev;
+  G.join_ev = h->join_ev;
+  return G;
+}
+
+static int stage_multi(ew_handle* h, int nvec, const double* d_dq, double* d_shift,
+                       double* d_energy, Staged* S) {
+  const size_t n = (size_t)h->n;
+  std::vector<IoSlot> io((size_t)nvec);
+  for (int v = 0; v < nvec; ++v)
+    io[v] = IoSlot{d_dq + v * n, d_shift + v * n, d_energy + v, v, 0};
+  if (!stage_group(h->views_host.data(), io.data(), nvec, h->fp32, !h->half, S))
+    return h->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
+  return EW_OK;
+}
+
+static int launch_multi(ew_handle* h, const GroupLaunch& G, hipStream_t s) {
+  HIPCHK(h, h->fp32 ? launch_group<float>(G, s) : launch_group<double>(G, s));
+  return EW_OK;
+}
+
+// nvec evaluations on stream s: captured once per (nvec, buffers) and replayed as a graph
+// unless eager or profiling.
+static int run_multi(ew_handle* h, int nvec, const double* d_dq, double* d_shift,
+                     double* d_energy, hipStream_t s) {
+  CHK(ensure_views(h, nvec));
+  const bool eager = h->opt.eager || h->opt.profile;
+  if (!eager)
+    for (auto& g : h->mgraphs)
+      if (g.nvec == nvec && g.dq == d_dq && g.shift == d_shift && g.energy == d_energy) {
+        HIPCHK(h, hipGraphLaunch(g.exec, s));
+        return EW_OK;
+      }
+  Staged S;
+  CHK(stage_multi(h, nvec, d_dq, d_shift, d_energy, &S));
+  const size_t bytes = S.words.size() * sizeof(int);
+  if (eager) {
+    if (h->views_staging_cap < bytes) {
+      if (h->views.staging) h->free_buf(h->views.staging);
+      h->views.staging = nullptr;
+      h->views_staging_cap = 0;
+      unsigned char* p;
+      CHK(h->alloc(&p, bytes));
+      h->views.staging = p;
+      h->views_staging_cap = bytes;
+    }
+    // a pageable source: the copy has read the host image when the call returns
+    HIPCHK(h, hipMemcpyAsync(h->views.staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
+    return launch_multi(h, multi_launch(h, nvec, S, h->views.staging), s);
+  }
+  if (h->mgraphs.size() >= 8) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (auto& g : h->mgraphs) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipFree(g.staging);
+    }
+    h->mgraphs.clear();
+  }
+  ew_handle::CapturedMulti c{nvec, d_dq, d_shift, d_energy, nullptr, nullptr};
+  if (hipMalloc(&c.staging, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return h->fail(EW_ERR_MEMORY, "multi-vector staging allocation failed");
+  }
+  if (hipMemcpy(c.staging, S.words.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(c.staging);
+    return h->fail(EW_ERR_DEVICE, "multi-vector staging copy failed");
+  }
+  const GroupLaunch G = multi_launch(h, nvec, S, c.staging);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int st = launch_multi(h, G, h->stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+  if (st != EW_OK || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(c.staging);
+    if (st != EW_OK) return st;
+    return h->fail(EW_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ec));
+  }
+  const hipError_t ei = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    (void)hipFree(c.staging);
+    return h->fail(EW_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(ei));
+  }
+  h->mgraphs.push_back(c);
+  HIPCHK(h, hipGraphLaunch(c.exec, s));
+  return EW_OK;
+}
+
+extern "C" int ew_shift_multi_device(ew_handle* h, int nvec, const double* d_dq, double* d_shift,
+                                     double* d_energy, void* stream) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_shift_multi_device: handle is NULL");
+  if (!h->have_geom)
+    return h->fail(EW_ERR_NO_GEOMETRY, "ew_shift_multi_device before ew_set_geometry");
+  if (!d_dq || !d_shift) return h->fail(EW_ERR_ARGUMENT, "ew_shift_multi_device: NULL array");
+  if (nvec < 1 || nvec > EW_MAX_VECTORS)
+    return h->fail(EW_ERR_ARGUMENT, "ew_shift_multi_device: nvec outside 1.." +
+                                        std::to_string(EW_MAX_VECTORS));
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  CHK(ensure_views(h, nvec));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  return run_multi(h, nvec, d_dq, d_shift, d_energy ? d_energy : h->views.energy, s);
+}
+
+extern "C" int ew_shift_multi(ew_handle* h, int nvec, const double* dq, double* shift,
+                              double* energy) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_shift_multi: handle is NULL");
+  if (!h->have_geom) return h->fail(EW_ERR_NO_GEOMETRY, "ew_shift_multi before ew_set_geometry");
+  if (!dq || !shift) return h->fail(EW_ERR_ARGUMENT, "ew_shift_multi: NULL array");
+  if (nvec < 1 || nvec > EW_MAX_VECTORS)
+    return h->fail(EW_ERR_ARGUMENT,
+                   "ew_shift_multi: nvec outside 1.." + std::to_string(EW_MAX_VECTORS));
+  const size_t n = (size_t)h->n;
+  for (int v = 0; v < nvec; ++v) {
+    double net = 0.0, absq = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      const double q = dq[v * n + i];
+      if (!std::isfinite(q)) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+      net += q;
+      absq += std::fabs(q);
+    }
+    if (!h->opt.charged && std::fabs(net) > kNeutralityTol * absq)
+      return h->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral (or set opts.charged)");
+  }
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  CHK(ensure_views(h, nvec));
+  hipStream_t s = h->stream;
+  auto& V = h->views;
+  HIPCHK(h, hipMemcpyAsync(V.dq, dq, sizeof(double) * n * nvec, hipMemcpyHostToDevice, s));
+  CHK(run_multi(h, nvec, V.dq, V.shift, V.energy, s));
+  HIPCHK(h, hipMemcpyAsync(shift, V.shift, sizeof(double) * n * nvec, hipMemcpyDeviceToHost, s));
+  std::vector<double> e((size_t)nvec);
+  HIPCHK(h, hipMemcpyAsync(e.data(), V.energy, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  if (energy)
+    for (int v = 0; v < nvec; ++v) energy[v] = e[v];
   return EW_OK;
 }
 

@@ -93,9 +93,44 @@ struct ew_handle {
     hipGraphExec_t exec;
   };
   std::vector<Captured> graphs;
+
+  // Multi-vector evaluation (several charge vectors through this geometry in one grouped
+  // launch): per-vector charge, potential, mesh and partial buffers, allocated on first use.
+  struct Views {
+    int cap = 0;                 // vectors the buffers hold
+    double* qs = nullptr;        // [cap][n] sorted charges (contiguous: the pair sum reads them so)
+    double* phi = nullptr;       // [cap][n] real-space potentials
+    double* epart = nullptr;     // [cap][nparts]
+    double* qsum = nullptr;      // [cap] (opt.charged)
+    void* grid = nullptr;        // [cap][K0 K1 K2]
+    void* cgrid = nullptr;       // [cap][Kh K1 K2] complex
+    void* region = nullptr;      // [cap][blocks][region_stride] privatised spreading
+    double* dq = nullptr;        // [cap][n] host-array staging
+    double* shift = nullptr;     // [cap][n]
+    double* energy = nullptr;    // [cap]
+    ewamd::StructDesc* desc = nullptr;  // [cap] device descriptors, one per vector
+    void* staging = nullptr;     // maps and IoSlots of eager launches
+  } views;
+  std::vector<ewamd::StructDesc> views_host;  // host image of views.desc
+  size_t views_staging_cap = 0;
+  struct CapturedMulti {
+    int nvec;
+    const double* dq;
+    double* shift;
+    double* energy;
+    hipGraphExec_t exec;
+    void* staging;  // maps and IoSlots the graph's kernels read
+  };
+  std::vector<CapturedMulti> mgraphs;
+
   void drop_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
+    for (auto& g : mgraphs) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipFree(g.staging);
+    }
+    mgraphs.clear();
   }
 
   // per-phase timing (opt.profile)
@@ -117,6 +152,9 @@ struct ew_handle {
       if (b.p) (void)hipFree(b.p);
     bufs.clear();
     device_bytes = 0;
+    views = Views{};
+    views_host.clear();
+    views_staging_cap = 0;
     have_geom = false;
   }
   void free_buf(void* p) {

@@ -1058,6 +1058,70 @@ __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __res
   k_pair_spmv_body<CT, IT>(n, row_ptr, col, coef, q, out, blockIdx.x);
 }
 
+// Multi-vector pair sum: one wave per row, the row's coefficients and offsets loaded once for
+// the NV charge vectors (the same sweep as k_pair_spmv).
+template <typename CT, typename IT, int NV>
+__global__ __launch_bounds__(256) void k_pair_spmm(PairMulti pm) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int n = pm.n;
+  if (row >= n) return;  // uniform across the wave
+  const long long* __restrict__ row_ptr = pm.row_ptr;
+  const IT* __restrict__ col = static_cast<const IT*>(pm.col);
+  const CT* __restrict__ coef = static_cast<const CT*>(pm.coef);
+  const double* __restrict__ q = pm.q;
+  const long long a = row_ptr[row], b = row_ptr[row + 1];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (long long g = a + lane; g < b; g += 512) {
+    IT c[8];
+    CT w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long k = g + 64 * u;
+      c[u] = k < b ? col[k] : IT(0);
+      w[u] = k < b ? coef[k] : CT(0);
+    }
+    int j[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) j[u] = g + 64 * u < b ? partner(row, c[u], n) : -1;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double* __restrict__ qv = q + (size_t)v * n;
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = j[u] >= 0 ? qv[j[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[v] += double(w[u]) * x[u];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double t = wave_sum(acc[v]);
+    if (lane == 0) pm.out[(size_t)v * n + row] = t;
+  }
+}
+
+template <typename CT, typename IT>
+static hipError_t launch_spmm_nv(const PairMulti& pm, hipStream_t st) {
+  const dim3 grid((pm.n + 3) / 4), block(256);
+  switch (pm.nvec) {
+#define EW_NV(V) \
+  case V: hipLaunchKernelGGL((k_pair_spmm<CT, IT, V>), grid, block, 0, st, pm); break;
+    EW_NV(1) EW_NV(2) EW_NV(3) EW_NV(4) EW_NV(5) EW_NV(6) EW_NV(7) EW_NV(8)
+#undef EW_NV
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_spmm(const PairMulti& pm, bool fp32, hipStream_t st) {
+  if (pm.col16)
+    return fp32 ? launch_spmm_nv<float, short>(pm, st) : launch_spmm_nv<double, short>(pm, st);
+  return fp32 ? launch_spmm_nv<float, int>(pm, st) : launch_spmm_nv<double, int>(pm, st);
+}
+
 // ---- symmetric pair storage ----
 
 // Partner slot of tile B in tile A's ascending partner list [k0, k1), or -1.
@@ -1895,13 +1959,18 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
   if (run(kGkChargeSum))
     hipLaunchKernelGGL(kg_charge_sum, grid(kGkChargeSum), dim3(256), 0, st, D, io,
                        G.map[kGkChargeSum]);
-  const bool fork = G.aux && run(kGkPairs);
+  const bool fork = G.aux && (run(kGkPairs) || G.has_pm);
   if (fork) {
     hipError_t e = hipEventRecord(G.fork_ev, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(G.aux, G.fork_ev, 0);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, G.aux, D, io,
-                       G.map[kGkPairs]);
+    if (G.has_pm) {
+      e = launch_pair_spmm(G.pm, sizeof(R) == sizeof(float), G.aux);
+      if (e != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, G.aux, D, io,
+                         G.map[kGkPairs]);
+    }
     e = hipEventRecord(G.join_ev, G.aux);
     if (e != hipSuccess) return e;
   }
@@ -1956,6 +2025,9 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
                        G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
   if (fork) {
     const hipError_t e = hipStreamWaitEvent(st, G.join_ev, 0);
+    if (e != hipSuccess) return e;
+  } else if (G.has_pm) {
+    const hipError_t e = launch_pair_spmm(G.pm, sizeof(R) == sizeof(float), st);
     if (e != hipSuccess) return e;
   } else if (run(kGkPairs)) {
     hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, st, D, io,

@@ -219,10 +219,27 @@ enum GroupKind {
   kGroupKinds
 };
 
+// Pair sums of nvec charge vectors through one row list (the coefficients are read once):
+// out[v * n + i] = sum_j coef_ij q[v * n + j].
+struct PairMulti {
+  int n;
+  int nvec;  // 1..8
+  const long long* row_ptr;
+  const void* col;
+  int col16;
+  const void* coef;
+  const double* q;
+  double* out;
+};
+
 struct GroupLaunch {
   const StructDesc* desc;
   const IoSlot* io;
   int order;
+  // optional: the pair sums of all evaluations as one multi-vector sum (all evaluations are
+  // views of one structure); the per-structure pair kind then has no blocks
+  int has_pm;
+  PairMulti pm;
   // optional: the pair sums run on `aux` beside the SPME chain (fork after the charges are
   // gathered, join before the gather)
   hipStream_t aux;
@@ -234,6 +251,7 @@ struct GroupLaunch {
 
 template <typename R>
 hipError_t launch_group(const GroupLaunch& G, hipStream_t st);
+hipError_t launch_pair_spmm(const PairMulti& pm, bool fp32, hipStream_t st);
 
 // ev (optional, 6 events): recorded after spread, x, y, z, inverse y, inverse x.
 template <typename R>

@@ -21,6 +21,7 @@
 #include "engine.hpp"
 #include "handle.hpp"
 #include "kernels.hpp"
+#include "group.hpp"
 #include "pool.hpp"
 
 using namespace ewamd;
@@ -84,54 +85,6 @@ struct ew_pool {
 
 namespace {
 
-size_t smem_of(const StructDesc& d, int kind, bool fp32) {
-  const size_t rs = fp32 ? sizeof(float) : sizeof(double);
-  const size_t cs = 2 * rs;
-  const int p = d.spread.order;
-  switch (kind) {
-    case kGkRegion:
-      return sizeof(double) * (size_t)(d.region.S[0] + p - 1) * (d.region.S[1] + p - 1) *
-             (d.region.S[2] + p - 1);
-    case kGkSlotted:
-      return sizeof(unsigned long long) * kSpreadRows * 4 + sizeof(int4) * (size_t)d.spread.slots +
-             rs * 3 * (size_t)d.spread.slots * p;
-    case kGkFftX:
-      return cs * ((size_t)d.px.lines * d.px.pitch + d.px.n);
-    case kGkFftY:
-      return cs * ((size_t)d.py.lines * d.py.pitch + d.py.n);
-    case kGkFftZ:
-      return cs * ((size_t)d.pz.lines * d.pz.pitch + d.pz.n);
-    case kGkGatherTile:
-      return gather_tile_smem(d.spread.T, p, rs);
-    case kGkPlane:
-      return d.plane ? plane_smem(d.px, d.py, cs) : 0;
-    default:
-      return 0;
-  }
-}
-
-int blocks_of(const StructDesc& d, int kind) {
-  const int Kh = d.K[0] / 2 + 1;
-  switch (kind) {
-    case kGkCharges: return (d.n + 255) / 256;
-    case kGkRegion: return d.spread_mode == 0 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] : 0;
-    case kGkAssemble:
-      return d.spread_mode == 0 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] : 0;
-    case kGkSlotted: return d.spread_mode == 1 ? d.ntiles : 0;
-    case kGkChecked: return d.spread_mode == 2 ? d.ntiles : 0;
-    case kGkFftX: return d.plane ? 0 : (d.K[1] * d.K[2] + 2 * d.px.lines - 1) / (2 * d.px.lines);
-    case kGkFftY: return d.plane ? 0 : (Kh * d.K[2] + d.py.lines - 1) / d.py.lines;
-    case kGkPlane: return d.plane ? d.K[2] : 0;
-    case kGkFftZ: return (Kh * d.K[1] + d.pz.lines - 1) / d.pz.lines;
-    case kGkPairs: return d.half ? d.ntiles : (d.n + 3) / 4;
-    case kGkGatherTile: return d.gather_tile ? d.ntiles : 0;
-    case kGkGatherAtom: return d.gather_tile ? 0 : (d.n + 255) / 256;
-    case kGkEnergy: return 1;
-    case kGkChargeSum: return d.ga.qsum ? 1 : 0;
-    default: return 0;
-  }
-}
-
 int ensure(ew_pool* P, void** buf, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return EW_OK;
   const size_t want = std::max(bytes, 2 * *cap);
@@ -166,65 +119,17 @@ int sync_desc(ew_pool* P) {
   return EW_OK;
 }
 
-// Host image of the map and io arrays of one launch, and the GroupLaunch that points into a
-// device copy of it at `dev`.
-struct Staged {
-  std::vector<int> words;  // per kind: prefix[count+1], slot[count]; then the IoSlots
-  size_t io_offset = 0;    // in bytes
-  int offs[kGroupKinds] = {}, counts[kGroupKinds] = {}, blocks[kGroupKinds] = {};
-  size_t smem[kGroupKinds] = {};
-};
-
 int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double* const* shift,
           double* energies, Staged* S) {
   std::vector<IoSlot> io((size_t)count);
   for (int i = 0; i < count; ++i) io[i] = IoSlot{dq[i], shift[i], energies + i, ids[i], 0};
-  S->words.clear();
-  for (int k = 0; k < kGroupKinds; ++k) {
-    std::vector<int> prefix, slot;
-    int run = 0;
-    size_t sm = 0;
-    for (int i = 0; i < count; ++i) {
-      const StructDesc& d = P->desc[ids[i]];
-      const int b = blocks_of(d, k);
-      if (b <= 0) continue;
-      prefix.push_back(run);
-      slot.push_back(i);
-      run += b;
-      sm = std::max(sm, smem_of(d, k, P->fp32));
-    }
-    prefix.push_back(run);
-    S->offs[k] = (int)S->words.size();
-    S->counts[k] = (int)slot.size();
-    S->blocks[k] = run;
-    S->smem[k] = sm;
-    S->words.insert(S->words.end(), prefix.begin(), prefix.end());
-    S->words.insert(S->words.end(), slot.begin(), slot.end());
-  }
-  while ((S->words.size() * sizeof(int)) % alignof(IoSlot)) S->words.push_back(0);
-  S->io_offset = S->words.size() * sizeof(int);
-  const size_t nw = (S->io_offset + sizeof(IoSlot) * io.size() + sizeof(int) - 1) / sizeof(int);
-  S->words.resize(nw, 0);
-  std::memcpy(reinterpret_cast<char*>(S->words.data()) + S->io_offset, io.data(),
-              sizeof(IoSlot) * io.size());
-  for (int k = 0; k < kGroupKinds; ++k)
-    if (S->smem[k] > (k == kGkPlane ? kPlaneMaxLds : 65536))
-      return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
+  if (!stage_group(P->desc.data(), io.data(), count, P->fp32, false, S))
+    return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
   return EW_OK;
 }
 
 GroupLaunch launch_of(const ew_pool* P, const Staged& S, const void* dev) {
-  GroupLaunch G{};
-  G.desc = P->desc_dev;
-  const int* w = static_cast<const int*>(dev);
-  G.io = reinterpret_cast<const IoSlot*>(static_cast<const char*>(dev) + S.io_offset);
-  G.order = P->order;
-  for (int k = 0; k < kGroupKinds; ++k) {
-    G.map[k] = GroupMap{w + S.offs[k], w + S.offs[k] + S.counts[k] + 1, S.counts[k]};
-    G.blocks[k] = S.blocks[k];
-    G.smem[k] = S.smem[k];
-  }
-  return G;
+  return group_launch_of(S, P->desc_dev, dev, P->order);
 }
 
 int check_ids(ew_pool* P, int count, const int* ids) {

@@ -34,9 +34,13 @@ EW_ERR_PENDING = 10
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
+#: most charge vectors of one ew_shift_multi call
+MAX_VECTORS = 8
+
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
-            "ew_shift", "ew_shift_device", "ew_gradients", "ew_stress", "ew_phase_times", "ew_potential_force",
+            "ew_shift", "ew_shift_device", "ew_shift_multi", "ew_shift_multi_device",
+            "ew_gradients", "ew_stress", "ew_phase_times", "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
             "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync",
@@ -136,6 +140,9 @@ def lib() -> C.CDLL:
         L.ew_get_parameters.argtypes = [H, C.POINTER(Parameters)]
         L.ew_shift.argtypes = [H, _D1, _D1, C.POINTER(C.c_double)]
         L.ew_shift_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ew_shift_multi.argtypes = [H, C.c_int, _D2, _D2, _D1]
+        L.ew_shift_multi_device.argtypes = [H, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]
         L.ew_gradients.argtypes = [H, _D1, _D2]
         L.ew_stress.argtypes = [H, _D1, _D2]
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
@@ -273,6 +280,23 @@ class EwaldEngine:
                                      C.c_void_p(energy_ptr or None), C.c_void_p(stream or None)),
                self._h)
 
+    def shift_multi(self, dqs):
+        """Returns (shift[nvec, n], energy[nvec]) for the charge vectors dqs[nvec, n], evaluated
+        together in one grouped launch (nvec <= MAX_VECTORS)."""
+        dqs = _f64(dqs, 2)
+        nvec = dqs.shape[0]
+        if dqs.shape[1] != self.n:
+            raise ValueError(f"charge vectors must have {self.n} entries, got {dqs.shape[1]}")
+        out = np.empty((nvec, self.n))
+        e = np.empty(nvec)
+        _check(lib().ew_shift_multi(self._h, nvec, dqs, out, e), self._h)
+        return out, e
+
+    def shift_multi_device(self, nvec: int, dq_ptr: int, shift_ptr: int, energy_ptr: int = 0,
+                           stream: int = 0):
+        _check(lib().ew_shift_multi_device(self._h, int(nvec), C.c_void_p(dq_ptr),
+                                           C.c_void_p(shift_ptr), C.c_void_p(energy_ptr or None),
+                                           C.c_void_p(stream or None)), self._h)
 
     def gradients(self, dq, grad=None):
         """grad + dU/dx (Ewald part) at charges dq; grad defaults to zeros (n, 3)."""

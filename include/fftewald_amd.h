@@ -135,6 +135,22 @@ int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
 int ew_shift_device(ew_handle *h, const double *d_dq, double *d_shift, double *d_energy,
                     void *stream);
 
+/* Several charge vectors through the current geometry in one grouped launch: the analogue of
+ * nvec calls of TCoulomb%updateShifts on one structure (coulomb.F90:786-880), e.g. the charge
+ * vectors of independent SCC cycles that share a geometry (spin or k-point replicas, trial
+ * charges of a mixer).  The pair coefficients are read once for all vectors.  dq, shift:
+ * [nvec][n] row-major; energy: nvec doubles (may be NULL).  1 <= nvec <= EW_MAX_VECTORS.  Each
+ * vector's result is ew_shift's for that vector alone (to the round-off of the spreading's
+ * accumulation order).  Extra device memory per vector: about one mesh, one spreading scratch
+ * and four atom arrays, held until the next ew_set_geometry. */
+#define EW_MAX_VECTORS 8
+int ew_shift_multi(ew_handle *h, int nvec, const double *dq, double *shift, double *energy);
+
+/* Same with device pointers (d_dq, d_shift: [nvec][n]; d_energy: nvec doubles or NULL),
+ * enqueued on `stream` (NULL = the handle's stream); asynchronous, no neutrality check. */
+int ew_shift_multi_device(ew_handle *h, int nvec, const double *d_dq, double *d_shift,
+                          double *d_energy, void *stream);
+
 /* Energy gradient of the Ewald part at charges dq, ADDED into grad[n][3] as +dU/dx: the
  * fftEwaldAddGradients convention (coulomb.F90:361-385, which subtracts the physical force
  * from DFTB+'s deriv array).  The short-range gamma gradient stays with TShortGamma. */
