@@ -29,10 +29,11 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma):
+def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False):
     """Algorithmic (compulsory) bytes of one launch of each phase.
 
-    rs: mesh real size, cs: pair coefficient size, ib: partner index size (2 or 4 bytes)."""
+    rs: mesh real size, cs: pair coefficient size, ib: partner index size (2 or 4 bytes);
+    planes: the x and y passes are one fused kernel per direction (the y phases move nothing)."""
     G = K[0] * K[1] * K[2]
     Gh = (K[0] // 2 + 1) * K[1] * K[2]
     th = 3 * order * rs
@@ -40,9 +41,9 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma):
         "gather_charges": 20 * n,
         "spread": (16 + th + 8) * n + rs * G,
         "fft_x_r2c": rs * G + 2 * rs * Gh,
-        "fft_y": 4 * rs * Gh,
+        "fft_y": 0 if planes else 4 * rs * Gh,
         "fft_z_influence": 4 * rs * Gh,
-        "fft_y_inverse": 4 * rs * Gh,
+        "fft_y_inverse": 0 if planes else 4 * rs * Gh,
         "fft_x_c2r": 2 * rs * Gh + rs * G,
         "pair_sum": nnz * (ib + cs) + n * (8 + 8 + 8),
         "gather_combine": (16 + th + 8 + 8 + 4 + 8 + (8 if gamma else 0)) * n + rs * G,
@@ -52,17 +53,19 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spme_triclinic98k_hbm_traffic.json")
 
 
-def kernel_symbol(phase, prec, index_bytes, pair_mode=0):
+def kernel_symbol(phase, prec, index_bytes, pair_mode=0, planes=False):
     """The kernel that carries a phase (the single-kernel phases only), as rocprofv3 names it."""
     r = "float" if prec == "mixed" else "double"
     i = "short" if index_bytes == 2 else "int"
     return {"pair_sum": f"void ewamd::k_pair_spmv<{r}, {i}>" if pair_mode != 2 else
             f"void ewamd::k_pair_half<{r}>",
-            "fft_x_r2c": f"void ewamd::k_fft_x_r2c<{r}>",
-            "fft_y": f"void ewamd::k_fft_lines<{r}, 0>",
+            "fft_x_r2c": f"void ewamd::k_fft_xy_r2c<{r}>" if planes else
+            f"void ewamd::k_fft_x_r2c<{r}>",
+            "fft_y": None if planes else f"void ewamd::k_fft_lines<{r}, 0>",
             "fft_z_influence": f"void ewamd::k_fft_lines<{r}, 2>",
-            "fft_y_inverse": f"void ewamd::k_fft_lines<{r}, 1>",
-            "fft_x_c2r": f"void ewamd::k_fft_x_c2r<{r}>",
+            "fft_y_inverse": None if planes else f"void ewamd::k_fft_lines<{r}, 1>",
+            "fft_x_c2r": f"void ewamd::k_fft_xy_c2r<{r}>" if planes else
+            f"void ewamd::k_fft_x_c2r<{r}>",
             "gather_charges": "ewamd::k_gather_charges"}.get(phase)
 
 
@@ -159,7 +162,7 @@ def copy_calibration(torch, mib=512, reps=5):
 def batch_bytes(prm, n, rs):
     """Algorithmic bytes of one evaluation of one structure (all phases)."""
     pb = phase_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs, prm["index_bytes"],
-                     False)
+                     False, bool(prm.get("fft_planes", 0)))
     return sum(pb.values())
 
 
@@ -504,13 +507,17 @@ def main():
                 e.shift_device(dqs[i % 4].data_ptr(), out.data_ptr(), en.data_ptr(), sp)
             phases, calls = e.phase_times()
         rs = 4 if prec == "mixed" else 8
+        planes = bool(prm.get("fft_planes", 0))
         pb = phase_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
-                         prm["index_bytes"], False)
+                         prm["index_bytes"], False, planes)
         ph = {}
         for k, v in phases.items():
             avg_ms = v / calls
             ph[k] = {"ms": round(avg_ms, 5), "bytes": pb[k],
-                     "GBs": round(pb[k] / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
+                     "GBs": round(pb[k] / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 and pb[k]
+                     else None}
+            if planes and k in ("fft_y", "fft_y_inverse"):
+                ph[k]["note"] = "fused into the x-y plane kernels (event overhead only)"
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
                      "multi": multi,
                      "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph,
@@ -527,7 +534,7 @@ def main():
     name, d = dom
     achieved = d["GBs"]
     sym = kernel_symbol(name, args.precision, head["params"]["index_bytes"],
-                        head["params"]["pair_mode"])
+                        head["params"]["pair_mode"], bool(head["params"].get("fft_planes", 0)))
     roofline = {"bound": "hbm", "kernel": sym or name, "phase": name, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(sym) if sym else None,

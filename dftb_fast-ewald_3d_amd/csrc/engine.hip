@@ -710,6 +710,7 @@ extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
   out->precision = h->prm.precision;
   out->pair_mode = h->half ? 2 : h->pair_mode;
   out->index_bytes = (h->col16 || h->half) ? 2 : 4;
+  out->fft_planes = h->plane ? 1 : 0;
   out->n_pairs = h->half ? h->half_entries : h->nnz;
   out->device_bytes = (long long)h->device_bytes;
   return EW_OK;
@@ -927,7 +928,7 @@ static int ensure_views(ew_handle* h, int nvec) {
     (void)hipFree(g.staging);
   }
   h->mgraphs.clear();
-  for (void* p : {(void*)V.qs, (void*)V.phi, (void*)V.epart, (void*)V.qsum, V.grid, V.cgrid,
+  for (void* p : {(void*)V.qs, (void*)V.phi, (void*)V.qil, (void*)V.epart, (void*)V.qsum, V.grid, V.cgrid,
                   V.region, (void*)V.dq, (void*)V.shift, (void*)V.energy, (void*)V.desc})
     if (p) h->free_buf(p);
   const void* staging = V.staging;
@@ -947,6 +948,7 @@ static int ensure_views(ew_handle* h, int nvec) {
   unsigned char *g, *cg, *rg = nullptr;
   CHK(h->alloc(&V.qs, cap * n));
   CHK(h->alloc(&V.phi, cap * n));
+  CHK(h->alloc(&V.qil, (size_t)spmm_stride(nvec) * n));
   CHK(h->alloc(&V.epart, cap * nparts));
   if (h->qsum) CHK(h->alloc(&V.qsum, cap));
   CHK(h->alloc(&g, cap * ngrid * rs));
@@ -992,9 +994,11 @@ static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, c
                      h->col16 != nullptr,
                      h->coef,
                      h->views.qs,
-                     h->views.phi};
+                     h->views.phi,
+                     h->views.qil};
   }
-  G.aux = h->aux;
+  // profiling runs every kernel on the evaluation stream, one after the other
+  G.aux = h->opt.profile ? nullptr : h->aux;
   G.fork_ev = h->fork_ev;
   G.join_ev = h->join_ev;
   return G;

@@ -100,6 +100,7 @@ struct ew_handle {
     int cap = 0;                 // vectors the buffers hold
     double* qs = nullptr;        // [cap][n] sorted charges (contiguous: the pair sum reads them so)
     double* phi = nullptr;       // [cap][n] real-space potentials
+    double* qil = nullptr;       // [n][spmm_stride(cap)] charges interleaved for the pair sum
     double* epart = nullptr;     // [cap][nparts]
     double* qsum = nullptr;      // [cap] (opt.charged)
     void* grid = nullptr;        // [cap][K0 K1 K2]

@@ -1026,8 +1026,9 @@ __device__ __forceinline__ void k_pair_spmv_body(int n, const long long* __restr
                                                    const double* __restrict__ q,
                                                    double* __restrict__ out, int bid) {
   const int lane = threadIdx.x & 63;
-  const int row = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= n) return;  // uniform across the wave
+  // wave-uniform row: its bounds come through the scalar cache
+  const int row = __builtin_amdgcn_readfirstlane(bid * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (row >= n) return;
   const long long a = row_ptr[row], b = row_ptr[row + 1];
   double acc = 0.0;
   for (long long g = a + lane; g < b; g += 512) {
@@ -1049,6 +1050,85 @@ __device__ __forceinline__ void k_pair_spmv_body(int n, const long long* __restr
   if (lane == 0) out[row] = acc;
 }
 
+// Streaming form of the same sum for large row lists: a bounded grid of waves walks the rows
+// (wave w takes rows w, w + W, ...), and while it gathers the charges of one row it already
+// loads the bounds, partner offsets and coefficients of its next row, so that a row costs one
+// memory latency instead of three.  Products are summed in k_pair_spmv_body's order (sweeps
+// of 512 entries, lane l taking entries l + 64u), so both give the same result bit for bit.
+// The bounded grid also leaves room on every CU for the kernels that run beside it.
+template <typename CT, typename IT>
+__global__ __launch_bounds__(256) void k_pair_spmv_stream(int n,
+                                                          const long long* __restrict__ row_ptr,
+                                                          const IT* __restrict__ col,
+                                                          const CT* __restrict__ coef,
+                                                          const double* __restrict__ q,
+                                                          double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (row >= n) return;  // uniform across the wave
+  long long a = row_ptr[row], b = row_ptr[row + 1];
+  IT c[8];
+  CT w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const long long k = a + lane + 64 * u;
+    c[u] = k < b ? col[k] : IT(0);
+    w[u] = k < b ? coef[k] : CT(0);
+  }
+  while (true) {
+    const int nrow = row + nw;
+    const bool more = nrow < n;
+    // this row's partners' charges
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = a + lane + 64 * u < b ? q[partner(row, c[u], n)] : 0.0;
+    // the next row's bounds and first sweep, in flight during this row's sums
+    long long na = 0, nb = 0;
+    IT nc[8];
+    CT nwt[8];
+    if (more) {
+      na = row_ptr[nrow];
+      nb = row_ptr[nrow + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long k = na + lane + 64 * u;
+      nc[u] = k < nb ? col[k] : IT(0);
+      nwt[u] = k < nb ? coef[k] : CT(0);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += double(w[u]) * x[u];
+    // rows longer than one sweep (dense image sums, long short-range lists)
+    for (long long g = a + lane + 512; g < b; g += 512) {
+      IT cc[8];
+      CT ww[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long long k = g + 64 * u;
+        cc[u] = k < b ? col[k] : IT(0);
+        ww[u] = k < b ? coef[k] : CT(0);
+      }
+      double qv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) qv[u] = g + 64 * u < b ? q[partner(row, cc[u], n)] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += double(ww[u]) * qv[u];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[row] = acc;
+    if (!more) break;
+    row = nrow;
+    a = na;
+    b = nb;
+    for (int u = 0; u < 8; ++u) {
+      c[u] = nc[u];
+      w[u] = nwt[u];
+    }
+  }
+}
+
 template <typename CT, typename IT>
 __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __restrict__ row_ptr,
                                                    const IT* __restrict__ col,
@@ -1058,57 +1138,138 @@ __global__ __launch_bounds__(256) void k_pair_spmv(int n, const long long* __res
   k_pair_spmv_body<CT, IT>(n, row_ptr, col, coef, q, out, blockIdx.x);
 }
 
-// Multi-vector pair sum: one wave per row, the row's coefficients and offsets loaded once for
-// the NV charge vectors (the same sweep as k_pair_spmv).
+#ifndef EW_PAIR_MULTI_BLOCKS
+#define EW_PAIR_MULTI_BLOCKS 256  // 1 workgroup (4 waves) per CU on 256 CUs
+#endif
+
+// Charges of NV vectors interleaved per atom (row j holds the NV charges of atom j, padded to
+// an even count) so that one gather of a partner fetches all of its charges.
+template <int NV>
+__global__ __launch_bounds__(256) void k_interleave_charges(int n, const double* __restrict__ q,
+                                                            double* __restrict__ qil) {
+  constexpr int S = spmm_stride(NV);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int v = 0; v < S; ++v) qil[(size_t)i * S + v] = v < NV ? q[(size_t)v * n + i] : 0.0;
+}
+
+// Multi-vector pair sum: a bounded grid of waves walks the rows as k_pair_spmv_stream does
+// (the next row's bounds, offsets and coefficients in flight while this row's charges are
+// gathered); the coefficients and offsets of a row are loaded once for the NV vectors and each
+// partner's NV charges come with one gather of its interleaved row.  Per vector the products
+// are summed in k_pair_spmv_body's order, so every vector's result is the single-vector sum
+// bit for bit.
 template <typename CT, typename IT, int NV>
 __global__ __launch_bounds__(256) void k_pair_spmm(PairMulti pm) {
+  constexpr int S = spmm_stride(NV);
+  constexpr int H = S == 1 ? 1 : S / 2;  // 16-byte chunks per partner
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int n = pm.n;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  int row =
+      __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (row >= n) return;  // uniform across the wave
   const long long* __restrict__ row_ptr = pm.row_ptr;
   const IT* __restrict__ col = static_cast<const IT*>(pm.col);
   const CT* __restrict__ coef = static_cast<const CT*>(pm.coef);
-  const double* __restrict__ q = pm.q;
-  const long long a = row_ptr[row], b = row_ptr[row + 1];
-  double acc[NV];
+  const double* __restrict__ qil = pm.qil;
+  auto gather = [&](int j, bool ok, double* x) {
+    if constexpr (S == 1) {
+      x[0] = ok ? qil[j] : 0.0;
+    } else {
+      const double2* __restrict__ src = reinterpret_cast<const double2*>(qil + (size_t)j * S);
 #pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (long long g = a + lane; g < b; g += 512) {
-    IT c[8];
-    CT w[8];
+      for (int hh = 0; hh < H; ++hh) {
+        const double2 t = ok ? src[hh] : make_double2(0.0, 0.0);
+        x[2 * hh] = t.x;
+        x[2 * hh + 1] = t.y;
+      }
+    }
+  };
+  long long a = row_ptr[row], b = row_ptr[row + 1];
+  IT c[8];
+  CT w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const long long k = a + lane + 64 * u;
+    c[u] = k < b ? col[k] : IT(0);
+    w[u] = k < b ? coef[k] : CT(0);
+  }
+  while (true) {
+    const int nrow = row + nw;
+    const bool more = nrow < n;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    double x[8][S];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const long long k = g + 64 * u;
-      c[u] = k < b ? col[k] : IT(0);
-      w[u] = k < b ? coef[k] : CT(0);
+      const bool ok = a + lane + 64 * u < b;
+      gather(ok ? partner(row, c[u], n) : 0, ok, x[u]);
     }
-    int j[8];
+    long long na = 0, nb = 0;
+    IT nc[8];
+    CT nwt[8];
+    if (more) {
+      na = row_ptr[nrow];
+      nb = row_ptr[nrow + 1];
+    }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) j[u] = g + 64 * u < b ? partner(row, c[u], n) : -1;
+    for (int u = 0; u < 8; ++u) {
+      const long long k = na + lane + 64 * u;
+      nc[u] = k < nb ? col[k] : IT(0);
+      nwt[u] = k < nb ? coef[k] : CT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += double(w[u]) * x[u][v];
+    for (long long g = a + lane + 512; g < b; g += 512) {
+      IT cc[8];
+      CT ww[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long long k = g + 64 * u;
+        cc[u] = k < b ? col[k] : IT(0);
+        ww[u] = k < b ? coef[k] : CT(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool ok = g + 64 * u < b;
+        gather(ok ? partner(row, cc[u], n) : 0, ok, x[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] += double(ww[u]) * x[u][v];
+    }
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      const double* __restrict__ qv = q + (size_t)v * n;
-      double x[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) x[u] = j[u] >= 0 ? qv[j[u]] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc[v] += double(w[u]) * x[u];
+      const double t = wave_sum(acc[v]);
+      if (lane == 0) pm.out[(size_t)v * n + row] = t;
     }
-  }
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    const double t = wave_sum(acc[v]);
-    if (lane == 0) pm.out[(size_t)v * n + row] = t;
+    if (!more) break;
+    row = nrow;
+    a = na;
+    b = nb;
+    for (int u = 0; u < 8; ++u) {
+      c[u] = nc[u];
+      w[u] = nwt[u];
+    }
   }
 }
 
 template <typename CT, typename IT>
 static hipError_t launch_spmm_nv(const PairMulti& pm, hipStream_t st) {
-  const dim3 grid((pm.n + 3) / 4), block(256);
+  const dim3 grid(std::min((pm.n + 3) / 4, EW_PAIR_MULTI_BLOCKS)), block(256),
+      igrid((pm.n + 255) / 256);
   switch (pm.nvec) {
-#define EW_NV(V) \
-  case V: hipLaunchKernelGGL((k_pair_spmm<CT, IT, V>), grid, block, 0, st, pm); break;
+#define EW_NV(V)                                                                             \
+  case V:                                                                                    \
+    hipLaunchKernelGGL((k_interleave_charges<V>), igrid, block, 0, st, pm.n, pm.q, pm.qil);  \
+    hipLaunchKernelGGL((k_pair_spmm<CT, IT, V>), grid, block, 0, st, pm);                    \
+    break;
     EW_NV(1) EW_NV(2) EW_NV(3) EW_NV(4) EW_NV(5) EW_NV(6) EW_NV(7) EW_NV(8)
 #undef EW_NV
     default: return hipErrorInvalidValue;
@@ -2362,27 +2523,31 @@ hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t 
   return hipGetLastError();
 }
 
+#ifndef EW_PAIR_STREAM_BLOCKS
+#define EW_PAIR_STREAM_BLOCKS 512  // 2 workgroups (8 waves) per CU on 256 CUs
+#endif
+
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
                             const void* coef, bool fp32, const double* q, double* out,
                             hipStream_t st) {
-  const dim3 grid((n + 3) / 4), block(256);
+  const int rows_blocks = (n + 3) / 4;
+  const dim3 block(256);
+  const dim3 grid(std::min(rows_blocks, EW_PAIR_STREAM_BLOCKS));
+#define EW_SPMV(CT, IT)                                                                      \
+  hipLaunchKernelGGL((k_pair_spmv_stream<CT, IT>), grid, block, 0, st, n, row_ptr,           \
+                     static_cast<const IT*>(col), static_cast<const CT*>(coef), q, out)
   if (col16) {
-    const short* c = static_cast<const short*>(col);
     if (fp32)
-      hipLaunchKernelGGL((k_pair_spmv<float, short>), grid, block, 0, st, n, row_ptr, c,
-                         static_cast<const float*>(coef), q, out);
+      EW_SPMV(float, short);
     else
-      hipLaunchKernelGGL((k_pair_spmv<double, short>), grid, block, 0, st, n, row_ptr, c,
-                         static_cast<const double*>(coef), q, out);
+      EW_SPMV(double, short);
   } else {
-    const int* c = static_cast<const int*>(col);
     if (fp32)
-      hipLaunchKernelGGL((k_pair_spmv<float, int>), grid, block, 0, st, n, row_ptr, c,
-                         static_cast<const float*>(coef), q, out);
+      EW_SPMV(float, int);
     else
-      hipLaunchKernelGGL((k_pair_spmv<double, int>), grid, block, 0, st, n, row_ptr, c,
-                         static_cast<const double*>(coef), q, out);
+      EW_SPMV(double, int);
   }
+#undef EW_SPMV
   return hipGetLastError();
 }
 

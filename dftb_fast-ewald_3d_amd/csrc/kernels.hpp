@@ -230,7 +230,11 @@ struct PairMulti {
   const void* coef;
   const double* q;
   double* out;
+  double* qil;  // [n][spmm_stride(nvec)] scratch: the charges interleaved per atom
 };
+
+// Row length of the interleaved charges: nvec rounded up to an even count (16-byte rows).
+constexpr int spmm_stride(int nvec) { return nvec == 1 ? 1 : (nvec + 1) & ~1; }
 
 struct GroupLaunch {
   const StructDesc* desc;

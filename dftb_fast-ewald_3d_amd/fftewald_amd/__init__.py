@@ -85,14 +85,14 @@ class Parameters(C.Structure):
     _fields_ = [("alpha", C.c_double), ("r_cut", C.c_double), ("k_cut", C.c_double),
                 ("r_list", C.c_double), ("grid", C.c_int * 3), ("order", C.c_int),
                 ("precision", C.c_int), ("pair_mode", C.c_int), ("index_bytes", C.c_int),
-                ("n_pairs", C.c_longlong),
+                ("fft_planes", C.c_int), ("n_pairs", C.c_longlong),
                 ("device_bytes", C.c_longlong)]
 
     def as_dict(self) -> dict:
         return {"alpha": self.alpha, "r_cut": self.r_cut, "k_cut": self.k_cut,
                 "r_list": self.r_list, "grid": tuple(self.grid), "order": self.order,
                 "precision": self.precision, "pair_mode": self.pair_mode,
-                "index_bytes": self.index_bytes,
+                "index_bytes": self.index_bytes, "fft_planes": self.fft_planes,
                 "n_pairs": self.n_pairs, "device_bytes": self.device_bytes}
 
 

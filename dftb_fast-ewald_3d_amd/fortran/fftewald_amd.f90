@@ -55,6 +55,7 @@ module fftewald_amd
       integer(c_int) :: precision = 0
       integer(c_int) :: pair_mode = 0
       integer(c_int) :: index_bytes = 0
+      integer(c_int) :: fft_planes = 0
       integer(c_long_long) :: n_pairs = 0
       integer(c_long_long) :: device_bytes = 0
    end type ew_parameters

@@ -94,6 +94,8 @@ typedef struct ew_parameters {
                         1 = dense image sum (small cells); 2 = linked tiles, each pair stored
                         once (symmetric, tile-blocked)                                         */
   int index_bytes;   /* bytes per stored partner index: 2 (offsets from the row) or 4          */
+  int fft_planes;    /* 1: fused x-y plane transforms (a whole plane in one workgroup's LDS);
+                        0: separate x and y passes                                             */
   long long n_pairs; /* stored real-space pair coefficients                                      */
   long long device_bytes; /* device memory held by the handle for this geometry                 */
 } ew_parameters;
