@@ -45,7 +45,9 @@ EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_g
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
             "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync",
             "ew_server_create", "ew_server_destroy", "ew_server_last_error", "ew_server_submit",
-            "ew_server_wait", "ew_server_drain", "ew_server_stats")
+            "ew_server_wait", "ew_server_drain", "ew_server_stats", "ew_mixer_create",
+            "ew_mixer_destroy", "ew_mixer_last_error", "ew_mixer_reset", "ew_mixer_mix",
+            "ew_mixer_mix_device", "ew_mixer_iteration")
 
 #: phases reported by ew_phase_times, in order
 PHASES = ("gather_charges", "spread", "fft_x_r2c", "fft_y", "fft_z_influence", "fft_y_inverse",
@@ -70,6 +72,16 @@ class Options(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("max_batch", C.c_int), ("max_inflight", C.c_int), ("reserved", C.c_int * 6)]
+
+
+MIXER_BROYDEN = 0
+MIXER_SIMPLE = 1
+
+
+class MixerOptions(C.Structure):
+    _fields_ = [("kind", C.c_int), ("max_iter", C.c_int), ("mix_param", C.c_double),
+                ("omega0", C.c_double), ("min_weight", C.c_double), ("max_weight", C.c_double),
+                ("weight_fac", C.c_double), ("device", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class ServerReport(C.Structure):
@@ -177,6 +189,14 @@ def lib() -> C.CDLL:
         L.ew_server_wait.argtypes = [H, C.c_longlong, C.c_double]
         L.ew_server_drain.argtypes = [H]
         L.ew_server_stats.argtypes = [H, C.POINTER(ServerReport), C.c_int]
+        L.ew_mixer_create.argtypes = [C.POINTER(MixerOptions), C.POINTER(H)]
+        L.ew_mixer_destroy.argtypes = [H]
+        L.ew_mixer_last_error.restype = C.c_char_p
+        L.ew_mixer_last_error.argtypes = [H]
+        L.ew_mixer_reset.argtypes = [H, C.c_int]
+        L.ew_mixer_mix.argtypes = [H, _D1, _D1]
+        L.ew_mixer_mix_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ew_mixer_iteration.argtypes = [H, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -530,3 +550,74 @@ def ewald_grid_memory(charges, lattice, **opts) -> float:
     o = make_options(**opts)
     _check(lib().ew_grid_memory(len(charges), charges, lattice, C.byref(o), C.byref(g)))
     return g.value
+
+
+class EwaldMixer:
+    """SCC charge mixer on the device: DFTB+'s modified Broyden (default) or simple mixer.
+
+    ``reset(n)`` starts an SCC cycle with vectors of n entries; ``mix(q_inp, q_diff)`` returns
+    the mixed charges (host arrays); ``mix_device(q_ptr, q_diff_ptr)`` mixes device arrays in
+    place (q_inp -> mixed charges) without leaving the device.
+    """
+
+    def __init__(self, kind: str = "broyden", mix_param: float = 0.2, max_iter: int = 100,
+                 omega0: float = 0.01, min_weight: float = 1.0, max_weight: float = 1.0e5,
+                 weight_fac: float = 1.0e-2, device: int = 0):
+        kinds = {"broyden": MIXER_BROYDEN, "simple": MIXER_SIMPLE}
+        if kind not in kinds:
+            raise ValueError(f"kind must be one of {sorted(kinds)}")
+        o = MixerOptions()
+        o.kind = kinds[kind]
+        o.max_iter = max_iter
+        o.mix_param = mix_param
+        o.omega0 = omega0
+        o.min_weight = min_weight
+        o.max_weight = max_weight
+        o.weight_fac = weight_fac
+        o.device = device
+        self._m = C.c_void_p()
+        _check(lib().ew_mixer_create(C.byref(o), C.byref(self._m)))
+        self.n = 0
+
+    def _check(self, status):
+        if status != EW_OK:
+            raise EwaldError(status, lib().ew_mixer_last_error(self._m).decode())
+
+    def close(self):
+        if self._m:
+            lib().ew_mixer_destroy(self._m)
+            self._m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def reset(self, n_elem: int):
+        self._check(lib().ew_mixer_reset(self._m, int(n_elem)))
+        self.n = int(n_elem)
+
+    def mix(self, q_inp, q_diff) -> np.ndarray:
+        q = np.array(q_inp, dtype=np.float64, copy=True)
+        d = _f64(q_diff, 1)
+        if self.n and (q.ndim != 1 or q.shape != d.shape or len(q) != self.n):
+            raise ValueError(f"expected two vectors of {self.n} entries")
+        self._check(lib().ew_mixer_mix(self._m, q, d))
+        return q
+
+    def mix_device(self, q_ptr: int, q_diff_ptr: int, stream: int = 0):
+        self._check(lib().ew_mixer_mix_device(self._m, C.c_void_p(q_ptr), C.c_void_p(q_diff_ptr),
+                                              C.c_void_p(stream or None)))
+
+    @property
+    def iteration(self) -> int:
+        it = C.c_int()
+        self._check(lib().ew_mixer_iteration(self._m, C.byref(it)))
+        return it.value

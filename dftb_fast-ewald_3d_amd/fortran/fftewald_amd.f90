@@ -19,12 +19,15 @@ module fftewald_amd
    public :: ew_options, ew_parameters, TEwaldEngine
    public :: ewald_potential_force, ewald_parameters, ewald_grid_memory
    public :: engine_create, engine_set_geometry, engine_shift, engine_destroy, engine_error
-   public :: engine_add_gradients, engine_add_stress
-   public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED
+   public :: engine_add_gradients, engine_add_stress, engine_shift_multi
+   public :: ew_mixer_options, TEwaldMixer, mixer_create, mixer_reset, mixer_mix, mixer_destroy
+   public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED, EW_MIXER_BROYDEN, EW_MIXER_SIMPLE
 
    integer(c_int), parameter :: EW_OK = 0
    integer(c_int), parameter :: EW_PRECISION_FP64 = 0
    integer(c_int), parameter :: EW_PRECISION_MIXED = 1
+   integer(c_int), parameter :: EW_MIXER_BROYDEN = 0
+   integer(c_int), parameter :: EW_MIXER_SIMPLE = 1
 
    !> Mirrors struct ew_options.
    type, bind(C) :: ew_options
@@ -59,6 +62,24 @@ module fftewald_amd
       integer(c_long_long) :: n_pairs = 0
       integer(c_long_long) :: device_bytes = 0
    end type ew_parameters
+
+   !> Mirrors struct ew_mixer_options (TBroydenMixerInp, broydenmixer.F90:31-50).
+   type, bind(C) :: ew_mixer_options
+      integer(c_int) :: kind = 0
+      integer(c_int) :: max_iter = 0
+      real(c_double) :: mix_param = 0.0_c_double
+      real(c_double) :: omega0 = 0.0_c_double
+      real(c_double) :: min_weight = 0.0_c_double
+      real(c_double) :: max_weight = 0.0_c_double
+      real(c_double) :: weight_fac = 0.0_c_double
+      integer(c_int) :: device = 0
+      integer(c_int) :: reserved(3) = 0
+   end type ew_mixer_options
+
+   !> Device charge mixer handle (the TMixerReal analogue).
+   type :: TEwaldMixer
+      type(c_ptr) :: handle = c_null_ptr
+   end type TEwaldMixer
 
    !> Resident engine handle.
    type :: TEwaldEngine
@@ -141,6 +162,40 @@ module fftewald_amd
          type(ew_options), intent(in) :: opts
          real(c_double), intent(out) :: gigabytes
       end function c_ew_grid_memory
+
+      integer(c_int) function c_ew_shift_multi(h, nvec, dq, shift, energy) &
+            bind(C, name="ew_shift_multi")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: nvec
+         real(c_double), intent(in) :: dq(*)
+         real(c_double), intent(out) :: shift(*)
+         real(c_double), intent(out) :: energy(*)
+      end function c_ew_shift_multi
+
+      integer(c_int) function c_ew_mixer_create(opts, m) bind(C, name="ew_mixer_create")
+         import :: c_int, c_ptr, ew_mixer_options
+         type(ew_mixer_options), intent(in) :: opts
+         type(c_ptr), intent(out) :: m
+      end function c_ew_mixer_create
+
+      integer(c_int) function c_ew_mixer_destroy(m) bind(C, name="ew_mixer_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m
+      end function c_ew_mixer_destroy
+
+      integer(c_int) function c_ew_mixer_reset(m, n) bind(C, name="ew_mixer_reset")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m
+         integer(c_int), value :: n
+      end function c_ew_mixer_reset
+
+      integer(c_int) function c_ew_mixer_mix(m, q, qd) bind(C, name="ew_mixer_mix")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: m
+         real(c_double), intent(inout) :: q(*)
+         real(c_double), intent(in) :: qd(*)
+      end function c_ew_mixer_mix
 
       integer(c_int) function c_strlen(s) bind(C, name="strlen")
          import :: c_int, c_ptr
@@ -333,6 +388,54 @@ contains
 
       st = c_ew_shift(this%handle, deltaQAtom, shift, energy)
    end function engine_shift
+
+   !> nVec SCC iterations on one geometry in one launch: deltaQ(nAtom, nVec) -> shift(nAtom,
+   !> nVec), energy(nVec) (column-major (nAtom, nVec) is the C [nvec][n] layout).
+   function engine_shift_multi(this, deltaQ, shift, energy) result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: deltaQ(:, :)
+      real(c_double), intent(out) :: shift(:, :)
+      real(c_double), intent(out) :: energy(:)
+      integer(c_int) :: st
+
+      st = c_ew_shift_multi(this%handle, int(size(deltaQ, 2), c_int), deltaQ, shift, energy)
+   end function engine_shift_multi
+
+   !> TBroydenMixerReal_init / TSimpleMixerReal_init (broydenmixer.F90:114-140).
+   function mixer_create(this, opts) result(st)
+      type(TEwaldMixer), intent(out) :: this
+      type(ew_mixer_options), intent(in) :: opts
+      integer(c_int) :: st
+
+      st = c_ew_mixer_create(opts, this%handle)
+   end function mixer_create
+
+   !> TMixer%reset (broydenmixer.F90:144-169).
+   function mixer_reset(this, nElem) result(st)
+      type(TEwaldMixer), intent(inout) :: this
+      integer, intent(in) :: nElem
+      integer(c_int) :: st
+
+      st = c_ew_mixer_reset(this%handle, int(nElem, c_int))
+   end function mixer_reset
+
+   !> TMixer%mix (broydenmixer.F90:180-200): qInpResult in -> mixed charges out.
+   function mixer_mix(this, qInpResult, qDiff) result(st)
+      type(TEwaldMixer), intent(inout) :: this
+      real(c_double), intent(inout) :: qInpResult(:)
+      real(c_double), intent(in) :: qDiff(:)
+      integer(c_int) :: st
+
+      st = c_ew_mixer_mix(this%handle, qInpResult, qDiff)
+   end function mixer_mix
+
+   subroutine mixer_destroy(this)
+      type(TEwaldMixer), intent(inout) :: this
+      integer(c_int) :: st
+
+      if (c_associated(this%handle)) st = c_ew_mixer_destroy(this%handle)
+      this%handle = c_null_ptr
+   end subroutine mixer_destroy
 
    subroutine engine_destroy(this)
       type(TEwaldEngine), intent(inout) :: this

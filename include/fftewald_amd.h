@@ -270,6 +270,49 @@ int ew_server_drain(ew_server *server);
 /* Counters and latency percentiles since the last reset. */
 int ew_server_stats(ew_server *server, ew_server_report *out, int reset);
 
+/* ---- SCC charge mixing on the device ----
+ *
+ * DFTB+'s charge mixers (src/dftbp/mixer/: the modified Broyden mixer TBroydenMixerReal,
+ * broydenmixer.F90:114-342, and the simple mixer, simplemixer.F90:77-92) with their history
+ * resident on the device, so that an SCC cycle keeps the charges on the GPU between
+ * ew_shift_device and the mixer (the call in TScc / the SCC loop that mixes qInpRed with
+ * qDiffRed).  A Broyden mix is three kernels on the given stream and no host round trip. */
+typedef struct ew_mixer ew_mixer;
+
+#define EW_MIXER_BROYDEN 0    /* modified Broyden (Johnson, PRB 38, 12807 (1988)), the default */
+#define EW_MIXER_SIMPLE 1     /* qInpResult + mixParam * qDiff                                  */
+
+/* Mirrors TBroydenMixerInp (broydenmixer.F90:31-50); zero fields take its defaults. */
+typedef struct ew_mixer_options {
+  int kind;            /* EW_MIXER_BROYDEN or EW_MIXER_SIMPLE                                 */
+  int max_iter;        /* maxSccIter: mixes per reset (Broyden keeps max_iter - 1 vectors of
+                          history); 0 -> 100, at most 1024                                      */
+  double mix_param;    /* mixParam; 0 -> 0.2                                                    */
+  double omega0;       /* weight of the Jacobian differences; 0 -> 0.01                        */
+  double min_weight;   /* 0 -> 1.0                                                              */
+  double max_weight;   /* 0 -> 1e5                                                              */
+  double weight_fac;   /* numerator of the weights; 0 -> 1e-2                                   */
+  int device;          /* HIP device ordinal                                                    */
+  int reserved[3];
+} ew_mixer_options;
+
+int ew_mixer_create(const ew_mixer_options *opts, ew_mixer **out);
+int ew_mixer_destroy(ew_mixer *mixer);
+const char *ew_mixer_last_error(const ew_mixer *mixer);
+/* TMixer%reset (broydenmixer.F90:144-169): vectors of n_elem entries, history cleared. */
+int ew_mixer_reset(ew_mixer *mixer, int n_elem);
+/* TMixer%mix (broydenmixer.F90:180-200): q_inp_result holds the input charges on entry and the
+ * mixed charges on exit; q_diff = output - input charges.  More than max_iter mixes since the
+ * reset fail with EW_ERR_ARGUMENT, as the reference stops ("Maximal nr. of steps exceeded"). */
+int ew_mixer_mix(ew_mixer *mixer, double *q_inp_result, const double *q_diff);
+/* Same on device arrays of n_elem doubles, enqueued on `stream` (NULL = the mixer's own
+ * stream, which like the engine's is ordered with the null stream); asynchronous.  The calls
+ * on one mixer must be ordered (one stream, or synchronised). */
+int ew_mixer_mix_device(ew_mixer *mixer, double *d_q_inp_result, const double *d_q_diff,
+                        void *stream);
+/* Mixes since the last reset. */
+int ew_mixer_iteration(const ew_mixer *mixer, int *iter);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,11 @@
 !> native coords(3,N) / latVecs(3,3).  Reads input.bin, writes output.bin (stream access).
 !>   input : n, coords(3,n), charges(n), latvecs(3,3), tol, alpha, grid, order, nsp, species(n), U(nsp)
 !>   output: pot(n), shift(n), energy
+!> With a third argument "mixer" it drives the device Broyden mixer as DFTB+'s SCC loop drives
+!> TMixer%mix instead:
+!>   input : nElem, nIter, maxIter, mixParam, omega0, minWeight, maxWeight, weightFac,
+!>           q0(nElem), qDiff(nElem, nIter)
+!>   output: qInpResult(nElem, nIter) after each mix
 program fortran_check
    use, intrinsic :: iso_c_binding, only: c_double, c_int
    use fftewald_amd
@@ -18,6 +23,10 @@ program fortran_check
 
    call get_command_argument(1, inpath)
    call get_command_argument(2, outpath)
+   if (command_argument_count() >= 3) then
+      call mixer_check(trim(inpath), trim(outpath))
+      stop
+   end if
    open (newunit=u, file=trim(inpath), access="stream", form="unformatted", status="old")
    read (u) n
    allocate (coords(3, n), charges(n), pot(n), shift(n), species(n))
@@ -50,4 +59,44 @@ program fortran_check
    write (u) pot, shift, energy
    close (u)
    print '(A,I0,A,ES22.14)', "fortran_check: n=", n, " energy=", energy
+
+contains
+
+   subroutine mixer_check(inpath, outpath)
+      character(len=*), intent(in) :: inpath, outpath
+      integer(c_int) :: nElem, nIter, maxIter, st
+      real(c_double) :: prm(5)
+      real(c_double), allocatable :: q(:), qDiff(:, :), hist(:, :)
+      type(ew_mixer_options) :: mopts
+      type(TEwaldMixer) :: mixer
+      integer :: u, it
+
+      open (newunit=u, file=inpath, access="stream", form="unformatted", status="old")
+      read (u) nElem, nIter, maxIter, prm
+      allocate (q(nElem), qDiff(nElem, nIter), hist(nElem, nIter))
+      read (u) q, qDiff
+      close (u)
+      mopts%kind = EW_MIXER_BROYDEN
+      mopts%max_iter = maxIter
+      mopts%mix_param = prm(1)
+      mopts%omega0 = prm(2)
+      mopts%min_weight = prm(3)
+      mopts%max_weight = prm(4)
+      mopts%weight_fac = prm(5)
+      st = mixer_create(mixer, mopts)
+      if (st /= EW_OK) error stop "mixer_create failed"
+      st = mixer_reset(mixer, nElem)
+      if (st /= EW_OK) error stop "mixer_reset failed"
+      do it = 1, nIter
+         st = mixer_mix(mixer, q, qDiff(:, it))
+         if (st /= EW_OK) error stop "mixer_mix failed"
+         hist(:, it) = q
+      end do
+      call mixer_destroy(mixer)
+      open (newunit=u, file=outpath, access="stream", form="unformatted", status="replace")
+      write (u) hist
+      close (u)
+      print '(A,I0,A,I0)', "fortran_check mixer: nElem=", nElem, " nIter=", nIter
+   end subroutine mixer_check
+
 end program fortran_check

@@ -43,3 +43,33 @@ def test_fortran_host_matches_reference(tmp_path, oracle_mod):
     want = g["pot"] + sg
     assert rel_max(shift, want) < 1e-10
     assert energy == pytest.approx(0.5 * float(np.dot(g["charges"], want)), rel=1e-10)
+
+
+def test_fortran_broyden_mixer_reference_vectors(tmp_path):
+    """DFTB+'s own Broyden regression vectors (test/src/dftbp/unit/mixer/broyden.F90:25-64)
+    through the Fortran host side and the device mixer, within that test's atol = 100 eps."""
+    import json
+    from conftest import GOLDEN
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} not built (run __graft_entry__.build())")
+    with open(os.path.join(GOLDEN, "ref_broyden_h2o.json")) as f:
+        g = json.load(f)
+    p = g["params"]
+    n, niter = 3, len(g["difference"])
+    inp, out = tmp_path / "mixer_in.bin", tmp_path / "mixer_out.bin"
+    with open(inp, "wb") as f:
+        np.array([n, niter, p["maxSccIter"]], dtype=np.int32).tofile(f)
+        np.array([p["mixParam"], p["omega0"], p["minWeight"], p["maxWeight"], p["weightFac"]],
+                 dtype=np.float64).tofile(f)
+        np.array(g["expected_charges"][0], dtype=np.float64).tofile(f)
+        np.array(g["difference"], dtype=np.float64).tofile(f)   # C [iter][n] == Fortran (n, iter)
+    print("\n[gpu-case] fortran_check mixer: reference Broyden vectors", flush=True)
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = os.path.join(PKG, "lib") + ":" + env.get("LD_LIBRARY_PATH", "")
+    r = subprocess.run([BIN, str(inp), str(out), "mixer"], env=env, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    hist = np.fromfile(out, dtype=np.float64).reshape(niter, n)
+    want = np.array(g["expected_charges"][1:])
+    assert np.abs(hist - want).max() <= g["atol_eps"] * np.finfo(np.float64).eps
