@@ -773,7 +773,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     return h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real,
                                       st)
                    : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
-                                      h->qs, h->phi_real, st);
+                                      h->qs, h->phi_real, fork, st);
   };
   if (fork) {
     HIPCHK(h, hipEventRecord(h->fork_ev, s));
@@ -995,10 +995,12 @@ static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, c
                      h->coef,
                      h->views.qs,
                      h->views.phi,
-                     h->views.qil};
+                     h->views.qil,
+                     0};
   }
   // profiling runs every kernel on the evaluation stream, one after the other
   G.aux = h->opt.profile ? nullptr : h->aux;
+  G.pm.beside = G.aux != nullptr;
   G.fork_ev = h->fork_ev;
   G.join_ev = h->join_ev;
   return G;

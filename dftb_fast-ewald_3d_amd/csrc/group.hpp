@@ -43,7 +43,9 @@ inline int blocks_of(const StructDesc& d, int kind) {
     case kGkCharges: return (d.n + 255) / 256;
     case kGkRegion: return d.spread_mode == 0 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] : 0;
     case kGkAssemble:
-      return d.spread_mode == 0 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] : 0;
+      return d.spread_mode == 0
+                 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] * kAssembleSplit
+                 : 0;
     case kGkSlotted: return d.spread_mode == 1 ? d.ntiles : 0;
     case kGkChecked: return d.spread_mode == 2 ? d.ntiles : 0;
     case kGkFftX: return d.plane ? 0 : (d.K[1] * d.K[2] + 2 * d.px.lines - 1) / (2 * d.px.lines);

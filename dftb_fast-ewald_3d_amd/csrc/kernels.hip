@@ -919,9 +919,10 @@ __device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ reg
                                                          RegionArgs g, R* __restrict__ grid, int bid) {
   const int p = g.order;
   const int RX = g.S[0] + p - 1, RY = g.S[1] + p - 1;
+  const int part = bid % kAssembleSplit;
   int b[3];
   {
-    int t = bid;
+    int t = bid / kAssembleSplit;
     b[0] = t % g.nb[0];
     t /= g.nb[0];
     b[1] = t % g.nb[1];
@@ -951,25 +952,52 @@ __device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ reg
       nc[a] = d + 1;
     }
   }
-  const int npl = w[0] * w[1];
-  for (int xy = threadIdx.x; xy < npl; xy += blockDim.x) {
-    const int ly = xy / w[0], lx = xy - ly * w[0];
-    for (int lz = 0; lz < w[2]; ++lz) {
-      R sum = R(0);
-      for (int d3 = 0; d3 < nc[2]; ++d3) {
-        if (lz >= lim[2][d3]) continue;
-        for (int d2 = 0; d2 < nc[1]; ++d2) {
-          if (ly >= lim[1][d2]) continue;
-          const size_t base = (size_t)(cb[2][d3] * g.nb[1] + cb[1][d2]) * g.nb[0];
-          const size_t roff = ((size_t)(lz + sh[2][d3]) * RY + (ly + sh[1][d2])) * RX;
-          for (int d1 = 0; d1 < nc[0]; ++d1) {
-            if (lx >= lim[0][d1]) continue;
-            sum += regions[(base + cb[0][d1]) * g.region_stride + roff + lx + sh[0][d1]];
+  // every mesh point of the block, x fastest; this workgroup takes part `part` of them.  The
+  // candidate loops have block-uniform bounds and the loads are predicated, so a thread keeps
+  // the loads of kAssembleUnroll points in flight at once; per point the copies are summed in
+  // the same (d3, d2, d1) order whatever the split.
+  const int wxy = w[0] * w[1];
+  const int npts = wxy * w[2];
+  const int chunk = (npts + kAssembleSplit - 1) / kAssembleSplit;
+  const int p0 = part * chunk, p1 = min(npts, p0 + chunk);
+  for (int base = p0 + (int)threadIdx.x; base < p1; base += kAssembleUnroll * blockDim.x) {
+    int lx[kAssembleUnroll], ly[kAssembleUnroll], lz[kAssembleUnroll];
+    bool ok[kAssembleUnroll];
+    R sum[kAssembleUnroll];
+#pragma unroll
+    for (int u = 0; u < kAssembleUnroll; ++u) {
+      const int idx = base + u * (int)blockDim.x;
+      ok[u] = idx < p1;
+      const int z = idx / wxy, r = idx - z * wxy;
+      lz[u] = z;
+      ly[u] = r / w[0];
+      lx[u] = r - ly[u] * w[0];
+      sum[u] = R(0);
+    }
+    for (int d3 = 0; d3 < nc[2]; ++d3) {
+      for (int d2 = 0; d2 < nc[1]; ++d2) {
+        const size_t base_copy = (size_t)(cb[2][d3] * g.nb[1] + cb[1][d2]) * g.nb[0];
+        for (int d1 = 0; d1 < nc[0]; ++d1) {
+          const R* __restrict__ copy = regions + (base_copy + cb[0][d1]) * g.region_stride;
+          R v[kAssembleUnroll];
+#pragma unroll
+          for (int u = 0; u < kAssembleUnroll; ++u) {
+            const bool in = ok[u] && lz[u] < lim[2][d3] && ly[u] < lim[1][d2] &&
+                            lx[u] < lim[0][d1];
+            const size_t off = ((size_t)(lz[u] + sh[2][d3]) * RY + (ly[u] + sh[1][d2])) * RX +
+                               lx[u] + sh[0][d1];
+            v[u] = in ? copy[off] : R(0);
           }
+#pragma unroll
+          for (int u = 0; u < kAssembleUnroll; ++u) sum[u] += v[u];
         }
       }
-      grid[((size_t)(lo[2] + lz) * g.K[1] + (lo[1] + ly)) * g.K[0] + lo[0] + lx] = sum;
     }
+#pragma unroll
+    for (int u = 0; u < kAssembleUnroll; ++u)
+      if (ok[u])
+        grid[((size_t)(lo[2] + lz[u]) * g.K[1] + (lo[1] + ly[u])) * g.K[0] + lo[0] + lx[u]] =
+            sum[u];
   }
 }
 
@@ -1262,8 +1290,8 @@ __global__ __launch_bounds__(256) void k_pair_spmm(PairMulti pm) {
 
 template <typename CT, typename IT>
 static hipError_t launch_spmm_nv(const PairMulti& pm, hipStream_t st) {
-  const dim3 grid(std::min((pm.n + 3) / 4, EW_PAIR_MULTI_BLOCKS)), block(256),
-      igrid((pm.n + 255) / 256);
+  const dim3 grid(pm.beside ? std::min((pm.n + 3) / 4, EW_PAIR_MULTI_BLOCKS) : (pm.n + 3) / 4),
+      block(256), igrid((pm.n + 255) / 256);
   switch (pm.nvec) {
 #define EW_NV(V)                                                                             \
   case V:                                                                                    \
@@ -2416,7 +2444,7 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
       default: EW_REGION(0); break;
     }
 #undef EW_REGION
-    hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk * kAssembleSplit), dim3(256), 0, st,
                        static_cast<const R*>(scr), rg, grid);
   } else if (L.spread.slots > 0) {
     const size_t smem = sizeof(unsigned long long) * kSpreadRows * 4 +
@@ -2529,10 +2557,10 @@ hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t 
 
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
                             const void* coef, bool fp32, const double* q, double* out,
-                            hipStream_t st) {
+                            bool beside, hipStream_t st) {
   const int rows_blocks = (n + 3) / 4;
   const dim3 block(256);
-  const dim3 grid(std::min(rows_blocks, EW_PAIR_STREAM_BLOCKS));
+  const dim3 grid(beside ? std::min(rows_blocks, EW_PAIR_STREAM_BLOCKS) : rows_blocks);
 #define EW_SPMV(CT, IT)                                                                      \
   hipLaunchKernelGGL((k_pair_spmv_stream<CT, IT>), grid, block, 0, st, n, row_ptr,           \
                      static_cast<const IT*>(col), static_cast<const CT*>(coef), q, out)

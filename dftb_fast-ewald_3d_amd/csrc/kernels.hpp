@@ -108,6 +108,12 @@ struct RegionArgs {
 
 // LDS budget of a block copy
 constexpr size_t kRegionMaxBytes = 60 * 1024;
+// Assembly of the block copies: workgroups per mesh block, mesh points in flight per thread
+#ifndef EW_ASSEMBLE_SPLIT
+#define EW_ASSEMBLE_SPLIT 1
+#endif
+constexpr int kAssembleSplit = EW_ASSEMBLE_SPLIT;
+constexpr int kAssembleUnroll = 4;
 
 // Fused plane transforms: one workgroup of kPlaneBlock threads holds a whole x-y plane of the
 // half spectrum in LDS (up to kPlaneMaxLds, the gfx950 workgroup limit) and runs the x R2C and
@@ -231,6 +237,7 @@ struct PairMulti {
   const double* q;
   double* out;
   double* qil;  // [n][spmm_stride(nvec)] scratch: the charges interleaved per atom
+  int beside;   // 1: runs next to other kernels (bounded grid)
 };
 
 // Row length of the interleaved charges: nvec rounded up to an even count (16-byte rows).
@@ -266,9 +273,11 @@ hipError_t launch_gather_charges(int n, const int* perm, const double* dq, doubl
                                  double* zero, hipStream_t st);
 // Total charge of qs[0..n) in a fixed order (one workgroup) into *qsum.
 hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t st);
+// beside: the sum runs next to other kernels (a bounded grid leaves them CU room); otherwise
+// it has the device to itself and takes a workgroup per four rows.
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
                             const void* coef, bool fp32, const double* q, double* out,
-                            hipStream_t st);
+                            bool beside, hipStream_t st);
 hipError_t launch_compress_cols(int n, const long long* row_ptr, const int* col, short* off,
                                 int* too_far, hipStream_t st);
 
