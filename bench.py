@@ -57,7 +57,7 @@ def kernel_symbol(phase, prec, index_bytes, pair_mode=0, planes=False):
     """The kernel that carries a phase (the single-kernel phases only), as rocprofv3 names it."""
     r = "float" if prec == "mixed" else "double"
     i = "short" if index_bytes == 2 else "int"
-    return {"pair_sum": f"void ewamd::k_pair_spmv<{r}, {i}>" if pair_mode != 2 else
+    return {"pair_sum": f"void ewamd::k_pair_spmv_stream<{r}, {i}>" if pair_mode != 2 else
             f"void ewamd::k_pair_half<{r}>",
             "fft_x_r2c": f"void ewamd::k_fft_xy_r2c<{r}>" if planes else
             f"void ewamd::k_fft_x_r2c<{r}>",
