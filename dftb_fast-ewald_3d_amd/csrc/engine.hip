@@ -128,6 +128,10 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
   *out = nullptr;
   if (o.precision != EW_PRECISION_FP64 && o.precision != EW_PRECISION_MIXED)
     return set_global_error(EW_ERR_ARGUMENT, "unknown precision mode");
+  if (o.fft_mode < 0 || o.fft_mode > 2)
+    return set_global_error(EW_ERR_ARGUMENT, "fft_mode must be 0, 1 or 2");
+  if (o.pair_storage < 0 || o.pair_storage > 1)
+    return set_global_error(EW_ERR_ARGUMENT, "pair_storage must be 0 or 1");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     (void)hipGetLastError();
@@ -584,7 +588,10 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->px = make_plan(K[0], (K[1] * K[2] + 1) / 2);
   h->py = make_plan(K[1], 16);
   h->pz = make_plan(K[2], 16);
-  h->plane = h->opt.fft_mode == 0 && plane_fits(h->px, h->py, 2 * rs);
+  // fused plane transforms: by default in mixed precision only (an FP64 plane needs up to
+  // 133 KiB of LDS, one workgroup per CU, and the separate x and y passes are faster there)
+  h->plane = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32)) &&
+             plane_fits(h->px, h->py, 2 * rs);
   if (h->plane) HIPCHK(h, prepare_plane_kernels());
 
   // B-spline weights

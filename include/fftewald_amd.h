@@ -72,8 +72,10 @@ typedef struct ew_options {
                       -pi/(V alpha^2) * sum(q) to every shift, as DFTB+'s stock Ewald does
                       (src/dftbp/dftb/coulomb.F90:1292-1296); 0 (default): reject them, as the
                       fast path does (coulomb.F90:255-265, fe/common/ewald_validation.f90:26-48) */
-  int fft_mode;    /* 0 (default): fused x-y plane transforms where a plane fits one workgroup's
-                      LDS, separate x and y passes elsewhere; 1: always separate passes        */
+  int fft_mode;    /* 0 (default): fused x-y plane transforms in mixed precision where a plane
+                      fits one workgroup's LDS, separate x and y passes otherwise (FP64 planes
+                      hold one workgroup per CU and are slower); 1: always separate passes;
+                      2: fused planes wherever a plane fits                                   */
   int pair_storage; /* 0 (default): the row list (every linked pair in both rows, the fastest
                        pair sum); 1: each linked pair stored once (symmetric, tile-blocked):
                        half the pair memory, about twice the pair-sum time                    */

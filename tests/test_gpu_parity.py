@@ -395,3 +395,23 @@ def test_stress_charged_cell_closed_form():
     assert np.trace(st) == pytest.approx(energy / L ** 3, rel=1e-9)
     assert np.allclose(np.diag(st), np.trace(st) / 3, rtol=1e-9, atol=0)
     assert np.abs(st - np.diag(np.diag(st))).max() < 1e-12 * abs(np.trace(st))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_plane_and_separate_fft_passes_agree(precision, tol):
+    w = W.water()
+    say(f"water: fused x-y plane FFT (fft_mode 2) against separate passes (1), {precision}")
+    out = {}
+    for mode in (2, 1, 0):
+        with ew.EwaldEngine(tol=1e-10, r_cut=w["r_cut"], grid=w["grid"], order=w["order"],
+                            precision=precision, fft_mode=mode) as e:
+            e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+            out[mode] = e.shift(w["charges"]) + (e.parameters()["fft_planes"],)
+    assert out[2][2] == 1 and out[1][2] == 0
+    # the default: planes in mixed precision, separate passes in FP64
+    assert out[0][2] == (1 if precision == "mixed" else 0)
+    assert rel_max(out[2][0], out[1][0]) < tol
+    assert out[2][1] == pytest.approx(out[1][1], rel=tol)
+    with pytest.raises(ew.EwaldError) as exc:
+        ew.EwaldEngine(fft_mode=3)
+    assert exc.value.code == ew.EW_ERR_ARGUMENT
