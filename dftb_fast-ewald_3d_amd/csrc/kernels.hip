@@ -1,15 +1,19 @@
 // kernels.hip -- gfx950 device kernels of the periodic Coulomb engine.
 //
 // Per SCC iteration (charges change, geometry resident):
-//   k_gather_charges   dq (caller order) -> q (sorted order)
-//   k_spread           B-spline charge spreading, one workgroup per mesh tile, LDS accumulation
-//   k_fft_x_r2c        mesh rows -> half spectrum (two real rows per complex transform)
-//   k_fft_lines        Y pass; Z pass fused with the influence multiply and the inverse Z pass
-//   k_fft_x_c2r        half spectrum -> convolved mesh
-//   k_pair_spmv        real-space pair sum over the resident coefficients, one wave per atom
-//   k_gather_combine   B-spline gather + real space + on-site + self -> shift, energy partials
-//   k_energy_final     fixed-order reduction of the energy partials
-// Per geometry: k_bspline (weights), k_pair_count/k_pair_fill (linked tiles), k_pair_dense.
+//   k_gather_charges     dq (caller order) -> q (sorted order)
+//   k_spread_region      B-spline spreading into privatised block copies (FP64 LDS accumulation)
+//   k_spread_assemble    block copies -> mesh, fixed order per point
+//   k_fft_xy_r2c         x-y plane of the mesh -> half spectrum (or k_fft_x_r2c + k_fft_lines<0>)
+//   k_fft_lines<2>       z pass fused with the influence multiply and the inverse z pass
+//   k_fft_xy_c2r         half spectrum -> convolved mesh plane (or k_fft_lines<1> + k_fft_x_c2r)
+//   k_pair_spmv_stream   real-space pair sum over the resident coefficients (auxiliary stream)
+//   k_gather_tile        B-spline gather + real space + on-site + self -> shift, energy partials
+//   k_energy_final       fixed-order reduction of the energy partials
+// Several vectors / structures: the kg_* kernels (same bodies, a block prefix per kind) and
+// k_pair_spmm.  Per geometry: k_bspline (weights), k_pair_tiles (linked tiles), k_pair_dense,
+// k_compress_cols, k_half_pairs.  Stress and forces: k_recip_virial, k_real_virial,
+// k_force_recip, k_force_real.
 //
 // All FFTs are Stockham autosort transforms of one or more lines staged in LDS: every pass
 // loads its butterflies into registers, synchronises, and stores them back in place.
