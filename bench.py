@@ -422,7 +422,7 @@ def main():
     ap.add_argument("--workload", default="triclinic", choices=["triclinic", "batch", "serving"],
                     help="triclinic: the 98,304-atom cell (headline); batch: 256 structures; "
                          "serving: 1,000 resident structures under Poisson load")
-    ap.add_argument("--qps", default="5000,10000,20000,40000,60000,80000,100000",
+    ap.add_argument("--qps", default="5000,10000,20000,40000,60000,80000,100000,120000,140000",
                     help="serving: offered request rates")
     ap.add_argument("--seconds", type=float, default=2.0, help="serving: seconds per rate")
     ap.add_argument("--max-batch", type=int, default=64, help="serving: structures per launch")

@@ -588,9 +588,11 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->px = make_plan(K[0], (K[1] * K[2] + 1) / 2);
   h->py = make_plan(K[1], 16);
   h->pz = make_plan(K[2], 16);
-  // fused plane transforms: by default in mixed precision only (an FP64 plane needs up to
-  // 133 KiB of LDS, one workgroup per CU, and the separate x and y passes are faster there)
-  h->plane = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32)) &&
+  // fused plane transforms: by default in mixed precision on planes that occupy a
+  // 1024-thread workgroup (an FP64 plane needs up to 133 KiB of LDS, one workgroup per CU, and
+  // small planes leave most of the workgroup idle; the separate x and y passes are faster there)
+  const bool big_plane = (long long)(K[0] / 2 + 1) * K[1] >= kPlaneMinPoints;
+  h->plane = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32 && big_plane)) &&
              plane_fits(h->px, h->py, 2 * rs);
   if (h->plane) HIPCHK(h, prepare_plane_kernels());
 

@@ -119,6 +119,7 @@ constexpr int kAssembleUnroll = 4;
 // half spectrum in LDS (up to kPlaneMaxLds, the gfx950 workgroup limit) and runs the x R2C and
 // y passes of that plane (and the inverse y and x C2R passes) in one launch.
 constexpr int kPlaneBlock = 1024;
+constexpr int kPlaneMinPoints = 4096;  // half-spectrum points of a plane worth a fused workgroup
 constexpr int kPlaneEpt = 10;  // complex elements per thread: planes up to 10240 elements
 constexpr size_t kPlaneMaxLds = 160 * 1024;
 inline size_t plane_smem(const FftPlan& px, const FftPlan& py, size_t cs) {
