@@ -418,3 +418,57 @@ def test_plane_and_separate_fft_passes_agree(precision, tol):
     with pytest.raises(ew.EwaldError) as exc:
         ew.EwaldEngine(fft_mode=3)
     assert exc.value.code == ew.EW_ERR_ARGUMENT
+
+
+# ---- ragged and degenerate inputs ----
+
+def _clustered(n=1500, seed=21):
+    """Atoms crowded into one corner slab of a large triclinic cell: most SPME tiles are empty,
+    the occupied ones hold many atoms, and pair-list rows are long and uneven."""
+    rng = np.random.default_rng(seed)
+    lat = W.lattice_from_parameters(40.0 * W.AA, 36.0 * W.AA, 44.0 * W.AA, 80.0, 95.0, 70.0)
+    frac = rng.uniform(0.0, 1.0, (n, 3)) * np.array([0.35, 0.3, 1.0])
+    coords = frac @ lat
+    # keep a minimum separation so no two charges coincide
+    keep = [0]
+    for i in range(1, n):
+        d = coords[keep] - coords[i]
+        f = d @ np.linalg.inv(lat)
+        f -= np.rint(f)
+        if np.min(np.linalg.norm(f @ lat, axis=1)) > 0.9 * W.AA:
+            keep.append(i)
+    coords = coords[keep]
+    q = W.neutralise(rng.uniform(-1.0, 1.0, len(coords)))
+    return {"coords": coords, "lattice": lat, "charges": q}
+
+
+@pytest.mark.parametrize("storage", [0, 1])
+def test_clustered_atoms_empty_tiles_match_cpu_spme(oracle_mod, storage):
+    w = _clustered()
+    say(f"{len(w['charges'])} atoms clustered in a corner slab (empty tiles), storage {storage}")
+    s, en, prm = engine_shift(w, tol=1e-10, r_cut=10.0 * W.AA, grid=(48, 45, 50), order=6,
+                              pair_storage=storage)
+    assert prm["pair_mode"] == (2 if storage else 0)
+    want = _spme_reference(oracle_mod, w, prm)
+    assert rel_max(s, want) < 1e-11
+    assert en == pytest.approx(0.5 * float(np.dot(w["charges"], want)), rel=1e-11)
+
+
+def test_degenerate_charges_and_two_atoms(oracle_mod):
+    say("all-zero charges, linearity, and a two-atom cell (image-summed pairs)")
+    w = W.water(n_side=4)
+    with ew.EwaldEngine(tol=1e-10) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        s, en = e.shift(np.zeros(len(w["charges"])))
+        assert np.all(s == 0.0) and en == 0.0
+        # linearity: shift(2 q) = 2 shift(q) up to round-off
+        s1, e1 = e.shift(w["charges"])
+        s2, e2 = e.shift(2.0 * w["charges"])
+        assert rel_max(s2, 2.0 * s1) < 1e-13
+        assert e2 == pytest.approx(4.0 * e1, rel=1e-13)
+    lat = np.eye(3) * 12.0
+    two = {"coords": np.array([[1.0, 2.0, 3.0], [5.0, 4.0, 9.0]]), "lattice": lat,
+           "charges": np.array([0.7, -0.7])}
+    s, en, prm = engine_shift(two, tol=1e-12, grid=24, order=8)
+    want = _spme_reference(oracle_mod, two, prm)
+    assert rel_max(s, want) < 1e-11
