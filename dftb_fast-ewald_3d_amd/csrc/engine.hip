@@ -764,18 +764,24 @@ static SpmeLaunch make_spme_launch(const ew_handle* h) {
   return L;
 }
 
+// The root fork (see enqueue_eval): privatised spreading on an own-stream handle.
+static bool root_fork(const ew_handle* h) {
+  return !h->opt.profile && h->aux && h->region.scratch != nullptr;
+}
+
 static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
                         hipStream_t s) {
   const int n = h->n;
   const int* K = h->prm.K;
   const bool prof = h->opt.profile != 0;
-  if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
-  HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, h->half ? h->phi_real : nullptr, s));
-  if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, s));
-  if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
-  // the real-space pair sum only needs the sorted charges: it runs on the auxiliary stream,
-  // beside the spreading and FFTs (which leave CUs idle), and joins before the gather
+  // The real-space pair sum runs on the auxiliary stream beside the spreading and FFTs (which
+  // leave CUs idle) and joins before the gather.  With privatised spreading the two branches
+  // start together: the charge gather, charge sum and pair sum run on the auxiliary stream,
+  // and the spreading reads the caller's charges through the sort permutation, so the chain
+  // on stream s waits for nothing until the gather.  Otherwise the pair sum forks after the
+  // charge gather.
   const bool fork = !prof && h->aux;
+  const bool root = root_fork(h);
   const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
   const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
   auto pairs = [&](hipStream_t st) {
@@ -784,13 +790,33 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
                    : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
                                       h->qs, h->phi_real, fork, st);
   };
-  if (fork) {
+  auto charges = [&](hipStream_t st) {
+    HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, h->half ? h->phi_real : nullptr, st));
+    if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, st));
+    return EW_OK;
+  };
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
+  if (root) {
     HIPCHK(h, hipEventRecord(h->fork_ev, s));
     HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
+    CHK(charges(h->aux));
     HIPCHK(h, pairs(h->aux));
     HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+  } else {
+    CHK(charges(s));
+    if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
+    if (fork) {
+      HIPCHK(h, hipEventRecord(h->fork_ev, s));
+      HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
+      HIPCHK(h, pairs(h->aux));
+      HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+    }
   }
   SpmeLaunch L = make_spme_launch(h);
+  if (root) {
+    L.q = d_dq;
+    L.qperm = h->perm;
+  }
   if (h->fp32)
     HIPCHK(h, launch_spme<float>(L, s, prof ? &h->ev[2] : nullptr));
   else
@@ -842,12 +868,18 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   return EW_OK;
 }
 
-// One evaluation on stream s.  Unless eager or profiling, the kernel sequence is captured once
-// per buffer triple on the handle's own stream and replayed as a graph on s afterwards (the
+// One evaluation on stream s.  Large evaluations that fork at their start (privatised
+// spreading) are launched directly: their two branches and the join replay slower from a
+// captured graph than from direct launches on this runtime (98,304 atoms: 0.183 against
+// 0.176 ms), and their kernels are long enough to hide the launch cost.  Other evaluations are
+// captured once per buffer triple on the handle's own stream and replayed as a graph (the
 // sequence has no memset or host work, so every replay is the same work).
+constexpr int kDirectLaunchAtoms = 32768;
+
 static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
                     hipStream_t s) {
-  if (h->opt.eager || h->opt.profile) return enqueue_eval(h, d_dq, d_shift, d_energy, s);
+  if (h->opt.eager || h->opt.profile || (root_fork(h) && h->n >= kDirectLaunchAtoms))
+    return enqueue_eval(h, d_dq, d_shift, d_energy, s);
   hipGraphExec_t exec = nullptr;
   for (auto& g : h->graphs)
     if (g.dq == d_dq && g.shift == d_shift && g.energy == d_energy) exec = g.exec;

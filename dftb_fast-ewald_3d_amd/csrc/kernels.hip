@@ -845,7 +845,8 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
                                                        const R* __restrict__ theta,
                                                        const double* __restrict__ q,
                                                        const int* __restrict__ tile_start,
-                                                       RegionArgs g, R* __restrict__ regions, int bid) {
+                                                       RegionArgs g, R* __restrict__ regions, int bid,
+                                                       const int* __restrict__ qperm = nullptr) {
   // the private copy accumulates in FP64 in both precision modes (LDS FP64 adds are native
   // and fast; the FP32 ones are not), and is stored in the mesh precision R
   extern __shared__ unsigned char smem_raw[];
@@ -875,7 +876,8 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
         const int a = a0 + it / p, o3 = it % p;
         const int4 kk = k0[a];
         const R* th = theta + (size_t)a * 3 * p;
-        const double c3 = q[a] * double(th[2 * p + o3]);
+        // sorted charges, or the caller's through the sort permutation
+        const double c3 = (qperm ? q[qperm[a]] : q[a]) * double(th[2 * p + o3]);
         double* plane = reg + ((size_t)(kk.z - lo3 + o3) * RY + (kk.y - lo2)) * RX + (kk.x - lo1);
         if (P > 0) {
           double w1[P > 0 ? P : 1], w2[P > 0 ? P : 1];
@@ -910,8 +912,9 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
                                                        const R* __restrict__ theta,
                                                        const double* __restrict__ q,
                                                        const int* __restrict__ tile_start,
-                                                       RegionArgs g, R* __restrict__ regions) {
-  k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x);
+                                                       RegionArgs g, R* __restrict__ regions,
+                                                       const int* __restrict__ qperm) {
+  k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x, qperm);
 }
 
 // Assembly of the privatised copies, one workgroup per block of the mesh: a point at offset l
@@ -2437,7 +2440,7 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
     R* scr = static_cast<R*>(rg.scratch);
 #define EW_REGION(PP)                                                                         \
   hipLaunchKernelGGL((k_spread_region<R, PP>), dim3(nblk), dim3(256), smem, st, L.k0, th, L.q, \
-                     L.tile_start, rg, scr)
+                     L.tile_start, rg, scr, L.qperm)
     switch (rg.order) {
       case 4: EW_REGION(4); break;
       case 5: EW_REGION(5); break;

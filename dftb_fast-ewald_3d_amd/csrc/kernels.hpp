@@ -142,6 +142,7 @@ struct SpmeLaunch {
   const int4* k0;
   const void* theta;
   const double* q;
+  const int* qperm;  // privatised spreading only: q is in caller order, read through qperm
   const int* tile_start;
   SpreadArgs spread;
   RegionArgs region;

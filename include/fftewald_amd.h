@@ -67,7 +67,9 @@ typedef struct ew_options {
                       r_cut = sqrt(-ln tol)/alpha (fe/common/ewald_truncation.f90:36-45)     */
   int profile;     /* 1: record HIP events around every phase of each evaluation (ew_phase_times) */
   int eager;       /* 1: launch the evaluation's kernels one by one; 0 (default): replay a HIP
-                      graph captured once per (dq, shift, energy) buffer triple               */
+                      graph captured once per (dq, shift, energy) buffer triple, except for
+                      large structures (>= 32768 atoms with privatised spreading), whose
+                      direct launches measured faster than replay                             */
   int charged;     /* 1: accept non-neutral charges and add the neutralising background
                       -pi/(V alpha^2) * sum(q) to every shift, as DFTB+'s stock Ewald does
                       (src/dftbp/dftb/coulomb.F90:1292-1296); 0 (default): reject them, as the
