@@ -184,6 +184,7 @@ extern "C" int ew_destroy(ew_handle* h) {
   if (!h) return EW_OK;
   (void)hipSetDevice(h->opt.device);
   h->release();
+  h->trim();
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
@@ -351,6 +352,11 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   HIPCHK(h, hipSetDevice(h->opt.device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->release();
+  // whatever of the previous geometry's buffers the new one does not reuse is freed on return
+  struct TrimOnReturn {
+    ew_handle* h;
+    ~TrimOnReturn() { h->trim(); }
+  } trim_on_return{h};
 
   Cell c;
   int st = cell_metrics(latvecs, &c);

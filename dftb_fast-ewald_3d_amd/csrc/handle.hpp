@@ -52,6 +52,9 @@ struct ew_handle {
 
   // device state of the current geometry
   std::vector<ewamd::DevBuf> bufs;
+  // buffers of the previous geometry, reused by alloc() while the next one is built (a geometry
+  // step of a trajectory needs the same sizes again) and freed when it is done
+  std::vector<ewamd::DevBuf> spare;
   int* perm = nullptr;
   int4* k0 = nullptr;
   double* frac = nullptr;
@@ -150,13 +153,18 @@ struct ew_handle {
     if (aux) (void)hipStreamSynchronize(aux);
     drop_graphs();
     for (auto& b : bufs)
-      if (b.p) (void)hipFree(b.p);
+      if (b.p) spare.push_back(b);
     bufs.clear();
     device_bytes = 0;
     views = Views{};
     views_host.clear();
     views_staging_cap = 0;
     have_geom = false;
+  }
+  // frees the spare buffers (nothing queued uses them: release() synchronised the streams)
+  void trim() {
+    for (auto& b : spare) (void)hipFree(b.p);
+    spare.clear();
   }
   void free_buf(void* p) {
     for (size_t k = 0; k < bufs.size(); ++k)
@@ -171,6 +179,20 @@ struct ew_handle {
   int alloc(T** out, size_t count) {
     void* p = nullptr;
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    // the smallest spare buffer that fits and wastes at most half of itself
+    size_t best = spare.size();
+    for (size_t k = 0; k < spare.size(); ++k)
+      if (spare[k].bytes >= bytes && spare[k].bytes <= 2 * bytes + (1u << 20) &&
+          (best == spare.size() || spare[k].bytes < spare[best].bytes))
+        best = k;
+    if (best < spare.size()) {
+      const ewamd::DevBuf b = spare[best];
+      spare.erase(spare.begin() + best);
+      bufs.push_back(b);
+      device_bytes += b.bytes;
+      *out = static_cast<T*>(b.p);
+      return EW_OK;
+    }
     if (hipMalloc(&p, bytes) != hipSuccess) {
       (void)hipGetLastError();
       *out = nullptr;

@@ -1435,12 +1435,15 @@ __global__ __launch_bounds__(256) void k_pair_half(const int* __restrict__ tile_
 
 // Rewrites absolute partner indices as signed 16-bit offsets from the row; flags rows whose
 // partners lie further than +-32767 atoms away (then the 32-bit list is kept).
-__global__ void k_compress_cols(int n, const long long* __restrict__ row_ptr,
-                                const int* __restrict__ col, short* __restrict__ off,
-                                int* __restrict__ too_far) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_compress_cols(int n, const long long* __restrict__ row_ptr,
+                                                       const int* __restrict__ col,
+                                                       short* __restrict__ off,
+                                                       int* __restrict__ too_far) {
+  // one wave per row, lanes along the row
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (i >= n) return;
-  for (long long k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+  const int lane = threadIdx.x & 63;
+  for (long long k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) {
     int d = col[k] - i;
     if (d >= (n + 1) / 2) d -= n;
     if (d < -(n / 2)) d += n;
@@ -1726,14 +1729,22 @@ __device__ __forceinline__ double pair_term(double r2, const PairArgs& g, int si
 }
 
 // Linked tiles: every partner tile within +-D along each axis, minimum image by fractional
-// rounding (fe/fft/ewald_fft_3d_real.f90:150-157).  COUNT: row lengths; else fill.
+// rounding (fe/fft/ewald_fft_3d_real.f90:150-157).  COUNT: row lengths; else fill.  One wave
+// per atom: the lanes take 64 consecutive candidates of a partner tile at a time (coalesced
+// coordinate loads), and a ballot compacts the pairs inside r_list, so a row lists its partners
+// in candidate order -- partner tiles in (z, y, x) order, atoms ascending within a tile.
 template <bool COUNT, typename CT>
-__global__ void k_pair_tiles(int n, const double* __restrict__ frac, const int* __restrict__ sp,
-                             const int* __restrict__ tile_of, const int* __restrict__ tile_start,
-                             PairArgs g, long long* __restrict__ row_ptr, int* __restrict__ col,
-                             CT* __restrict__ coef, int* __restrict__ err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(256) void k_pair_tiles(int n, const double* __restrict__ frac,
+                                                    const int* __restrict__ sp,
+                                                    const int* __restrict__ tile_of,
+                                                    const int* __restrict__ tile_start,
+                                                    PairArgs g, long long* __restrict__ row_ptr,
+                                                    int* __restrict__ col,
+                                                    CT* __restrict__ coef,
+                                                    int* __restrict__ err) {
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (i >= n) return;  // uniform across the wave
   const double fi0 = frac[3 * i], fi1 = frac[3 * i + 1], fi2 = frac[3 * i + 2];
   const int si = sp ? sp[i] : 0;
   int t = tile_of[i];
@@ -1742,7 +1753,8 @@ __global__ void k_pair_tiles(int n, const double* __restrict__ frac, const int* 
   const int t2 = t % g.nt[1], t3 = t / g.nt[1];
   long long pos = COUNT ? 0 : row_ptr[i];
   const long long end = COUNT ? 0 : row_ptr[i + 1];
-  int cnt = 0;
+  long long cnt = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
   const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
   const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
   const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
@@ -1753,36 +1765,43 @@ __global__ void k_pair_tiles(int n, const double* __restrict__ frac, const int* 
       for (int c1 = 0; c1 < n1; ++c1) {
         const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
         const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
-        for (int j = tile_start[tt]; j < tile_start[tt + 1]; ++j) {
-          if (j == i) continue;
-          double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
-          s0 -= rint(s0);
-          s1 -= rint(s1);
-          s2 -= rint(s2);
-          const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
-          const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
-          const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
-          const double r2 = x * x + y * y + z * z;
-          if (r2 > g.rl2) continue;
-          if (r2 < kMinSeparationSq) {
-            *err = 1;
-            continue;
+        const int j1 = tile_start[tt + 1];
+        for (int j0 = tile_start[tt]; j0 < j1; j0 += 64) {
+          const int j = j0 + lane;
+          bool in = false;
+          double c = 0.0;
+          if (j < j1 && j != i) {
+            double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
+            s0 -= rint(s0);
+            s1 -= rint(s1);
+            s2 -= rint(s2);
+            const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+            const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+            const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+            const double r2 = x * x + y * y + z * z;
+            if (r2 <= g.rl2) {
+              if (r2 < kMinSeparationSq)
+                *err = 1;
+              else
+                c = pair_term(r2, g, si, sp ? sp[j] : 0, &in);
+            }
           }
-          bool in;
-          const double c = pair_term(r2, g, si, sp ? sp[j] : 0, &in);
-          if (!in) continue;
+          const unsigned long long mask = __ballot(in);
           if (COUNT) {
-            ++cnt;
-          } else if (pos < end) {  // the fill never runs past the counted row length
-            col[pos] = j;
-            coef[pos] = CT(c);
-            ++pos;
+            cnt += __popcll(mask);
+          } else {
+            const long long k = pos + __popcll(mask & below);
+            if (in && k < end) {  // the fill never runs past the counted row length
+              col[k] = j;
+              coef[k] = CT(c);
+            }
+            pos += __popcll(mask);
           }
         }
       }
     }
   }
-  if (COUNT) row_ptr[i] = cnt;
+  if (COUNT && lane == 0) row_ptr[i] = cnt;
 }
 
 // Small cells: every (i, j) pair summed over all images within r_list
@@ -2624,7 +2643,7 @@ hipError_t launch_pair_half(int ntiles, const int* tile_start, const HalfPairs& 
 
 hipError_t launch_compress_cols(int n, const long long* row_ptr, const int* col, short* off,
                                 int* too_far, hipStream_t st) {
-  hipLaunchKernelGGL(k_compress_cols, dim3((n + 255) / 256), dim3(256), 0, st, n, row_ptr, col,
+  hipLaunchKernelGGL(k_compress_cols, dim3((n + 3) / 4), dim3(256), 0, st, n, row_ptr, col,
                      off, too_far);
   return hipGetLastError();
 }
@@ -2698,7 +2717,7 @@ hipError_t launch_bspline(int n, const double* frac, const BsplineArgs& g, void*
 hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int* tile_of,
                              const int* tile_start, const PairArgs& g, long long* counts,
                              int* err, hipStream_t st) {
-  hipLaunchKernelGGL((k_pair_tiles<true, double>), dim3((n + 127) / 128), dim3(128), 0, st, n,
+  hipLaunchKernelGGL((k_pair_tiles<true, double>), dim3((n + 3) / 4), dim3(256), 0, st, n,
                      frac, sp, tile_of, tile_start, g, counts, nullptr, nullptr, err);
   return hipGetLastError();
 }
@@ -2707,12 +2726,12 @@ hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int*
                             const int* tile_start, const PairArgs& g, long long* row_ptr,
                             int* col, void* coef, bool fp32, int* err, hipStream_t st) {
   if (fp32)
-    hipLaunchKernelGGL((k_pair_tiles<false, float>), dim3((n + 127) / 128), dim3(128), 0, st, n,
+    hipLaunchKernelGGL((k_pair_tiles<false, float>), dim3((n + 3) / 4), dim3(256), 0, st, n,
                        frac, sp, tile_of, tile_start, g, row_ptr, col, static_cast<float*>(coef),
                        err);
   else
-    hipLaunchKernelGGL((k_pair_tiles<false, double>), dim3((n + 127) / 128), dim3(128), 0, st,
-                       n, frac, sp, tile_of, tile_start, g, row_ptr, col,
+    hipLaunchKernelGGL((k_pair_tiles<false, double>), dim3((n + 3) / 4), dim3(256), 0, st, n,
+                       frac, sp, tile_of, tile_start, g, row_ptr, col,
                        static_cast<double*>(coef), err);
   return hipGetLastError();
 }
