@@ -443,6 +443,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     tile[i] = (tt[2] * h->nt[1] + tt[1]) * h->nt[0] + tt[0];
     count[tile[i] + 1]++;
   }
+  int max_tile_atoms = 0;
+  for (int t = 0; t < ntiles; ++t) max_tile_atoms = std::max(max_tile_atoms, count[t + 1]);
   for (int t = 0; t < ntiles; ++t) count[t + 1] += count[t];
   std::vector<int> perm(n), fill(count.begin(), count.end() - 1);
   for (int i = 0; i < n; ++i) perm[fill[tile[i]]++] = i;
@@ -634,7 +636,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   if (linked) {
     h->pair_mode = 0;
     HIPCHK(h, launch_pair_count(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
-                                h->row_ptr, h->err_flag, s));
+                                max_tile_atoms, h->row_ptr, h->err_flag, s));
     std::vector<long long> rp((size_t)n + 1);
     HIPCHK(h, hipMemcpyAsync(rp.data(), h->row_ptr, sizeof(long long) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
@@ -653,8 +655,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     h->coef = cp;
     HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
                              hipMemcpyHostToDevice, s));
-    HIPCHK(h, launch_pair_fill(n, h->frac, h->species, h->tile_of, h->tile_start, pa, h->row_ptr,
-                               h->col, h->coef, cf, h->err_flag, s));
+    HIPCHK(h, launch_pair_fill(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
+                               max_tile_atoms, h->row_ptr, h->col, h->coef, cf, h->err_flag, s));
     HIPCHK(h, hipStreamSynchronize(s));
   } else {
     h->pair_mode = 1;

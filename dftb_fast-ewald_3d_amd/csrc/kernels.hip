@@ -1804,6 +1804,129 @@ __global__ __launch_bounds__(256) void k_pair_tiles(int n, const double* __restr
   if (COUNT && lane == 0) row_ptr[i] = cnt;
 }
 
+// The same list, one workgroup per sort tile: the partner candidates every atom of the tile
+// shares (the atoms of its partner tiles, in the same order) are staged in LDS a chunk at a
+// time, and each wave scans them for one atom of the tile after another, so the coordinates
+// are read from HBM once per tile instead of once per atom.  Rows come out exactly as from
+// k_pair_tiles.  Needs at most kPairMaxCandTiles partner tiles and kPairMaxTileAtoms atoms in
+// a tile.
+constexpr int kPairStage = 1024;        // candidates staged per chunk
+constexpr int kPairMaxCandTiles = 1024;
+constexpr int kPairMaxTileAtoms = 1024;
+
+template <bool COUNT, typename CT>
+__global__ __launch_bounds__(256) void k_pair_tiles_lds(const double* __restrict__ frac,
+                                                        const int* __restrict__ sp,
+                                                        const int* __restrict__ tile_start,
+                                                        PairArgs g, long long* __restrict__ row_ptr,
+                                                        int* __restrict__ col,
+                                                        CT* __restrict__ coef,
+                                                        int* __restrict__ err) {
+  __shared__ double cxs[kPairStage], cys[kPairStage], czs[kPairStage];
+  __shared__ int cjs[kPairStage], css[kPairStage];
+  __shared__ int ctile[kPairMaxCandTiles], cstart[kPairMaxCandTiles + 1];
+  __shared__ long long apos[kPairMaxTileAtoms];
+  const int A = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  int t = A;
+  const int t1 = t % g.nt[0];
+  t /= g.nt[0];
+  const int t2 = t % g.nt[1], t3 = t / g.nt[1];
+  const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
+  const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
+  const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
+  const int ntc = n1 * n2 * n3;  // <= kPairMaxCandTiles (checked by the launcher)
+  // partner tiles in (z, y, x) order, with the running count of their atoms
+  for (int c = threadIdx.x; c < ntc; c += blockDim.x) {
+    const int c1 = c % n1, c2 = (c / n1) % n2, c3 = c / (n1 * n2);
+    const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
+    const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
+    const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
+    ctile[c] = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int c = 0; c < ntc; ++c) {
+      cstart[c] = run;
+      run += tile_start[ctile[c] + 1] - tile_start[ctile[c]];
+    }
+    cstart[ntc] = run;
+  }
+  const int ia0 = tile_start[A], na = tile_start[A + 1] - ia0;  // na <= kPairMaxTileAtoms
+  if (na == 0) return;  // uniform: an empty tile has no rows
+  for (int a = threadIdx.x; a < na; a += blockDim.x) apos[a] = COUNT ? 0 : row_ptr[ia0 + a];
+  __syncthreads();
+  const int total = cstart[ntc];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k0 = 0; k0 < total; k0 += kPairStage) {
+    const int m = min(kPairStage, total - k0);
+    __syncthreads();  // the previous chunk is no longer read
+    for (int idx = threadIdx.x; idx < m; idx += blockDim.x) {
+      const int k = k0 + idx;
+      int lo = 0, hi = ntc - 1;  // the partner tile holding candidate k
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cstart[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      const int j = tile_start[ctile[lo]] + (k - cstart[lo]);
+      cxs[idx] = frac[3 * j];
+      cys[idx] = frac[3 * j + 1];
+      czs[idx] = frac[3 * j + 2];
+      cjs[idx] = j;
+      css[idx] = sp ? sp[j] : 0;
+    }
+    __syncthreads();
+    for (int a = wv; a < na; a += nwv) {
+      const int i = ia0 + a;
+      const double fi0 = frac[3 * i], fi1 = frac[3 * i + 1], fi2 = frac[3 * i + 2];
+      const int si = sp ? sp[i] : 0;
+      long long pos = apos[a];
+      const long long end = COUNT ? 0 : row_ptr[i + 1];
+      for (int c0 = 0; c0 < m; c0 += 64) {
+        const int c = c0 + lane;
+        bool in = false;
+        double v = 0.0;
+        int j = -1;
+        if (c < m) {
+          j = cjs[c];
+          if (j != i) {
+            double s0 = fi0 - cxs[c], s1 = fi1 - cys[c], s2 = fi2 - czs[c];
+            s0 -= rint(s0);
+            s1 -= rint(s1);
+            s2 -= rint(s2);
+            const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+            const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+            const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+            const double r2 = x * x + y * y + z * z;
+            if (r2 <= g.rl2) {
+              if (r2 < kMinSeparationSq)
+                *err = 1;
+              else
+                v = pair_term(r2, g, si, css[c], &in);
+            }
+          }
+        }
+        const unsigned long long mask = __ballot(in);
+        if (!COUNT) {
+          const long long k = pos + __popcll(mask & below);
+          if (in && k < end) {  // the fill never runs past the counted row length
+            col[k] = j;
+            coef[k] = CT(v);
+          }
+        }
+        pos += __popcll(mask);
+      }
+      if (lane == 0) apos[a] = pos;
+    }
+  }
+  if (COUNT) {
+    __syncthreads();
+    for (int a = threadIdx.x; a < na; a += blockDim.x) row_ptr[ia0 + a] = apos[a];
+  }
+}
+
 // Small cells: every (i, j) pair summed over all images within r_list
 // (fe/direct/ewald_direct_3d.f90:237-300).  Row i holds all n partners.
 template <typename CT>
@@ -2714,17 +2837,40 @@ hipError_t launch_bspline(int n, const double* frac, const BsplineArgs& g, void*
   return hipGetLastError();
 }
 
+// The tile-staged build when its bounds hold, else one wave per atom.
+static bool pair_lds_fits(const PairArgs& g, int max_tile_atoms) {
+  long long ntc = 1;
+  for (int a = 0; a < 3; ++a) ntc *= g.full[a] ? g.nt[a] : 2 * g.D[a] + 1;
+  return ntc <= kPairMaxCandTiles && max_tile_atoms <= kPairMaxTileAtoms;
+}
+
 hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int* tile_of,
-                             const int* tile_start, const PairArgs& g, long long* counts,
-                             int* err, hipStream_t st) {
-  hipLaunchKernelGGL((k_pair_tiles<true, double>), dim3((n + 3) / 4), dim3(256), 0, st, n,
-                     frac, sp, tile_of, tile_start, g, counts, nullptr, nullptr, err);
+                             const int* tile_start, const PairArgs& g, int max_tile_atoms,
+                             long long* counts, int* err, hipStream_t st) {
+  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  if (pair_lds_fits(g, max_tile_atoms))
+    hipLaunchKernelGGL((k_pair_tiles_lds<true, double>), dim3(ntiles), dim3(256), 0, st, frac,
+                       sp, tile_start, g, counts, nullptr, nullptr, err);
+  else
+    hipLaunchKernelGGL((k_pair_tiles<true, double>), dim3((n + 3) / 4), dim3(256), 0, st, n,
+                       frac, sp, tile_of, tile_start, g, counts, nullptr, nullptr, err);
   return hipGetLastError();
 }
 
 hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
-                            const int* tile_start, const PairArgs& g, long long* row_ptr,
-                            int* col, void* coef, bool fp32, int* err, hipStream_t st) {
+                            const int* tile_start, const PairArgs& g, int max_tile_atoms,
+                            long long* row_ptr, int* col, void* coef, bool fp32, int* err,
+                            hipStream_t st) {
+  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  if (pair_lds_fits(g, max_tile_atoms)) {
+    if (fp32)
+      hipLaunchKernelGGL((k_pair_tiles_lds<false, float>), dim3(ntiles), dim3(256), 0, st, frac,
+                         sp, tile_start, g, row_ptr, col, static_cast<float*>(coef), err);
+    else
+      hipLaunchKernelGGL((k_pair_tiles_lds<false, double>), dim3(ntiles), dim3(256), 0, st,
+                         frac, sp, tile_start, g, row_ptr, col, static_cast<double*>(coef), err);
+    return hipGetLastError();
+  }
   if (fp32)
     hipLaunchKernelGGL((k_pair_tiles<false, float>), dim3((n + 3) / 4), dim3(256), 0, st, n,
                        frac, sp, tile_of, tile_start, g, row_ptr, col, static_cast<float*>(coef),

@@ -309,12 +309,14 @@ hipError_t launch_gather_tile(const int4* k0, const void* theta, const void* gri
                               double* shift, double* epart, double* energy, hipStream_t st);
 hipError_t launch_bspline(int n, const double* frac, const BsplineArgs& g, void* theta, bool fp32,
                           hipStream_t st);
+// Linked-tile pair list: row lengths, then the rows (max_tile_atoms: most atoms in a sort tile).
 hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int* tile_of,
-                             const int* tile_start, const PairArgs& g, long long* counts,
-                             int* err, hipStream_t st);
+                             const int* tile_start, const PairArgs& g, int max_tile_atoms,
+                             long long* counts, int* err, hipStream_t st);
 hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
-                            const int* tile_start, const PairArgs& g, long long* row_ptr,
-                            int* col, void* coef, bool fp32, int* err, hipStream_t st);
+                            const int* tile_start, const PairArgs& g, int max_tile_atoms,
+                            long long* row_ptr, int* col, void* coef, bool fp32, int* err,
+                            hipStream_t st);
 // Virial W_ab = -dE/d(strain_ab) of the Ewald energy, as per-block partials of 6 components
 // (xx, yy, zz, xy, xz, yz) that the caller sums: the reciprocal part from the forward spectrum
 // left in L.cgrid by a forward_only launch_spme (nparts = number of blocks written), the
