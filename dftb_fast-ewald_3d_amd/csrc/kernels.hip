@@ -1826,6 +1826,10 @@ __global__ __launch_bounds__(256) void k_pair_tiles_lds(const double* __restrict
   __shared__ int cjs[kPairStage], css[kPairStage];
   __shared__ int ctile[kPairMaxCandTiles], cstart[kPairMaxCandTiles + 1];
   __shared__ long long apos[kPairMaxTileAtoms];
+  // fill: per wave, the candidates inside r_list queue up (in candidate order) so that the
+  // pair terms are evaluated 64 at a time on full waves
+  __shared__ int qj[COUNT ? 1 : 4][128], qs[COUNT ? 1 : 4][128];
+  __shared__ double qr[COUNT ? 1 : 4][128];
   const int A = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   int t = A;
@@ -1884,6 +1888,75 @@ __global__ __launch_bounds__(256) void k_pair_tiles_lds(const double* __restrict
       const int si = sp ? sp[i] : 0;
       long long pos = apos[a];
       const long long end = COUNT ? 0 : row_ptr[i + 1];
+      if constexpr (!COUNT) {
+        // the queue of this wave: qn entries (j, r^2, species of j), flushed 64 at a time
+        int* wj = qj[wv];
+        int* ws = qs[wv];
+        double* wr = qr[wv];
+        int qn = 0;
+        auto flush = [&](int cnt) {
+          bool in = false;
+          double v = 0.0;
+          int j = -1;
+          if (lane < cnt) {
+            j = wj[lane];
+            v = pair_term(wr[lane], g, si, ws[lane], &in);
+          }
+          const unsigned long long mask = __ballot(in);
+          const long long k = pos + __popcll(mask & below);
+          if (in && k < end) {  // the fill never runs past the counted row length
+            col[k] = j;
+            coef[k] = CT(v);
+          }
+          pos += __popcll(mask);
+        };
+        for (int c0 = 0; c0 < m; c0 += 64) {
+          const int c = c0 + lane;
+          bool near = false;
+          double r2 = 0.0;
+          int j = -1;
+          if (c < m) {
+            j = cjs[c];
+            if (j != i) {
+              double s0 = fi0 - cxs[c], s1 = fi1 - cys[c], s2 = fi2 - czs[c];
+              s0 -= rint(s0);
+              s1 -= rint(s1);
+              s2 -= rint(s2);
+              const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+              const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+              const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+              r2 = x * x + y * y + z * z;
+              if (r2 <= g.rl2) {
+                if (r2 < kMinSeparationSq)
+                  *err = 1;
+                else
+                  near = true;
+              }
+            }
+          }
+          const unsigned long long mask = __ballot(near);
+          if (near) {
+            const int slot = qn + __popcll(mask & below);
+            wj[slot] = j;
+            wr[slot] = r2;
+            ws[slot] = css[c];
+          }
+          qn += __popcll(mask);
+          if (qn >= 64) {
+            flush(64);
+            // move the overflow (fewer than 64 entries) to the front
+            if (lane < qn - 64) {
+              wj[lane] = wj[64 + lane];
+              wr[lane] = wr[64 + lane];
+              ws[lane] = ws[64 + lane];
+            }
+            qn -= 64;
+          }
+        }
+        if (qn > 0) flush(qn);
+        if (lane == 0) apos[a] = pos;
+        continue;
+      }
       for (int c0 = 0; c0 < m; c0 += 64) {
         const int c = c0 + lane;
         bool in = false;

@@ -472,3 +472,18 @@ def test_degenerate_charges_and_two_atoms(oracle_mod):
     s, en, prm = engine_shift(two, tol=1e-12, grid=24, order=8)
     want = _spme_reference(oracle_mod, two, prm)
     assert rel_max(s, want) < 1e-11
+
+
+def test_crowded_tiles_use_the_per_atom_pair_build(oracle_mod):
+    """More than 1024 atoms in a sort tile (a coarse 16^3 mesh over 5000 atoms): the pair list
+    is built one wave per atom instead of through the tile-staged kernel; same rows."""
+    rng = np.random.default_rng(31)
+    lat = np.eye(3) * 45.0 * W.AA
+    n = 5000
+    w = {"coords": W.poisson_disc(n, lat, 1.0 * W.AA, rng), "lattice": lat,
+         "charges": W.neutralise(rng.uniform(-1, 1, n))}
+    say("5000 atoms, 16^3 mesh: > 1024 atoms per sort tile (per-atom pair build)")
+    s, en, prm = engine_shift(w, tol=1e-8, r_cut=9.0 * W.AA, grid=16, order=6)
+    assert prm["pair_mode"] == 0
+    want = _spme_reference(oracle_mod, w, prm)
+    assert rel_max(s, want) < 1e-11
