@@ -133,6 +133,32 @@ __device__ __forceinline__ int fdiv(int x, FastDiv f) {
   return int((__umulhi(unsigned(x), f.m) + unsigned(x)) >> f.s);
 }
 
+// Coordinates (x, y, z) of the flat index i0, i0 + S, i0 + 2S, ... of an E1 x E2 x E3 box (x
+// fastest), without a division per step: S is split once into box coordinates and each step
+// adds them with carries (x + sx < 2 E1 and y + sy + 1 < 2 E2, so one subtraction each).
+struct BoxWalk {
+  int x, y, z, sx, sy, sz, E1, E2;
+  __device__ __forceinline__ BoxWalk(int i0, int stride, int e1, int e2) : E1(e1), E2(e2) {
+    const int r = i0 / e1;
+    x = i0 - r * e1;
+    z = r / e2;
+    y = r - z * e2;
+    const int rs = stride / e1;
+    sx = stride - rs * e1;
+    sz = rs / e2;
+    sy = rs - sz * e2;
+  }
+  __device__ __forceinline__ void next() {
+    x += sx;
+    int c = x >= E1;
+    x -= c ? E1 : 0;
+    y += sy + c;
+    c = y >= E2;
+    y -= c ? E2 : 0;
+    z += sz + c;
+  }
+};
+
 // tw: this pass's table, exp(-2 pi i t k / (Ns RAD)) at [(t - 1) * Ns + k].  BLOCK threads
 // cover nl lines of n elements with at most EPT elements per thread (nl * n <= BLOCK * EPT).
 template <int RAD, typename R, int BLOCK, int EPT>
@@ -967,18 +993,18 @@ __device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ reg
   const int npts = wxy * w[2];
   const int chunk = (npts + kAssembleSplit - 1) / kAssembleSplit;
   const int p0 = part * chunk, p1 = min(npts, p0 + chunk);
+  BoxWalk wk(p0 + (int)threadIdx.x, blockDim.x, w[0], w[1]);
   for (int base = p0 + (int)threadIdx.x; base < p1; base += kAssembleUnroll * blockDim.x) {
     int lx[kAssembleUnroll], ly[kAssembleUnroll], lz[kAssembleUnroll];
     bool ok[kAssembleUnroll];
     R sum[kAssembleUnroll];
 #pragma unroll
     for (int u = 0; u < kAssembleUnroll; ++u) {
-      const int idx = base + u * (int)blockDim.x;
-      ok[u] = idx < p1;
-      const int z = idx / wxy, r = idx - z * wxy;
-      lz[u] = z;
-      ly[u] = r / w[0];
-      lx[u] = r - ly[u] * w[0];
+      ok[u] = base + u * (int)blockDim.x < p1;
+      lx[u] = wk.x;
+      ly[u] = wk.y;
+      lz[u] = wk.z;
+      wk.next();
       sum[u] = R(0);
     }
     for (int d3 = 0; d3 < nc[2]; ++d3) {
@@ -991,8 +1017,9 @@ __device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ reg
           for (int u = 0; u < kAssembleUnroll; ++u) {
             const bool in = ok[u] && lz[u] < lim[2][d3] && ly[u] < lim[1][d2] &&
                             lx[u] < lim[0][d1];
-            const size_t off = ((size_t)(lz[u] + sh[2][d3]) * RY + (ly[u] + sh[1][d2])) * RX +
-                               lx[u] + sh[0][d1];
+            // one copy holds (S + p - 1)^3 points: 32-bit offsets
+            const int off = ((lz[u] + sh[2][d3]) * RY + (ly[u] + sh[1][d2])) * RX + lx[u] +
+                            sh[0][d1];
             v[u] = in ? copy[off] : R(0);
           }
 #pragma unroll
@@ -1003,7 +1030,7 @@ __device__ __forceinline__ void k_spread_assemble_body(const R* __restrict__ reg
 #pragma unroll
     for (int u = 0; u < kAssembleUnroll; ++u)
       if (ok[u])
-        grid[((size_t)(lo[2] + lz[u]) * g.K[1] + (lo[1] + ly[u])) * g.K[0] + lo[0] + lx[u]] =
+        grid[(size_t)((lo[2] + lz[u]) * g.K[1] + (lo[1] + ly[u])) * g.K[0] + lo[0] + lx[u]] =
             sum[u];
   }
 }
@@ -1546,6 +1573,7 @@ __device__ __forceinline__ void k_gather_tile_body(
   const int E1 = min(g.T[0], g.K[0] - lo1) + p - 1, E2 = min(g.T[1], g.K[1] - lo2) + p - 1,
             E3 = min(g.T[2], g.K[2] - lo3) + p - 1;
   const int nreg = E1 * E2 * E3;
+  BoxWalk wk(threadIdx.x, 256, E1, E2);
   for (int i0 = 0; i0 < nreg; i0 += 16 * 256) {
     // sixteen loads in flight per thread before the LDS stores
     R v[16];
@@ -1554,13 +1582,13 @@ __device__ __forceinline__ void k_gather_tile_body(
       const int i = i0 + threadIdx.x + k * 256;
       v[k] = R(0);
       if (i < nreg) {
-        const int x = i % E1, y = (i / E1) % E2, z = i / (E1 * E2);
-        int gx = lo1 + x, gy = lo2 + y, gz = lo3 + z;
+        int gx = lo1 + wk.x, gy = lo2 + wk.y, gz = lo3 + wk.z;
         while (gx >= g.K[0]) gx -= g.K[0];
         while (gy >= g.K[1]) gy -= g.K[1];
         while (gz >= g.K[2]) gz -= g.K[2];
-        v[k] = grid[((size_t)gz * g.K[1] + gy) * g.K[0] + gx];
+        v[k] = grid[(size_t)(gz * g.K[1] + gy) * g.K[0] + gx];
       }
+      wk.next();
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
