@@ -344,6 +344,60 @@ static int build_half_pairs(ew_handle* h, const std::vector<int>& ts, hipStream_
   return EW_OK;
 }
 
+// Points in the order the kernels see them: fractional coordinates wrapped into [0, 1), the
+// B-spline stencil origin per axis (floor(s K) - (p - 1), modulo K), the SPME tile of that
+// origin, and a stable counting sort by tile.  Needs the handle's mesh and tiles.
+struct TileOrder {
+  std::vector<int> perm;      // sorted position -> caller index
+  std::vector<int> start;     // [ntiles + 1] first sorted position of each tile
+  std::vector<double> frac;   // sorted [k][3]
+  std::vector<int4> k0;       // sorted stencil origins
+  std::vector<int> tile;      // sorted tile indices
+  int max_tile = 0;           // most points in one tile
+};
+
+static void order_by_tile(const ew_handle* h, int n, const double* coords, TileOrder& o) {
+  const Cell& c = h->cell;
+  const int* K = h->prm.K;
+  const int p = h->prm.order;
+  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  std::vector<double> fr(3 * (size_t)n);
+  std::vector<int> k0o(3 * (size_t)n), tile(n);
+  o.start.assign(ntiles + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    int tt[3];
+    for (int a = 0; a < 3; ++a) {
+      double s = c.rec[a][0] * coords[3 * i] + c.rec[a][1] * coords[3 * i + 1] +
+                 c.rec[a][2] * coords[3 * i + 2];
+      s -= std::floor(s);
+      fr[3 * (size_t)i + a] = s;
+      int fl = int(std::floor(s * K[a]));
+      if (fl >= K[a]) fl -= K[a];
+      int k0 = fl - (p - 1);
+      if (k0 < 0) k0 += K[a];
+      k0o[3 * (size_t)i + a] = k0;
+      tt[a] = k0 / h->T[a];
+    }
+    tile[i] = (tt[2] * h->nt[1] + tt[1]) * h->nt[0] + tt[0];
+    o.start[tile[i] + 1]++;
+  }
+  o.max_tile = 0;
+  for (int t = 0; t < ntiles; ++t) o.max_tile = std::max(o.max_tile, o.start[t + 1]);
+  for (int t = 0; t < ntiles; ++t) o.start[t + 1] += o.start[t];
+  o.perm.assign(n, 0);
+  std::vector<int> fill(o.start.begin(), o.start.end() - 1);
+  for (int i = 0; i < n; ++i) o.perm[fill[tile[i]]++] = i;
+  o.frac.assign(3 * (size_t)n, 0.0);
+  o.k0.assign(n, make_int4(0, 0, 0, 0));
+  o.tile.assign(n, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = o.perm[k];
+    for (int a = 0; a < 3; ++a) o.frac[3 * (size_t)k + a] = fr[3 * (size_t)i + a];
+    o.k0[k] = make_int4(k0o[3 * (size_t)i], k0o[3 * (size_t)i + 1], k0o[3 * (size_t)i + 2], 0);
+    o.tile[k] = tile[i];
+  }
+}
+
 extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const double* latvecs,
                                const int* species, const double* hubbard_u, int n_species,
                                const double* charges) {
@@ -424,44 +478,21 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
 
   // fractional coordinates, stencil origins, tiles; counting sort by tile
   const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
-  std::vector<double> fr(3 * (size_t)n);
-  std::vector<int> k0o(3 * (size_t)n), tile(n), count(ntiles + 1, 0);
-  for (int i = 0; i < n; ++i) {
-    int tt[3];
-    for (int a = 0; a < 3; ++a) {
-      double s = c.rec[a][0] * coords[3 * i] + c.rec[a][1] * coords[3 * i + 1] +
-                 c.rec[a][2] * coords[3 * i + 2];
-      s -= std::floor(s);
-      fr[3 * (size_t)i + a] = s;
-      int fl = int(std::floor(s * K[a]));
-      if (fl >= K[a]) fl -= K[a];
-      int k0 = fl - (p - 1);
-      if (k0 < 0) k0 += K[a];
-      k0o[3 * (size_t)i + a] = k0;
-      tt[a] = k0 / h->T[a];
-    }
-    tile[i] = (tt[2] * h->nt[1] + tt[1]) * h->nt[0] + tt[0];
-    count[tile[i] + 1]++;
-  }
-  int max_tile_atoms = 0;
-  for (int t = 0; t < ntiles; ++t) max_tile_atoms = std::max(max_tile_atoms, count[t + 1]);
-  for (int t = 0; t < ntiles; ++t) count[t + 1] += count[t];
-  std::vector<int> perm(n), fill(count.begin(), count.end() - 1);
-  for (int i = 0; i < n; ++i) perm[fill[tile[i]]++] = i;
-  std::vector<double> frs(3 * (size_t)n);
-  std::vector<int4> k0s(n);
-  std::vector<int> tiles(n), sps(gamma ? n : 0);
+  TileOrder ord;
+  order_by_tile(h, n, coords, ord);
+  const int max_tile_atoms = ord.max_tile;
+  const std::vector<int>& perm = ord.perm;
+  const std::vector<int>& count = ord.start;
+  const std::vector<double>& frs = ord.frac;
+  const std::vector<int4>& k0s = ord.k0;
+  const std::vector<int>& tiles = ord.tile;
+  std::vector<int> sps(gamma ? n : 0);
   std::vector<double> ons(gamma ? n : 0);
-  for (int k = 0; k < n; ++k) {
-    const int i = perm[k];
-    for (int a = 0; a < 3; ++a) frs[3 * (size_t)k + a] = fr[3 * (size_t)i + a];
-    k0s[k] = make_int4(k0o[3 * (size_t)i], k0o[3 * (size_t)i + 1], k0o[3 * (size_t)i + 2], 0);
-    tiles[k] = tile[i];
-    if (gamma) {
-      sps[k] = species[i];
-      ons[k] = hubbard_u[species[i]];  // -expGamma(0, U, U) = U  (shortgammafuncs.F90:128-131)
+  if (gamma)
+    for (int k = 0; k < n; ++k) {
+      sps[k] = species[perm[k]];
+      ons[k] = hubbard_u[species[perm[k]]];  // -expGamma(0, U, U) = U  (shortgammafuncs.F90:128-131)
     }
-  }
 
   // device buffers
   const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
@@ -772,6 +803,40 @@ static SpmeLaunch make_spme_launch(const ew_handle* h) {
   return L;
 }
 
+// The mesh part at the resident atoms plus phi_real, on-site and self terms (and the background
+// of a charged cell), written in caller order; energy = 1/2 sum shift q.
+static GatherArgs gather_args(const ew_handle* h) {
+  GatherArgs ga{};
+  for (int a = 0; a < 3; ++a) {
+    ga.K[a] = h->prm.K[a];
+    ga.T[a] = h->T[a];
+    ga.nt[a] = h->nt[a];
+  }
+  ga.order = h->prm.order;
+  ga.self_coef = h->self_coef;
+  ga.bg = h->background;
+  ga.qsum = h->qsum;
+  return ga;
+}
+
+static hipError_t launch_gather(const ew_handle* h, const GatherArgs& ga, const double* q,
+                                const double* phi_real, const double* onsite, double* d_out,
+                                double* d_energy, hipStream_t s) {
+  const size_t region = gather_tile_smem(h->T, h->prm.order, h->fp32 ? sizeof(float) : sizeof(double));
+  if (region <= kGatherRegionMaxBytes)
+    return h->fp32 ? launch_gather_tile<float>(h->k0, h->theta, h->grid, q, phi_real, onsite,
+                                               h->perm, h->tile_start, ga, d_out, h->epart,
+                                               d_energy, s)
+                   : launch_gather_tile<double>(h->k0, h->theta, h->grid, q, phi_real, onsite,
+                                                h->perm, h->tile_start, ga, d_out, h->epart,
+                                                d_energy, s);
+  return h->fp32 ? launch_gather_combine<float>(h->n, h->k0, h->theta, h->grid, q, phi_real,
+                                                onsite, h->perm, ga, d_out, h->epart, d_energy, s)
+                 : launch_gather_combine<double>(h->n, h->k0, h->theta, h->grid, q, phi_real,
+                                                 onsite, h->perm, ga, d_out, h->epart, d_energy,
+                                                 s);
+}
+
 // The root fork (see enqueue_eval): privatised spreading on an own-stream handle.
 static bool root_fork(const ew_handle* h) {
   return !h->opt.profile && h->aux && h->region.scratch != nullptr;
@@ -780,7 +845,6 @@ static bool root_fork(const ew_handle* h) {
 static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
                         hipStream_t s) {
   const int n = h->n;
-  const int* K = h->prm.K;
   const bool prof = h->opt.profile != 0;
   // The real-space pair sum runs on the auxiliary stream beside the spreading and FFTs (which
   // leave CUs idle) and joins before the gather.  With privatised spreading the two branches
@@ -834,35 +898,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   else
     HIPCHK(h, pairs(s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
-  GatherArgs ga{};
-  for (int a = 0; a < 3; ++a) {
-    ga.K[a] = K[a];
-    ga.T[a] = h->T[a];
-    ga.nt[a] = h->nt[a];
-  }
-  ga.order = h->prm.order;
-  ga.self_coef = h->self_coef;
-  ga.bg = h->background;
-  ga.qsum = h->qsum;
-  const int p = h->prm.order;
-  const size_t region = gather_tile_smem(h->T, p, h->fp32 ? sizeof(float) : sizeof(double));
-  if (region <= kGatherRegionMaxBytes) {
-    if (h->fp32)
-      HIPCHK(h, launch_gather_tile<float>(h->k0, h->theta, h->grid, h->qs, h->phi_real, h->onsite,
-                                          h->perm, h->tile_start, ga, d_shift, h->epart, d_energy,
-                                          s));
-    else
-      HIPCHK(h, launch_gather_tile<double>(h->k0, h->theta, h->grid, h->qs, h->phi_real,
-                                           h->onsite, h->perm, h->tile_start, ga, d_shift,
-                                           h->epart, d_energy, s));
-  } else if (h->fp32) {
-    HIPCHK(h, launch_gather_combine<float>(n, h->k0, h->theta, h->grid, h->qs, h->phi_real,
-                                           h->onsite, h->perm, ga, d_shift, h->epart, d_energy, s));
-  } else {
-    HIPCHK(h, launch_gather_combine<double>(n, h->k0, h->theta, h->grid, h->qs, h->phi_real,
-                                            h->onsite, h->perm, ga, d_shift, h->epart, d_energy,
-                                            s));
-  }
+  HIPCHK(h, launch_gather(h, gather_args(h), h->qs, h->phi_real, h->onsite, d_shift, d_energy, s));
   if (prof) {
     HIPCHK(h, hipEventRecord(h->ev[9], s));
     HIPCHK(h, hipEventSynchronize(h->ev[9]));
@@ -1288,6 +1324,149 @@ extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
     stress[3 * ia[c] + ib[c]] += w[c] * iv;
     if (ia[c] != ib[c]) stress[3 * ib[c] + ia[c]] += w[c] * iv;
   }
+  return EW_OK;
+}
+
+// Potential of external point charges at the resident atoms (caller order, overwritten), the
+// periodic form of TCoulomb%getPotential (src/dftbp/dftb/coulomb.F90:3121-3160 ->
+// sumInvRPeriodicAsymm 1340-1418): potential_i = sum_s q_s ewald(r_i - r_s) with ewald of
+// coulomb.F90:2311-2345 -- reciprocal part, real part (optionally Gaussian-blurred sources and a
+// softened denominator), the background -pi / (V alpha^2) per source, and the self term
+// -2 alpha / sqrt(pi) for a source on an atom.  The reciprocal part is the SPME convolution of
+// the sources on the engine's mesh; the real part sums the source images within
+// max(r_cut, w_max alpha r_cut) (a blurred source's erfc(d / w) has decayed as far there as
+// erfc(alpha d) at r_cut).
+extern "C" int ew_external_potential(ew_handle* h, int m, const double* ext_coords,
+                                     const double* ext_charges, const double* blur_widths,
+                                     double eps_soften, double* potential) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_external_potential: handle is NULL");
+  if (!h->have_geom)
+    return h->fail(EW_ERR_NO_GEOMETRY, "ew_external_potential before ew_set_geometry");
+  if (!potential || m < 0 || (m > 0 && (!ext_coords || !ext_charges)))
+    return h->fail(EW_ERR_ARGUMENT, "ew_external_potential: bad arguments");
+  if (!std::isfinite(eps_soften) || eps_soften < 0.0)
+    return h->fail(EW_ERR_ARGUMENT, "ew_external_potential: eps_soften must be finite and >= 0");
+  const int n = h->n;
+  if (m == 0) {
+    std::fill(potential, potential + n, 0.0);
+    return EW_OK;
+  }
+  double wmax = 0.0, qsum = 0.0;
+  for (int s = 0; s < m; ++s) {
+    if (!std::isfinite(ext_charges[s]) || !std::isfinite(ext_coords[3 * s]) ||
+        !std::isfinite(ext_coords[3 * s + 1]) || !std::isfinite(ext_coords[3 * s + 2]))
+      return h->fail(EW_ERR_NOT_FINITE, "non-finite external charge or coordinate");
+    if (blur_widths) {
+      if (!(blur_widths[s] > 0.0) || !std::isfinite(blur_widths[s]))
+        return h->fail(EW_ERR_ARGUMENT, "blur widths must be finite and > 0");
+      wmax = std::max(wmax, blur_widths[s]);
+    }
+  }
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipStreamSynchronize(s));
+
+  TileOrder o;
+  order_by_tile(h, m, ext_coords, o);
+  std::vector<double> qs(m), ws(blur_widths ? m : 0);
+  for (int k = 0; k < m; ++k) {
+    qs[k] = ext_charges[o.perm[k]];
+    if (blur_widths) ws[k] = blur_widths[o.perm[k]];
+  }
+  for (int k = 0; k < m; ++k) qsum += qs[k];
+
+  // cutoff and search shape of the real part
+  const Cell& c = h->cell;
+  const Params& prm = h->prm;
+  const double r_ext = std::max(prm.r_cut, wmax * prm.alpha * prm.r_cut);
+  ExtArgs ea{};
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) ea.lat[3 * a + d] = c.lat[a][d];
+  ea.alpha = prm.alpha;
+  ea.rc2 = r_ext * r_ext;
+  ea.eps2 = eps_soften * eps_soften;
+  ea.self_coef = h->self_coef;
+  ea.m = m;
+  bool linked = true;
+  for (int a = 0; a < 3; ++a) {
+    ea.nt[a] = h->nt[a];
+    const int K = prm.K[a];
+    const double wmin = double(std::min(h->T[a], K - (h->nt[a] - 1) * h->T[a])) / K;
+    ea.D[a] = int(std::ceil(r_ext * c.reclen[a] / wmin));
+    ea.full[a] = 2 * ea.D[a] + 1 > h->nt[a] ? 1 : 0;
+    if (!(r_ext * c.reclen[a] < 0.5)) linked = false;
+  }
+  if (!linked)
+    for (int a = 0; a < 3; ++a) ea.D[a] = int(std::ceil(r_ext * c.reclen[a])) + 1;
+
+  // device scratch of this call: one allocation, carved
+  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  const int p = prm.order;
+  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t b_frac = up(sizeof(double) * 3 * (size_t)m), b_q = up(sizeof(double) * m),
+               b_w = blur_widths ? up(sizeof(double) * m) : 0, b_k0 = up(sizeof(int4) * m),
+               b_start = up(sizeof(int) * (ntiles + 1)), b_th = up(3 * (size_t)m * p * rs),
+               b_n = up(sizeof(double) * n), b_one = up(sizeof(double));
+  const size_t total = b_frac + b_q + b_w + b_k0 + b_start + b_th + 3 * b_n + 2 * b_one;
+  struct Scratch {
+    unsigned char* p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } scr;
+  if (hipMalloc(&scr.p, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return h->fail(EW_ERR_MEMORY, "ew_external_potential: device allocation failed");
+  }
+  unsigned char* cur = scr.p;
+  auto take = [&](size_t b) {
+    unsigned char* r = cur;
+    cur += b;
+    return r;
+  };
+  double* d_frac = reinterpret_cast<double*>(take(b_frac));
+  double* d_q = reinterpret_cast<double*>(take(b_q));
+  double* d_w = blur_widths ? reinterpret_cast<double*>(take(b_w)) : nullptr;
+  int4* d_k0 = reinterpret_cast<int4*>(take(b_k0));
+  int* d_start = reinterpret_cast<int*>(take(b_start));
+  void* d_theta = take(b_th);
+  double* d_zero = reinterpret_cast<double*>(take(b_n));
+  double* d_phi = reinterpret_cast<double*>(take(b_n));
+  double* d_out = reinterpret_cast<double*>(take(b_n));
+  double* d_qsum = reinterpret_cast<double*>(take(b_one));
+  double* d_energy = reinterpret_cast<double*>(take(b_one));
+  HIPCHK(h, hipMemcpyAsync(d_frac, o.frac.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(d_q, qs.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  if (d_w) HIPCHK(h, hipMemcpyAsync(d_w, ws.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(d_k0, o.k0.data(), sizeof(int4) * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(d_start, o.start.data(), sizeof(int) * (ntiles + 1),
+                           hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(d_qsum, &qsum, sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemsetAsync(d_zero, 0, sizeof(double) * n, s));
+
+  // reciprocal part: the sources' SPME mesh, convolved with the influence function
+  BsplineArgs ba{{prm.K[0], prm.K[1], prm.K[2]}, p};
+  HIPCHK(h, launch_bspline(m, d_frac, ba, d_theta, h->fp32, s));
+  SpmeLaunch L = make_spme_launch(h);
+  L.k0 = d_k0;
+  L.theta = d_theta;
+  L.q = d_q;
+  L.qperm = nullptr;
+  L.tile_start = d_start;
+  HIPCHK(h, h->fp32 ? launch_spme<float>(L, s, nullptr) : launch_spme<double>(L, s, nullptr));
+  // real part at the atoms, then mesh part + real part + background per source
+  ea.frac = d_frac;
+  ea.q = d_q;
+  ea.blur = d_w;
+  ea.start = d_start;
+  HIPCHK(h, launch_ext_real(n, h->frac, h->tile_of, ea, !linked, d_phi, s));
+  GatherArgs ga = gather_args(h);
+  ga.self_coef = 0.0;
+  ga.qsum = d_qsum;
+  HIPCHK(h, launch_gather(h, ga, d_zero, d_phi, nullptr, d_out, d_energy, s));
+  HIPCHK(h, hipMemcpyAsync(potential, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
   return EW_OK;
 }
 

@@ -13,7 +13,8 @@
 // Several vectors / structures: the kg_* kernels (same bodies, a block prefix per kind) and
 // k_pair_spmm.  Per geometry: k_bspline (weights), k_pair_tiles (linked tiles), k_pair_dense,
 // k_compress_cols, k_half_pairs.  Stress and forces: k_recip_virial, k_real_virial,
-// k_force_recip, k_force_real.
+// k_force_recip, k_force_real.  External point charges: k_ext_real (with the spreading, FFTs
+// and gather above run on the sources and the resident atoms).
 //
 // All FFTs are Stockham autosort transforms of one or more lines staged in LDS: every pass
 // loads its butterflies into registers, synchronises, and stores them back in place.
@@ -3018,6 +3019,92 @@ hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const Pai
   else
     hipLaunchKernelGGL(k_pair_dense<double>, grid, dim3(256), 0, st, n, frac, sp, g, col,
                        static_cast<double*>(coef), err);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------------------
+// external point charges
+// ----------------------------------------------------------------------------------------
+
+// Real-space potential of the external charges at each resident atom (sorted order), one thread
+// per atom: sum_s q_s [erfc(alpha d) - erfc(d / w_s) (blurred, d < 10 w_s)] / sqrt(d^2 + eps^2)
+// over the source images within the cutoff (ewaldReal, src/dftbp/dftb/coulomb.F90:2523-2575:
+// erf(d/w) - erf(alpha d) written through erfc).  An image closer than tolSameDist (1e-5,
+// src/dftbp/common/accuracy.F90:69) contributes the self term -2 alpha / sqrt(pi) q_s instead
+// (ewald, coulomb.F90:2341-2343).  Linked mode walks the source tiles around the atom's tile
+// with the minimum image; dense mode sums every source over explicit image shells.
+template <bool DENSE>
+__global__ __launch_bounds__(128) void k_ext_real(int n, const double* __restrict__ frac,
+                                                  const int* __restrict__ tile_of, ExtArgs g,
+                                                  double* __restrict__ phi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr double kTolSameDist2 = 1.0e-10;
+  const double fi0 = frac[3 * i], fi1 = frac[3 * i + 1], fi2 = frac[3 * i + 2];
+  double acc = 0.0;
+  auto add = [&](double s0, double s1, double s2, int j) {
+    const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+    const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+    const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+    const double r2 = x * x + y * y + z * z;
+    if (r2 > g.rc2) return;
+    if (r2 < kTolSameDist2) {
+      acc += g.self_coef * g.q[j];
+      return;
+    }
+    const double r = sqrt(r2);
+    double t = erfc(g.alpha * r);
+    if (g.blur) {
+      const double w = g.blur[j];
+      if (r < 10.0 * w) t -= erfc(r / w);  // erfArgLimit_ = 10, coulomb.F90:214
+    }
+    acc += g.q[j] * t / (g.eps2 > 0.0 ? sqrt(r2 + g.eps2) : r);
+  };
+  if (DENSE) {
+    for (int j = 0; j < g.m; ++j) {
+      const double s0 = fi0 - g.frac[3 * j], s1 = fi1 - g.frac[3 * j + 1],
+                   s2 = fi2 - g.frac[3 * j + 2];
+      for (int a = -g.D[0]; a <= g.D[0]; ++a)
+        for (int b = -g.D[1]; b <= g.D[1]; ++b)
+          for (int e = -g.D[2]; e <= g.D[2]; ++e) add(s0 + a, s1 + b, s2 + e, j);
+    }
+  } else {
+    int t = tile_of[i];
+    const int t1 = t % g.nt[0];
+    t /= g.nt[0];
+    const int t2 = t % g.nt[1], t3 = t / g.nt[1];
+    const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
+    const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
+    const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
+    for (int c3 = 0; c3 < n3; ++c3) {
+      const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
+      for (int c2 = 0; c2 < n2; ++c2) {
+        const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
+        for (int c1 = 0; c1 < n1; ++c1) {
+          const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
+          const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
+          for (int j = g.start[tt]; j < g.start[tt + 1]; ++j) {
+            double s0 = fi0 - g.frac[3 * j], s1 = fi1 - g.frac[3 * j + 1],
+                   s2 = fi2 - g.frac[3 * j + 2];
+            s0 -= rint(s0);
+            s1 -= rint(s1);
+            s2 -= rint(s2);
+            add(s0, s1, s2, j);
+          }
+        }
+      }
+    }
+  }
+  phi[i] = acc;
+}
+
+hipError_t launch_ext_real(int n, const double* frac, const int* tile_of, const ExtArgs& g,
+                           bool dense, double* phi, hipStream_t st) {
+  const dim3 grd((n + 127) / 128), blk(128);
+  if (dense)
+    hipLaunchKernelGGL(k_ext_real<true>, grd, blk, 0, st, n, frac, tile_of, g, phi);
+  else
+    hipLaunchKernelGGL(k_ext_real<false>, grd, blk, 0, st, n, frac, tile_of, g, phi);
   return hipGetLastError();
 }
 

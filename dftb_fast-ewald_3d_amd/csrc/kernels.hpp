@@ -337,4 +337,26 @@ hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* 
 hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const PairArgs& g,
                              int* col, void* coef, bool fp32, int* err, hipStream_t st);
 
+// External point charges (sources) acting on the resident atoms (targets): the real-space part
+// of the periodic Ewald sum of TCoulomb%getPotential (src/dftbp/dftb/coulomb.F90:3121-3160 ->
+// sumInvRPeriodicAsymm 1340-1418 -> ewald 2311-2345, ewaldReal 2523-2575).
+struct ExtArgs {
+  double lat[9];        // rows a_i
+  double alpha;
+  double rc2;           // squared cutoff of the source sum
+  double eps2;          // squared softening length (0: none)
+  double self_coef;     // -2 alpha / sqrt(pi): a source on a target (or on one of its images)
+  int D[3];             // source tiles (linked mode) or image shells (dense mode) per side
+  int full[3];          // linked mode: 1 = sweep every tile of this axis once
+  int nt[3];
+  int m;                // sources
+  const double* frac;   // sources in tile order, fractional [m][3]
+  const double* q;      // their charges
+  const double* blur;   // their Gaussian widths, or null (point charges)
+  const int* start;     // [ntiles + 1] first source of each tile
+};
+// phi[k] = real-space potential of the sources at sorted target k (overwritten).
+hipError_t launch_ext_real(int n, const double* frac, const int* tile_of, const ExtArgs& g,
+                           bool dense, double* phi, hipStream_t st);
+
 }  // namespace ewamd

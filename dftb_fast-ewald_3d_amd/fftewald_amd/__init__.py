@@ -40,7 +40,8 @@ MAX_VECTORS = 8
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
             "ew_shift", "ew_shift_device", "ew_shift_multi", "ew_shift_multi_device",
-            "ew_gradients", "ew_stress", "ew_phase_times", "ew_potential_force",
+            "ew_gradients", "ew_stress", "ew_external_potential", "ew_phase_times",
+            "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
             "ew_pool_usage", "ew_pool_shift", "ew_pool_shift_device", "ew_pool_sync",
@@ -157,6 +158,8 @@ def lib() -> C.CDLL:
                                             C.c_void_p]
         L.ew_gradients.argtypes = [H, _D1, _D2]
         L.ew_stress.argtypes = [H, _D1, _D2]
+        L.ew_external_potential.argtypes = [H, C.c_int, _optional(_D2), _optional(_D1),
+                                            _optional(_D1), C.c_double, _D1]
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
@@ -331,6 +334,23 @@ class EwaldEngine:
         st = np.zeros((3, 3)) if stress is None else np.array(stress, dtype=np.float64, copy=True)
         _check(lib().ew_stress(self._h, dq, st), self._h)
         return st
+
+    def external_potential(self, coords, charges, blur_widths=None, eps_soften=0.0):
+        """Potential (n,) of external point charges at the resident atoms, caller order
+        (TCoulomb%getPotential, periodic).  coords (m, 3) Bohr, charges (m,); blur_widths (m,)
+        for Gaussian sources; eps_soften > 0 softens the real-space denominator."""
+        q = _f64(charges, 1)
+        m = len(q)
+        xy = _f64(coords, 2) if m else None
+        if m and xy.shape != (m, 3):
+            raise ValueError("coords must be (m, 3) for m charges")
+        w = None if blur_widths is None else _f64(blur_widths, 1)
+        if w is not None and len(w) != m:
+            raise ValueError("blur_widths must have one entry per charge")
+        pot = np.empty(self.n)
+        _check(lib().ew_external_potential(self._h, m, xy, q if m else None, w,
+                                           float(eps_soften), pot), self._h)
+        return pot
 
     def phase_times(self, reset: bool = False):
         """({phase: ms}, calls) accumulated since the last reset (needs profile=True)."""

@@ -19,7 +19,7 @@ module fftewald_amd
    public :: ew_options, ew_parameters, TEwaldEngine
    public :: ewald_potential_force, ewald_parameters, ewald_grid_memory
    public :: engine_create, engine_set_geometry, engine_shift, engine_destroy, engine_error
-   public :: engine_add_gradients, engine_add_stress, engine_shift_multi
+   public :: engine_add_gradients, engine_add_stress, engine_shift_multi, engine_get_potential
    public :: ew_mixer_options, TEwaldMixer, mixer_create, mixer_reset, mixer_mix, mixer_destroy
    public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED, EW_MIXER_BROYDEN, EW_MIXER_SIMPLE
 
@@ -134,6 +134,18 @@ module fftewald_amd
          real(c_double), intent(in) :: dq(*)
          real(c_double), intent(inout) :: stress(3, 3)
       end function c_ew_stress
+
+      integer(c_int) function c_ew_external_potential(h, m, coords, charges, blur, eps, pot) &
+         bind(C, name="ew_external_potential")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: m
+         real(c_double), intent(in) :: coords(3, *)
+         real(c_double), intent(in) :: charges(*)
+         type(c_ptr), value :: blur
+         real(c_double), value :: eps
+         real(c_double), intent(out) :: pot(*)
+      end function c_ew_external_potential
 
       integer(c_int) function c_ew_potential_force(n, coords, charges, latvecs, opts, pot, force) &
          bind(C, name="ew_potential_force")
@@ -378,6 +390,29 @@ contains
 
       st = c_ew_stress(this%handle, deltaQAtom, stress)
    end function engine_add_stress
+
+   !> TCoulomb%getPotential, periodic branch (coulomb.F90:3121-3160): potential(nAtom) of the
+   !> point charges chargeCoords(3, m), charges(m) at the resident atoms, with the reference's
+   !> optional Gaussian blurWidths(m) and real-space softening epsSoften.
+   function engine_get_potential(this, chargeCoords, charges, potential, blurWidths, epsSoften) &
+      result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: chargeCoords(:, :)
+      real(c_double), intent(in) :: charges(:)
+      real(c_double), intent(out) :: potential(:)
+      real(c_double), intent(in), optional, target, contiguous :: blurWidths(:)
+      real(c_double), intent(in), optional :: epsSoften
+      integer(c_int) :: st
+      type(c_ptr) :: pb
+      real(c_double) :: eps
+
+      pb = c_null_ptr
+      if (present(blurWidths)) pb = c_loc(blurWidths)
+      eps = 0.0_c_double
+      if (present(epsSoften)) eps = epsSoften
+      st = c_ew_external_potential(this%handle, int(size(charges), c_int), chargeCoords, charges, &
+         pb, eps, potential)
+   end function engine_get_potential
 
    function engine_shift(this, deltaQAtom, shift, energy) result(st)
       type(TEwaldEngine), intent(inout) :: this

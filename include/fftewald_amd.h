@@ -177,6 +177,23 @@ int ew_phase_times(ew_handle *h, double *ms, long long *calls, int reset);
  * for charged cells (opts.charged), the neutralising background.  Short gamma not included. */
 int ew_stress(ew_handle *h, const double *dq, double *stress);
 
+/* TCoulomb%getPotential, periodic branch (src/dftbp/dftb/coulomb.F90:3121-3160 ->
+ * sumInvRPeriodicAsymm 1340-1418): the potential of m external point charges at the resident
+ * atoms, potential[i] = sum_s q_s ewald(x_i - y_s) (caller order, overwritten), with ewald()
+ * of coulomb.F90:2311-2345: Ewald reciprocal + real part, the background -pi/(V alpha^2) per
+ * source (so a non-neutral set of charges is allowed), and -2 alpha/sqrt(pi) for a source on an
+ * atom.  ext_coords[m][3] in Bohr (DFTB+'s chargeCoords(3, m)); blur_widths[m] (NULL: point
+ * charges) gives Gaussian sources, erf(d/w)/d in place of 1/d within 10 w (ewaldReal,
+ * 2523-2575); eps_soften > 0 replaces d by sqrt(d^2 + eps^2) in the real part (which, as in
+ * the reference, makes the result depend on alpha: opts.alpha_rule = 1 gives DFTB+'s alpha).
+ * The reciprocal part is the SPME convolution on the engine's mesh (its precision mode
+ * applies).  A source within 1e-5 Bohr of an atom or of one of its periodic images counts as on
+ * it (the reference tests the unwrapped difference only).  Uses the handle's mesh buffers: not
+ * concurrent with other calls on the handle. */
+int ew_external_potential(ew_handle *h, int m, const double *ext_coords,
+                          const double *ext_charges, const double *blur_widths,
+                          double eps_soften, double *potential);
+
 /* ---- one-shot calls, the reference library's own entry points ---- */
 
 /* ewald_potential_force (fe/fft/ewald_fft_3d.f90:36-134): pot[i] = phi(x_i) and, when force
