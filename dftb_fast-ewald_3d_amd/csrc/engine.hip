@@ -1327,15 +1327,151 @@ extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
   return EW_OK;
 }
 
+// External point charges staged on the device for one call: sorted by the engine's sort tiles
+// (coordinates, charges, widths, stencil origins, tile starts, B-spline weights), with the search
+// shape of their real-space sums.  Device memory is one allocation freed with the object.
+struct ExtStage {
+  int m = 0;
+  bool linked = true;
+  ExtArgs ea{};
+  std::vector<int> perm;             // sorted -> caller index
+  double qsum = 0.0;
+  unsigned char* mem = nullptr;
+  double *frac = nullptr, *q = nullptr, *w = nullptr;
+  int4* k0 = nullptr;
+  int *start = nullptr, *tile = nullptr, *perm_d = nullptr;
+  void* theta = nullptr;
+  unsigned char* cur = nullptr;
+  ~ExtStage() {
+    if (mem) (void)hipFree(mem);
+  }
+  template <typename T>
+  T* take(size_t count) {
+    T* r = reinterpret_cast<T*>(cur);
+    cur += (sizeof(T) * count + 255) & ~size_t(255);
+    return r;
+  }
+};
+
+// Validates and stages the sources; `extra` bytes of further scratch follow the staged arrays
+// (taken by the caller with st.take).  The cutoff of the real-space sums is
+// max(r_cut, w_max alpha r_cut): a blurred source's erfc(d / w) has decayed as far there as
+// erfc(alpha d) at r_cut.
+static int stage_sources(ew_handle* h, int m, const double* coords, const double* charges,
+                         const double* widths, size_t extra, ExtStage& st) {
+  double wmax = 0.0;
+  for (int s = 0; s < m; ++s) {
+    if (!std::isfinite(charges[s]) || !std::isfinite(coords[3 * s]) ||
+        !std::isfinite(coords[3 * s + 1]) || !std::isfinite(coords[3 * s + 2]))
+      return h->fail(EW_ERR_NOT_FINITE, "non-finite external charge or coordinate");
+    if (widths) {
+      if (!(widths[s] > 0.0) || !std::isfinite(widths[s]))
+        return h->fail(EW_ERR_ARGUMENT, "blur widths must be finite and > 0");
+      wmax = std::max(wmax, widths[s]);
+    }
+  }
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipStreamSynchronize(s));
+  st.m = m;
+  TileOrder o;
+  order_by_tile(h, m, coords, o);
+  std::vector<double> qs(m), ws(widths ? m : 0);
+  for (int k = 0; k < m; ++k) {
+    qs[k] = charges[o.perm[k]];
+    if (widths) ws[k] = widths[o.perm[k]];
+  }
+  for (int k = 0; k < m; ++k) st.qsum += qs[k];
+  st.perm = o.perm;
+
+  const Cell& c = h->cell;
+  const Params& prm = h->prm;
+  const double r_ext = std::max(prm.r_cut, wmax * prm.alpha * prm.r_cut);
+  ExtArgs& ea = st.ea;
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) ea.lat[3 * a + d] = c.lat[a][d];
+  ea.alpha = prm.alpha;
+  ea.rc2 = r_ext * r_ext;
+  ea.self_coef = h->self_coef;
+  ea.m = m;
+  for (int a = 0; a < 3; ++a) {
+    ea.nt[a] = h->nt[a];
+    const int K = prm.K[a];
+    const double wmin = double(std::min(h->T[a], K - (h->nt[a] - 1) * h->T[a])) / K;
+    ea.D[a] = int(std::ceil(r_ext * c.reclen[a] / wmin));
+    ea.full[a] = 2 * ea.D[a] + 1 > h->nt[a] ? 1 : 0;
+    if (!(r_ext * c.reclen[a] < 0.5)) st.linked = false;
+  }
+  if (!st.linked)
+    for (int a = 0; a < 3; ++a) ea.D[a] = int(std::ceil(r_ext * c.reclen[a])) + 1;
+
+  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
+  const int p = prm.order;
+  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t total = up(sizeof(double) * 3 * (size_t)m) + 2 * up(sizeof(double) * m) +
+                       up(sizeof(int4) * m) + up(sizeof(int) * (ntiles + 1)) +
+                       2 * up(sizeof(int) * m) + up(3 * (size_t)m * p * rs) + extra;
+  if (hipMalloc(&st.mem, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return h->fail(EW_ERR_MEMORY, "external charges: device allocation failed");
+  }
+  st.cur = st.mem;
+  st.frac = st.take<double>(3 * (size_t)m);
+  st.q = st.take<double>(m);
+  st.w = st.take<double>(m);
+  st.k0 = st.take<int4>(m);
+  st.start = st.take<int>(ntiles + 1);
+  st.tile = st.take<int>(m);
+  st.perm_d = st.take<int>(m);
+  st.theta = st.take<unsigned char>(3 * (size_t)m * p * rs);
+  HIPCHK(h, hipMemcpyAsync(st.frac, o.frac.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(st.q, qs.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  if (widths) HIPCHK(h, hipMemcpyAsync(st.w, ws.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(st.k0, o.k0.data(), sizeof(int4) * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(st.start, o.start.data(), sizeof(int) * (ntiles + 1),
+                           hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(st.tile, o.tile.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(st.perm_d, o.perm.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+  BsplineArgs ba{{prm.K[0], prm.K[1], prm.K[2]}, p};
+  HIPCHK(h, launch_bspline(m, st.frac, ba, st.theta, h->fp32, s));
+  ea.frac = st.frac;
+  ea.q = st.q;
+  ea.blur = widths ? st.w : nullptr;
+  ea.start = st.start;
+  return EW_OK;
+}
+
+// The mesh of the staged sources, convolved with the influence function (left in h->grid).
+static int spme_of_sources(ew_handle* h, const ExtStage& st) {
+  SpmeLaunch L = make_spme_launch(h);
+  L.k0 = st.k0;
+  L.theta = st.theta;
+  L.q = st.q;
+  L.qperm = nullptr;
+  L.tile_start = st.start;
+  HIPCHK(h, h->fp32 ? launch_spme<float>(L, h->stream, nullptr)
+                    : launch_spme<double>(L, h->stream, nullptr));
+  return EW_OK;
+}
+
+static ForceArgs force_args(const ew_handle* h) {
+  ForceArgs fa{};
+  for (int a = 0; a < 3; ++a) {
+    fa.K[a] = h->prm.K[a];
+    for (int d = 0; d < 3; ++d) fa.rec[3 * a + d] = h->cell.rec[a][d];
+  }
+  fa.order = h->prm.order;
+  return fa;
+}
+
 // Potential of external point charges at the resident atoms (caller order, overwritten), the
 // periodic form of TCoulomb%getPotential (src/dftbp/dftb/coulomb.F90:3121-3160 ->
 // sumInvRPeriodicAsymm 1340-1418): potential_i = sum_s q_s ewald(r_i - r_s) with ewald of
 // coulomb.F90:2311-2345 -- reciprocal part, real part (optionally Gaussian-blurred sources and a
 // softened denominator), the background -pi / (V alpha^2) per source, and the self term
 // -2 alpha / sqrt(pi) for a source on an atom.  The reciprocal part is the SPME convolution of
-// the sources on the engine's mesh; the real part sums the source images within
-// max(r_cut, w_max alpha r_cut) (a blurred source's erfc(d / w) has decayed as far there as
-// erfc(alpha d) at r_cut).
+// the sources on the engine's mesh.
 extern "C" int ew_external_potential(ew_handle* h, int m, const double* ext_coords,
                                      const double* ext_charges, const double* blur_widths,
                                      double eps_soften, double* potential) {
@@ -1351,122 +1487,119 @@ extern "C" int ew_external_potential(ew_handle* h, int m, const double* ext_coor
     std::fill(potential, potential + n, 0.0);
     return EW_OK;
   }
-  double wmax = 0.0, qsum = 0.0;
-  for (int s = 0; s < m; ++s) {
-    if (!std::isfinite(ext_charges[s]) || !std::isfinite(ext_coords[3 * s]) ||
-        !std::isfinite(ext_coords[3 * s + 1]) || !std::isfinite(ext_coords[3 * s + 2]))
-      return h->fail(EW_ERR_NOT_FINITE, "non-finite external charge or coordinate");
-    if (blur_widths) {
-      if (!(blur_widths[s] > 0.0) || !std::isfinite(blur_widths[s]))
-        return h->fail(EW_ERR_ARGUMENT, "blur widths must be finite and > 0");
-      wmax = std::max(wmax, blur_widths[s]);
-    }
-  }
-  HIPCHK(h, hipSetDevice(h->opt.device));
-  hipStream_t s = h->stream;
-  HIPCHK(h, hipStreamSynchronize(s));
-
-  TileOrder o;
-  order_by_tile(h, m, ext_coords, o);
-  std::vector<double> qs(m), ws(blur_widths ? m : 0);
-  for (int k = 0; k < m; ++k) {
-    qs[k] = ext_charges[o.perm[k]];
-    if (blur_widths) ws[k] = blur_widths[o.perm[k]];
-  }
-  for (int k = 0; k < m; ++k) qsum += qs[k];
-
-  // cutoff and search shape of the real part
-  const Cell& c = h->cell;
-  const Params& prm = h->prm;
-  const double r_ext = std::max(prm.r_cut, wmax * prm.alpha * prm.r_cut);
-  ExtArgs ea{};
-  for (int a = 0; a < 3; ++a)
-    for (int d = 0; d < 3; ++d) ea.lat[3 * a + d] = c.lat[a][d];
-  ea.alpha = prm.alpha;
-  ea.rc2 = r_ext * r_ext;
-  ea.eps2 = eps_soften * eps_soften;
-  ea.self_coef = h->self_coef;
-  ea.m = m;
-  bool linked = true;
-  for (int a = 0; a < 3; ++a) {
-    ea.nt[a] = h->nt[a];
-    const int K = prm.K[a];
-    const double wmin = double(std::min(h->T[a], K - (h->nt[a] - 1) * h->T[a])) / K;
-    ea.D[a] = int(std::ceil(r_ext * c.reclen[a] / wmin));
-    ea.full[a] = 2 * ea.D[a] + 1 > h->nt[a] ? 1 : 0;
-    if (!(r_ext * c.reclen[a] < 0.5)) linked = false;
-  }
-  if (!linked)
-    for (int a = 0; a < 3; ++a) ea.D[a] = int(std::ceil(r_ext * c.reclen[a])) + 1;
-
-  // device scratch of this call: one allocation, carved
-  const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
-  const int p = prm.order;
-  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
   auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t b_frac = up(sizeof(double) * 3 * (size_t)m), b_q = up(sizeof(double) * m),
-               b_w = blur_widths ? up(sizeof(double) * m) : 0, b_k0 = up(sizeof(int4) * m),
-               b_start = up(sizeof(int) * (ntiles + 1)), b_th = up(3 * (size_t)m * p * rs),
-               b_n = up(sizeof(double) * n), b_one = up(sizeof(double));
-  const size_t total = b_frac + b_q + b_w + b_k0 + b_start + b_th + 3 * b_n + 2 * b_one;
-  struct Scratch {
-    unsigned char* p = nullptr;
-    ~Scratch() {
-      if (p) (void)hipFree(p);
-    }
-  } scr;
-  if (hipMalloc(&scr.p, total) != hipSuccess) {
-    (void)hipGetLastError();
-    return h->fail(EW_ERR_MEMORY, "ew_external_potential: device allocation failed");
-  }
-  unsigned char* cur = scr.p;
-  auto take = [&](size_t b) {
-    unsigned char* r = cur;
-    cur += b;
-    return r;
-  };
-  double* d_frac = reinterpret_cast<double*>(take(b_frac));
-  double* d_q = reinterpret_cast<double*>(take(b_q));
-  double* d_w = blur_widths ? reinterpret_cast<double*>(take(b_w)) : nullptr;
-  int4* d_k0 = reinterpret_cast<int4*>(take(b_k0));
-  int* d_start = reinterpret_cast<int*>(take(b_start));
-  void* d_theta = take(b_th);
-  double* d_zero = reinterpret_cast<double*>(take(b_n));
-  double* d_phi = reinterpret_cast<double*>(take(b_n));
-  double* d_out = reinterpret_cast<double*>(take(b_n));
-  double* d_qsum = reinterpret_cast<double*>(take(b_one));
-  double* d_energy = reinterpret_cast<double*>(take(b_one));
-  HIPCHK(h, hipMemcpyAsync(d_frac, o.frac.data(), sizeof(double) * 3 * m, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(d_q, qs.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
-  if (d_w) HIPCHK(h, hipMemcpyAsync(d_w, ws.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(d_k0, o.k0.data(), sizeof(int4) * m, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(d_start, o.start.data(), sizeof(int) * (ntiles + 1),
-                           hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(d_qsum, &qsum, sizeof(double), hipMemcpyHostToDevice, s));
+  ExtStage st;
+  CHK(stage_sources(h, m, ext_coords, ext_charges, blur_widths,
+                    3 * up(sizeof(double) * n) + 2 * up(sizeof(double)), st));
+  double* d_zero = st.take<double>(n);
+  double* d_phi = st.take<double>(n);
+  double* d_out = st.take<double>(n);
+  double* d_qsum = st.take<double>(1);
+  double* d_energy = st.take<double>(1);
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(d_qsum, &st.qsum, sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemsetAsync(d_zero, 0, sizeof(double) * n, s));
-
-  // reciprocal part: the sources' SPME mesh, convolved with the influence function
-  BsplineArgs ba{{prm.K[0], prm.K[1], prm.K[2]}, p};
-  HIPCHK(h, launch_bspline(m, d_frac, ba, d_theta, h->fp32, s));
-  SpmeLaunch L = make_spme_launch(h);
-  L.k0 = d_k0;
-  L.theta = d_theta;
-  L.q = d_q;
-  L.qperm = nullptr;
-  L.tile_start = d_start;
-  HIPCHK(h, h->fp32 ? launch_spme<float>(L, s, nullptr) : launch_spme<double>(L, s, nullptr));
+  CHK(spme_of_sources(h, st));
   // real part at the atoms, then mesh part + real part + background per source
-  ea.frac = d_frac;
-  ea.q = d_q;
-  ea.blur = d_w;
-  ea.start = d_start;
-  HIPCHK(h, launch_ext_real(n, h->frac, h->tile_of, ea, !linked, d_phi, s));
+  st.ea.eps2 = eps_soften * eps_soften;
+  HIPCHK(h, launch_ext_real(n, h->frac, h->tile_of, st.ea, !st.linked, d_phi, s));
   GatherArgs ga = gather_args(h);
   ga.self_coef = 0.0;
   ga.qsum = d_qsum;
   HIPCHK(h, launch_gather(h, ga, d_zero, d_phi, nullptr, d_out, d_energy, s));
   HIPCHK(h, hipMemcpyAsync(potential, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(h, hipStreamSynchronize(s));
+  return EW_OK;
+}
+
+// Derivatives of the interaction of the resident atoms with external point charges, the periodic
+// forms of TCoulomb%getPotentialGradient and %addExternalPotGrad (src/dftbp/dftb/coulomb.F90:
+// 3162-3235 -> addInvRPrimePeriodicAsymm 1934-2092, derivEwaldReciprocal 2388-2425,
+// derivEwaldReal 2577-2620).  With atom charges dq: atom_grad[i] += dq_i sum_s q_s
+// grad ewald(x_i - y_s) and, if ext_grad is given, ext_grad[s] -= the same summed over the
+// atoms.  Without dq (the Hamiltonian derivative): atom_grad[i] += sum_s q_s grad ewald(x_i - y_s).
+// Reciprocal parts from the B-spline derivatives of the sources' mesh at the atoms and of the
+// atoms' mesh at the sources; real parts as in ew_external_potential (no softening, as in the
+// reference).
+extern "C" int ew_external_gradients(ew_handle* h, int m, const double* ext_coords,
+                                     const double* ext_charges, const double* blur_widths,
+                                     const double* dq, double* atom_grad, double* ext_grad) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_external_gradients: handle is NULL");
+  if (!h->have_geom)
+    return h->fail(EW_ERR_NO_GEOMETRY, "ew_external_gradients before ew_set_geometry");
+  if (!atom_grad || m < 0 || (m > 0 && (!ext_coords || !ext_charges)))
+    return h->fail(EW_ERR_ARGUMENT, "ew_external_gradients: bad arguments");
+  const int n = h->n;
+  if (dq)
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(dq[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+  if (m == 0) return EW_OK;
+  const bool want_ext = dq && ext_grad;
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  ExtStage st;
+  CHK(stage_sources(h, m, ext_coords, ext_charges, blur_widths,
+                    2 * up(sizeof(double) * 3 * (size_t)n) + up(sizeof(double) * n) +
+                        (want_ext ? 2 * up(sizeof(double) * 3 * (size_t)m) : 0),
+                    st));
+  double* d_frec = st.take<double>(3 * (size_t)n);
+  double* d_greal = st.take<double>(3 * (size_t)n);
+  double* d_scale = st.take<double>(n);
+  double* d_srec = want_ext ? st.take<double>(3 * (size_t)m) : nullptr;
+  double* d_sreal = want_ext ? st.take<double>(3 * (size_t)m) : nullptr;
+  hipStream_t s = h->stream;
+  // atom charges in sorted order (ones for the Hamiltonian derivative)
+  if (dq) {
+    HIPCHK(h, hipMemcpyAsync(h->dq_io, dq, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(h, launch_gather_charges(n, h->perm, h->dq_io, h->qs, nullptr, s));
+    HIPCHK(h, hipMemcpyAsync(d_scale, h->qs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  } else {
+    const std::vector<double> ones(n, 1.0);
+    HIPCHK(h, hipMemcpyAsync(d_scale, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipStreamSynchronize(s));  // ones is a host temporary
+  }
+  // atoms: -dq grad(sources' mesh potential), and dq times the real-space gradient
+  CHK(spme_of_sources(h, st));
+  const ForceArgs fa = force_args(h);
+  HIPCHK(h, h->fp32 ? launch_force_recip<float>(n, h->frac, h->k0, h->grid, d_scale, h->perm, fa,
+                                                 d_frec, s)
+                    : launch_force_recip<double>(n, h->frac, h->k0, h->grid, d_scale, h->perm,
+                                                 fa, d_frec, s));
+  HIPCHK(h, launch_ext_real_grad(n, h->frac, h->tile_of, h->perm, d_scale, st.ea, !st.linked,
+                                 d_greal, s));
+  if (want_ext) {
+    // sources: -q_s grad(atoms' mesh potential), and q_s times the real-space gradient of the
+    // atoms' potential (the Gaussian width is the source's own)
+    SpmeLaunch L = make_spme_launch(h);
+    HIPCHK(h, h->fp32 ? launch_spme<float>(L, s, nullptr) : launch_spme<double>(L, s, nullptr));
+    HIPCHK(h, h->fp32 ? launch_force_recip<float>(m, st.frac, st.k0, h->grid, st.q, st.perm_d,
+                                                   fa, d_srec, s)
+                      : launch_force_recip<double>(m, st.frac, st.k0, h->grid, st.q, st.perm_d,
+                                                   fa, d_srec, s));
+    ExtArgs ea = st.ea;
+    ea.m = n;
+    ea.frac = h->frac;
+    ea.q = h->qs;
+    ea.blur = nullptr;
+    ea.tblur = blur_widths ? st.w : nullptr;
+    ea.start = h->tile_start;
+    HIPCHK(h, launch_ext_real_grad(m, st.frac, st.tile, st.perm_d, st.q, ea, !st.linked, d_sreal,
+                                   s));
+  }
+  std::vector<double> frec(3 * (size_t)n), greal(3 * (size_t)n);
+  std::vector<double> srec(want_ext ? 3 * (size_t)m : 0), sreal(want_ext ? 3 * (size_t)m : 0);
+  HIPCHK(h, hipMemcpyAsync(frec.data(), d_frec, sizeof(double) * frec.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipMemcpyAsync(greal.data(), d_greal, sizeof(double) * greal.size(),
+                           hipMemcpyDeviceToHost, s));
+  if (want_ext) {
+    HIPCHK(h, hipMemcpyAsync(srec.data(), d_srec, sizeof(double) * srec.size(),
+                             hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(sreal.data(), d_sreal, sizeof(double) * sreal.size(),
+                             hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(h, hipStreamSynchronize(s));
+  for (size_t k = 0; k < frec.size(); ++k) atom_grad[k] += greal[k] - frec[k];
+  // d/dy_s ewald(x - y_s) = -grad ewald(x - y_s): ext_grad -= dq q_s grad ewald = q_s grad phi_atoms
+  for (size_t k = 0; k < srec.size(); ++k) ext_grad[k] += sreal[k] - srec[k];
   return EW_OK;
 }
 

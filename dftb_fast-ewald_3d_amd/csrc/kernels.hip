@@ -3026,29 +3026,72 @@ hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const Pai
 // external point charges
 // ----------------------------------------------------------------------------------------
 
+// Calls f(s0, s1, s2, j) for every source image a target at fractional (f0, f1, f2) in sort
+// tile `tile` may see: linked mode walks the source tiles around the target's tile with the
+// minimum image; dense mode takes every source over explicit image shells.
+template <bool DENSE, typename F>
+__device__ __forceinline__ void for_source_images(const ExtArgs& g, double f0, double f1,
+                                                  double f2, int tile, F&& f) {
+  if (DENSE) {
+    for (int j = 0; j < g.m; ++j) {
+      const double s0 = f0 - g.frac[3 * j], s1 = f1 - g.frac[3 * j + 1], s2 = f2 - g.frac[3 * j + 2];
+      for (int a = -g.D[0]; a <= g.D[0]; ++a)
+        for (int b = -g.D[1]; b <= g.D[1]; ++b)
+          for (int e = -g.D[2]; e <= g.D[2]; ++e) f(s0 + a, s1 + b, s2 + e, j);
+    }
+    return;
+  }
+  int t = tile;
+  const int t1 = t % g.nt[0];
+  t /= g.nt[0];
+  const int t2 = t % g.nt[1], t3 = t / g.nt[1];
+  const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
+  const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
+  const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
+  for (int c3 = 0; c3 < n3; ++c3) {
+    const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
+    for (int c2 = 0; c2 < n2; ++c2) {
+      const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
+      for (int c1 = 0; c1 < n1; ++c1) {
+        const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
+        const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
+        for (int j = g.start[tt]; j < g.start[tt + 1]; ++j) {
+          double s0 = f0 - g.frac[3 * j], s1 = f1 - g.frac[3 * j + 1], s2 = f2 - g.frac[3 * j + 2];
+          s0 -= rint(s0);
+          s1 -= rint(s1);
+          s2 -= rint(s2);
+          f(s0, s1, s2, j);
+        }
+      }
+    }
+  }
+}
+
+// An image closer than tolSameDist (1e-5, src/dftbp/common/accuracy.F90:69) is the source on the
+// target; erfArgLimit_ = 10 (coulomb.F90:214) bounds the Gaussian correction of a blurred source.
+constexpr double kExtSameDist2 = 1.0e-10;
+constexpr double kErfArgLimit = 10.0;
+
 // Real-space potential of the external charges at each resident atom (sorted order), one thread
 // per atom: sum_s q_s [erfc(alpha d) - erfc(d / w_s) (blurred, d < 10 w_s)] / sqrt(d^2 + eps^2)
 // over the source images within the cutoff (ewaldReal, src/dftbp/dftb/coulomb.F90:2523-2575:
-// erf(d/w) - erf(alpha d) written through erfc).  An image closer than tolSameDist (1e-5,
-// src/dftbp/common/accuracy.F90:69) contributes the self term -2 alpha / sqrt(pi) q_s instead
-// (ewald, coulomb.F90:2341-2343).  Linked mode walks the source tiles around the atom's tile
-// with the minimum image; dense mode sums every source over explicit image shells.
+// erf(d/w) - erf(alpha d) written through erfc).  A source on the atom contributes the self
+// term -2 alpha / sqrt(pi) q_s instead (ewald, coulomb.F90:2341-2343).
 template <bool DENSE>
 __global__ __launch_bounds__(128) void k_ext_real(int n, const double* __restrict__ frac,
                                                   const int* __restrict__ tile_of, ExtArgs g,
                                                   double* __restrict__ phi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  constexpr double kTolSameDist2 = 1.0e-10;
-  const double fi0 = frac[3 * i], fi1 = frac[3 * i + 1], fi2 = frac[3 * i + 2];
   double acc = 0.0;
-  auto add = [&](double s0, double s1, double s2, int j) {
+  for_source_images<DENSE>(g, frac[3 * i], frac[3 * i + 1], frac[3 * i + 2], DENSE ? 0 : tile_of[i],
+                           [&](double s0, double s1, double s2, int j) {
     const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
     const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
     const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
     const double r2 = x * x + y * y + z * z;
     if (r2 > g.rc2) return;
-    if (r2 < kTolSameDist2) {
+    if (r2 < kExtSameDist2) {
       acc += g.self_coef * g.q[j];
       return;
     }
@@ -3056,46 +3099,56 @@ __global__ __launch_bounds__(128) void k_ext_real(int n, const double* __restric
     double t = erfc(g.alpha * r);
     if (g.blur) {
       const double w = g.blur[j];
-      if (r < 10.0 * w) t -= erfc(r / w);  // erfArgLimit_ = 10, coulomb.F90:214
+      if (r < kErfArgLimit * w) t -= erfc(r / w);
     }
     acc += g.q[j] * t / (g.eps2 > 0.0 ? sqrt(r2 + g.eps2) : r);
-  };
-  if (DENSE) {
-    for (int j = 0; j < g.m; ++j) {
-      const double s0 = fi0 - g.frac[3 * j], s1 = fi1 - g.frac[3 * j + 1],
-                   s2 = fi2 - g.frac[3 * j + 2];
-      for (int a = -g.D[0]; a <= g.D[0]; ++a)
-        for (int b = -g.D[1]; b <= g.D[1]; ++b)
-          for (int e = -g.D[2]; e <= g.D[2]; ++e) add(s0 + a, s1 + b, s2 + e, j);
-    }
-  } else {
-    int t = tile_of[i];
-    const int t1 = t % g.nt[0];
-    t /= g.nt[0];
-    const int t2 = t % g.nt[1], t3 = t / g.nt[1];
-    const int b1 = g.full[0] ? 0 : t1 - g.D[0], n1 = g.full[0] ? g.nt[0] : 2 * g.D[0] + 1;
-    const int b2 = g.full[1] ? 0 : t2 - g.D[1], n2 = g.full[1] ? g.nt[1] : 2 * g.D[1] + 1;
-    const int b3 = g.full[2] ? 0 : t3 - g.D[2], n3 = g.full[2] ? g.nt[2] : 2 * g.D[2] + 1;
-    for (int c3 = 0; c3 < n3; ++c3) {
-      const int u3 = ((b3 + c3) % g.nt[2] + g.nt[2]) % g.nt[2];
-      for (int c2 = 0; c2 < n2; ++c2) {
-        const int u2 = ((b2 + c2) % g.nt[1] + g.nt[1]) % g.nt[1];
-        for (int c1 = 0; c1 < n1; ++c1) {
-          const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
-          const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
-          for (int j = g.start[tt]; j < g.start[tt + 1]; ++j) {
-            double s0 = fi0 - g.frac[3 * j], s1 = fi1 - g.frac[3 * j + 1],
-                   s2 = fi2 - g.frac[3 * j + 2];
-            s0 -= rint(s0);
-            s1 -= rint(s1);
-            s2 -= rint(s2);
-            add(s0, s1, s2, j);
-          }
-        }
-      }
-    }
-  }
+  });
   phi[i] = acc;
+}
+
+// Real-space gradient with respect to the target position of the same sum (derivEwaldReal,
+// src/dftbp/dftb/coulomb.F90:2577-2620): per image v = x_t - y_s + L, |v| = d,
+//   q_s v [-(2/sqrt(pi)) alpha d exp(-alpha^2 d^2) - erfc(alpha d)] / d^3
+//   + q_s v [(2/sqrt(pi)) f exp(-f^2) + erfc(f)] / d^3,  f = d / w  (blurred, d < 10 w),
+// skipping a source on the target.  The Gaussian width is the source's (blur) or the
+// target's (tblur: the external charges as targets of the atoms).  out[3 perm[i] + c] =
+// scale[i] times the sum (scale null: 1).
+template <bool DENSE>
+__global__ __launch_bounds__(128) void k_ext_real_grad(int n, const double* __restrict__ frac,
+                                                       const int* __restrict__ tile_of,
+                                                       const int* __restrict__ perm,
+                                                       const double* __restrict__ scale,
+                                                       ExtArgs g, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double two_sqrtpi = 2.0 / kSqrtPi;
+  const double wt = g.tblur ? g.tblur[i] : 0.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for_source_images<DENSE>(g, frac[3 * i], frac[3 * i + 1], frac[3 * i + 2], DENSE ? 0 : tile_of[i],
+                           [&](double s0, double s1, double s2, int j) {
+    const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+    const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+    const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+    const double r2 = x * x + y * y + z * z;
+    if (r2 > g.rc2 || r2 < kExtSameDist2) return;
+    const double r = sqrt(r2);
+    const double ar = g.alpha * r;
+    double c = -two_sqrtpi * ar * exp(-ar * ar) - erfc(ar);
+    const double w = g.tblur ? wt : (g.blur ? g.blur[j] : 0.0);
+    if (w > 0.0 && r < kErfArgLimit * w) {
+      const double f = r / w;
+      c += two_sqrtpi * f * exp(-f * f) + erfc(f);
+    }
+    c *= g.q[j] / (r2 * r);
+    a0 += c * x;
+    a1 += c * y;
+    a2 += c * z;
+  });
+  const double sc = scale ? scale[i] : 1.0;
+  const int o = perm[i];
+  out[3 * o] = sc * a0;
+  out[3 * o + 1] = sc * a1;
+  out[3 * o + 2] = sc * a2;
 }
 
 hipError_t launch_ext_real(int n, const double* frac, const int* tile_of, const ExtArgs& g,
@@ -3107,5 +3160,33 @@ hipError_t launch_ext_real(int n, const double* frac, const int* tile_of, const 
     hipLaunchKernelGGL(k_ext_real<false>, grd, blk, 0, st, n, frac, tile_of, g, phi);
   return hipGetLastError();
 }
+
+hipError_t launch_ext_real_grad(int n, const double* frac, const int* tile_of, const int* perm,
+                                const double* scale, const ExtArgs& g, bool dense, double* out,
+                                hipStream_t st) {
+  const dim3 grd((n + 127) / 128), blk(128);
+  if (dense)
+    hipLaunchKernelGGL(k_ext_real_grad<true>, grd, blk, 0, st, n, frac, tile_of, perm, scale, g,
+                       out);
+  else
+    hipLaunchKernelGGL(k_ext_real_grad<false>, grd, blk, 0, st, n, frac, tile_of, perm, scale, g,
+                       out);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_force_recip(int n, const double* frac, const int4* k0, const void* grid,
+                              const double* q, const int* perm, const ForceArgs& fa,
+                              double* force, hipStream_t st) {
+  hipLaunchKernelGGL(k_force_recip<R>, dim3((n + 127) / 128), dim3(128), 0, st, n, frac, k0,
+                     static_cast<const R*>(grid), q, perm, fa, force);
+  return hipGetLastError();
+}
+template hipError_t launch_force_recip<double>(int, const double*, const int4*, const void*,
+                                               const double*, const int*, const ForceArgs&,
+                                               double*, hipStream_t);
+template hipError_t launch_force_recip<float>(int, const double*, const int4*, const void*,
+                                              const double*, const int*, const ForceArgs&,
+                                              double*, hipStream_t);
 
 }  // namespace ewamd

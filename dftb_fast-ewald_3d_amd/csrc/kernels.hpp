@@ -353,10 +353,20 @@ struct ExtArgs {
   const double* frac;   // sources in tile order, fractional [m][3]
   const double* q;      // their charges
   const double* blur;   // their Gaussian widths, or null (point charges)
+  const double* tblur;  // gradient only: the targets' Gaussian widths (used in place of blur)
   const int* start;     // [ntiles + 1] first source of each tile
 };
 // phi[k] = real-space potential of the sources at sorted target k (overwritten).
 hipError_t launch_ext_real(int n, const double* frac, const int* tile_of, const ExtArgs& g,
                            bool dense, double* phi, hipStream_t st);
+// out[perm[k]][3] = scale[k] * (gradient of that real-space sum at sorted target k), scale null: 1.
+hipError_t launch_ext_real_grad(int n, const double* frac, const int* tile_of, const int* perm,
+                                const double* scale, const ExtArgs& g, bool dense, double* out,
+                                hipStream_t st);
+// Reciprocal force only: force[perm[k]][3] = -q_k grad(mesh potential) at sorted point k.
+template <typename R>
+hipError_t launch_force_recip(int n, const double* frac, const int4* k0, const void* grid,
+                              const double* q, const int* perm, const ForceArgs& fa,
+                              double* force, hipStream_t st);
 
 }  // namespace ewamd

@@ -40,7 +40,8 @@ MAX_VECTORS = 8
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
             "ew_shift", "ew_shift_device", "ew_shift_multi", "ew_shift_multi_device",
-            "ew_gradients", "ew_stress", "ew_external_potential", "ew_phase_times",
+            "ew_gradients", "ew_stress", "ew_external_potential", "ew_external_gradients",
+            "ew_phase_times",
             "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
@@ -160,6 +161,8 @@ def lib() -> C.CDLL:
         L.ew_stress.argtypes = [H, _D1, _D2]
         L.ew_external_potential.argtypes = [H, C.c_int, _optional(_D2), _optional(_D1),
                                             _optional(_D1), C.c_double, _D1]
+        L.ew_external_gradients.argtypes = [H, C.c_int, _optional(_D2), _optional(_D1),
+                                            _optional(_D1), _optional(_D1), _D2, _optional(_D2)]
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
@@ -351,6 +354,23 @@ class EwaldEngine:
         _check(lib().ew_external_potential(self._h, m, xy, q if m else None, w,
                                            float(eps_soften), pot), self._h)
         return pot
+
+    def external_gradients(self, coords, charges, dq=None, blur_widths=None):
+        """Derivatives of the atoms' interaction with external point charges.  With atom charges
+        dq: (atom_grad (n, 3), ext_grad (m, 3)) = (dq_i sum_s q_s grad ewald(x_i - y_s), minus
+        its sum over the atoms per source); without: (sum_s q_s grad ewald(x_i - y_s), None)."""
+        q = _f64(charges, 1)
+        m = len(q)
+        xy = _f64(coords, 2) if m else None
+        if m and xy.shape != (m, 3):
+            raise ValueError("coords must be (m, 3) for m charges")
+        w = None if blur_widths is None else _f64(blur_widths, 1)
+        d = None if dq is None else _f64(dq, 1)
+        ga = np.zeros((self.n, 3))
+        gs = np.zeros((m, 3)) if d is not None else None
+        _check(lib().ew_external_gradients(self._h, m, xy, q if m else None, w, d, ga,
+                                           gs if (gs is not None and m) else None), self._h)
+        return ga, gs
 
     def phase_times(self, reset: bool = False):
         """({phase: ms}, calls) accumulated since the last reset (needs profile=True)."""

@@ -20,6 +20,7 @@ module fftewald_amd
    public :: ewald_potential_force, ewald_parameters, ewald_grid_memory
    public :: engine_create, engine_set_geometry, engine_shift, engine_destroy, engine_error
    public :: engine_add_gradients, engine_add_stress, engine_shift_multi, engine_get_potential
+   public :: engine_add_external_pot_grad, engine_get_potential_gradient
    public :: ew_mixer_options, TEwaldMixer, mixer_create, mixer_reset, mixer_mix, mixer_destroy
    public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED, EW_MIXER_BROYDEN, EW_MIXER_SIMPLE
 
@@ -146,6 +147,19 @@ module fftewald_amd
          real(c_double), value :: eps
          real(c_double), intent(out) :: pot(*)
       end function c_ew_external_potential
+
+      integer(c_int) function c_ew_external_gradients(h, m, coords, charges, blur, dq, agrad, &
+         egrad) bind(C, name="ew_external_gradients")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: m
+         real(c_double), intent(in) :: coords(3, *)
+         real(c_double), intent(in) :: charges(*)
+         type(c_ptr), value :: blur
+         type(c_ptr), value :: dq
+         real(c_double), intent(inout) :: agrad(3, *)
+         type(c_ptr), value :: egrad
+      end function c_ew_external_gradients
 
       integer(c_int) function c_ew_potential_force(n, coords, charges, latvecs, opts, pot, force) &
          bind(C, name="ew_potential_force")
@@ -413,6 +427,45 @@ contains
       st = c_ew_external_potential(this%handle, int(size(charges), c_int), chargeCoords, charges, &
          pb, eps, potential)
    end function engine_get_potential
+
+   !> TCoulomb%addExternalPotGrad, periodic (coulomb.F90:3197-3235): atomGrads(3, nAtom) +=
+   !> dq_i sum_s q_s grad ewald(x_i - y_s), extChargeGrads(3, m) -= the same per point charge.
+   function engine_add_external_pot_grad(this, chargeCoords, charges, deltaQAtom, atomGrads, &
+      extChargeGrads, blurWidths) result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: chargeCoords(:, :)
+      real(c_double), intent(in) :: charges(:)
+      real(c_double), intent(in), target, contiguous :: deltaQAtom(:)
+      real(c_double), intent(inout) :: atomGrads(:, :)
+      real(c_double), intent(inout), target, contiguous :: extChargeGrads(:, :)
+      real(c_double), intent(in), optional, target, contiguous :: blurWidths(:)
+      integer(c_int) :: st
+      type(c_ptr) :: pb
+
+      pb = c_null_ptr
+      if (present(blurWidths)) pb = c_loc(blurWidths)
+      st = c_ew_external_gradients(this%handle, int(size(charges), c_int), chargeCoords, charges, &
+         pb, c_loc(deltaQAtom), atomGrads, c_loc(extChargeGrads))
+   end function engine_add_external_pot_grad
+
+   !> TCoulomb%getPotentialGradient, periodic (coulomb.F90:3162-3195): gradient(3, nAtom) =
+   !> sum_s q_s grad ewald(x_i - y_s) (overwritten, as the reference zeroes it first).
+   function engine_get_potential_gradient(this, chargeCoords, charges, gradient, blurWidths) &
+      result(st)
+      type(TEwaldEngine), intent(inout) :: this
+      real(c_double), intent(in) :: chargeCoords(:, :)
+      real(c_double), intent(in) :: charges(:)
+      real(c_double), intent(out) :: gradient(:, :)
+      real(c_double), intent(in), optional, target, contiguous :: blurWidths(:)
+      integer(c_int) :: st
+      type(c_ptr) :: pb
+
+      pb = c_null_ptr
+      if (present(blurWidths)) pb = c_loc(blurWidths)
+      gradient(:, :) = 0.0_c_double
+      st = c_ew_external_gradients(this%handle, int(size(charges), c_int), chargeCoords, charges, &
+         pb, c_null_ptr, gradient, c_null_ptr)
+   end function engine_get_potential_gradient
 
    function engine_shift(this, deltaQAtom, shift, energy) result(st)
       type(TEwaldEngine), intent(inout) :: this

@@ -194,6 +194,18 @@ int ew_external_potential(ew_handle *h, int m, const double *ext_coords,
                           const double *ext_charges, const double *blur_widths,
                           double eps_soften, double *potential);
 
+/* TCoulomb%getPotentialGradient and %addExternalPotGrad, periodic branch
+ * (src/dftbp/dftb/coulomb.F90:3162-3235 -> addInvRPrimePeriodicAsymm 1934-2092): derivatives of
+ * the interaction of the resident atoms with m external point charges (same sources, widths
+ * and conventions as ew_external_potential; no softening, as in the reference).
+ *   dq != NULL (forces):  atom_grad[n][3] += dq_i sum_s q_s grad ewald(x_i - y_s) and, when
+ *                         ext_grad != NULL, ext_grad[m][3] -= the same summed over the atoms;
+ *   dq == NULL (tHamDeriv): atom_grad[n][3] += sum_s q_s grad ewald(x_i - y_s)  (ext_grad unused).
+ * Both arrays are accumulated into, in DFTB+'s deriv(3, N) layout. */
+int ew_external_gradients(ew_handle *h, int m, const double *ext_coords,
+                          const double *ext_charges, const double *blur_widths,
+                          const double *dq, double *atom_grad, double *ext_grad);
+
 /* ---- one-shot calls, the reference library's own entry points ---- */
 
 /* ewald_potential_force (fe/fft/ewald_fft_3d.f90:36-134): pot[i] = phi(x_i) and, when force

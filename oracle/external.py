@@ -18,11 +18,17 @@ within cutoffs chosen so that the dropped terms are below `tol` (the sums are th
 rounding, and independent of alpha -- except with eps_soften > 0, which softens only the
 real-space denominator, so that the result depends on alpha as the reference's does).
 
+Gradients follow addInvRPrimePeriodicAsymm (1934-2092) with derivEwaldReciprocal (2388-2425)
+and derivEwaldReal (2577-2620): deriv0(:, t) += q_t q_s grad ewald(x_t - y_s), deriv1(:, s) -=
+the same (q_t = 1 and no deriv1 for the Hamiltonian derivative); derivEwaldReal skips images
+closer than tolSameDist^2 in distance, as written.
+
 Pinned in tests/test_external_oracle.py: with the sources on the atoms the sum is the Ewald
 potential of the direct-Ewald restatement (itself pinned by Madelung constants); the total is
 alpha-independent for a non-neutral set of sources only with the background term; a blurred
 source differs from a point source by the closed form -q erfc(d/w)/d over its images; the
-softened form differs by q erfc(alpha d) (1/sqrt(d^2+eps^2) - 1/d).
+softened form differs by q erfc(alpha d) (1/sqrt(d^2+eps^2) - 1/d); the gradients are the
+central differences of sum_t q_t potential_t with respect to the atoms and the sources.
 """
 from __future__ import annotations
 
@@ -91,3 +97,56 @@ def external_potential(targets, sources, charges, lat, alpha, blur=None, eps_sof
                 val -= 2.0 * alpha / np.sqrt(np.pi)
             pot[t] += val * q[s]
     return pot
+
+
+def _lists(lat, alpha, tol, wmax, span):
+    rec = 2.0 * np.pi * np.linalg.inv(lat).T
+    lnt = np.sqrt(-np.log(tol))
+    g = _lattice_points(rec, 2.0 * alpha * lnt)
+    g = g[np.einsum("ij,ij->i", g, g) > 0.0]
+    key = np.where(g[:, 0] != 0, g[:, 0], np.where(g[:, 1] != 0, g[:, 1], g[:, 2]))
+    g = g[key > 0]
+    rmax = max(lnt / alpha, wmax * lnt)
+    return g, _lattice_points(lat, rmax + span)
+
+
+def external_gradients(targets, sources, charges, lat, alpha, target_charges=None, blur=None,
+                       tol=1.0e-17):
+    """(deriv0 (n, 3), deriv1 (m, 3)) of addInvRPrimePeriodicAsymm; target_charges None is the
+    Hamiltonian derivative (deriv0 = sum_s q_s grad ewald(x_t - y_s), deriv1 None)."""
+    x = np.asarray(targets, dtype=np.float64)
+    y = np.asarray(sources, dtype=np.float64)
+    q = np.asarray(charges, dtype=np.float64)
+    lat = np.asarray(lat, dtype=np.float64)
+    w = None if blur is None else np.asarray(blur, dtype=np.float64)
+    qt = np.ones(len(x)) if target_charges is None else np.asarray(target_charges, np.float64)
+    vol = abs(np.linalg.det(lat))
+    diff = x[:, None, :] - y[None, :, :]
+    g, lvec = _lists(lat, alpha, tol, 0.0 if w is None else w.max(),
+                     np.abs(diff).max() * np.sqrt(3.0))
+    g2 = np.einsum("ij,ij->i", g, g)
+    gw = np.exp(-g2 / (4.0 * alpha * alpha)) / g2
+    d0 = np.zeros((len(x), 3))
+    d1 = np.zeros((len(y), 3))
+    for t in range(len(x)):
+        for s in range(len(y)):
+            rr = diff[t, s]
+            # derivEwaldReciprocal (2388-2425)
+            rec = 2.0 * ((-np.sin(g @ rr) * gw) @ g) * 4.0 * np.pi / vol
+            # derivEwaldReal (2577-2620)
+            v = rr + lvec
+            d = np.sqrt(np.einsum("ij,ij->i", v, v))
+            keep = d >= TOL_SAME_DIST * TOL_SAME_DIST
+            v, d = v[keep], d[keep]
+            f = alpha * d
+            c = (-2.0 / np.sqrt(np.pi) * np.exp(-f * f) * f - erfc(f)) / d ** 3
+            if w is not None:
+                near = d < ERF_ARG_LIMIT * w[s]
+                fb = d / w[s]
+                c = c + np.where(near, (2.0 / np.sqrt(np.pi) * np.exp(-fb * fb) * fb + erfc(fb))
+                                 / d ** 3, 0.0)
+            real = c @ v
+            ft = (rec + real) * q[s] * qt[t]
+            d0[t] += ft
+            d1[s] -= ft
+    return d0, (None if target_charges is None else d1)

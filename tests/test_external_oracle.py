@@ -7,14 +7,15 @@ used as the checker of ew_external_potential.
   * alpha independence with non-neutral sources (fails without the background term);
   * closed forms of the two variants: a blurred source differs from a point source by
     -q erfc(d/w)/d summed over its images, a softened one by q erfc(alpha d)(1/sqrt(d^2+eps^2)-1/d);
-  * translation of sources and targets together by a lattice vector changes nothing.
+  * translation of sources and targets together by a lattice vector changes nothing;
+  * the gradients are central differences of the energy sum_t q_t potential_t.
 """
 import numpy as np
 import pytest
 from conftest import load_golden, rel_max
 from scipy.special import erfc
 
-from oracle.external import external_potential
+from oracle.external import external_gradients, external_potential
 
 
 @pytest.mark.parametrize("name", ["ref_triclinic.npz", "ref_small_cell.npz"])
@@ -88,3 +89,28 @@ def test_lattice_translation_invariance():
     p = external_potential(x, y, q, lat, alpha=0.4)
     shifted = external_potential(x + lat[0] - 2 * lat[2], y - lat[1], q, lat, alpha=0.4)
     assert rel_max(shifted, p) < 1e-12
+
+
+@pytest.mark.parametrize("blurred", [False, True])
+def test_gradients_are_energy_derivatives(blurred):
+    lat, x, y, q = _cell(13)
+    qt = np.random.default_rng(14).normal(size=len(x))
+    w = np.array([0.5, 0.8, 0.3, 1.0, 0.6]) if blurred else None
+    alpha = 0.4
+    d0, d1 = external_gradients(x, y, q, lat, alpha, target_charges=qt, blur=w)
+    h = 1e-5
+    energy = lambda xx, yy: float(qt @ external_potential(xx, yy, q, lat, alpha=alpha, blur=w))
+    for t, c in [(0, 0), (2, 1), (5, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[t, c] += h
+        xm[t, c] -= h
+        assert (energy(xp, y) - energy(xm, y)) / (2 * h) == pytest.approx(d0[t, c], abs=1e-8)
+    for s, c in [(0, 2), (3, 0), (4, 1)]:
+        yp, ym = y.copy(), y.copy()
+        yp[s, c] += h
+        ym[s, c] -= h
+        assert (energy(x, yp) - energy(x, ym)) / (2 * h) == pytest.approx(d1[s, c], abs=1e-8)
+    # the Hamiltonian derivative is the gradient of the potential itself
+    g0, none = external_gradients(x, y, q, lat, alpha, blur=w)
+    assert none is None
+    assert np.allclose(g0 * qt[:, None], d0, rtol=1e-12, atol=1e-15)

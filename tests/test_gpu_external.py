@@ -12,6 +12,10 @@ Checks, FP64 unless said otherwise:
   * 98,304 atoms with 20,000 sources against the CPU SPME of the same mesh, order, alpha and
     r_cut (sources and zero-charge targets in one SPME + linked-cell real space): 1e-11;
   * mixed precision against FP64: 1e-6;
+  * gradients (ew_external_gradients) against the restatement's derivatives (themselves pinned
+    by central differences): 1e-9, both forms (with atom charges, and the Hamiltonian
+    derivative), with and without blur; the atom and source gradients sum to zero (1e-8); at
+    98,304 atoms against the CPU SPME force difference with and without the sources: 1e-10;
   * no sources, and the error paths.
 """
 import sys
@@ -22,7 +26,7 @@ from conftest import load_golden, rel_max
 
 import fftewald_amd as ew
 from fftewald_amd import workloads as W
-from oracle.external import external_potential
+from oracle.external import external_gradients, external_potential
 
 pytestmark = pytest.mark.gpu
 
@@ -165,3 +169,63 @@ def test_no_sources_and_errors():
         # the handle still evaluates after the failed calls
         s, _ = e.shift(w["charges"])
         assert np.all(np.isfinite(s))
+
+
+def _grad_oracle(x, y, q, lat, alpha, **kw):
+    return external_gradients(x, y, q, lat, alpha, **kw)
+
+
+@pytest.mark.parametrize("case,blurred", [("linked", False), ("linked", True),
+                                          ("image_sum", False)])
+def test_gradients_against_reference_restatement(case, blurred):
+    if case == "linked":
+        lat = W.lattice_from_parameters(16.0 * W.AA, 15.0 * W.AA, 17.0 * W.AA, 80.0, 95.0, 72.0)
+        x, qa, y, qs = _scene(lat, 240, 9, 41, on_atoms=0)
+    else:
+        g = load_golden("ref_small_cell.npz")
+        lat, x, qa = g["lattice"], g["coords"], g["charges"]
+        rng = np.random.default_rng(42)
+        y = rng.random((6, 3)) @ lat
+        qs = rng.normal(size=6)
+    w = np.random.default_rng(43).uniform(0.3, 1.1, len(qs)) if blurred else None
+    say(f"external-charge gradients, {case}, blur={blurred}: {len(x)} atoms, {len(y)} sources")
+    with ew.EwaldEngine(tol=1e-12, order=12) as e:
+        e.set_geometry(x, lat, charges=qa)
+        ga, gs = e.external_gradients(y, qs, dq=qa, blur_widths=w)
+        gh, none = e.external_gradients(y, qs, blur_widths=w)
+    assert none is None
+    alpha = np.sqrt(np.pi) / abs(np.linalg.det(lat)) ** (1.0 / 3.0)
+    d0, d1 = _grad_oracle(x, y, qs, lat, alpha, target_charges=qa, blur=w)
+    h0, _ = _grad_oracle(x, y, qs, lat, alpha, blur=w)
+    assert rel_max(ga, d0) < 1e-9
+    assert rel_max(gs, d1) < 1e-9
+    assert rel_max(gh, h0) < 1e-9
+    # Newton's third law: the interaction is translation invariant
+    assert np.abs(ga.sum(0) + gs.sum(0)).max() < 1e-8 * np.abs(ga).max()
+
+
+def test_gradients_c3_sized_against_cpu_spme(oracle_mod):
+    w = W.triclinic()
+    n = len(w["charges"])
+    m = 20000
+    rng = np.random.default_rng(51)
+    y = rng.random((m, 3)) @ w["lattice"]
+    qs = rng.normal(size=m)
+    say(f"triclinic {n} atoms, {m} external charges: Hamiltonian-derivative gradient, FP64")
+    opts = dict(tol=1e-10, r_cut=w["r_cut"], grid=w["grid"], order=w["order"])
+    with ew.EwaldEngine(**opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+        prm = e.parameters()
+        gh, _ = e.external_gradients(y, qs)
+    # unit charges on the atoms with and without the sources: the force difference is
+    # -grad of the sources' potential (SPME and the linked-cell real space are linear in q)
+    pos = np.concatenate([w["coords"], y])
+    args = (w["lattice"], prm["grid"], prm["order"], prm["alpha"])
+    q1 = np.concatenate([np.ones(n), qs])
+    q0 = np.concatenate([np.ones(n), np.zeros(m)])
+    f1 = oracle_mod.spme_reciprocal(pos, q1, *args, want_force=True)[1]
+    f0 = oracle_mod.spme_reciprocal(pos, q0, *args, want_force=True)[1]
+    r1 = oracle_mod.real_space(pos, q1, w["lattice"], prm["alpha"], prm["r_cut"], want_force=True)[1]
+    r0 = oracle_mod.real_space(pos, q0, w["lattice"], prm["alpha"], prm["r_cut"], want_force=True)[1]
+    want = -((f1 - f0) + (r1 - r0))[:n]
+    assert rel_max(gh, want) < 1e-10
