@@ -8,6 +8,10 @@
 !>   input : nElem, nIter, maxIter, mixParam, omega0, minWeight, maxWeight, weightFac,
 !>           q0(nElem), qDiff(nElem, nIter)
 !>   output: qInpResult(nElem, nIter) after each mix
+!> With "external" it drives the external-charge calls as TCoulomb%getPotential,
+!> %getPotentialGradient and %addExternalPotGrad would be driven:
+!>   input : n, coords(3,n), charges(n), latvecs(3,3), tol, m, chargeCoords(3,m), q(m), blur(m), eps
+!>   output: pot(n), potBlurSoft(n), atomGrads(3,n), extGrads(3,m), potGrad(3,n)
 program fortran_check
    use, intrinsic :: iso_c_binding, only: c_double, c_int
    use fftewald_amd
@@ -18,13 +22,18 @@ program fortran_check
    real(c_double) :: lat(3, 3), tol, alpha, energy
    type(TEwaldEngine) :: engine
    type(ew_options) :: opts
-   character(len=512) :: inpath, outpath
+   character(len=512) :: inpath, outpath, mode
    integer :: u
 
    call get_command_argument(1, inpath)
    call get_command_argument(2, outpath)
    if (command_argument_count() >= 3) then
-      call mixer_check(trim(inpath), trim(outpath))
+      call get_command_argument(3, mode)
+      if (trim(mode) == "external") then
+         call external_check(trim(inpath), trim(outpath))
+      else
+         call mixer_check(trim(inpath), trim(outpath))
+      end if
       stop
    end if
    open (newunit=u, file=trim(inpath), access="stream", form="unformatted", status="old")
@@ -98,5 +107,44 @@ contains
       close (u)
       print '(A,I0,A,I0)', "fortran_check mixer: nElem=", nElem, " nIter=", nIter
    end subroutine mixer_check
+
+   subroutine external_check(inpath, outpath)
+      character(len=*), intent(in) :: inpath, outpath
+      integer(c_int) :: n, m, st
+      real(c_double), allocatable :: coords(:, :), charges(:), ext(:, :), qext(:), blur(:)
+      real(c_double), allocatable :: pot(:), potb(:), agrad(:, :), egrad(:, :), pgrad(:, :)
+      real(c_double) :: lat(3, 3), tol, eps
+      type(TEwaldEngine) :: engine
+      type(ew_options) :: opts
+      integer :: u
+
+      open (newunit=u, file=inpath, access="stream", form="unformatted", status="old")
+      read (u) n
+      allocate (coords(3, n), charges(n), pot(n), potb(n), agrad(3, n), pgrad(3, n))
+      read (u) coords, charges, lat, tol, m
+      allocate (ext(3, m), qext(m), blur(m), egrad(3, m))
+      read (u) ext, qext, blur, eps
+      close (u)
+      opts%tol = tol
+      st = engine_create(engine, opts)
+      if (st /= EW_OK) error stop "engine_create failed"
+      st = engine_set_geometry(engine, coords, lat, charges=charges)
+      if (st /= EW_OK) error stop "engine_set_geometry failed"
+      st = engine_get_potential(engine, ext, qext, pot)
+      if (st /= EW_OK) error stop "engine_get_potential failed"
+      st = engine_get_potential(engine, ext, qext, potb, blurWidths=blur, epsSoften=eps)
+      if (st /= EW_OK) error stop "engine_get_potential (blur) failed"
+      agrad(:, :) = 0.0_c_double
+      egrad(:, :) = 0.0_c_double
+      st = engine_add_external_pot_grad(engine, ext, qext, charges, agrad, egrad)
+      if (st /= EW_OK) error stop "engine_add_external_pot_grad failed"
+      st = engine_get_potential_gradient(engine, ext, qext, pgrad)
+      if (st /= EW_OK) error stop "engine_get_potential_gradient failed"
+      call engine_destroy(engine)
+      open (newunit=u, file=outpath, access="stream", form="unformatted", status="replace")
+      write (u) pot, potb, agrad, egrad, pgrad
+      close (u)
+      print '(A,I0,A,I0)', "fortran_check external: n=", n, " m=", m
+   end subroutine external_check
 
 end program fortran_check

@@ -73,3 +73,59 @@ def test_fortran_broyden_mixer_reference_vectors(tmp_path):
     hist = np.fromfile(out, dtype=np.float64).reshape(niter, n)
     want = np.array(g["expected_charges"][1:])
     assert np.abs(hist - want).max() <= g["atol_eps"] * np.finfo(np.float64).eps
+
+
+def test_fortran_external_charges(tmp_path, oracle_mod):
+    """TCoulomb%getPotential / %getPotentialGradient / %addExternalPotGrad as DFTB+ would call
+    them from Fortran (chargeCoords(3, m), deriv(3, N)), against the CPU restatement of the
+    reference's ewald() and its derivatives (oracle/external.py)."""
+    from oracle.external import external_gradients, external_potential
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} not built (run __graft_entry__.build())")
+    g = load_golden("ref_triclinic.npz")
+    x, q, lat = g["coords"], g["charges"], g["lattice"]
+    n = len(q)
+    rng = np.random.default_rng(61)
+    m = 6
+    y = rng.random((m, 3)) @ lat
+    qe = rng.normal(size=m)
+    blur = rng.uniform(0.4, 1.0, m)
+    eps = 0.3
+    inp, out = tmp_path / "ext_in.bin", tmp_path / "ext_out.bin"
+    with open(inp, "wb") as f:
+        np.array([n], dtype=np.int32).tofile(f)
+        x.astype(np.float64).tofile(f)
+        q.astype(np.float64).tofile(f)
+        lat.astype(np.float64).tofile(f)
+        np.array([1e-12], dtype=np.float64).tofile(f)
+        np.array([m], dtype=np.int32).tofile(f)
+        y.tofile(f)
+        qe.tofile(f)
+        blur.tofile(f)
+        np.array([eps], dtype=np.float64).tofile(f)
+    print("\n[gpu-case] fortran_check external: 300 atoms, 6 point charges", flush=True)
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = os.path.join(PKG, "lib") + ":" + env.get("LD_LIBRARY_PATH", "")
+    r = subprocess.run([BIN, str(inp), str(out), "external"], env=env, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    data = np.fromfile(out, dtype=np.float64)
+    pot, potb = data[:n], data[n:2 * n]
+    agrad = data[2 * n:5 * n].reshape(n, 3)
+    egrad = data[5 * n:5 * n + 3 * m].reshape(m, 3)
+    pgrad = data[5 * n + 3 * m:].reshape(n, 3)
+    a_bal = np.sqrt(np.pi) / abs(np.linalg.det(lat)) ** (1.0 / 3.0)
+    assert rel_max(pot, external_potential(x, y, qe, lat, alpha=a_bal)) < 1e-10
+    d0, d1 = external_gradients(x, y, qe, lat, a_bal, target_charges=q)
+    h0, _ = external_gradients(x, y, qe, lat, a_bal)
+    assert rel_max(agrad, d0) < 1e-9
+    assert rel_max(egrad, d1) < 1e-9
+    assert rel_max(pgrad, h0) < 1e-9
+    # the softened potential depends on alpha: the restatement at the engine's own alpha
+    import fftewald_amd as ew
+    with ew.EwaldEngine(tol=1e-12) as e:
+        e.set_geometry(x, lat, charges=q)
+        alpha = e.parameters()["alpha"]
+    want = external_potential(x[:60], y, qe, lat, alpha=alpha, blur=blur, eps_soften=eps)
+    assert rel_max(potb[:60], want) < 1e-10
