@@ -428,6 +428,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=64, help="serving: structures per launch")
     ap.add_argument("--vectors", default="2,4,8",
                     help="triclinic: charge vectors per multi-vector call (empty: skip)")
+    ap.add_argument("--external", type=int, default=20000,
+                    help="triclinic: external point charges timed once per precision (0: skip)")
     args = ap.parse_args()
     args.vectors = [int(v) for v in args.vectors.split(",") if v.strip()]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -497,6 +499,28 @@ def main():
                 print(f"[bench] {prec} multi-vector x{nv}: {msm:.4f} ms/call, "
                       f"{nv * 1e3 / msm:.1f} evals/s", file=sys.stderr, flush=True)
                 del dqm, outm, enm
+            # external point charges (once per geometry in DFTB+): potential at the atoms
+            # and both sides' gradients, host arrays, averaged over a few calls
+            ext = None
+            if args.external > 0:
+                ry = np.random.default_rng(777)
+                ey = ry.random((args.external, 3)) @ w["lattice"]
+                eq = ry.normal(size=args.external)
+                e.external_potential(ey, eq)
+                t = time.perf_counter()
+                for _ in range(3):
+                    e.external_potential(ey, eq)
+                tp = (time.perf_counter() - t) / 3
+                e.external_gradients(ey, eq, dq=qs[0])
+                t = time.perf_counter()
+                for _ in range(3):
+                    e.external_gradients(ey, eq, dq=qs[0])
+                tg = (time.perf_counter() - t) / 3
+                ext = {"charges": args.external, "potential_ms": round(tp * 1e3, 3),
+                       "gradients_ms": round(tg * 1e3, 3)}
+                print(f"[bench] {prec} external charges x{args.external}: potential "
+                      f"{tp * 1e3:.2f} ms, gradients {tg * 1e3:.2f} ms", file=sys.stderr,
+                      flush=True)
         with ew.EwaldEngine(precision=prec, profile=True, **opts) as e:
             e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
             sp = torch.cuda.current_stream().cuda_stream
@@ -519,7 +543,7 @@ def main():
             if planes and k in ("fft_y", "fft_y_inverse"):
                 ph[k]["note"] = "fused into the x-y plane kernels (event overhead only)"
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
-                     "multi": multi,
+                     "multi": multi, "external": ext,
                      "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph,
                      "host_api_ms": host_ms}
         print(f"[bench] {prec}: {ms:.4f} ms/eval, {1e3 / ms:.1f} evals/s, geometry {geom_s:.2f} s,"
@@ -577,6 +601,7 @@ def main():
             max(m["evals_per_s"] for m in res["mixed"]["multi"].values()) /
             res["fp64"]["evals_per_s"], 3) if res["mixed"]["multi"] else None,
         "geometry_setup_s": {k: round(v["geometry_s"], 3) for k, v in res.items()},
+        "external_charges": {k: v["external"] for k, v in res.items()},
         "host_arrays_ms_per_eval": {k: round(v["host_api_ms"], 4) for k, v in res.items()},
         "phases": {k: v["phases"] for k, v in res.items()},
         "roofline": roofline,
