@@ -83,3 +83,13 @@ def test_create_without_device_fails_loudly():
     with pytest.raises(ew.EwaldError) as e:
         ew.EwaldEngine()
     assert e.value.code == ew.EW_ERR_DEVICE
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it exits with a message instead of running."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
