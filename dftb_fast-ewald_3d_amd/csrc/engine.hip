@@ -1226,6 +1226,18 @@ extern "C" int ew_shift_multi(ew_handle* h, int nvec, const double* dq, double* 
   return EW_OK;
 }
 
+// The resident row list when the real-space force and virial can walk it (linked mode, rows
+// stored in full); otherwise they walk the tiles.
+static PairRows pair_rows(const ew_handle* h) {
+  PairRows r;
+  if (h->pair_mode == 0 && !h->half && (h->col16 || h->col)) {
+    r.row_ptr = h->row_ptr;
+    r.col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
+    r.col16 = h->col16 != nullptr;
+  }
+  return r;
+}
+
 // Physical Ewald force -dU/dx (caller order, host array) at the charges of the last evaluation,
 // which must have run on this handle's stream: the mesh still holds the convolved potential.
 static int forces_after_eval(ew_handle* h, double* force_host) {
@@ -1238,12 +1250,15 @@ static int forces_after_eval(ew_handle* h, double* force_host) {
   }
   fa.order = h->prm.order;
   hipStream_t s = h->stream;
+  const PairRows rows = pair_rows(h);
   if (h->fp32)
     HIPCHK(h, launch_forces<float>(n, h->frac, h->k0, h->grid, h->qs, h->perm, h->tile_of,
-                                   h->tile_start, fa, h->pa, h->pair_mode == 1, h->force, s));
+                                   h->tile_start, fa, h->pa, h->pair_mode == 1, h->force, s,
+                                   rows));
   else
     HIPCHK(h, launch_forces<double>(n, h->frac, h->k0, h->grid, h->qs, h->perm, h->tile_of,
-                                    h->tile_start, fa, h->pa, h->pair_mode == 1, h->force, s));
+                                    h->tile_start, fa, h->pa, h->pair_mode == 1, h->force, s,
+                                    rows));
   HIPCHK(h, hipMemcpyAsync(force_host, h->force, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
   HIPCHK(h, hipStreamSynchronize(s));
   return EW_OK;
@@ -1286,7 +1301,9 @@ extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
   SpmeLaunch L = make_spme_launch(h);
   L.forward_only = 1;
   const long long gh = (long long)(h->prm.K[0] / 2 + 1) * h->prm.K[1] * h->prm.K[2];
-  const size_t nrec = (size_t)((gh + 255) / 256), nreal = (size_t)((n + 255) / 256);
+  const PairRows rows = pair_rows(h);
+  const size_t nrec = (size_t)((gh + 255) / 256),
+               nreal = (size_t)real_virial_parts(n, rows.row_ptr && h->pair_mode != 1);
   double* part = nullptr;
   if (hipMalloc(&part, sizeof(double) * 6 * (nrec + nreal)) != hipSuccess) {
     (void)hipGetLastError();
@@ -1299,7 +1316,7 @@ extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
                 : launch_recip_virial<double>(L, part, &nb1, s);
   if (e == hipSuccess)
     e = launch_real_virial(n, h->frac, h->tile_of, h->tile_start, h->qs, h->pa, h->pair_mode == 1,
-                           part + 6 * nrec, &nb2, s);
+                           part + 6 * nrec, &nb2, s, rows);
   std::vector<double> hp(6 * (nrec + nreal), 0.0);
   if (e == hipSuccess)
     e = hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s);

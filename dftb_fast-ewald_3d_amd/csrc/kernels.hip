@@ -12,8 +12,9 @@
 //   k_energy_final       fixed-order reduction of the energy partials
 // Several vectors / structures: the kg_* kernels (same bodies, a block prefix per kind) and
 // k_pair_spmm.  Per geometry: k_bspline (weights), k_pair_tiles (linked tiles), k_pair_dense,
-// k_compress_cols, k_half_pairs.  Stress and forces: k_recip_virial, k_real_virial,
-// k_force_recip, k_force_real.  External point charges: k_ext_real (with the spreading, FFTs
+// k_compress_cols, k_half_pairs.  Stress and forces: k_recip_virial, k_force_recip, and the
+// real-space parts over the pair list (k_pair_real_terms) or the tiles (k_real_virial,
+// k_force_real).  External point charges: k_ext_real (with the spreading, FFTs
 // and gather above run on the sources and the resident atoms).
 //
 // All FFTs are Stockham autosort transforms of one or more lines staged in LDS: every pass
@@ -2640,10 +2641,88 @@ hipError_t launch_recip_virial(const SpmeLaunch& L, double* partials, int* npart
 template hipError_t launch_recip_virial<double>(const SpmeLaunch&, double*, int*, hipStream_t);
 template hipError_t launch_recip_virial<float>(const SpmeLaunch&, double*, int*, hipStream_t);
 
+// Real-space force and virial over the resident pair list (linked mode), one wave per row:
+// the partners' displacements come from the sorted fractional coordinates (minimum image, the
+// list's own convention), and only pairs within r_cut contribute (the list reaches r_list).
+// Per pair c = q_j (2 alpha / sqrt(pi) exp(-alpha^2 r^2) + erfc(alpha r) / r) / r^2, the same
+// term as k_force_real / k_real_virial.
+template <typename IT, bool VIRIAL>
+__global__ __launch_bounds__(256) void k_pair_real_terms(int n, const long long* __restrict__ row_ptr,
+                                                         const IT* __restrict__ col,
+                                                         const double* __restrict__ frac,
+                                                         const double* __restrict__ q,
+                                                         const int* __restrict__ perm, PairArgs g,
+                                                         double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (row < n) {
+    const double fi0 = frac[3 * row], fi1 = frac[3 * row + 1], fi2 = frac[3 * row + 2];
+    const double two_a_sqrtpi = 2.0 * g.alpha / kSqrtPi;
+    const long long a = row_ptr[row], b = row_ptr[row + 1];
+    for (long long k = a + lane; k < b; k += 64) {
+      const int j = partner(row, col[k], n);
+      double s0 = fi0 - frac[3 * j], s1 = fi1 - frac[3 * j + 1], s2 = fi2 - frac[3 * j + 2];
+      s0 -= rint(s0);
+      s1 -= rint(s1);
+      s2 -= rint(s2);
+      const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
+      const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
+      const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
+      const double r2 = x * x + y * y + z * z;
+      if (r2 > g.rc2 || r2 < kMinSeparationSq) continue;
+      const double r = sqrt(r2);
+      const double c =
+          q[j] * (two_a_sqrtpi * exp(-g.alpha * g.alpha * r2) + erfc(g.alpha * r) / r) / r2;
+      if (VIRIAL) {
+        v[0] += c * x * x;
+        v[1] += c * y * y;
+        v[2] += c * z * z;
+        v[3] += c * x * y;
+        v[4] += c * x * z;
+        v[5] += c * y * z;
+      } else {
+        v[0] += c * x;
+        v[1] += c * y;
+        v[2] += c * z;
+      }
+    }
+  }
+  if (VIRIAL) {
+    const double h = row < n ? 0.5 * q[row] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] *= h;
+    block_sum6(v, out);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = wave_sum(v[c]);
+    if (row < n && lane == 0) {
+      const double qi = q[row];
+      const int o = perm[row];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[3 * o + c] += qi * v[c];
+    }
+  }
+}
+
 hipError_t launch_real_virial(int n, const double* frac, const int* tile_of,
                               const int* tile_start, const double* q, const PairArgs& pa,
-                              bool dense, double* partials, int* nparts, hipStream_t st) {
-  const int nb = (n + 255) / 256;
+                              bool dense, double* partials, int* nparts, hipStream_t st,
+                              const PairRows& rows) {
+  if (rows.row_ptr && !dense) {
+    const int nb = real_virial_parts(n, true);
+    if (rows.col16)
+      hipLaunchKernelGGL((k_pair_real_terms<short, true>), dim3(nb), dim3(256), 0, st, n,
+                         rows.row_ptr, static_cast<const short*>(rows.col), frac, q,
+                         static_cast<const int*>(nullptr), pa, partials);
+    else
+      hipLaunchKernelGGL((k_pair_real_terms<int, true>), dim3(nb), dim3(256), 0, st, n,
+                         rows.row_ptr, static_cast<const int*>(rows.col), frac, q,
+                         static_cast<const int*>(nullptr), pa, partials);
+    *nparts = nb;
+    return hipGetLastError();
+  }
+  const int nb = real_virial_parts(n, false);
   if (dense)
     hipLaunchKernelGGL(k_real_virial<true>, dim3(nb), dim3(256), 0, st, n, frac, tile_of,
                        tile_start, q, pa, partials);
@@ -2988,11 +3067,19 @@ template <typename R>
 hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* grid,
                          const double* q, const int* perm, const int* tile_of,
                          const int* tile_start, const ForceArgs& fa, const PairArgs& pa,
-                         bool dense, double* force, hipStream_t st) {
+                         bool dense, double* force, hipStream_t st, const PairRows& rows) {
   const dim3 grd((n + 127) / 128), blk(128);
   hipLaunchKernelGGL(k_force_recip<R>, grd, blk, 0, st, n, frac, k0, static_cast<const R*>(grid),
                      q, perm, fa, force);
-  if (dense)
+  if (rows.row_ptr && !dense) {
+    const dim3 rgrd((n + 3) / 4), rblk(256);
+    if (rows.col16)
+      hipLaunchKernelGGL((k_pair_real_terms<short, false>), rgrd, rblk, 0, st, n, rows.row_ptr,
+                         static_cast<const short*>(rows.col), frac, q, perm, pa, force);
+    else
+      hipLaunchKernelGGL((k_pair_real_terms<int, false>), rgrd, rblk, 0, st, n, rows.row_ptr,
+                         static_cast<const int*>(rows.col), frac, q, perm, pa, force);
+  } else if (dense)
     hipLaunchKernelGGL(k_force_real<true>, grd, blk, 0, st, n, frac, tile_of, tile_start, q, perm,
                        pa, force);
   else
@@ -3003,11 +3090,11 @@ hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* 
 template hipError_t launch_forces<double>(int, const double*, const int4*, const void*,
                                           const double*, const int*, const int*, const int*,
                                           const ForceArgs&, const PairArgs&, bool, double*,
-                                          hipStream_t);
+                                          hipStream_t, const PairRows&);
 template hipError_t launch_forces<float>(int, const double*, const int4*, const void*,
                                          const double*, const int*, const int*, const int*,
                                          const ForceArgs&, const PairArgs&, bool, double*,
-                                         hipStream_t);
+                                         hipStream_t, const PairRows&);
 
 hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const PairArgs& g,
                              int* col, void* coef, bool fp32, int* err, hipStream_t st) {

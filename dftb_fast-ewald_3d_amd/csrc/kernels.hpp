@@ -88,6 +88,14 @@ struct PairArgs {
   const double* gcut2;  // [ns*ns] squared short-gamma cutoffs
 };
 
+// The resident pair list of the linked mode (rows in sort order, each partner once, minimum
+// image within r_list): the real-space force and virial walk it instead of the tiles.
+struct PairRows {
+  const long long* row_ptr = nullptr;
+  const void* col = nullptr;  // int16 offsets (col16) or int32 partners
+  bool col16 = false;
+};
+
 struct ForceArgs {
   int K[3];
   int order;
@@ -326,14 +334,18 @@ hipError_t launch_recip_virial(const SpmeLaunch& L, double* partials, int* npart
                                hipStream_t st);
 hipError_t launch_real_virial(int n, const double* frac, const int* tile_of,
                               const int* tile_start, const double* q, const PairArgs& pa,
-                              bool dense, double* partials, int* nparts, hipStream_t st);
+                              bool dense, double* partials, int* nparts, hipStream_t st,
+                              const PairRows& rows = {});
 
-// Reciprocal force (overwrites force, caller order) then the real-space Ewald force (adds).
+// Reciprocal force (overwrites force, caller order) then the real-space Ewald force (adds);
+// with rows.row_ptr set (linked mode) the real-space part walks the pair list.
 template <typename R>
 hipError_t launch_forces(int n, const double* frac, const int4* k0, const void* grid,
                          const double* q, const int* perm, const int* tile_of,
                          const int* tile_start, const ForceArgs& fa, const PairArgs& pa,
-                         bool dense, double* force, hipStream_t st);
+                         bool dense, double* force, hipStream_t st, const PairRows& rows = {});
+// Workgroups (partials) the real-space virial writes.
+inline int real_virial_parts(int n, bool pair_rows) { return pair_rows ? (n + 3) / 4 : (n + 255) / 256; }
 hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const PairArgs& g,
                              int* col, void* coef, bool fp32, int* err, hipStream_t st);
 
