@@ -557,7 +557,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   if (h->opt.charged) CHK(h->alloc(&h->qsum, 1));
   CHK(h->alloc(&h->dq_io, n));
   CHK(h->alloc(&h->shift_io, n));
-  CHK(h->alloc(&h->err_flag, 1));
+  CHK(h->alloc(&h->err_flag, 2));  // [0] coincident charges, [1] a partner offset beyond 16 bits
   if (h->region.region_stride > 0) {
     const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
     unsigned char* scr;
@@ -661,9 +661,11 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     if (!(prm.r_list * c.reclen[a] < 0.5)) linked = false;
   }
   h->pa = pa;
-  HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));
+  HIPCHK(h, hipMemsetAsync(h->err_flag, 0, 2 * sizeof(int), s));
   const bool cf = h->fp32;
   const size_t cs = cf ? sizeof(float) : sizeof(double);
+  h->col16 = nullptr;
+  h->col = nullptr;
   if (linked) {
     h->pair_mode = 0;
     HIPCHK(h, launch_pair_count(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
@@ -679,16 +681,38 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     }
     rp[n] = run;
     h->nnz = run;
-    // 8 spare entries: the pair sum reads aligned groups of eight
-    CHK(h->alloc(&h->col, (size_t)run + 8));
+    // 8 spare entries: the pair sum reads aligned groups of eight.  Rows are in tile order, so
+    // partners sit close to the row index: the fill writes 16-bit offsets directly (int32
+    // partners when one does not fit, and for the symmetric storage, which is built from them).
     unsigned char* cp;
     CHK(h->alloc(&cp, ((size_t)run + 8) * cs));
     h->coef = cp;
     HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
                              hipMemcpyHostToDevice, s));
-    HIPCHK(h, launch_pair_fill(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
-                               max_tile_atoms, h->row_ptr, h->col, h->coef, cf, h->err_flag, s));
-    HIPCHK(h, hipStreamSynchronize(s));
+    bool need32 = h->opt.pair_storage == 1;
+    if (!need32) {
+      short* off = nullptr;
+      CHK(h->alloc(&off, (size_t)run + 8));
+      HIPCHK(h, launch_pair_fill(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
+                                 max_tile_atoms, h->row_ptr, nullptr, off, h->err_flag + 1,
+                                 h->coef, cf, h->err_flag, s));
+      int too_far = 0;
+      HIPCHK(h, hipMemcpyAsync(&too_far, h->err_flag + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(h, hipStreamSynchronize(s));
+      if (too_far) {
+        h->free_buf(off);
+        need32 = true;
+      } else {
+        h->col16 = off;
+      }
+    }
+    if (need32) {
+      CHK(h->alloc(&h->col, (size_t)run + 8));
+      HIPCHK(h, launch_pair_fill(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
+                                 max_tile_atoms, h->row_ptr, h->col, nullptr, nullptr, h->coef,
+                                 cf, h->err_flag, s));
+      HIPCHK(h, hipStreamSynchronize(s));
+    }
   } else {
     h->pair_mode = 1;
     for (int a = 0; a < 3; ++a) pa.D[a] = int(std::ceil(prm.r_list * c.reclen[a])) + 1;
@@ -714,14 +738,13 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     return h->fail(EW_ERR_COINCIDENT, "coincident charges");
   }
   // symmetric storage of the linked list (half the bytes per evaluation), else 16-bit partner
-  // offsets: rows are in tile order, so partners sit close to the row index.
-  h->col16 = nullptr;
+  // offsets (already written by the fill unless it needed int32 partners).
   h->half = false;
   h->hp = HalfPairs{};
   h->half_entries = 0;
   if (h->pair_mode == 0 && h->nnz > 0 && h->opt.pair_storage == 1)
     CHK(build_half_pairs(h, count, s));
-  if (h->pair_mode == 0 && h->nnz > 0 && !h->half) {
+  if (h->pair_mode == 0 && h->nnz > 0 && !h->half && h->opt.pair_storage == 1) {
     short* off = nullptr;
     CHK(h->alloc(&off, (size_t)h->nnz + 8));
     HIPCHK(h, hipMemsetAsync(h->err_flag, 0, sizeof(int), s));

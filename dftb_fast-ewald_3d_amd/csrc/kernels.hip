@@ -1763,13 +1763,31 @@ __device__ __forceinline__ double pair_term(double r2, const PairArgs& g, int si
 // per atom: the lanes take 64 consecutive candidates of a partner tile at a time (coalesced
 // coordinate loads), and a ballot compacts the pairs inside r_list, so a row lists its partners
 // in candidate order -- partner tiles in (z, y, x) order, atoms ascending within a tile.
+// Stores partner j of row i at entry k: the index itself (out.col), or its signed 16-bit offset
+// from the row taken modulo n into [-n/2, n/2) (out.off; out.too_far is set and 0 stored when
+// it does not fit), the layout the pair sums read.
+__device__ __forceinline__ void store_partner(const PartnerOut& out, long long k, int i, int j) {
+  if (out.col) {
+    out.col[k] = j;
+    return;
+  }
+  int d = j - i;
+  if (d >= (out.n + 1) / 2) d -= out.n;
+  if (d < -(out.n / 2)) d += out.n;
+  if (d < -32768 || d > 32767) {
+    *out.too_far = 1;
+    d = 0;
+  }
+  out.off[k] = short(d);
+}
+
 template <bool COUNT, typename CT>
 __global__ __launch_bounds__(256) void k_pair_tiles(int n, const double* __restrict__ frac,
                                                     const int* __restrict__ sp,
                                                     const int* __restrict__ tile_of,
                                                     const int* __restrict__ tile_start,
                                                     PairArgs g, long long* __restrict__ row_ptr,
-                                                    int* __restrict__ col,
+                                                    PartnerOut po,
                                                     CT* __restrict__ coef,
                                                     int* __restrict__ err) {
   const int lane = threadIdx.x & 63;
@@ -1822,7 +1840,7 @@ __global__ __launch_bounds__(256) void k_pair_tiles(int n, const double* __restr
           } else {
             const long long k = pos + __popcll(mask & below);
             if (in && k < end) {  // the fill never runs past the counted row length
-              col[k] = j;
+              store_partner(po, k, i, j);
               coef[k] = CT(c);
             }
             pos += __popcll(mask);
@@ -1849,7 +1867,7 @@ __global__ __launch_bounds__(256) void k_pair_tiles_lds(const double* __restrict
                                                         const int* __restrict__ sp,
                                                         const int* __restrict__ tile_start,
                                                         PairArgs g, long long* __restrict__ row_ptr,
-                                                        int* __restrict__ col,
+                                                        PartnerOut po,
                                                         CT* __restrict__ coef,
                                                         int* __restrict__ err) {
   __shared__ double cxs[kPairStage], cys[kPairStage], czs[kPairStage];
@@ -1935,7 +1953,7 @@ __global__ __launch_bounds__(256) void k_pair_tiles_lds(const double* __restrict
           const unsigned long long mask = __ballot(in);
           const long long k = pos + __popcll(mask & below);
           if (in && k < end) {  // the fill never runs past the counted row length
-            col[k] = j;
+            store_partner(po, k, i, j);
             coef[k] = CT(v);
           }
           pos += __popcll(mask);
@@ -2015,7 +2033,7 @@ __global__ __launch_bounds__(256) void k_pair_tiles_lds(const double* __restrict
         if (!COUNT) {
           const long long k = pos + __popcll(mask & below);
           if (in && k < end) {  // the fill never runs past the counted row length
-            col[k] = j;
+            store_partner(po, k, i, j);
             coef[k] = CT(v);
           }
         }
@@ -3031,35 +3049,36 @@ hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int
   const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
   if (pair_lds_fits(g, max_tile_atoms))
     hipLaunchKernelGGL((k_pair_tiles_lds<true, double>), dim3(ntiles), dim3(256), 0, st, frac,
-                       sp, tile_start, g, counts, nullptr, nullptr, err);
+                       sp, tile_start, g, counts, PartnerOut{}, nullptr, err);
   else
     hipLaunchKernelGGL((k_pair_tiles<true, double>), dim3((n + 3) / 4), dim3(256), 0, st, n,
-                       frac, sp, tile_of, tile_start, g, counts, nullptr, nullptr, err);
+                       frac, sp, tile_of, tile_start, g, counts, PartnerOut{}, nullptr, err);
   return hipGetLastError();
 }
 
 hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
                             const int* tile_start, const PairArgs& g, int max_tile_atoms,
-                            long long* row_ptr, int* col, void* coef, bool fp32, int* err,
-                            hipStream_t st) {
+                            long long* row_ptr, int* col, short* off, int* too_far, void* coef,
+                            bool fp32, int* err, hipStream_t st) {
   const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  const PartnerOut po{col, off, too_far, n};
   if (pair_lds_fits(g, max_tile_atoms)) {
     if (fp32)
       hipLaunchKernelGGL((k_pair_tiles_lds<false, float>), dim3(ntiles), dim3(256), 0, st, frac,
-                         sp, tile_start, g, row_ptr, col, static_cast<float*>(coef), err);
+                         sp, tile_start, g, row_ptr, po, static_cast<float*>(coef), err);
     else
       hipLaunchKernelGGL((k_pair_tiles_lds<false, double>), dim3(ntiles), dim3(256), 0, st,
-                         frac, sp, tile_start, g, row_ptr, col, static_cast<double*>(coef), err);
+                         frac, sp, tile_start, g, row_ptr, po, static_cast<double*>(coef), err);
     return hipGetLastError();
   }
   if (fp32)
     hipLaunchKernelGGL((k_pair_tiles<false, float>), dim3((n + 3) / 4), dim3(256), 0, st, n,
-                       frac, sp, tile_of, tile_start, g, row_ptr, col, static_cast<float*>(coef),
+                       frac, sp, tile_of, tile_start, g, row_ptr, po, static_cast<float*>(coef),
                        err);
   else
     hipLaunchKernelGGL((k_pair_tiles<false, double>), dim3((n + 3) / 4), dim3(256), 0, st, n,
-                       frac, sp, tile_of, tile_start, g, row_ptr, col,
-                       static_cast<double*>(coef), err);
+                       frac, sp, tile_of, tile_start, g, row_ptr, po, static_cast<double*>(coef),
+                       err);
   return hipGetLastError();
 }
 

@@ -96,6 +96,14 @@ struct PairRows {
   bool col16 = false;
 };
 
+// Where the pair-list fill writes the partners of a row (see store_partner).
+struct PartnerOut {
+  int* col = nullptr;      // partner indices, or null for 16-bit offsets
+  short* off = nullptr;
+  int* too_far = nullptr;  // set when an offset does not fit 16 bits
+  int n = 0;
+};
+
 struct ForceArgs {
   int K[3];
   int order;
@@ -321,10 +329,12 @@ hipError_t launch_bspline(int n, const double* frac, const BsplineArgs& g, void*
 hipError_t launch_pair_count(int n, const double* frac, const int* sp, const int* tile_of,
                              const int* tile_start, const PairArgs& g, int max_tile_atoms,
                              long long* counts, int* err, hipStream_t st);
+// Partners go to col (int32) or, when col is null, as 16-bit offsets to off, with *too_far set
+// when one does not fit.
 hipError_t launch_pair_fill(int n, const double* frac, const int* sp, const int* tile_of,
                             const int* tile_start, const PairArgs& g, int max_tile_atoms,
-                            long long* row_ptr, int* col, void* coef, bool fp32, int* err,
-                            hipStream_t st);
+                            long long* row_ptr, int* col, short* off, int* too_far, void* coef,
+                            bool fp32, int* err, hipStream_t st);
 // Virial W_ab = -dE/d(strain_ab) of the Ewald energy, as per-block partials of 6 components
 // (xx, yy, zz, xy, xz, yz) that the caller sums: the reciprocal part from the forward spectrum
 // left in L.cgrid by a forward_only launch_spme (nparts = number of blocks written), the

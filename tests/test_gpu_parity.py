@@ -487,3 +487,23 @@ def test_crowded_tiles_use_the_per_atom_pair_build(oracle_mod):
     assert prm["pair_mode"] == 0
     want = _spme_reference(oracle_mod, w, prm)
     assert rel_max(s, want) < 1e-11
+
+
+def test_int32_partners_when_offsets_exceed_16_bits(oracle_mod):
+    """A cell with four sort-tile layers along z holds ~35,000 atoms per layer, so partners in
+    the next layer lie more than 32,767 rows away: the fill falls back to int32 partners.  The
+    shift must still match the CPU SPME at the same parameters, and the mixed engine FP64."""
+    rng = np.random.default_rng(77)
+    side = 112.0 * W.AA
+    lat = np.eye(3) * side
+    n = 140_000
+    w = {"coords": rng.random((n, 3)) @ lat, "lattice": lat,
+         "charges": W.neutralise(rng.uniform(-1.0, 1.0, n))}
+    say(f"{n} random atoms, grid (128, 128, 32): int32 partner fallback")
+    opts = dict(tol=1e-8, r_cut=8.0 * W.AA, grid=(128, 128, 32), order=6)
+    s_rows, _, prm = engine_shift(w, **opts)
+    assert prm["pair_mode"] == 0 and prm["index_bytes"] == 4
+    assert rel_max(s_rows, _spme_reference(oracle_mod, w, prm)) < 1e-11
+    s_mixed, _, prm_m = engine_shift(w, precision="mixed", **opts)
+    assert prm_m["index_bytes"] == 4
+    assert rel_max(s_mixed, s_rows) < 1e-6
