@@ -476,23 +476,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     if (!build_source_table(K[a], h->T[a], h->nt[a], p, h->ntmax, a, srctab))
       return h->fail(EW_ERR_ARGUMENT, "spreading tile table overflow (order too high)");
 
-  // fractional coordinates, stencil origins, tiles; counting sort by tile
   const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
-  TileOrder ord;
-  order_by_tile(h, n, coords, ord);
-  const int max_tile_atoms = ord.max_tile;
-  const std::vector<int>& perm = ord.perm;
-  const std::vector<int>& count = ord.start;
-  const std::vector<double>& frs = ord.frac;
-  const std::vector<int4>& k0s = ord.k0;
-  const std::vector<int>& tiles = ord.tile;
-  std::vector<int> sps(gamma ? n : 0);
-  std::vector<double> ons(gamma ? n : 0);
-  if (gamma)
-    for (int k = 0; k < n; ++k) {
-      sps[k] = species[perm[k]];
-      ons[k] = hubbard_u[species[perm[k]]];  // -expGamma(0, U, U) = U  (shortgammafuncs.F90:128-131)
-    }
 
   // device buffers
   const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
@@ -578,21 +562,69 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     CHK(h->alloc(&h->onsite, n));
   }
   hipStream_t s = h->stream;
-  HIPCHK(h, hipMemcpyAsync(h->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->k0, k0s.data(), sizeof(int4) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->frac, frs.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->tile_of, tiles.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->tile_start, count.data(), sizeof(int) * (ntiles + 1),
-                           hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemcpyAsync(h->src_tab, srctab.data(), sizeof(int) * srctab.size(),
                            hipMemcpyHostToDevice, s));
   if (gamma) {
-    HIPCHK(h, hipMemcpyAsync(h->species, sps.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     HIPCHK(h, hipMemcpyAsync(h->hub, hubbard_u, sizeof(double) * n_species,
                              hipMemcpyHostToDevice, s));
     HIPCHK(h, hipMemcpyAsync(h->gcut2, gcut2.data(), sizeof(double) * gcut2.size(),
                              hipMemcpyHostToDevice, s));
-    HIPCHK(h, hipMemcpyAsync(h->onsite, ons.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  }
+  // fractional coordinates, stencil origins and sort tiles of the atoms, their stable sort by
+  // tile and the per-atom arrays in that order, on the device (the coordinates go up once)
+  int max_tile_atoms = 0;
+  {
+    OrderArgs og{};
+    for (int a = 0; a < 3; ++a) {
+      for (int d = 0; d < 3; ++d) og.rec[3 * a + d] = c.rec[a][d];
+      og.K[a] = K[a];
+      og.T[a] = h->T[a];
+      og.nt[a] = h->nt[a];
+    }
+    og.order = p;
+    og.n = n;
+    og.ntiles = ntiles;
+    OrderBufs ob{};
+    double* d_coords = nullptr;
+    int *key_in = nullptr, *idx_in = nullptr, *counts = nullptr, *d_max = nullptr,
+        *sp_raw = nullptr;
+    int4* k0_raw = nullptr;
+    double* frac_raw = nullptr;
+    unsigned char* temp = nullptr;
+    const size_t temp_bytes = order_temp_bytes(n, ntiles);
+    CHK(h->alloc(&d_coords, 3 * (size_t)n));
+    CHK(h->alloc(&frac_raw, 3 * (size_t)n));
+    CHK(h->alloc(&k0_raw, n));
+    CHK(h->alloc(&key_in, n));
+    CHK(h->alloc(&idx_in, n));
+    CHK(h->alloc(&counts, ntiles + 1));
+    CHK(h->alloc(&d_max, 1));
+    CHK(h->alloc(&temp, temp_bytes));
+    if (gamma) CHK(h->alloc(&sp_raw, n));
+    HIPCHK(h, hipMemcpyAsync(d_coords, coords, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    if (gamma)
+      HIPCHK(h, hipMemcpyAsync(sp_raw, species, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    ob.coords = d_coords;
+    ob.sp_raw = sp_raw;
+    ob.hub = h->hub;
+    ob.frac_raw = frac_raw;
+    ob.k0_raw = k0_raw;
+    ob.key_in = key_in;
+    ob.idx_in = idx_in;
+    ob.counts = counts;
+    ob.temp = temp;
+    ob.temp_bytes = temp_bytes;
+    ob.perm = h->perm;
+    ob.tile_of = h->tile_of;
+    ob.tile_start = h->tile_start;
+    ob.max_tile = d_max;
+    ob.frac = h->frac;
+    ob.k0 = h->k0;
+    ob.species = h->species;
+    ob.onsite = h->onsite;
+    HIPCHK(h, launch_order(og, ob, s));
+    HIPCHK(h, hipMemcpyAsync(&max_tile_atoms, d_max, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
   }
   // FFT tables
   for (int a = 0; a < 3; ++a) {
@@ -742,8 +774,12 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->half = false;
   h->hp = HalfPairs{};
   h->half_entries = 0;
-  if (h->pair_mode == 0 && h->nnz > 0 && h->opt.pair_storage == 1)
+  if (h->pair_mode == 0 && h->nnz > 0 && h->opt.pair_storage == 1) {
+    std::vector<int> count(ntiles + 1);
+    HIPCHK(h, hipMemcpy(count.data(), h->tile_start, sizeof(int) * (ntiles + 1),
+                        hipMemcpyDeviceToHost));
     CHK(build_half_pairs(h, count, s));
+  }
   if (h->pair_mode == 0 && h->nnz > 0 && !h->half && h->opt.pair_storage == 1) {
     short* off = nullptr;
     CHK(h->alloc(&off, (size_t)h->nnz + 8));

@@ -11,7 +11,8 @@
 //   k_gather_tile        B-spline gather + real space + on-site + self -> shift, energy partials
 //   k_energy_final       fixed-order reduction of the energy partials
 // Several vectors / structures: the kg_* kernels (same bodies, a block prefix per kind) and
-// k_pair_spmm.  Per geometry: k_bspline (weights), k_pair_tiles (linked tiles), k_pair_dense,
+// k_pair_spmm.  Per geometry: the ordering of order.hip, k_bspline (weights), k_pair_tiles
+// (linked tiles), k_pair_dense,
 // k_compress_cols, k_half_pairs.  Stress and forces: k_recip_virial, k_force_recip, and the
 // real-space parts over the pair list (k_pair_real_terms) or the tiles (k_real_virial,
 // k_force_real).  External point charges: k_ext_real (with the spreading, FFTs

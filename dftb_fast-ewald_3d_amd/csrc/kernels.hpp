@@ -96,6 +96,34 @@ struct PairRows {
   bool col16 = false;
 };
 
+// Per-geometry ordering of the atoms on the device (order.hip).
+struct OrderArgs {
+  double rec[9];  // dual basis rows b_i (b_i . a_j = delta_ij)
+  int K[3], T[3], nt[3];
+  int order;
+  int n, ntiles;
+};
+struct OrderBufs {
+  const double* coords;   // [n][3] caller order
+  const int* sp_raw;      // species in caller order, or null (no short gamma)
+  const double* hub;      // Hubbard U per species (with sp_raw)
+  double* frac_raw;       // [n][3] scratch
+  int4* k0_raw;           // [n] scratch
+  int *key_in, *idx_in;   // [n] scratch
+  int* counts;            // [ntiles + 1] scratch
+  void* temp;             // hipCUB scratch of order_temp_bytes(n, ntiles)
+  size_t temp_bytes;
+  // results: sorted position -> caller index, tile of each sorted atom, tile starts, most atoms
+  // in one tile, and the per-atom arrays in sorted order
+  int *perm, *tile_of, *tile_start, *max_tile;
+  double* frac;
+  int4* k0;
+  int* species;
+  double* onsite;
+};
+size_t order_temp_bytes(int n, int ntiles);
+hipError_t launch_order(const OrderArgs& g, const OrderBufs& o, hipStream_t st);
+
 // Where the pair-list fill writes the partners of a row (see store_partner).
 struct PartnerOut {
   int* col = nullptr;      // partner indices, or null for 16-bit offsets
