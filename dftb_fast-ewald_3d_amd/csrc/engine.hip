@@ -626,9 +626,10 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     HIPCHK(h, hipMemcpyAsync(&max_tile_atoms, d_max, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
   }
-  // FFT tables
+  // FFT tables, from host copies that outlive their asynchronous uploads
   for (int a = 0; a < 3; ++a) {
-    std::vector<double> bq(K[a]);
+    std::vector<double>& bq = h->tab_bsq[a];
+    bq.assign(K[a], 0.0);
     bspline_moduli(K[a], p, bq.data());
     CHK(h->alloc(&h->bsq[a], K[a]));
     HIPCHK(h, hipMemcpyAsync(h->bsq[a], bq.data(), sizeof(double) * K[a], hipMemcpyHostToDevice, s));
@@ -636,15 +637,13 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     CHK(h->alloc(&t, 2 * (size_t)K[a] * rs));
     h->tw[a] = t;
     if (h->fp32) {
-      std::vector<float> tv;
+      std::vector<float>& tv = h->tab_tw32[a];
       twiddles(K[a], tv);
       HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice, s));
-      HIPCHK(h, hipStreamSynchronize(s));
     } else {
-      std::vector<double> tv;
+      std::vector<double>& tv = h->tab_tw64[a];
       twiddles(K[a], tv);
       HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice, s));
-      HIPCHK(h, hipStreamSynchronize(s));
     }
   }
   for (int a = 0; a < 3; ++a)
@@ -700,18 +699,19 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   h->col = nullptr;
   if (linked) {
     h->pair_mode = 0;
+    // row lengths, then their exclusive sum on the device; only the total comes back
+    long long* counts = nullptr;
+    unsigned char* scan_tmp = nullptr;
+    const size_t scan_bytes = row_scan_temp_bytes(n);
+    CHK(h->alloc(&counts, (size_t)n + 1));
+    CHK(h->alloc(&scan_tmp, scan_bytes));
+    HIPCHK(h, hipMemsetAsync(counts + n, 0, sizeof(long long), s));
     HIPCHK(h, launch_pair_count(n, h->frac, h->species, h->tile_of, h->tile_start, pa,
-                                max_tile_atoms, h->row_ptr, h->err_flag, s));
-    std::vector<long long> rp((size_t)n + 1);
-    HIPCHK(h, hipMemcpyAsync(rp.data(), h->row_ptr, sizeof(long long) * n, hipMemcpyDeviceToHost, s));
-    HIPCHK(h, hipStreamSynchronize(s));
+                                max_tile_atoms, counts, h->err_flag, s));
+    HIPCHK(h, launch_row_scan(counts, h->row_ptr, n, scan_tmp, scan_bytes, s));
     long long run = 0;
-    for (int i = 0; i < n; ++i) {
-      const long long c_i = rp[i];
-      rp[i] = run;
-      run += c_i;
-    }
-    rp[n] = run;
+    HIPCHK(h, hipMemcpyAsync(&run, h->row_ptr + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
     h->nnz = run;
     // 8 spare entries: the pair sum reads aligned groups of eight.  Rows are in tile order, so
     // partners sit close to the row index: the fill writes 16-bit offsets directly (int32
@@ -719,8 +719,6 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     unsigned char* cp;
     CHK(h->alloc(&cp, ((size_t)run + 8) * cs));
     h->coef = cp;
-    HIPCHK(h, hipMemcpyAsync(h->row_ptr, rp.data(), sizeof(long long) * (n + 1),
-                             hipMemcpyHostToDevice, s));
     bool need32 = h->opt.pair_storage == 1;
     if (!need32) {
       short* off = nullptr;

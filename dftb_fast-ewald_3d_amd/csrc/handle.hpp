@@ -55,6 +55,11 @@ struct ew_handle {
   // buffers of the previous geometry, reused by alloc() while the next one is built (a geometry
   // step of a trajectory needs the same sizes again) and freed when it is done
   std::vector<ewamd::DevBuf> spare;
+  // host copies of the FFT tables of the current geometry, kept until its uploads are done (the
+  // next ew_set_geometry synchronises the stream before rebuilding them)
+  std::vector<double> tab_bsq[3];
+  std::vector<float> tab_tw32[3];
+  std::vector<double> tab_tw64[3];
   int* perm = nullptr;
   int4* k0 = nullptr;
   double* frac = nullptr;

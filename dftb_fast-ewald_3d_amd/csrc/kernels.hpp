@@ -123,6 +123,10 @@ struct OrderBufs {
 };
 size_t order_temp_bytes(int n, int ntiles);
 hipError_t launch_order(const OrderArgs& g, const OrderBufs& o, hipStream_t st);
+// row_ptr[0..n] = exclusive sum of counts[0..n] (counts[n] = 0): the rows of the pair list.
+size_t row_scan_temp_bytes(int n);
+hipError_t launch_row_scan(const long long* counts, long long* row_ptr, int n, void* temp,
+                           size_t temp_bytes, hipStream_t st);
 
 // Where the pair-list fill writes the partners of a row (see store_partner).
 struct PartnerOut {

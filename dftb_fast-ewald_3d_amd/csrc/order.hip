@@ -101,4 +101,17 @@ hipError_t launch_order(const OrderArgs& g, const OrderBufs& o, hipStream_t st) 
   return hipGetLastError();
 }
 
+size_t row_scan_temp_bytes(int n) {
+  size_t b = 0;
+  long long* p = nullptr;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, p, p, n + 1);
+  return b;
+}
+
+hipError_t launch_row_scan(const long long* counts, long long* row_ptr, int n, void* temp,
+                           size_t temp_bytes, hipStream_t st) {
+  size_t bytes = temp_bytes;
+  return hipcub::DeviceScan::ExclusiveSum(temp, bytes, counts, row_ptr, n + 1, st);
+}
+
 }  // namespace ewamd
