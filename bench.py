@@ -495,7 +495,7 @@ def main():
     args = ap.parse_args()
     args.vectors = [int(v) for v in args.vectors.split(",") if v.strip()]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
               "(structures shard across GPUs by footprint)", file=sys.stderr)
         sys.exit(2)
 
