@@ -3133,14 +3133,16 @@ hipError_t launch_pair_dense(int n, const double* frac, const int* sp, const Pai
 // external point charges
 // ----------------------------------------------------------------------------------------
 
-// Calls f(s0, s1, s2, j) for every source image a target at fractional (f0, f1, f2) in sort
-// tile `tile` may see: linked mode walks the source tiles around the target's tile with the
-// minimum image; dense mode takes every source over explicit image shells.
+// Calls f(s0, s1, s2, j) for the source images a target at fractional (f0, f1, f2) in sort tile
+// `tile` may see, sources lane, lane + lanes, ... of each tile (the lanes of a wave share a
+// target): linked mode walks the source tiles around the target's tile with the minimum image;
+// dense mode takes every source over explicit image shells.
 template <bool DENSE, typename F>
 __device__ __forceinline__ void for_source_images(const ExtArgs& g, double f0, double f1,
-                                                  double f2, int tile, F&& f) {
+                                                  double f2, int tile, int lane, int lanes,
+                                                  F&& f) {
   if (DENSE) {
-    for (int j = 0; j < g.m; ++j) {
+    for (int j = lane; j < g.m; j += lanes) {
       const double s0 = f0 - g.frac[3 * j], s1 = f1 - g.frac[3 * j + 1], s2 = f2 - g.frac[3 * j + 2];
       for (int a = -g.D[0]; a <= g.D[0]; ++a)
         for (int b = -g.D[1]; b <= g.D[1]; ++b)
@@ -3162,7 +3164,7 @@ __device__ __forceinline__ void for_source_images(const ExtArgs& g, double f0, d
       for (int c1 = 0; c1 < n1; ++c1) {
         const int u1 = ((b1 + c1) % g.nt[0] + g.nt[0]) % g.nt[0];
         const int tt = (u3 * g.nt[1] + u2) * g.nt[0] + u1;
-        for (int j = g.start[tt]; j < g.start[tt + 1]; ++j) {
+        for (int j = g.start[tt] + lane; j < g.start[tt + 1]; j += lanes) {
           double s0 = f0 - g.frac[3 * j], s1 = f1 - g.frac[3 * j + 1], s2 = f2 - g.frac[3 * j + 2];
           s0 -= rint(s0);
           s1 -= rint(s1);
@@ -3179,20 +3181,33 @@ __device__ __forceinline__ void for_source_images(const ExtArgs& g, double f0, d
 constexpr double kExtSameDist2 = 1.0e-10;
 constexpr double kErfArgLimit = 10.0;
 
-// Real-space potential of the external charges at each resident atom (sorted order), one thread
-// per atom: sum_s q_s [erfc(alpha d) - erfc(d / w_s) (blurred, d < 10 w_s)] / sqrt(d^2 + eps^2)
+// Real-space potential of the external charges at each resident atom (sorted order), one wave
+// or one thread per atom (ext_wave): sum_s q_s [erfc(alpha d) - erfc(d / w_s) (blurred, d < 10 w_s)] / sqrt(d^2 + eps^2)
 // over the source images within the cutoff (ewaldReal, src/dftbp/dftb/coulomb.F90:2523-2575:
 // erf(d/w) - erf(alpha d) written through erfc).  A source on the atom contributes the self
 // term -2 alpha / sqrt(pi) q_s instead (ewald, coulomb.F90:2341-2343).
-template <bool DENSE>
-__global__ __launch_bounds__(128) void k_ext_real(int n, const double* __restrict__ frac,
+// WAVE: one wave per target, its lanes along the sources (many sources per tile); else one
+// thread per target (few sources per tile, where a wave's lanes would mostly idle).
+template <bool WAVE>
+__device__ __forceinline__ int ext_target(int& lane) {
+  if (WAVE) {
+    lane = threadIdx.x & 63;
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  }
+  lane = 0;
+  return blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+template <bool DENSE, bool WAVE>
+__global__ __launch_bounds__(256) void k_ext_real(int n, const double* __restrict__ frac,
                                                   const int* __restrict__ tile_of, ExtArgs g,
                                                   double* __restrict__ phi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int lane;
+  const int i = ext_target<WAVE>(lane);
   if (i >= n) return;
   double acc = 0.0;
   for_source_images<DENSE>(g, frac[3 * i], frac[3 * i + 1], frac[3 * i + 2], DENSE ? 0 : tile_of[i],
-                           [&](double s0, double s1, double s2, int j) {
+                           lane, WAVE ? 64 : 1, [&](double s0, double s1, double s2, int j) {
     const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
     const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
     const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
@@ -3210,7 +3225,8 @@ __global__ __launch_bounds__(128) void k_ext_real(int n, const double* __restric
     }
     acc += g.q[j] * t / (g.eps2 > 0.0 ? sqrt(r2 + g.eps2) : r);
   });
-  phi[i] = acc;
+  if (WAVE) acc = wave_sum(acc);
+  if (lane == 0) phi[i] = acc;
 }
 
 // Real-space gradient with respect to the target position of the same sum (derivEwaldReal,
@@ -3220,19 +3236,20 @@ __global__ __launch_bounds__(128) void k_ext_real(int n, const double* __restric
 // skipping a source on the target.  The Gaussian width is the source's (blur) or the
 // target's (tblur: the external charges as targets of the atoms).  out[3 perm[i] + c] =
 // scale[i] times the sum (scale null: 1).
-template <bool DENSE>
-__global__ __launch_bounds__(128) void k_ext_real_grad(int n, const double* __restrict__ frac,
+template <bool DENSE, bool WAVE>
+__global__ __launch_bounds__(256) void k_ext_real_grad(int n, const double* __restrict__ frac,
                                                        const int* __restrict__ tile_of,
                                                        const int* __restrict__ perm,
                                                        const double* __restrict__ scale,
                                                        ExtArgs g, double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int lane;
+  const int i = ext_target<WAVE>(lane);
   if (i >= n) return;
   const double two_sqrtpi = 2.0 / kSqrtPi;
   const double wt = g.tblur ? g.tblur[i] : 0.0;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for_source_images<DENSE>(g, frac[3 * i], frac[3 * i + 1], frac[3 * i + 2], DENSE ? 0 : tile_of[i],
-                           [&](double s0, double s1, double s2, int j) {
+                           lane, WAVE ? 64 : 1, [&](double s0, double s1, double s2, int j) {
     const double x = s0 * g.lat[0] + s1 * g.lat[3] + s2 * g.lat[6];
     const double y = s0 * g.lat[1] + s1 * g.lat[4] + s2 * g.lat[7];
     const double z = s0 * g.lat[2] + s1 * g.lat[5] + s2 * g.lat[8];
@@ -3251,33 +3268,59 @@ __global__ __launch_bounds__(128) void k_ext_real_grad(int n, const double* __re
     a1 += c * y;
     a2 += c * z;
   });
-  const double sc = scale ? scale[i] : 1.0;
-  const int o = perm[i];
-  out[3 * o] = sc * a0;
-  out[3 * o + 1] = sc * a1;
-  out[3 * o + 2] = sc * a2;
+  if (WAVE) {
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+  }
+  if (lane == 0) {
+    const double sc = scale ? scale[i] : 1.0;
+    const int o = perm[i];
+    out[3 * o] = sc * a0;
+    out[3 * o + 1] = sc * a1;
+    out[3 * o + 2] = sc * a2;
+  }
+}
+
+// A wave per target pays off from about 16 sources per sort tile.
+static bool ext_wave(const ExtArgs& g) {
+  return g.m >= 16 * g.nt[0] * g.nt[1] * g.nt[2];
 }
 
 hipError_t launch_ext_real(int n, const double* frac, const int* tile_of, const ExtArgs& g,
                            bool dense, double* phi, hipStream_t st) {
-  const dim3 grd((n + 127) / 128), blk(128);
-  if (dense)
-    hipLaunchKernelGGL(k_ext_real<true>, grd, blk, 0, st, n, frac, tile_of, g, phi);
-  else
-    hipLaunchKernelGGL(k_ext_real<false>, grd, blk, 0, st, n, frac, tile_of, g, phi);
+  const dim3 blk(256), wgrd((n + 3) / 4), tgrd((n + 255) / 256);
+  if (ext_wave(g)) {
+    if (dense)
+      hipLaunchKernelGGL((k_ext_real<true, true>), wgrd, blk, 0, st, n, frac, tile_of, g, phi);
+    else
+      hipLaunchKernelGGL((k_ext_real<false, true>), wgrd, blk, 0, st, n, frac, tile_of, g, phi);
+  } else if (dense) {
+    hipLaunchKernelGGL((k_ext_real<true, false>), tgrd, blk, 0, st, n, frac, tile_of, g, phi);
+  } else {
+    hipLaunchKernelGGL((k_ext_real<false, false>), tgrd, blk, 0, st, n, frac, tile_of, g, phi);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_ext_real_grad(int n, const double* frac, const int* tile_of, const int* perm,
                                 const double* scale, const ExtArgs& g, bool dense, double* out,
                                 hipStream_t st) {
-  const dim3 grd((n + 127) / 128), blk(128);
-  if (dense)
-    hipLaunchKernelGGL(k_ext_real_grad<true>, grd, blk, 0, st, n, frac, tile_of, perm, scale, g,
-                       out);
-  else
-    hipLaunchKernelGGL(k_ext_real_grad<false>, grd, blk, 0, st, n, frac, tile_of, perm, scale, g,
-                       out);
+  const dim3 blk(256), wgrd((n + 3) / 4), tgrd((n + 255) / 256);
+  if (ext_wave(g)) {
+    if (dense)
+      hipLaunchKernelGGL((k_ext_real_grad<true, true>), wgrd, blk, 0, st, n, frac, tile_of, perm,
+                         scale, g, out);
+    else
+      hipLaunchKernelGGL((k_ext_real_grad<false, true>), wgrd, blk, 0, st, n, frac, tile_of, perm,
+                         scale, g, out);
+  } else if (dense) {
+    hipLaunchKernelGGL((k_ext_real_grad<true, false>), tgrd, blk, 0, st, n, frac, tile_of, perm,
+                       scale, g, out);
+  } else {
+    hipLaunchKernelGGL((k_ext_real_grad<false, false>), tgrd, blk, 0, st, n, frac, tile_of, perm,
+                       scale, g, out);
+  }
   return hipGetLastError();
 }
 
