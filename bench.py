@@ -564,6 +564,30 @@ def main():
                 print(f"[bench] {prec} multi-vector x{nv}: {msm:.4f} ms/call, "
                       f"{nv * 1e3 / msm:.1f} evals/s", file=sys.stderr, flush=True)
                 del dqm, outm, enm
+            # the per-geometry work of a molecular-dynamics step: a new geometry (the atoms moved
+            # a little), the Ewald gradients and the stress at the current charges
+            rg = np.random.default_rng(4242)
+            geo = []
+            for k in range(4):
+                xk = w["coords"] + rg.normal(scale=0.01, size=w["coords"].shape)
+                t = time.perf_counter()
+                e.set_geometry(xk, w["lattice"], charges=w["charges"])
+                geo.append((time.perf_counter() - t) * 1e3)
+            e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+            e.gradients(qs[0])
+            t = time.perf_counter()
+            for _ in range(3):
+                e.gradients(qs[0])
+            grad_ms = (time.perf_counter() - t) / 3 * 1e3
+            e.stress(qs[0])
+            t = time.perf_counter()
+            for _ in range(3):
+                e.stress(qs[0])
+            stress_ms = (time.perf_counter() - t) / 3 * 1e3
+            md = {"set_geometry_ms": round(float(np.mean(geo[1:])), 3),
+                  "gradients_ms": round(grad_ms, 3), "stress_ms": round(stress_ms, 3)}
+            print(f"[bench] {prec} geometry step {md['set_geometry_ms']:.2f} ms, gradients "
+                  f"{grad_ms:.2f} ms, stress {stress_ms:.2f} ms", file=sys.stderr, flush=True)
             # external point charges (once per geometry in DFTB+): potential at the atoms
             # and both sides' gradients, host arrays, averaged over a few calls
             ext = None
@@ -608,7 +632,7 @@ def main():
             if planes and k in ("fft_y", "fft_y_inverse"):
                 ph[k]["note"] = "fused into the x-y plane kernels (event overhead only)"
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
-                     "multi": multi, "external": ext,
+                     "multi": multi, "external": ext, "md_step": md,
                      "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph,
                      "host_api_ms": host_ms}
         print(f"[bench] {prec}: {ms:.4f} ms/eval, {1e3 / ms:.1f} evals/s, geometry {geom_s:.2f} s,"
@@ -667,6 +691,7 @@ def main():
             res["fp64"]["evals_per_s"], 3) if res["mixed"]["multi"] else None,
         "geometry_setup_s": {k: round(v["geometry_s"], 3) for k, v in res.items()},
         "external_charges": {k: v["external"] for k, v in res.items()},
+        "per_geometry_ms": {k: v["md_step"] for k, v in res.items()},
         "host_arrays_ms_per_eval": {k: round(v["host_api_ms"], 4) for k, v in res.items()},
         "phases": {k: v["phases"] for k, v in res.items()},
         "roofline": roofline,
