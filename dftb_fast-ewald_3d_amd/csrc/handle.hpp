@@ -161,6 +161,30 @@ struct ew_handle {
       if (b.p) spare.push_back(b);
     bufs.clear();
     device_bytes = 0;
+    // every pointer below pointed into bufs: the next geometry may hand that memory to another
+    // buffer or free it, and the optional ones (species terms, forces) are tested for null
+    perm = nullptr;
+    k0 = nullptr;
+    frac = nullptr;
+    theta = nullptr;
+    tile_of = tile_start = src_tab = nullptr;
+    species = nullptr;
+    hub = gcut2 = onsite = nullptr;
+    row_ptr = nullptr;
+    col = nullptr;
+    col16 = nullptr;
+    half = false;
+    hp = ewamd::HalfPairs{};
+    half_entries = 0;
+    coef = grid = cgrid = nullptr;
+    for (int a = 0; a < 3; ++a) {
+      tw[a] = nullptr;
+      bsq[a] = nullptr;
+    }
+    qs = qsum = phi_real = epart = energy = dq_io = shift_io = nullptr;
+    err_flag = nullptr;
+    force = nullptr;
+    region = ewamd::RegionArgs{};
     views = Views{};
     views_host.clear();
     views_staging_cap = 0;
