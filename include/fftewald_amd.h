@@ -122,7 +122,10 @@ const char *ew_last_error(const ew_handle *h);
  *             short-range gamma of shortgamma.F90 is folded into the pair coefficients and the
  *             returned shift is the SCC shift (Coulomb + short gamma); NULL -> Coulomb only.
  *   charges   optional [n]: only sum q^2 is read, for the automatic alpha (as
- *             ewald_parameters does); NULL -> a unit-charge estimate. */
+ *             ewald_parameters does); NULL -> a unit-charge estimate.
+ * A later call replaces the geometry (n, cell and species may all change, as along a
+ * trajectory): the previous device buffers are reused where the sizes fit, and every captured
+ * evaluation is dropped, so the caller's buffers may be passed again to the entry points below. */
 int ew_set_geometry(ew_handle *h, int n, const double *coords, const double *latvecs,
                     const int *species, const double *hubbard_u, int n_species,
                     const double *charges);
