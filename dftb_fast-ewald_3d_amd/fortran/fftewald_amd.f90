@@ -22,9 +22,13 @@ module fftewald_amd
    public :: engine_add_gradients, engine_add_stress, engine_shift_multi, engine_get_potential
    public :: engine_add_external_pot_grad, engine_get_potential_gradient
    public :: ew_mixer_options, TEwaldMixer, mixer_create, mixer_reset, mixer_mix, mixer_destroy
-   public :: EW_OK, EW_PRECISION_FP64, EW_PRECISION_MIXED, EW_MIXER_BROYDEN, EW_MIXER_SIMPLE
+   public :: EW_OK, EW_ERR_ARGUMENT, EW_PRECISION_FP64, EW_PRECISION_MIXED, EW_MIXER_BROYDEN
+   public :: EW_MIXER_SIMPLE
 
    integer(c_int), parameter :: EW_OK = 0
+   !> An array shorter than the resident geometry (the engine wrappers check the sizes that the
+   !> C entry points take on trust), a null pointer, a bad option value.
+   integer(c_int), parameter :: EW_ERR_ARGUMENT = 1
    integer(c_int), parameter :: EW_PRECISION_FP64 = 0
    integer(c_int), parameter :: EW_PRECISION_MIXED = 1
    integer(c_int), parameter :: EW_MIXER_BROYDEN = 0
@@ -391,6 +395,8 @@ contains
       real(c_double), intent(inout) :: gradient(:, :)
       integer(c_int) :: st
 
+      st = EW_ERR_ARGUMENT
+      if (size(deltaQAtom) < this%nAtom .or. .not. atoms3(this, gradient)) return
       st = c_ew_gradients(this%handle, deltaQAtom, gradient)
    end function engine_add_gradients
 
@@ -402,6 +408,8 @@ contains
       real(c_double), intent(inout) :: stress(3, 3)
       integer(c_int) :: st
 
+      st = EW_ERR_ARGUMENT
+      if (size(deltaQAtom) < this%nAtom) return
       st = c_ew_stress(this%handle, deltaQAtom, stress)
    end function engine_add_stress
 
@@ -420,6 +428,9 @@ contains
       type(c_ptr) :: pb
       real(c_double) :: eps
 
+      st = EW_ERR_ARGUMENT
+      if (.not. sources_ok(chargeCoords, charges, blurWidths) .or. size(potential) < this%nAtom) &
+         return
       pb = c_null_ptr
       if (present(blurWidths)) pb = c_loc(blurWidths)
       eps = 0.0_c_double
@@ -442,6 +453,10 @@ contains
       integer(c_int) :: st
       type(c_ptr) :: pb
 
+      st = EW_ERR_ARGUMENT
+      if (.not. sources_ok(chargeCoords, charges, blurWidths)) return
+      if (size(deltaQAtom) < this%nAtom .or. .not. atoms3(this, atomGrads)) return
+      if (size(extChargeGrads, 1) /= 3 .or. size(extChargeGrads, 2) < size(charges)) return
       pb = c_null_ptr
       if (present(blurWidths)) pb = c_loc(blurWidths)
       st = c_ew_external_gradients(this%handle, int(size(charges), c_int), chargeCoords, charges, &
@@ -460,6 +475,9 @@ contains
       integer(c_int) :: st
       type(c_ptr) :: pb
 
+      st = EW_ERR_ARGUMENT
+      if (.not. sources_ok(chargeCoords, charges, blurWidths) .or. .not. atoms3(this, gradient)) &
+         return
       pb = c_null_ptr
       if (present(blurWidths)) pb = c_loc(blurWidths)
       gradient(:, :) = 0.0_c_double
@@ -474,6 +492,8 @@ contains
       real(c_double), intent(out) :: energy
       integer(c_int) :: st
 
+      st = EW_ERR_ARGUMENT
+      if (size(deltaQAtom) < this%nAtom .or. size(shift) < this%nAtom) return
       st = c_ew_shift(this%handle, deltaQAtom, shift, energy)
    end function engine_shift
 
@@ -486,6 +506,10 @@ contains
       real(c_double), intent(out) :: energy(:)
       integer(c_int) :: st
 
+      ! the C layout [nvec][n] is column-major (nAtom, nVec) with exactly nAtom rows
+      st = EW_ERR_ARGUMENT
+      if (size(deltaQ, 1) /= this%nAtom .or. size(shift, 1) /= this%nAtom) return
+      if (size(shift, 2) < size(deltaQ, 2) .or. size(energy) < size(deltaQ, 2)) return
       st = c_ew_shift_multi(this%handle, int(size(deltaQ, 2), c_int), deltaQ, shift, energy)
    end function engine_shift_multi
 
@@ -524,6 +548,22 @@ contains
       if (c_associated(this%handle)) st = c_ew_mixer_destroy(this%handle)
       this%handle = c_null_ptr
    end subroutine mixer_destroy
+
+   !> gradient-like array of shape (3, >= nAtom)
+   logical function atoms3(this, a)
+      type(TEwaldEngine), intent(in) :: this
+      real(c_double), intent(in) :: a(:, :)
+      atoms3 = size(a, 1) == 3 .and. size(a, 2) >= this%nAtom
+   end function atoms3
+
+   !> point charges: coordinates (3, >= m), optional blur widths (>= m), m = size(charges)
+   logical function sources_ok(chargeCoords, charges, blurWidths)
+      real(c_double), intent(in) :: chargeCoords(:, :)
+      real(c_double), intent(in) :: charges(:)
+      real(c_double), intent(in), optional :: blurWidths(:)
+      sources_ok = size(chargeCoords, 1) == 3 .and. size(chargeCoords, 2) >= size(charges)
+      if (present(blurWidths)) sources_ok = sources_ok .and. size(blurWidths) >= size(charges)
+   end function sources_ok
 
    subroutine engine_destroy(this)
       type(TEwaldEngine), intent(inout) :: this

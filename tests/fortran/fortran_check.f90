@@ -60,6 +60,9 @@ program fortran_check
       print *, engine_error(engine%handle)
       error stop "engine_set_geometry failed"
    end if
+   ! arrays shorter than the geometry are refused before the C side reads them
+   if (engine_shift(engine, charges(1:size(charges) - 1), shift, energy) /= EW_ERR_ARGUMENT) &
+      error stop "engine_shift accepted a short charge array"
    st = engine_shift(engine, charges, shift, energy)
    if (st /= EW_OK) error stop "engine_shift failed"
    call engine_destroy(engine)
