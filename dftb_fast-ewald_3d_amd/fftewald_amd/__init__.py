@@ -293,9 +293,17 @@ class EwaldEngine:
         _check(lib().ew_get_parameters(self._h, C.byref(p)), self._h)
         return p.as_dict()
 
+    def _per_atom(self, a, name):
+        """a as a contiguous float64 vector with one entry per atom of the current geometry (the
+        C side reads n entries; without a geometry it reports that itself)."""
+        a = _f64(a, 1)
+        if self.n and len(a) != self.n:
+            raise ValueError(f"{name} must have {self.n} entries (one per atom), got {len(a)}")
+        return a
+
     def shift(self, dq):
         """Returns (shift[n], energy) for charges dq (caller order)."""
-        dq = _f64(dq, 1)
+        dq = self._per_atom(dq, "dq")
         out = np.empty(self.n)
         e = C.c_double()
         _check(lib().ew_shift(self._h, dq, out, C.byref(e)), self._h)
@@ -326,15 +334,21 @@ class EwaldEngine:
 
     def gradients(self, dq, grad=None):
         """grad + dU/dx (Ewald part) at charges dq; grad defaults to zeros (n, 3)."""
-        dq = _f64(dq, 1)
-        g = np.zeros((self.n, 3)) if grad is None else np.array(grad, dtype=np.float64, copy=True)
+        dq = self._per_atom(dq, "dq")
+        g = (np.zeros((self.n, 3)) if grad is None
+             else np.array(grad, dtype=np.float64, order="C", copy=True))
+        if g.shape != (self.n, 3):
+            raise ValueError(f"grad must be ({self.n}, 3), got {g.shape}")
         _check(lib().ew_gradients(self._h, dq, g), self._h)
         return g
 
     def stress(self, dq, stress=None):
         """stress + the Ewald stress (3, 3) at charges dq (TCoulomb%addStress convention)."""
-        dq = _f64(dq, 1)
-        st = np.zeros((3, 3)) if stress is None else np.array(stress, dtype=np.float64, copy=True)
+        dq = self._per_atom(dq, "dq")
+        st = (np.zeros((3, 3)) if stress is None
+              else np.array(stress, dtype=np.float64, order="C", copy=True))
+        if st.shape != (3, 3):
+            raise ValueError(f"stress must be (3, 3), got {st.shape}")
         _check(lib().ew_stress(self._h, dq, st), self._h)
         return st
 
@@ -365,7 +379,9 @@ class EwaldEngine:
         if m and xy.shape != (m, 3):
             raise ValueError("coords must be (m, 3) for m charges")
         w = None if blur_widths is None else _f64(blur_widths, 1)
-        d = None if dq is None else _f64(dq, 1)
+        if w is not None and len(w) != m:
+            raise ValueError("blur_widths must have one entry per charge")
+        d = None if dq is None else self._per_atom(dq, "dq")
         ga = np.zeros((self.n, 3))
         gs = np.zeros((m, 3)) if d is not None else None
         _check(lib().ew_external_gradients(self._h, m, xy, q if m else None, w, d, ga,

@@ -177,6 +177,17 @@ def test_error_paths():
         with pytest.raises(ew.EwaldError) as err:
             e.shift(q)
         assert err.value.code == ew.EW_ERR_NOT_FINITE
+        # arrays that do not match the geometry are refused before the C side reads them
+        with pytest.raises(ValueError):
+            e.shift(w["charges"][:-1])
+        with pytest.raises(ValueError):
+            e.gradients(w["charges"], grad=np.zeros((len(w["charges"]) - 1, 3)))
+        with pytest.raises(ValueError):
+            e.stress(w["charges"][:-1])
+        # a Fortran-ordered gradient array is accumulated in the right layout
+        gf = np.asfortranarray(np.arange(3.0 * len(w["charges"])).reshape(-1, 3))
+        diff = e.gradients(w["charges"], grad=gf) - e.gradients(w["charges"])
+        assert np.abs(diff - gf).max() < 1e-6
         bad = w["coords"].copy()
         bad[5] = bad[9]
         with pytest.raises(ew.EwaldError) as err:
