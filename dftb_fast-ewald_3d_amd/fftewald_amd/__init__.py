@@ -467,7 +467,16 @@ class EwaldPool:
     def shift(self, ids, dqs):
         """One grouped evaluation; dqs: one charge vector per id.  Returns (shifts, energies)."""
         ids = np.ascontiguousarray(ids, dtype=np.int32)
-        dq = np.concatenate([_f64(d, 1) for d in dqs]) if len(dqs) else np.zeros(0)
+        if len(dqs) != len(ids):
+            raise ValueError(f"{len(ids)} ids but {len(dqs)} charge vectors")
+        vecs = [_f64(d, 1) for d in dqs]
+        # the library walks the concatenation by each structure's atom count (an unknown id is
+        # refused there before anything is read)
+        for i, d in zip(ids, vecs):
+            n = self.sizes.get(int(i))
+            if n is not None and len(d) != n:
+                raise ValueError(f"structure {int(i)} has {n} atoms, got {len(d)} charges")
+        dq = np.concatenate(vecs) if vecs else np.zeros(0)
         out = np.empty_like(dq)
         en = np.empty(len(ids))
         self._check(lib().ew_pool_shift(self._p, len(ids), ids, dq, out, en))

@@ -169,6 +169,11 @@ def test_pool_errors(structures):
         with pytest.raises(ew.EwaldError) as exc:
             p.shift([0], [w["charges"] + 1.0])
         assert exc.value.code == ew.EW_ERR_NOT_NEUTRAL
+        # vectors that do not match their structures are refused before the library reads them
+        with pytest.raises(ValueError):
+            p.shift([0], [w["charges"][:-1]])
+        with pytest.raises(ValueError):
+            p.shift([0], [w["charges"], w["charges"]])
         with pytest.raises(ew.EwaldError) as exc:
             p.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"],
                   species=np.zeros(len(w["charges"]), dtype=np.int32),
