@@ -177,6 +177,16 @@ def lambert_w0(x: float) -> float:
     return lib().orc_lambert_w0(x)
 
 
+def splitting_from_real_budget(r_cut: float, q2sum: float, n: int, pseudo_volume: float,
+                               budget: float) -> float:
+    return lib().orc_splitting_from_real_budget(r_cut, q2sum, n, pseudo_volume, budget)
+
+
+def cutoff_from_fourier_budget(q2sum: float, n: int, box_edge: float, alpha: float,
+                               budget: float) -> float:
+    return lib().orc_cutoff_from_fourier_budget(q2sum, n, box_edge, alpha, budget)
+
+
 def balanced_alpha(minG, minR, volume, tol):
     a = C.c_double()
     st = lib().orc_balanced_alpha(minG, minR, volume, tol, C.byref(a))
@@ -395,6 +405,18 @@ def ref_balanced_alpha(minG, minR, volume, tol):
     st = C.c_int()
     ref().ref_balanced_alpha(minG, minR, volume, tol, C.byref(a), C.byref(st))
     return a.value, st.value
+
+
+def ref_lambert_w0(x: float) -> float:
+    return ref().ref_lambert_w0(x)
+
+
+def ref_splitting_from_real_budget(r_cut, q2sum, n, pseudo_volume, budget) -> float:
+    return ref().ref_splitting_from_real_budget(r_cut, q2sum, n, pseudo_volume, budget)
+
+
+def ref_cutoff_from_fourier_budget(q2sum, n, box_edge, alpha, budget) -> float:
+    return ref().ref_cutoff_from_fourier_budget(q2sum, n, box_edge, alpha, budget)
 
 
 def ref_balanced_splitting(lat, tol):

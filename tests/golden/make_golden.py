@@ -18,8 +18,19 @@ Fixtures:
   ref_small_cell.npz   40 charges in a cell too small for the linked cells (image-sum path).
   ref_alpha.npz        balanced_alpha on a table of (minG, minR, V, tol) and its status.
   ref_real_space.npz   the linked-cell real-space potential/force of 2000 charges.
+  ref_truncation.npz   the fast path's parameter chain at tol 1e-6 ... 1e-12 on the cells of the
+                       five workloads and on batch/serving-like cells: lambert_w0 on a table,
+                       r_cut by the mean-spacing rule, alpha = splitting_from_real_budget and
+                       k_cut = cutoff_from_fourier_budget (compiled reference functions, the rule's
+                       two constants read from fft/ewald_fft_3d_parameters.f90 as text), and
+                       balanced_alpha(2 pi min|b_i|, min|a_i|, V, tol), the balance condition
+                       getOptimalAlphaEwald bisects (src/dftbp/dftb/coulomb.F90:2097-2179).
+
+    python tests/golden/make_golden.py [--only truncation]
 """
+import math
 import os
+import re
 import sys
 
 import numpy as np
@@ -33,10 +44,70 @@ import oracle  # noqa: E402
 from fftewald_amd import workloads as W  # noqa: E402
 
 
+PARAMS_F90 = ("/root/reference/src/dftbp/extlibs/fftewald/fft/ewald_fft_3d_parameters.f90")
+
+
+def _reference_constant(name):
+    """A `real(dp), parameter :: name = value_dp` of the parameter module, read as text (the
+    module itself needs FFTW and cannot be compiled here)."""
+    text = open(PARAMS_F90).read()
+    m = re.search(r"real\(dp\),\s*parameter\s*::\s*" + name + r"\s*=\s*([0-9.eEdD+-]+)_dp", text)
+    if not m:
+        raise SystemExit(f"{name} not found in {PARAMS_F90}")
+    return float(m.group(1).replace("d", "e").replace("D", "e"))
+
+
+def truncation():
+    """Parameter chain of the fast path (fft/ewald_fft_3d_parameters.f90:96-131) through the
+    compiled reference functions; the composition (mean-spacing rule, linked-cell cap, budget
+    split) is the reference's, its constants read from the reference source."""
+    prefactor = _reference_constant("realCutoffPrefactor")
+    fraction = _reference_constant("realCutoffBudgetFraction")
+    rng = np.random.default_rng(20260107)
+    cells = []  # (lattice, n, q2sum)
+    for w in (W.nacl(), W.water()):
+        cells.append((w["lattice"], len(w["charges"]), float(np.sum(w["charges"] ** 2))))
+    lat3 = W.lattice_from_parameters(99.4 * W.AA, 99.4 * W.AA, 99.4 * W.AA, 90.0, 90.0, 75.0)
+    q3 = W.neutralise(np.random.default_rng(W.SEED0 + 3).uniform(-1.0, 1.0, 98304))
+    cells.append((lat3, 98304, float(np.sum(q3 ** 2))))
+    for n in (64, 300, 1500, 8192, 100, 4000, 20000):  # batch-like and serving-like cells
+        ang = rng.uniform(70.0, 110.0, 3)
+        side = W.cell_for_density(n, 0.1, ang)
+        lat = W.lattice_from_parameters(side * W.AA, side * W.AA, side * W.AA, *ang)
+        q = W.neutralise(rng.uniform(-1.0, 1.0, n))
+        cells.append((lat, n, float(np.sum(q * q))))
+    rows, lats = [], []
+    for lat, n, q2 in cells:
+        m = oracle.ref_cell_metrics(lat)
+        V = m["volume"]
+        spacing = (V / n) ** (1.0 / 3.0)
+        for tol in (1e-6, 1e-8, 1e-9, 1e-10, 1e-12):
+            r_cut = prefactor * spacing
+            limit = 0.999 / (3.0 * max(m["reclen"]))
+            if limit >= 1.5 * spacing:
+                r_cut = min(r_cut, limit)
+            alpha = oracle.ref_splitting_from_real_budget(r_cut, q2, n, V, fraction * tol)
+            k_cut = oracle.ref_cutoff_from_fourier_budget(q2, n, V ** (1.0 / 3.0), alpha, tol)
+            minG = 2.0 * math.pi * min(m["reclen"])
+            minR = min(m["latlen"])
+            a_bal, st = oracle.ref_balanced_alpha(minG, minR, V, tol)
+            rows.append((n, q2, tol, V, r_cut, alpha, k_cut, minG, minR, a_bal, st))
+            lats.append(lat)
+    xs = np.concatenate([[0.0, 1e-300, 1e-17, 1e-9, 0.5, 1.0, math.e, 10.0],
+                         np.logspace(-12, 200, 107)])
+    ws = np.array([oracle.ref_lambert_w0(float(x)) for x in xs])
+    np.savez(os.path.join(HERE, "ref_truncation.npz"), chain=np.array(rows, dtype=np.float64),
+             lattice=np.array(lats), prefactor=np.float64(prefactor),
+             budget_fraction=np.float64(fraction), lw_x=xs, lw_w=ws)
+
+
 def main():
     oracle.build(with_reference=True)
     if not oracle.have_reference():
         raise SystemExit("oracle/_ref/libewald_ref.so is missing: /root/reference not present")
+    truncation()
+    if "--only" in sys.argv:
+        return
 
     # 1. the NaCl workload
     w = W.nacl()

@@ -44,6 +44,23 @@ def test_parameter_chain_matches_reference(oracle_mod):
     assert p["r_cut"] == pytest.approx(math.sqrt(-math.log(1e-10)) / 0.3, rel=1e-15)
 
 
+def test_parameter_chain_matches_reference_fixture():
+    """The library's chain (csrc/params.cpp) against the chain composed from the compiled
+    reference functions (tests/golden/ref_truncation.npz): r_cut rule, alpha from the real-space
+    budget, and the DFTB+ balanced alpha (alpha_rule "dftb").  The charges are rebuilt from
+    sum q^2, which moves alpha by round-off only."""
+    from conftest import load_golden
+    g = load_golden("ref_truncation.npz")
+    for r, lat in zip(g["chain"], g["lattice"]):
+        n, q2, tol = int(r[0]), r[1], r[2]
+        q = np.full(n, math.sqrt(q2 / n))
+        p = ew.ewald_parameters(q, lat, tol=tol)
+        assert p["r_cut"] == pytest.approx(r[4], rel=1e-15)
+        assert p["alpha"] == pytest.approx(r[5], rel=1e-13)
+        p = ew.ewald_parameters(q, lat, tol=tol, alpha_rule="dftb")
+        assert p["alpha"] == pytest.approx(r[9], rel=1e-8)
+
+
 def test_dftb_alpha_rule_matches_getOptimalAlphaEwald(oracle_mod):
     w = W.nacl()
     p = ew.ewald_parameters(w["charges"], w["lattice"], tol=1e-10, alpha_rule="dftb",

@@ -52,6 +52,93 @@ def test_balanced_alpha_matches_reference_bitwise(oracle_mod):
         assert a == a_ref
 
 
+def test_lambert_w0_matches_reference_bitwise(oracle_mod):
+    g = load_golden("ref_truncation.npz")
+    for x, w in zip(g["lw_x"], g["lw_w"]):
+        assert oracle_mod.lambert_w0(float(x)) == w, x
+    # and the defining identity W e^W = x on the same table (x > 0, no overflow; exp
+    # amplifies the last-bit error of W by W)
+    for x, w in zip(g["lw_x"], g["lw_w"]):
+        if 1e-300 < x < 1e150:
+            assert w * math.exp(w) == pytest.approx(x, rel=2e-15 * (2.0 + w))
+
+
+def _chain_rows():
+    g = load_golden("ref_truncation.npz")
+    keys = ("n", "q2", "tol", "V", "r_cut", "alpha", "k_cut", "minG", "minR", "a_bal", "st")
+    return g, [dict(zip(keys, r), lattice=lat) for r, lat in zip(g["chain"], g["lattice"])]
+
+
+def test_budget_inversions_match_reference_bitwise(oracle_mod):
+    """splitting_from_real_budget / cutoff_from_fourier_budget (fe/common/ewald_truncation.f90:
+    53-104) against the compiled reference on the workload cells, tol 1e-6 ... 1e-12."""
+    g, rows = _chain_rows()
+    frac = float(g["budget_fraction"])
+    for r in rows:
+        n = int(r["n"])
+        a = oracle_mod.splitting_from_real_budget(r["r_cut"], r["q2"], n, r["V"], frac * r["tol"])
+        assert a == r["alpha"]
+        k = oracle_mod.cutoff_from_fourier_budget(r["q2"], n, r["V"] ** (1.0 / 3.0), r["alpha"],
+                                                 r["tol"])
+        assert k == r["k_cut"]
+
+
+def test_parameter_chain_matches_reference(oracle_mod):
+    """The oracle's whole chain (r_cut rule and linked-cell cap, alpha from 0.2 tol, k_cut;
+    fe/fft/ewald_fft_3d_parameters.f90:96-131) against the chain composed from the compiled
+    reference functions with the rule's constants read from the reference source."""
+    _, rows = _chain_rows()
+    for r in rows:
+        m = oracle_mod.cell_metrics(r["lattice"])
+        assert m["volume"] == pytest.approx(r["V"], rel=1e-15)
+        q = np.full(int(r["n"]), math.sqrt(r["q2"] / r["n"]))
+        p = oracle_mod.choose_parameters(r["lattice"], q, r["tol"])
+        assert p["r_cut"] == pytest.approx(r["r_cut"], rel=1e-15)
+        assert p["alpha"] == pytest.approx(r["alpha"], rel=1e-14)
+        assert p["k_cut"] == pytest.approx(r["k_cut"], rel=1e-14)
+
+
+def test_dftb_optimal_alpha_matches_reference_balance(oracle_mod):
+    """getOptimalAlphaEwald (src/dftbp/dftb/coulomb.F90:2097-2179, 2686-2748) bisects the same
+    balance as the compiled balanced_alpha (fe/common/ewald_alpha_search.f90:83-181) with
+    minG = |shortest reciprocal vector incl. 2 pi|, minR = |shortest lattice vector|; only the
+    rounding order of the Fourier term and the cap (30 against 100 steps) differ.  Agreement to
+    the last bisection bracket (relative 2^-30 of the starting bracket, here < 1e-8) whenever the
+    reference converged; exact agreement on most rows."""
+    _, rows = _chain_rows()
+    exact = 0
+    for r in rows:
+        assert int(r["st"]) == 0
+        a = oracle_mod.optimal_alpha_dftb(r["lattice"], r["tol"])
+        assert a == pytest.approx(r["a_bal"], rel=1e-8)
+        exact += a == r["a_bal"]
+    assert exact >= len(rows) // 2
+    # the 512-atom NaCl workload's alpha (written into ref_nacl_c1.npz) is the reference's
+    g = load_golden("ref_nacl_c1.npz")
+    nacl = [r for r in rows if int(r["n"]) == 512 and r["tol"] == 1e-10][0]
+    assert float(g["alpha"]) == pytest.approx(nacl["a_bal"], rel=1e-8)
+
+
+def test_planted_parameter_mistakes_miss_the_reference(oracle_mod):
+    """Plausible slips in the chain fail the fixtures above: minG without the 2 pi, the whole
+    tolerance as the real-space budget instead of its fraction, the cube edge replaced by the
+    first lattice length."""
+    g, rows = _chain_rows()
+    frac = float(g["budget_fraction"])
+    assert frac == 0.2 and float(g["prefactor"]) == 3.0
+    for r in rows:
+        n = int(r["n"])
+        a, st = oracle_mod.balanced_alpha(r["minG"] / (2.0 * math.pi), r["minR"], r["V"],
+                                          r["tol"])
+        assert st != 0 or abs(a / r["a_bal"] - 1.0) > 1e-3
+        a = oracle_mod.splitting_from_real_budget(r["r_cut"], r["q2"], n, r["V"], r["tol"])
+        assert abs(a / r["alpha"] - 1.0) > 1e-3
+        edge = float(np.linalg.norm(r["lattice"][0]))
+        k = oracle_mod.cutoff_from_fourier_budget(r["q2"], n, edge, r["alpha"], r["tol"])
+        if abs(edge / r["V"] ** (1.0 / 3.0) - 1.0) > 1e-3:
+            assert abs(k / r["k_cut"] - 1.0) > 1e-6
+
+
 def test_linked_cell_real_space_matches_reference(oracle_mod):
     g = load_golden("ref_real_space.npz")
     pot, force = oracle_mod.real_space(g["coords"], g["charges"], g["lattice"], float(g["alpha"]),
