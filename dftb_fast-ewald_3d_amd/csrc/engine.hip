@@ -421,8 +421,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     for (int i = 0; i < n; ++i) q2 += charges[i] * charges[i];
   Params prm;
   std::string perr;
-  st = choose_parameters(c, n, q2, &h->opt.tol, h->opt.alpha, h->opt.r_cut, h->opt.grid,
-                         h->opt.order, h->opt.alpha_rule, &prm, &perr);
+  st = choose_parameters(c, n, q2, &h->opt.tol, h->opt.alpha, h->opt.r_cut, h->opt.k_cut,
+                         h->opt.grid, h->opt.order, h->opt.alpha_rule, &prm, &perr);
   if (st) return h->fail(st, perr);
   prm.precision = h->opt.precision;
 
@@ -1706,8 +1706,8 @@ extern "C" int ew_parameters_for(int n, const double* charges, const double* lat
     for (int i = 0; i < n; ++i) q2 += charges[i] * charges[i];
   Params p;
   std::string err;
-  st = choose_parameters(c, n, q2, &o.tol, o.alpha, o.r_cut, o.grid, o.order, o.alpha_rule, &p,
-                         &err);
+  st = choose_parameters(c, n, q2, &o.tol, o.alpha, o.r_cut, o.k_cut, o.grid, o.order,
+                         o.alpha_rule, &p, &err);
   if (st) return set_global_error(st, err);
   std::memset(out, 0, sizeof(*out));
   out->alpha = p.alpha;

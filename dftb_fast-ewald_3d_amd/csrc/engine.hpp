@@ -52,8 +52,8 @@ struct Params {
 
 // Parameter chain (host).  Returns an EW_* status and fills p.
 int choose_parameters(const Cell& c, int n, double q2sum, const double* opt_tol,
-                      double opt_alpha, double opt_r_cut, const int* opt_grid, int opt_order,
-                      int alpha_rule, Params* p, std::string* err);
+                      double opt_alpha, double opt_r_cut, double opt_k_cut, const int* opt_grid,
+                      int opt_order, int alpha_rule, Params* p, std::string* err);
 
 // src/dftbp/dftb/shortgammafuncs.F90:97-148, 27-93
 double exp_gamma(double r, double ua, double ub);

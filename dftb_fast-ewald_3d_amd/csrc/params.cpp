@@ -125,7 +125,8 @@ int smooth235(int n) {
 }
 
 int choose_parameters(const Cell& c, int n, double q2sum, const double* opt_tol,
-                      double opt_alpha, double opt_r_cut, const int* opt_grid, int opt_order,
+                      double opt_alpha, double opt_r_cut, double opt_k_cut, const int* opt_grid,
+                      int opt_order,
                       int alpha_rule, Params* p, std::string* err) {
   double tol = (opt_tol && *opt_tol > 0.0) ? *opt_tol : 1.0e-10;
   if (!(tol > 0.0 && tol < 1.0)) {
@@ -134,6 +135,10 @@ int choose_parameters(const Cell& c, int n, double q2sum, const double* opt_tol,
   }
   if (n <= 0) {
     if (err) *err = "n must be positive";
+    return EW_ERR_ARGUMENT;
+  }
+  if (!std::isfinite(opt_k_cut) || !std::isfinite(opt_alpha) || !std::isfinite(opt_r_cut)) {
+    if (err) *err = "alpha, r_cut and k_cut overrides must be finite";
     return EW_ERR_ARGUMENT;
   }
   if (!(q2sum > 0.0)) q2sum = double(n);  // unit charges when none are known
@@ -167,8 +172,9 @@ int choose_parameters(const Cell& c, int n, double q2sum, const double* opt_tol,
     if (err) *err = "alpha and r_cut must be positive";
     return EW_ERR_ARGUMENT;
   }
-  // 3. Fourier cutoff (fe/fft/ewald_fft_3d_parameters.f90:127-130)
-  p->k_cut = kcut_from_fourier_budget(q2sum, n, std::cbrt(c.volume), p->alpha, tol);
+  // 3. Fourier cutoff (fe/fft/ewald_fft_3d_parameters.f90:127-130), or the caller's k_cut_in
+  p->k_cut = opt_k_cut > 0.0 ? opt_k_cut
+                             : kcut_from_fourier_budget(q2sum, n, std::cbrt(c.volume), p->alpha, tol);
   // 4. B-spline order and mesh.  Calibrated against the direct sum: the reciprocal error of
   //    order 10 with a mesh of 2*sigma*m_max points per axis (m_max = k_cut |a_i| / 2 pi)
   //    stays below tol for sigma = 1.25 / 1.5 / 2 at tol >= 1e-7 / >= 1e-9 / below.

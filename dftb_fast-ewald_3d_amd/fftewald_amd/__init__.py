@@ -69,7 +69,7 @@ class Options(C.Structure):
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
                 ("eager", C.c_int), ("charged", C.c_int), ("fft_mode", C.c_int),
-                ("pair_storage", C.c_int), ("reserved", C.c_int)]
+                ("pair_storage", C.c_int), ("reserved", C.c_int), ("k_cut", C.c_double)]
 
 
 class ServerOptions(C.Structure):
@@ -209,11 +209,12 @@ def lib() -> C.CDLL:
 
 def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
                  device=0, alpha_rule="fftewald", profile=False, eager=False,
-                 charged=False, fft_mode=0, pair_storage=0) -> Options:
+                 charged=False, fft_mode=0, pair_storage=0, k_cut=None) -> Options:
     o = Options()
     o.tol = tol or 0.0
     o.alpha = alpha or 0.0
     o.r_cut = r_cut or 0.0
+    o.k_cut = k_cut or 0.0
     if grid is not None:
         g = (grid, grid, grid) if np.isscalar(grid) else tuple(grid)
         for a in range(3):

@@ -50,6 +50,7 @@ module fftewald_amd
       integer(c_int) :: fft_mode = 0
       integer(c_int) :: pair_storage = 0
       integer(c_int) :: reserved = 0
+      real(c_double) :: k_cut = 0.0_c_double
    end type ew_options
 
    !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).
@@ -253,9 +254,11 @@ contains
    end function engine_error
 
    !> Options from the reference's optional overrides: nModes_in and windowCutoff_in become the
-   !> SPME mesh and B-spline order, the method this engine runs in place of the NFFT.
-   function options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in) result(o)
-      real(c_double), intent(in), optional :: tol, alpha_in, r_cut_in
+   !> SPME mesh and B-spline order, the method this engine runs in place of the NFFT; k_cut_in
+   !> replaces the Fourier cutoff the mesh is derived from (ewald_fft_3d_parameters.f90:127-130).
+   function options_from_overrides(tol, alpha_in, r_cut_in, k_cut_in, nModes_in, &
+                                   windowCutoff_in) result(o)
+      real(c_double), intent(in), optional :: tol, alpha_in, r_cut_in, k_cut_in
       integer, intent(in), optional :: nModes_in, windowCutoff_in
       type(ew_options) :: o
 
@@ -263,13 +266,14 @@ contains
       if (present(tol)) o%tol = tol
       if (present(alpha_in)) o%alpha = alpha_in
       if (present(r_cut_in)) o%r_cut = r_cut_in
+      if (present(k_cut_in)) o%k_cut = k_cut_in
       if (present(nModes_in)) o%grid = nModes_in
       if (present(windowCutoff_in)) o%order = windowCutoff_in
    end function options_from_overrides
 
    !> fe/fft/ewald_fft_3d.f90:36-134 with the same arguments: positions(i,:) is atom i,
    !> latVecs(i,:) is a_i; pot(i) = phi(x_i); force(:,i) = -dU/dx_i when present.
-   !> k_cut_in is accepted for signature compatibility; the mesh follows from tol or nModes_in.
+   !> k_cut_in replaces the Fourier cutoff; the mesh follows from it unless nModes_in is given.
    subroutine ewald_potential_force(positions, charges, nParticle, latVecs, tol, pot, force, &
                                     alpha_in, r_cut_in, k_cut_in, nModes_in, windowCutoff_in)
       real(c_double), intent(in) :: positions(:, :)
@@ -292,8 +296,7 @@ contains
       if (size(positions, 1) < nParticle .or. size(charges) < nParticle) &
          error stop "fftewald_amd: too few positions or charges"
       if (size(pot) < nParticle) error stop "fftewald_amd: pot must be sized (N)"
-      if (present(k_cut_in)) continue
-      o = options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in)
+      o = options_from_overrides(tol, alpha_in, r_cut_in, k_cut_in, nModes_in, windowCutoff_in)
       allocate (coords(3, nParticle))
       coords = transpose(positions(1:nParticle, :))
       lat = transpose(latVecs)
@@ -326,8 +329,7 @@ contains
       integer(c_int) :: st
 
       if (size(positions, 1) < nParticle) error stop "fftewald_amd: too few positions"
-      if (present(k_cut_in)) continue
-      o = options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in)
+      o = options_from_overrides(tol, alpha_in, r_cut_in, k_cut_in, nModes_in, windowCutoff_in)
       st = c_ew_parameters_for(int(nParticle, c_int), charges, transpose(latVecs), o, params)
       if (st /= EW_OK) error stop "fftewald_amd: ewald_parameters failed"
    end subroutine ewald_parameters
@@ -344,8 +346,7 @@ contains
       type(ew_options) :: o
       integer(c_int) :: st
 
-      if (present(k_cut_in)) continue
-      o = options_from_overrides(tol, alpha_in, r_cut_in, nModes_in, windowCutoff_in)
+      o = options_from_overrides(tol, alpha_in, r_cut_in, k_cut_in, nModes_in, windowCutoff_in)
       st = c_ew_grid_memory(int(nParticle, c_int), charges, transpose(latVecs), o, gigabytes)
       if (st /= EW_OK) error stop "fftewald_amd: ewald_grid_memory failed"
    end function ewald_grid_memory
@@ -360,24 +361,35 @@ contains
    end function engine_create
 
    !> Geometry step (updateCoords/updateLatVecs, coulomb.F90:509-620) with DFTB+'s native
-   !> coords(3,N) and latVecs(3,3) (columns a_i).  species/hubbardU enable the short gamma.
+   !> coords(3,N) and latVecs(3,3) (columns a_i).  species/hubbardU enable the short gamma:
+   !> species(i) is DFTB+'s 1-based species index of atom i (species0 in TScc%updateCoords,
+   !> scc.F90:368-400) and hubbardU(s) the U of species s; the C ABI takes 0-based indices, so
+   !> the wrapper shifts a copy.  An index outside 1..size(hubbardU) or a species array shorter
+   !> than the geometry returns EW_ERR_ARGUMENT.
    function engine_set_geometry(this, coords, latVecs, species, hubbardU, charges) result(st)
       type(TEwaldEngine), intent(inout) :: this
       real(c_double), intent(in) :: coords(:, :)
       real(c_double), intent(in) :: latVecs(3, 3)
-      integer(c_int), intent(in), optional, target :: species(:)
+      integer, intent(in), optional :: species(:)
       real(c_double), intent(in), optional, target :: hubbardU(:)
       real(c_double), intent(in), optional, target :: charges(:)
       integer(c_int) :: st
       type(c_ptr) :: ps, pu, pq
       integer(c_int) :: ns
+      integer(c_int), allocatable, target :: species0(:)
 
       ps = c_null_ptr
       pu = c_null_ptr
       pq = c_null_ptr
       ns = 0
-      if (present(species) .and. present(hubbardU)) then
-         ps = c_loc(species)
+      if (present(species) .and. present(hubbardU) .and. size(hubbardU) > 0) then
+         st = EW_ERR_ARGUMENT
+         if (size(species) < size(coords, 2)) return
+         if (any(species(1:size(coords, 2)) < 1) .or. &
+             any(species(1:size(coords, 2)) > size(hubbardU))) return
+         allocate (species0(size(coords, 2)))
+         species0(:) = int(species(1:size(coords, 2)) - 1, c_int)
+         ps = c_loc(species0)
          pu = c_loc(hubbardU)
          ns = int(size(hubbardU), c_int)
       end if

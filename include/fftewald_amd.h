@@ -82,6 +82,11 @@ typedef struct ew_options {
                        pair sum); 1: each linked pair stored once (symmetric, tile-blocked):
                        half the pair memory, about twice the pair-sum time                    */
   int reserved;
+  double k_cut;    /* Fourier cutoff override (k_cut_in), Bohr^-1: replaces the cutoff of the
+                      Fourier budget (fe/fft/ewald_fft_3d_parameters.f90:127-130) and the mesh
+                      follows from it as the modes follow from k_cut in modes_along_axis
+                      (:159-167): K_i = smallest 2^a 3^b 5^c >= 2 sigma k_cut |a_i| / 2 pi;
+                      an explicit grid still wins, as nModes_in does                          */
 } ew_options;
 
 /* Parameters an evaluation uses; mirrors TEwaldParameters3d

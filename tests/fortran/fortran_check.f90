@@ -8,6 +8,10 @@
 !>   input : nElem, nIter, maxIter, mixParam, omega0, minWeight, maxWeight, weightFac,
 !>           q0(nElem), qDiff(nElem, nIter)
 !>   output: qInpResult(nElem, nIter) after each mix
+!> With "params" it calls the reference-signature ewald_parameters without and with k_cut_in
+!> (no device needed):
+!>   input : n, charges(n), latvecs(3,3), tol, kcut
+!>   output: grid(3) without, grid(3) with k_cut_in, k_cut(2) reported
 !> With "external" it drives the external-charge calls as TCoulomb%getPotential,
 !> %getPotentialGradient and %addExternalPotGrad would be driven:
 !>   input : n, coords(3,n), charges(n), latvecs(3,3), tol, m, chargeCoords(3,m), q(m), blur(m), eps
@@ -18,7 +22,7 @@ program fortran_check
    implicit none
    integer(c_int) :: n, grid, order, nsp, st
    real(c_double), allocatable :: coords(:, :), charges(:), pot(:), shift(:), hub(:)
-   integer(c_int), allocatable :: species(:)
+   integer, allocatable :: species(:)
    real(c_double) :: lat(3, 3), tol, alpha, energy
    type(TEwaldEngine) :: engine
    type(ew_options) :: opts
@@ -31,6 +35,8 @@ program fortran_check
       call get_command_argument(3, mode)
       if (trim(mode) == "external") then
          call external_check(trim(inpath), trim(outpath))
+      else if (trim(mode) == "params") then
+         call params_check(trim(inpath), trim(outpath))
       else
          call mixer_check(trim(inpath), trim(outpath))
       end if
@@ -55,7 +61,14 @@ program fortran_check
    opts%order = order
    st = engine_create(engine, opts)
    if (st /= EW_OK) error stop "engine_create failed"
-   st = engine_set_geometry(engine, coords, lat, species, hub, charges)
+   ! input.bin holds 0-based species; DFTB+ numbers them from 1, and so does the wrapper, which
+   ! hands the C ABI 0-based indices.  An index out of range is refused, not read as another
+   ! species' U.
+   if (nsp > 0 .and. any(species == 0)) then
+      if (engine_set_geometry(engine, coords, lat, species, hub, charges) /= EW_ERR_ARGUMENT) &
+         error stop "engine_set_geometry accepted species index 0 (1-based input)"
+   end if
+   st = engine_set_geometry(engine, coords, lat, species + 1, hub, charges)
    if (st /= EW_OK) then
       print *, engine_error(engine%handle)
       error stop "engine_set_geometry failed"
@@ -73,6 +86,28 @@ program fortran_check
    print '(A,I0,A,ES22.14)', "fortran_check: n=", n, " energy=", energy
 
 contains
+
+   subroutine params_check(inpath, outpath)
+      character(len=*), intent(in) :: inpath, outpath
+      integer(c_int) :: n
+      real(c_double), allocatable :: charges(:), positions(:, :)
+      real(c_double) :: lat(3, 3), tol, kcut
+      type(ew_parameters) :: p0, p1
+      integer :: u
+
+      open (newunit=u, file=inpath, access="stream", form="unformatted", status="old")
+      read (u) n
+      allocate (charges(n), positions(n, 3))
+      read (u) charges, lat, tol, kcut
+      close (u)
+      positions = 0.0_c_double
+      call ewald_parameters(positions, charges, n, transpose(lat), tol, p0)
+      call ewald_parameters(positions, charges, n, transpose(lat), tol, p1, k_cut_in=kcut)
+      open (newunit=u, file=outpath, access="stream", form="unformatted", status="replace")
+      write (u) p0%grid, p1%grid, p0%k_cut, p1%k_cut
+      close (u)
+      print '(A,3I5,A,3I5)', "fortran_check params: grid", p0%grid, " with k_cut_in", p1%grid
+   end subroutine params_check
 
    subroutine mixer_check(inpath, outpath)
       character(len=*), intent(in) :: inpath, outpath

@@ -110,3 +110,57 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_k_cut_override_sets_the_mesh(oracle_mod):
+    """k_cut_in (fe/fft/ewald_fft_3d.f90:36-76, parameters.f90:127-130): the given Fourier cutoff
+    replaces the budget's and the mesh follows from it; the chain's own cutoff reproduces the
+    automatic mesh, a doubled one roughly doubles it, an explicit grid still wins, and a
+    non-finite value is refused."""
+    w = W.triclinic(n=3000, side=31.0)
+    q, lat = w["charges"], w["lattice"]
+    for tol in (1e-6, 1e-9, 1e-10):
+        auto = ew.ewald_parameters(q, lat, tol=tol)
+        m = oracle_mod.cell_metrics(lat)
+        kc = oracle_mod.cutoff_from_fourier_budget(float(np.sum(q * q)), len(q),
+                                                   m["volume"] ** (1.0 / 3.0), auto["alpha"], tol)
+        same = ew.ewald_parameters(q, lat, tol=tol, k_cut=kc)
+        assert same["grid"] == auto["grid"] and same["alpha"] == auto["alpha"]
+        big = ew.ewald_parameters(q, lat, tol=tol, k_cut=2.0 * kc)
+        for a in range(3):
+            assert 1.8 * auto["grid"][a] <= big["grid"][a] <= 2.6 * auto["grid"][a]
+        # the mesh resolves the requested cutoff along every axis
+        assert big["k_cut"] >= 2.0 * kc
+        pinned = ew.ewald_parameters(q, lat, tol=tol, k_cut=2.0 * kc, grid=(40, 40, 40))
+        assert pinned["grid"] == (40, 40, 40)
+    with pytest.raises(ew.EwaldError) as e:
+        ew.ewald_parameters(q, lat, tol=1e-8, k_cut=float("nan"))
+    assert e.value.code == ew.EW_ERR_ARGUMENT
+
+
+def test_fortran_k_cut_in_reaches_the_mesh(tmp_path):
+    """The Fortran ewald_parameters passes k_cut_in through (it used to drop it)."""
+    import subprocess
+    from conftest import PKG
+    binp = os.path.join(ROOT, "tests", "fortran", "bin", "fortran_check")
+    if not os.path.exists(binp):
+        pytest.fail(f"{binp} not built (run __graft_entry__.build())")
+    w = W.triclinic(n=3000, side=31.0)
+    inp, out = tmp_path / "p_in.bin", tmp_path / "p_out.bin"
+    auto = ew.ewald_parameters(w["charges"], w["lattice"], tol=1e-8)
+    with open(inp, "wb") as f:
+        np.array([len(w["charges"])], dtype=np.int32).tofile(f)
+        w["charges"].astype(np.float64).tofile(f)
+        w["lattice"].astype(np.float64).tofile(f)
+        np.array([1e-8, 2.0 * auto["k_cut"]], dtype=np.float64).tofile(f)
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = os.path.join(PKG, "lib") + ":" + env.get("LD_LIBRARY_PATH", "")
+    r = subprocess.run([binp, str(inp), str(out), "params"], env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = np.fromfile(out, dtype=np.int32, count=6)
+    kc = np.fromfile(out, dtype=np.float64, offset=24)
+    assert tuple(g[:3]) == auto["grid"]
+    want = ew.ewald_parameters(w["charges"], w["lattice"], tol=1e-8, k_cut=2.0 * auto["k_cut"])
+    assert tuple(g[3:]) == want["grid"] and tuple(g[3:]) != tuple(g[:3])
+    assert kc[1] == pytest.approx(want["k_cut"], rel=1e-15)
