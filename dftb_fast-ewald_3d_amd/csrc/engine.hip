@@ -159,6 +159,15 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     delete h;
     return set_global_error(EW_ERR_DEVICE, "hipStreamCreate failed");
   }
+  if (hipEventCreateWithFlags(&h->last_ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+    if (h->aux) (void)hipStreamDestroy(h->aux);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return set_global_error(EW_ERR_DEVICE, "hipEventCreate failed");
+  }
   if (o.profile)
     for (auto& e : h->ev)
       if (hipEventCreate(&e) != hipSuccess) {
@@ -189,6 +198,7 @@ extern "C" int ew_destroy(ew_handle* h) {
     if (e) (void)hipEventDestroy(e);
   if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
   if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+  if (h->last_ev) (void)hipEventDestroy(h->last_ev);
   if (h->aux) (void)hipStreamDestroy(h->aux);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -977,15 +987,31 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
 // sequence has no memset or host work, so every replay is the same work).
 constexpr int kDirectLaunchAtoms = 32768;
 
+static int launch_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                       hipStream_t s);
+
 static int run_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
                     hipStream_t s) {
+  HIPCHK(h, h->after_last(s));
+  CHK(launch_eval(h, d_dq, d_shift, d_energy, s));
+  HIPCHK(h, h->mark_last(s));
+  return EW_OK;
+}
+
+static int launch_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                       hipStream_t s) {
   if (h->opt.eager || h->opt.profile || (root_fork(h) && h->n >= kDirectLaunchAtoms))
     return enqueue_eval(h, d_dq, d_shift, d_energy, s);
   hipGraphExec_t exec = nullptr;
   for (auto& g : h->graphs)
     if (g.dq == d_dq && g.shift == d_shift && g.energy == d_energy) exec = g.exec;
   if (!exec) {
-    if (h->graphs.size() >= 8) h->drop_graphs();
+    if (h->graphs.size() >= 8) {
+      // the oldest capture goes once its launches are done
+      h->quiesce();
+      (void)hipGraphExecDestroy(h->graphs.front().exec);
+      h->graphs.erase(h->graphs.begin());
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     const int st = enqueue_eval(h, d_dq, d_shift, d_energy, h->stream);
@@ -1063,8 +1089,7 @@ static int ensure_views(ew_handle* h, int nvec) {
   auto& V = h->views;
   if (V.cap >= nvec) return EW_OK;
   // the smaller buffers go: nothing queued may still read them
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (h->aux) HIPCHK(h, hipStreamSynchronize(h->aux));
+  h->quiesce();
   for (auto& g : h->mgraphs) {
     (void)hipGraphExecDestroy(g.exec);
     (void)hipFree(g.staging);
@@ -1166,9 +1191,20 @@ static int launch_multi(ew_handle* h, const GroupLaunch& G, hipStream_t s) {
 
 // nvec evaluations on stream s: captured once per (nvec, buffers) and replayed as a graph
 // unless eager or profiling.
+static int launch_multi_eval(ew_handle* h, int nvec, const double* d_dq, double* d_shift,
+                             double* d_energy, hipStream_t s);
+
 static int run_multi(ew_handle* h, int nvec, const double* d_dq, double* d_shift,
                      double* d_energy, hipStream_t s) {
   CHK(ensure_views(h, nvec));
+  HIPCHK(h, h->after_last(s));
+  CHK(launch_multi_eval(h, nvec, d_dq, d_shift, d_energy, s));
+  HIPCHK(h, h->mark_last(s));
+  return EW_OK;
+}
+
+static int launch_multi_eval(ew_handle* h, int nvec, const double* d_dq, double* d_shift,
+                             double* d_energy, hipStream_t s) {
   const bool eager = h->opt.eager || h->opt.profile;
   if (!eager)
     for (auto& g : h->mgraphs)
@@ -1181,7 +1217,10 @@ static int run_multi(ew_handle* h, int nvec, const double* d_dq, double* d_shift
   const size_t bytes = S.words.size() * sizeof(int);
   if (eager) {
     if (h->views_staging_cap < bytes) {
-      if (h->views.staging) h->free_buf(h->views.staging);
+      if (h->views.staging) {
+        h->quiesce();  // an earlier launch may still read it
+        h->free_buf(h->views.staging);
+      }
       h->views.staging = nullptr;
       h->views_staging_cap = 0;
       unsigned char* p;
@@ -1194,12 +1233,11 @@ static int run_multi(ew_handle* h, int nvec, const double* d_dq, double* d_shift
     return launch_multi(h, multi_launch(h, nvec, S, h->views.staging), s);
   }
   if (h->mgraphs.size() >= 8) {
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    for (auto& g : h->mgraphs) {
-      (void)hipGraphExecDestroy(g.exec);
-      (void)hipFree(g.staging);
-    }
-    h->mgraphs.clear();
+    // the oldest capture goes once its launches are done
+    h->quiesce();
+    (void)hipGraphExecDestroy(h->mgraphs.front().exec);
+    (void)hipFree(h->mgraphs.front().staging);
+    h->mgraphs.erase(h->mgraphs.begin());
   }
   ew_handle::CapturedMulti c{nvec, d_dq, d_shift, d_energy, nullptr, nullptr};
   if (hipMalloc(&c.staging, bytes) != hipSuccess) {
@@ -1353,6 +1391,8 @@ extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
     return h->fail(EW_ERR_NOT_NEUTRAL, "the cell must be charge neutral (or set opts.charged)");
   HIPCHK(h, hipSetDevice(h->opt.device));
   hipStream_t s = h->stream;
+  // an evaluation on a caller's stream may still use the charge and mesh buffers
+  HIPCHK(h, h->after_last(s));
   HIPCHK(h, hipMemcpyAsync(h->dq_io, dq, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(h, launch_gather_charges(n, h->perm, h->dq_io, h->qs, nullptr, s));
   SpmeLaunch L = make_spme_launch(h);
@@ -1446,7 +1486,8 @@ static int stage_sources(ew_handle* h, int m, const double* coords, const double
   }
   HIPCHK(h, hipSetDevice(h->opt.device));
   hipStream_t s = h->stream;
-  HIPCHK(h, hipStreamSynchronize(s));
+  // an evaluation on a caller's stream may still use the charge and mesh buffers
+  h->quiesce();
   st.m = m;
   TileOrder o;
   order_by_tile(h, m, coords, o);

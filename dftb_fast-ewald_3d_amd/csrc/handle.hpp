@@ -31,6 +31,12 @@ struct ew_handle {
   bool own_stream = true;  // false: the stream belongs to the pool that holds this handle
   hipStream_t aux = nullptr;      // runs the pair sum beside the SPME chain (own-stream handles)
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // Completion of the last evaluation, on whichever stream it ran (ew_shift_device and
+  // ew_shift_multi_device take a caller's stream): the geometry's buffers, the multi-vector
+  // views and the captured graphs are released, rewritten or destroyed only after it.
+  hipEvent_t last_ev = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool last_pending = false;
   std::string err = "no error";
   bool have_geom = false;
 
@@ -132,7 +138,31 @@ struct ew_handle {
   };
   std::vector<CapturedMulti> mgraphs;
 
+  // Orders work about to be enqueued on s after the last evaluation when that ran on another
+  // stream (one handle's buffers serve one evaluation at a time).
+  hipError_t after_last(hipStream_t s) const {
+    if (!last_pending || last_stream == s) return hipSuccess;
+    return hipStreamWaitEvent(s, last_ev, 0);
+  }
+  hipError_t mark_last(hipStream_t s) {
+    const hipError_t e = hipEventRecord(last_ev, s);
+    if (e == hipSuccess) {
+      last_stream = s;
+      last_pending = true;
+    }
+    return e;
+  }
+  // Waits for the handle's streams and for the last evaluation on any stream.
+  void quiesce() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (aux) (void)hipStreamSynchronize(aux);
+    if (last_pending) (void)hipEventSynchronize(last_ev);
+    last_pending = false;
+  }
+
+  // destroys the captured graphs once nothing that launched them can still be running
   void drop_graphs() {
+    if (!graphs.empty() || !mgraphs.empty()) quiesce();
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
     for (auto& g : mgraphs) {
@@ -153,9 +183,9 @@ struct ew_handle {
     return code;
   }
   void release() {
-    // work queued on the handle's streams may still read the buffers
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (aux) (void)hipStreamSynchronize(aux);
+    // work queued on the handle's streams, or an evaluation on a caller's stream, may still
+    // read the buffers
+    quiesce();
     drop_graphs();
     for (auto& b : bufs)
       if (b.p) spare.push_back(b);

@@ -42,6 +42,12 @@ struct ew_pool {
   hipStream_t stream = nullptr;
   hipStream_t aux = nullptr;  // pair sums beside the SPME chain
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // completion of the last grouped evaluation on whichever stream it ran (ew_pool_eval_device
+  // takes a caller's stream): staging, the descriptor table and the captured graphs are reused
+  // or destroyed only after it
+  hipEvent_t last_ev = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool last_pending = false;
   std::string err = "no error";
   bool fp32 = false;
   int order = 0;  // B-spline order shared by every member (0 until the first member)
@@ -66,7 +72,27 @@ struct ew_pool {
     global_error() = msg;
     return code;
   }
+  hipError_t after_last(hipStream_t s) const {
+    if (!last_pending || last_stream == s) return hipSuccess;
+    return hipStreamWaitEvent(s, last_ev, 0);
+  }
+  hipError_t mark_last(hipStream_t s) {
+    const hipError_t e = hipEventRecord(last_ev, s);
+    if (e == hipSuccess) {
+      last_stream = s;
+      last_pending = true;
+    }
+    return e;
+  }
+  void quiesce() {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(aux);
+    if (last_pending) (void)hipEventSynchronize(last_ev);
+    last_pending = false;
+  }
+  // destroys the captured graphs once nothing that launched them can still be running
   void drop_graphs() {
+    if (!graphs.empty()) quiesce();
     for (auto& g : graphs) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
       if (g.staging) (void)hipFree(g.staging);
@@ -104,7 +130,8 @@ int sync_desc(ew_pool* P) {
   if (!P->desc_dirty) return EW_OK;
   const size_t bytes = sizeof(StructDesc) * std::max<size_t>(1, P->desc.size());
   if (P->desc_cap < bytes) {
-    // the captured graphs read the old table
+    // the captured graphs and the last evaluation read the old table
+    P->quiesce();
     P->drop_graphs();
     void* b = P->desc_dev;
     int st = ensure(P, &b, &P->desc_cap, bytes);
@@ -154,11 +181,26 @@ int enqueue(ew_pool* P, GroupLaunch G, hipStream_t s, hipStream_t aux, hipEvent_
   return EW_OK;
 }
 
+int launch_run(ew_pool* P, int count, const int* ids, const double* const* dq,
+               double* const* shift, double* energies, hipStream_t s);
+
 // One grouped evaluation on stream s: graph replay when the same ids and buffers were seen.
+// It is ordered after the pool's previous evaluation when that ran on another stream (the
+// members' buffers serve one evaluation at a time).
 int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* const* shift,
         double* energies, hipStream_t s) {
   int st = sync_desc(P);
   if (st) return st;
+  PCHK(P, P->after_last(s));
+  st = launch_run(P, count, ids, dq, shift, energies, s);
+  if (st) return st;
+  PCHK(P, P->mark_last(s));
+  return EW_OK;
+}
+
+int launch_run(ew_pool* P, int count, const int* ids, const double* const* dq,
+               double* const* shift, double* energies, hipStream_t s) {
+  int st = EW_OK;
   const bool eager = P->opt.eager || P->opt.profile;
   std::vector<const void*> ptrs;
   const std::vector<int> key(ids, ids + count);
@@ -180,13 +222,21 @@ int run(ew_pool* P, int count, const int* ids, const double* const* dq, double* 
   if (st) return st;
   const size_t bytes = S.words.size() * sizeof(int);
   if (eager) {
+    if (P->staging_cap < bytes) P->quiesce();  // an earlier launch may still read the old one
     st = ensure(P, &P->staging, &P->staging_cap, bytes);
     if (st) return st;
     // a pageable source: the copy has read the host image when the call returns
     PCHK(P, hipMemcpyAsync(P->staging, S.words.data(), bytes, hipMemcpyHostToDevice, s));
     return enqueue(P, launch_of(P, S, P->staging), s, P->aux, P->fork_ev, P->join_ev);
   }
-  if (P->graphs.size() >= 8) P->drop_graphs();
+  if (P->graphs.size() >= 8) {
+    // the oldest capture goes once its launches are done
+    P->quiesce();
+    Captured& old = P->graphs.front();
+    (void)hipGraphExecDestroy(old.exec);
+    (void)hipFree(old.staging);
+    P->graphs.erase(P->graphs.begin());
+  }
   Captured c;
   c.ids = key;
   c.ptrs = ptrs;
@@ -275,8 +325,10 @@ extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, 
   P->capacity = capacity_bytes;
   if (hipStreamCreate(&P->stream) != hipSuccess || hipStreamCreate(&P->aux) != hipSuccess ||
       hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->last_ev, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
+    if (P->last_ev) (void)hipEventDestroy(P->last_ev);
     if (P->join_ev) (void)hipEventDestroy(P->join_ev);
     if (P->fork_ev) (void)hipEventDestroy(P->fork_ev);
     if (P->aux) (void)hipStreamDestroy(P->aux);
@@ -291,8 +343,7 @@ extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, 
 extern "C" int ew_pool_destroy(ew_pool* P) {
   if (!P) return EW_OK;
   (void)hipSetDevice(P->opt.device);
-  (void)hipStreamSynchronize(P->stream);
-  (void)hipStreamSynchronize(P->aux);
+  P->quiesce();
   P->drop_graphs();
   for (auto* h : P->members)
     if (h) (void)ew_destroy(h);
@@ -303,6 +354,7 @@ extern "C" int ew_pool_destroy(ew_pool* P) {
   if (P->io_energy) (void)hipFree(P->io_energy);
   (void)hipEventDestroy(P->fork_ev);
   (void)hipEventDestroy(P->join_ev);
+  (void)hipEventDestroy(P->last_ev);
   (void)hipStreamDestroy(P->aux);
   (void)hipStreamDestroy(P->stream);
   delete P;
@@ -367,8 +419,8 @@ extern "C" int ew_pool_remove(ew_pool* P, int id) {
   if (id < 0 || id >= (int)P->members.size() || !P->members[id])
     return P->fail(EW_ERR_ARGUMENT, "unknown structure id " + std::to_string(id));
   PCHK(P, hipSetDevice(P->opt.device));
-  PCHK(P, hipStreamSynchronize(P->stream));
-  // graphs may name the structure's buffers
+  // the last evaluation and the graphs may name the structure's buffers
+  P->quiesce();
   P->drop_graphs();
   P->used -= (long long)P->members[id]->device_bytes;
   (void)ew_destroy(P->members[id]);
@@ -437,8 +489,8 @@ extern "C" int ew_pool_shift(ew_pool* P, int count, const int* ids, const double
   PCHK(P, hipSetDevice(P->opt.device));
   const size_t atoms = off[count];
   if (P->io_atoms_cap < atoms || P->io_struct_cap < (size_t)count) {
-    // graphs may point at the old buffers
-    PCHK(P, hipStreamSynchronize(P->stream));
+    // the last evaluation and the graphs may point at the old buffers
+    P->quiesce();
     P->drop_graphs();
     for (double** b : {&P->io_dq, &P->io_shift, &P->io_energy})
       if (*b) {

@@ -101,3 +101,85 @@ def test_trajectory_matches_fresh_engines(precision):
                 first = s
             elif name == "first again":
                 assert rel_max(s, first) < tol
+
+
+def test_caller_stream_evaluations_are_waited_for():
+    """Evaluations enqueued on a caller's (non-default) stream and not synchronised: a new
+    geometry, the gradients and a second evaluation on another stream must wait for them, since
+    they reuse the same device buffers.  Every result equals a fresh engine's."""
+    import torch
+    dev = torch.device("cuda:0")
+    a = W.water(n_side=8, box=8 * 3.1)
+    b = W.water(seed=5, n_side=6, box=6 * 3.1)
+    with ew.EwaldEngine(tol=1e-8) as f:
+        _set(f, a)
+        want_a = f.shift(a["charges"])
+        want_ga = f.gradients(a["charges"])
+        _set(f, b)
+        want_b = f.shift(b["charges"])
+    na, nb = len(a["charges"]), len(b["charges"])
+    dq_a = torch.from_numpy(a["charges"]).to(dev)
+    dq_b = torch.from_numpy(b["charges"]).to(dev)
+    out_a = torch.zeros(na, dtype=torch.float64, device=dev)
+    out_a2 = torch.zeros(na, dtype=torch.float64, device=dev)
+    out_b = torch.zeros(nb, dtype=torch.float64, device=dev)
+    en = torch.zeros(3, dtype=torch.float64, device=dev)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    say("caller streams: evaluation, gradients, second stream, new geometry, no sync between")
+    with ew.EwaldEngine(tol=1e-8) as e:
+        _set(e, a)
+        e.shift_device(dq_a.data_ptr(), out_a.data_ptr(), en.data_ptr(), stream=s1.cuda_stream)
+        g = e.gradients(a["charges"])  # host entry point on the handle's own stream
+        e.shift_device(dq_a.data_ptr(), out_a2.data_ptr(), en.data_ptr() + 8,
+                       stream=s2.cuda_stream)
+        _set(e, b)  # releases a's buffers: must wait for both evaluations
+        e.shift_device(dq_b.data_ptr(), out_b.data_ptr(), en.data_ptr() + 16,
+                       stream=s1.cuda_stream)
+        torch.cuda.synchronize()
+    assert rel_max(out_a.cpu().numpy(), want_a[0]) < 1e-12
+    assert rel_max(out_a2.cpu().numpy(), want_a[0]) < 1e-12
+    assert rel_max(out_b.cpu().numpy(), want_b[0]) < 1e-12
+    assert float(en[0]) == pytest.approx(want_a[1], rel=1e-12)
+    assert float(en[1]) == pytest.approx(want_a[1], rel=1e-12)
+    assert float(en[2]) == pytest.approx(want_b[1], rel=1e-12)
+    assert rel_max(g, want_ga) < 1e-12
+
+
+@pytest.mark.parametrize("multi", [False, True])
+def test_more_buffer_sets_than_captured_graphs(multi):
+    """Twelve distinct buffer sets (graphs are kept for eight) launched back to back on a
+    caller's stream with no synchronisation: evicting a captured graph must wait for its
+    launches.  Every result is checked."""
+    import torch
+    dev = torch.device("cuda:0")
+    w = W.water(n_side=6, box=6 * 3.1)
+    n = len(w["charges"])
+    rng = np.random.default_rng(5)
+    nv = 2 if multi else 1
+    qs = [np.stack([W.scc_update(w["charges"], rng) for _ in range(nv)]) for _ in range(12)]
+    with ew.EwaldEngine(tol=1e-8) as f:
+        _set(f, w)
+        want = [[f.shift(q[v]) for v in range(nv)] for q in qs]
+    dqs = [torch.from_numpy(q.reshape(-1)).to(dev) for q in qs]
+    outs = [torch.zeros(nv * n, dtype=torch.float64, device=dev) for _ in qs]
+    ens = [torch.zeros(nv, dtype=torch.float64, device=dev) for _ in qs]
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    say(f"12 buffer sets on a caller stream, multi={multi}")
+    with ew.EwaldEngine(tol=1e-8) as e:
+        _set(e, w)
+        for rep in range(2):
+            for k in range(12):
+                if multi:
+                    e.shift_multi_device(nv, dqs[k].data_ptr(), outs[k].data_ptr(),
+                                         ens[k].data_ptr(), stream=s.cuda_stream)
+                else:
+                    e.shift_device(dqs[k].data_ptr(), outs[k].data_ptr(), ens[k].data_ptr(),
+                                   stream=s.cuda_stream)
+            torch.cuda.synchronize()
+            for k in range(12):
+                got = outs[k].cpu().numpy().reshape(nv, n)
+                for v in range(nv):
+                    assert rel_max(got[v], want[k][v][0]) < 1e-12, (rep, k, v)
+                    assert float(ens[k][v]) == pytest.approx(want[k][v][1], rel=1e-12)
