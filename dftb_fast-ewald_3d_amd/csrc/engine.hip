@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,34 @@ extern "C" const char* ew_last_error(const ew_handle* h) {
   return h ? h->err.c_str() : global_error().c_str();
 }
 
+// ---- the allocation-free evaluation's scratch (ew_shift_device_noalloc) ----
+// Charges, potentials, energy partials, mesh, spectrum and spreading copies of one evaluation,
+// carved from one static device array that each device holds until the process ends.  In host
+// code `scratch` only names the symbol: its device address comes from hipGetSymbolAddress, once
+// per device (with that device current), and only that address is passed to kernels.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static void* g_scratch_dev[kMaxScratchDevices] = {};
+static std::mutex g_scratch_mutex;
+
+// Resolves the device address of `scratch` on the current device `device` (once per device).
+static int resolve_static_scratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return set_global_error(EW_ERR_DEVICE, "device ordinal beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_dev[device]) return EW_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return set_global_error(EW_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                               hipGetErrorString(e));
+  }
+  g_scratch_dev[device] = p;
+  return EW_OK;
+}
+
 int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** out) {
   *out = nullptr;
   if (o.precision != EW_PRECISION_FP64 && o.precision != EW_PRECISION_MIXED)
@@ -141,6 +170,7 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     return set_global_error(EW_ERR_ARGUMENT, "device ordinal out of range");
   if (hipSetDevice(o.device) != hipSuccess)
     return set_global_error(EW_ERR_DEVICE, "hipSetDevice failed");
+  if (const int st = resolve_static_scratch(o.device)) return st;
   ew_handle* h = new ew_handle();
   h->opt = o;
   h->fp32 = o.precision == EW_PRECISION_MIXED;
@@ -1053,6 +1083,82 @@ extern "C" int ew_shift_device(ew_handle* h, const double* d_dq, double* d_shift
   HIPCHK(h, hipSetDevice(h->opt.device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
   return run_eval(h, d_dq, d_shift, d_energy ? d_energy : h->energy, s);
+}
+
+// Bytes of static scratch one evaluation of h needs: sorted charges, real-space potentials,
+// energy partials and total, charge sum, mesh, spectrum and spreading copies, each on a
+// 256-byte boundary.  With `base` set, points the handle's per-evaluation buffers there.
+struct NoallocLayout {
+  size_t qs, phi, epart, energy, qsum, grid, cgrid, region, total;
+};
+static NoallocLayout noalloc_layout(const ew_handle* h) {
+  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  const size_t n = (size_t)h->n;
+  const size_t nparts = std::max<size_t>((n + 255) / 256, (size_t)h->nt[0] * h->nt[1] * h->nt[2]);
+  const size_t K0 = h->prm.K[0], K1 = h->prm.K[1], K2 = h->prm.K[2];
+  const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  NoallocLayout L{};
+  size_t off = 0;
+  L.qs = off;     off += up(8 * n);
+  L.phi = off;    off += up(8 * n);
+  L.epart = off;  off += up(8 * nparts);
+  L.energy = off; off += up(8);
+  L.qsum = off;   off += up(8);
+  L.grid = off;   off += up(K0 * K1 * K2 * rs);
+  L.cgrid = off;  off += up((K0 / 2 + 1) * K1 * K2 * 2 * rs);
+  L.region = off;
+  if (h->region.scratch) off += up(nblk * (size_t)h->region.region_stride * rs);
+  L.total = off;
+  return L;
+}
+
+extern "C" int ew_shift_device_noalloc(ew_handle* h, const double* d_dq, double* d_shift,
+                                       double* d_energy, void* stream) {
+  if (!h) return set_global_error(EW_ERR_ARGUMENT, "ew_shift_device_noalloc: handle is NULL");
+  if (!h->have_geom)
+    return h->fail(EW_ERR_NO_GEOMETRY, "ew_shift_device_noalloc before ew_set_geometry");
+  if (!d_dq || !d_shift) return h->fail(EW_ERR_ARGUMENT, "ew_shift_device_noalloc: NULL array");
+  if (h->opt.profile)
+    return h->fail(EW_ERR_ARGUMENT, "ew_shift_device_noalloc does not run with opts.profile");
+  const NoallocLayout L = noalloc_layout(h);
+  if (L.total > kStaticScratchBytes)
+    return h->fail(EW_ERR_MEMORY, "ew_shift_device_noalloc: this geometry needs " +
+                                      std::to_string(L.total) + " bytes of scratch, above the " +
+                                      std::to_string(kStaticScratchBytes) + "-byte static limit");
+  void* base = g_scratch_dev[h->opt.device];
+  if (!base) return h->fail(EW_ERR_DEVICE, "static scratch not resolved on this device");
+  HIPCHK(h, hipSetDevice(h->opt.device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  unsigned char* b = static_cast<unsigned char*>(base);
+  // the evaluation's buffers point into the scratch for this call only (eager launches: a
+  // captured graph would keep the handle's own buffers)
+  struct Swap {
+    ew_handle* h;
+    double *qs, *phi, *epart, *qsum;
+    void *grid, *cgrid, *region;
+    ~Swap() {
+      h->qs = qs;
+      h->phi_real = phi;
+      h->epart = epart;
+      h->qsum = qsum;
+      h->grid = grid;
+      h->cgrid = cgrid;
+      h->region.scratch = region;
+    }
+  } keep{h, h->qs, h->phi_real, h->epart, h->qsum, h->grid, h->cgrid, h->region.scratch};
+  h->qs = reinterpret_cast<double*>(b + L.qs);
+  h->phi_real = reinterpret_cast<double*>(b + L.phi);
+  h->epart = reinterpret_cast<double*>(b + L.epart);
+  if (h->qsum) h->qsum = reinterpret_cast<double*>(b + L.qsum);
+  h->grid = b + L.grid;
+  h->cgrid = b + L.cgrid;
+  if (h->region.scratch) h->region.scratch = b + L.region;
+  double* energy = d_energy ? d_energy : reinterpret_cast<double*>(b + L.energy);
+  HIPCHK(h, h->after_last(s));
+  CHK(enqueue_eval(h, d_dq, d_shift, energy, s));
+  HIPCHK(h, h->mark_last(s));
+  return EW_OK;
 }
 
 extern "C" int ew_shift(ew_handle* h, const double* dq, double* shift, double* energy) {

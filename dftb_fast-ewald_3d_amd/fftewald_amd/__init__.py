@@ -39,7 +39,7 @@ MAX_VECTORS = 8
 
 #: every symbol the C ABI (include/fftewald_amd.h) exports
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
-            "ew_shift", "ew_shift_device", "ew_shift_multi", "ew_shift_multi_device",
+            "ew_shift", "ew_shift_device", "ew_shift_device_noalloc", "ew_shift_multi", "ew_shift_multi_device",
             "ew_gradients", "ew_stress", "ew_external_potential", "ew_external_gradients",
             "ew_phase_times",
             "ew_potential_force",
@@ -154,6 +154,7 @@ def lib() -> C.CDLL:
         L.ew_get_parameters.argtypes = [H, C.POINTER(Parameters)]
         L.ew_shift.argtypes = [H, _D1, _D1, C.POINTER(C.c_double)]
         L.ew_shift_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ew_shift_device_noalloc.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ew_shift_multi.argtypes = [H, C.c_int, _D2, _D2, _D1]
         L.ew_shift_multi_device.argtypes = [H, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]

@@ -154,6 +154,20 @@ int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
 int ew_shift_device(ew_handle *h, const double *d_dq, double *d_shift, double *d_energy,
                     void *stream);
 
+/* ew_shift_device without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The evaluation's scratch (sorted charges, real-space
+ * potentials, energy partials, mesh, spectrum and spreading copies) lives in one static device
+ * array of 64 MiB per device, held until the process ends, instead of the handle's buffers; the
+ * kernels are ew_shift_device's, launched directly (no graph is captured or instantiated).  A
+ * geometry whose scratch exceeds 64 MiB gets EW_ERR_MEMORY (e.g. the 98,304-atom cell at 128^3
+ * fits in mixed precision, not in FP64).  The caller passes the inputs and outputs as for
+ * ew_shift_device (d_energy may be NULL).  Every call on one device shares that array, which
+ * keeps the last call's contents: calls that overlap on one device race (order them on one
+ * stream or synchronise between them). */
+int ew_shift_device_noalloc(ew_handle *h, const double *d_dq, double *d_shift, double *d_energy,
+                            void *stream);
+
 /* Several charge vectors through the current geometry in one grouped launch: the analogue of
  * nvec calls of TCoulomb%updateShifts on one structure (coulomb.F90:786-880), e.g. the charge
  * vectors of independent SCC cycles that share a geometry (spin or k-point replicas, trial
