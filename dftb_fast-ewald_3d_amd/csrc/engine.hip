@@ -161,6 +161,8 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     return set_global_error(EW_ERR_ARGUMENT, "fft_mode must be 0, 1 or 2");
   if (o.pair_storage < 0 || o.pair_storage > 1)
     return set_global_error(EW_ERR_ARGUMENT, "pair_storage must be 0 or 1");
+  if (o.profile < 0 || o.profile > 2)
+    return set_global_error(EW_ERR_ARGUMENT, "profile must be 0, 1 or 2");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     (void)hipGetLastError();
@@ -198,9 +200,18 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     delete h;
     return set_global_error(EW_ERR_DEVICE, "hipEventCreate failed");
   }
-  if (o.profile)
+  if (o.profile == 1)
     for (auto& e : h->ev)
       if (hipEventCreate(&e) != hipSuccess) {
+        if (h->own_stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return set_global_error(EW_ERR_DEVICE, "hipEventCreate failed");
+      }
+  // timing only: no system-scope fence (cache write-back) at each event, which would slow the
+  // kernels it separates
+  if (o.profile == 2)
+    for (auto& e : h->tev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return set_global_error(EW_ERR_DEVICE, "hipEventCreate failed");
@@ -225,6 +236,8 @@ extern "C" int ew_destroy(ew_handle* h) {
   h->release();
   h->trim();
   for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : h->tev)
     if (e) (void)hipEventDestroy(e);
   if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
   if (h->join_ev) (void)hipEventDestroy(h->join_ev);
@@ -934,15 +947,17 @@ static hipError_t launch_gather(const ew_handle* h, const GatherArgs& ga, const 
                                                  s);
 }
 
-// The root fork (see enqueue_eval): privatised spreading on an own-stream handle.
+// The root fork (see enqueue_eval): privatised spreading on an own-stream handle.  Phase
+// profiling (opt.profile = 1) runs every phase alone; tracing (2) keeps the launch shape.
 static bool root_fork(const ew_handle* h) {
-  return !h->opt.profile && h->aux && h->region.scratch != nullptr;
+  return h->opt.profile != 1 && h->aux && h->region.scratch != nullptr;
 }
 
 static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
                         hipStream_t s) {
   const int n = h->n;
-  const bool prof = h->opt.profile != 0;
+  const bool prof = h->opt.profile == 1;
+  const bool trace = h->opt.profile == 2;
   // The real-space pair sum runs on the auxiliary stream beside the spreading and FFTs (which
   // leave CUs idle) and joins before the gather.  With privatised spreading the two branches
   // start together: the charge gather, charge sum and pair sum run on the auxiliary stream,
@@ -954,17 +969,23 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
   const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
   auto pairs = [&](hipStream_t st) {
-    return h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real,
-                                      st)
-                   : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
-                                      h->qs, h->phi_real, fork, st);
+    h->record_trace(ew_handle::kTrPair0, st);
+    const hipError_t e =
+        h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real, st)
+                : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
+                                   h->qs, h->phi_real, fork, st);
+    h->record_trace(ew_handle::kTrPair1, st);
+    return e;
   };
   auto charges = [&](hipStream_t st) {
+    h->record_trace(ew_handle::kTrChg0, st);
     HIPCHK(h, launch_gather_charges(n, h->perm, d_dq, h->qs, h->half ? h->phi_real : nullptr, st));
     if (h->qsum) HIPCHK(h, launch_charge_sum(n, h->qs, h->qsum, st));
+    h->record_trace(ew_handle::kTrChg1, st);
     return EW_OK;
   };
   if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
+  h->record_trace(ew_handle::kTrStart, s);
   if (root) {
     HIPCHK(h, hipEventRecord(h->fork_ev, s));
     HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
@@ -986,16 +1007,36 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     L.q = d_dq;
     L.qperm = h->perm;
   }
+  h->record_trace(ew_handle::kTrChain0, s);
+  hipEvent_t* marks = prof ? &h->ev[2] : (trace ? &h->tev[ew_handle::kTrSpread] : nullptr);
+  hipEvent_t region_mark = trace ? h->tev[ew_handle::kTrRegion] : nullptr;
   if (h->fp32)
-    HIPCHK(h, launch_spme<float>(L, s, prof ? &h->ev[2] : nullptr));
+    HIPCHK(h, launch_spme<float>(L, s, marks, region_mark));
   else
-    HIPCHK(h, launch_spme<double>(L, s, prof ? &h->ev[2] : nullptr));
+    HIPCHK(h, launch_spme<double>(L, s, marks, region_mark));
   if (fork)
     HIPCHK(h, hipStreamWaitEvent(s, h->join_ev, 0));
   else
     HIPCHK(h, pairs(s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
+  h->record_trace(ew_handle::kTrJoined, s);
   HIPCHK(h, launch_gather(h, gather_args(h), h->qs, h->phi_real, h->onsite, d_shift, d_energy, s));
+  if (trace) {
+    h->record_trace(ew_handle::kTrEnd, s);
+    HIPCHK(h, hipEventSynchronize(h->tev[ew_handle::kTrEnd]));
+    using T = ew_handle;
+    const int span[EW_NUM_TRACE][2] = {
+        {T::kTrChg0, T::kTrChg1},    {T::kTrPair0, T::kTrPair1},  {T::kTrChain0, T::kTrRegion},
+        {T::kTrRegion, T::kTrSpread}, {T::kTrSpread, T::kTrX},    {T::kTrX, T::kTrY},
+        {T::kTrY, T::kTrZ},          {T::kTrZ, T::kTrYi},         {T::kTrYi, T::kTrXi},
+        {T::kTrXi, T::kTrJoined},    {T::kTrJoined, T::kTrEnd},   {T::kTrStart, T::kTrEnd}};
+    for (int i = 0; i < EW_NUM_TRACE; ++i) {
+      float ms = 0.0f;
+      HIPCHK(h, hipEventElapsedTime(&ms, h->tev[span[i][0]], h->tev[span[i][1]]));
+      h->trace_ms[i] += ms;
+    }
+    h->trace_calls++;
+  }
   if (prof) {
     HIPCHK(h, hipEventRecord(h->ev[9], s));
     HIPCHK(h, hipEventSynchronize(h->ev[9]));
@@ -1065,12 +1106,26 @@ static int launch_eval(ew_handle* h, const double* d_dq, double* d_shift, double
 
 extern "C" int ew_phase_times(ew_handle* h, double* ms, long long* calls, int reset) {
   if (!h || !ms) return set_global_error(EW_ERR_ARGUMENT, "ew_phase_times: NULL argument");
-  if (!h->opt.profile) return h->fail(EW_ERR_ARGUMENT, "ew_phase_times needs opts.profile = 1");
+  if (h->opt.profile != 1)
+    return h->fail(EW_ERR_ARGUMENT, "ew_phase_times needs opts.profile = 1");
   for (int i = 0; i < EW_NUM_PHASES; ++i) ms[i] = h->phase_ms[i];
   if (calls) *calls = h->phase_calls;
   if (reset) {
     for (int i = 0; i < EW_NUM_PHASES; ++i) h->phase_ms[i] = 0.0;
     h->phase_calls = 0;
+  }
+  return EW_OK;
+}
+
+extern "C" int ew_trace_times(ew_handle* h, double* ms, long long* calls, int reset) {
+  if (!h || !ms) return set_global_error(EW_ERR_ARGUMENT, "ew_trace_times: NULL argument");
+  if (h->opt.profile != 2)
+    return h->fail(EW_ERR_ARGUMENT, "ew_trace_times needs opts.profile = 2");
+  for (int i = 0; i < EW_NUM_TRACE; ++i) ms[i] = h->trace_ms[i];
+  if (calls) *calls = h->trace_calls;
+  if (reset) {
+    for (int i = 0; i < EW_NUM_TRACE; ++i) h->trace_ms[i] = 0.0;
+    h->trace_calls = 0;
   }
   return EW_OK;
 }
@@ -1272,7 +1327,7 @@ static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, c
                      0};
   }
   // profiling runs every kernel on the evaluation stream, one after the other
-  G.aux = h->opt.profile ? nullptr : h->aux;
+  G.aux = h->opt.profile == 1 ? nullptr : h->aux;
   G.pm.beside = G.aux != nullptr;
   G.fork_ev = h->fork_ev;
   G.join_ev = h->join_ev;

@@ -161,6 +161,9 @@ struct ew_handle {
   }
 
   // destroys the captured graphs once nothing that launched them can still be running
+  void record_trace(int i, hipStream_t s) {
+    if (opt.profile == 2) (void)hipEventRecord(tev[i], s);
+  }
   void drop_graphs() {
     if (!graphs.empty() || !mgraphs.empty()) quiesce();
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
@@ -172,10 +175,27 @@ struct ew_handle {
     mgraphs.clear();
   }
 
-  // per-phase timing (opt.profile)
+  // per-phase timing (opt.profile = 1: every phase alone on the stream)
   hipEvent_t ev[EW_NUM_PHASES + 1] = {};
   double phase_ms[EW_NUM_PHASES] = {};
   long long phase_calls = 0;
+  // timing in the evaluation's own launch shape (opt.profile = 2), see TraceEvent
+  enum TraceEvent {
+    kTrStart = 0,  // stream s: start of the evaluation
+    kTrChg0,       // charges: before the gather (on the stream that runs them)
+    kTrChg1,       //          after the gather and charge sum
+    kTrPair0,      // pair sum: before (auxiliary stream when it runs beside the chain)
+    kTrPair1,      //           after
+    kTrChain0,     // stream s: before the spreading
+    kTrRegion,     //           after the region kernel
+    kTrSpread, kTrX, kTrY, kTrZ, kTrYi, kTrXi,  // after each chain phase
+    kTrJoined,     //           after the wait for the pair sum
+    kTrEnd,        //           after the gather and energy
+    kTrEvents
+  };
+  hipEvent_t tev[kTrEvents] = {};
+  double trace_ms[EW_NUM_TRACE] = {};
+  long long trace_calls = 0;
 
   int fail(int code, const std::string& msg) {
     err = msg;

@@ -2762,10 +2762,12 @@ static size_t fft_smem(const FftPlan& pl) {
 }
 
 template <typename R>
-hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
+hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
+                       hipEvent_t ev_region) {
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
+  if (ev_region && !L.region.scratch) (void)hipEventRecord(ev_region, st);
   const int K1 = L.K[0], K2 = L.K[1], K3 = L.K[2], Kh = K1 / 2 + 1;
   R* grid = static_cast<R*>(L.grid);
   cx<R>* cg = static_cast<cx<R>*>(L.cgrid);
@@ -2793,6 +2795,7 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
       default: EW_REGION(0); break;
     }
 #undef EW_REGION
+    if (ev_region) (void)hipEventRecord(ev_region, st);
     hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk * kAssembleSplit), dim3(256), 0, st,
                        static_cast<const R*>(scr), rg, grid);
   } else if (L.spread.slots > 0) {
@@ -2885,8 +2888,8 @@ hipError_t prepare_plane_kernels() {
   return done;
 }
 
-template hipError_t launch_spme<double>(const SpmeLaunch&, hipStream_t, hipEvent_t*);
-template hipError_t launch_spme<float>(const SpmeLaunch&, hipStream_t, hipEvent_t*);
+template hipError_t launch_spme<double>(const SpmeLaunch&, hipStream_t, hipEvent_t*, hipEvent_t);
+template hipError_t launch_spme<float>(const SpmeLaunch&, hipStream_t, hipEvent_t*, hipEvent_t);
 
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,
                                  double* zero, hipStream_t st) {

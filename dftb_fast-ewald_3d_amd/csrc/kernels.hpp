@@ -314,9 +314,12 @@ template <typename R>
 hipError_t launch_group(const GroupLaunch& G, hipStream_t st);
 hipError_t launch_pair_spmm(const PairMulti& pm, bool fp32, hipStream_t st);
 
-// ev (optional, 6 events): recorded after spread, x, y, z, inverse y, inverse x.
+// ev (optional, 6 events): recorded after spread, x, y, z, inverse y, inverse x; ev_region
+// (optional): after the privatised spreading's region kernel, before its assembly (at the start
+// of the spreading when it has no region kernel).
 template <typename R>
-hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
+hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
+                       hipEvent_t ev_region = nullptr);
 
 // qs[k] = dq[perm[k]]; zero (when not null) is cleared for the symmetric pair sum.
 hipError_t launch_gather_charges(int n, const int* perm, const double* dq, double* qs,

@@ -41,7 +41,7 @@ MAX_VECTORS = 8
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
             "ew_shift", "ew_shift_device", "ew_shift_device_noalloc", "ew_shift_multi", "ew_shift_multi_device",
             "ew_gradients", "ew_stress", "ew_external_potential", "ew_external_gradients",
-            "ew_phase_times",
+            "ew_phase_times", "ew_trace_times",
             "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
@@ -54,6 +54,10 @@ EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_g
 #: phases reported by ew_phase_times, in order
 PHASES = ("gather_charges", "spread", "fft_x_r2c", "fft_y", "fft_z_influence", "fft_y_inverse",
           "fft_x_c2r", "pair_sum", "gather_combine")
+#: spans reported by ew_trace_times (the evaluation's own launch shape), in order
+TRACE = ("charges", "pair_sum", "spread_region", "spread_assemble", "fft_x_r2c", "fft_y",
+         "fft_z_influence", "fft_y_inverse", "fft_x_c2r", "join_wait", "gather_combine",
+         "evaluation")
 
 
 class EwaldError(RuntimeError):
@@ -167,6 +171,9 @@ def lib() -> C.CDLL:
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
+        L.ew_trace_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
+                                                                flags="C_CONTIGUOUS"),
+                                     C.POINTER(C.c_longlong), C.c_int]
         L.ew_potential_force.argtypes = [C.c_int, _D2, _D1, _D2, C.POINTER(Options), _D1,
                                          _optional(_D2)]
         L.ew_parameters_for.argtypes = [C.c_int, _optional(_D1), _D2, C.POINTER(Options),
@@ -226,7 +233,7 @@ def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precis
     o.device = device
     o.alpha_rule = {"fftewald": 0, "dftb": 1}[alpha_rule] if isinstance(alpha_rule, str) \
         else int(alpha_rule)
-    o.profile = 1 if profile else 0
+    o.profile = 2 if profile == "trace" else (1 if profile else 0)
     o.eager = 1 if eager else 0
     o.charged = 1 if charged else 0
     o.fft_mode = int(fft_mode)
@@ -396,6 +403,14 @@ class EwaldEngine:
         calls = C.c_longlong()
         _check(lib().ew_phase_times(self._h, ms, C.byref(calls), 1 if reset else 0), self._h)
         return dict(zip(PHASES, ms.tolist())), calls.value
+
+    def trace_times(self, reset: bool = False):
+        """({span: ms}, calls) accumulated since the last reset, timed in the evaluation's own
+        launch shape (needs profile="trace")."""
+        ms = np.zeros(len(TRACE))
+        calls = C.c_longlong()
+        _check(lib().ew_trace_times(self._h, ms, C.byref(calls), 1 if reset else 0), self._h)
+        return dict(zip(TRACE, ms.tolist())), calls.value
 
 
 class EwaldPool:

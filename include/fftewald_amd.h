@@ -65,7 +65,10 @@ typedef struct ew_options {
                       (fe/fft/ewald_fft_3d_parameters.f90:114-121); 1 = DFTB+'s
                       getOptimalAlphaEwald(tol) (src/dftbp/dftb/coulomb.F90:2097-2179) with
                       r_cut = sqrt(-ln tol)/alpha (fe/common/ewald_truncation.f90:36-45)     */
-  int profile;     /* 1: record HIP events around every phase of each evaluation (ew_phase_times) */
+  int profile;     /* 1: record HIP events around every phase of each evaluation, every phase alone
+                      on the handle's stream (ew_phase_times); 2: record HIP events in the
+                      evaluation's own launch shape -- pair sum beside the chain, direct
+                      launches -- without changing it (ew_trace_times)                       */
   int eager;       /* 1: launch the evaluation's kernels one by one; 0 (default): replay a HIP
                       graph captured once per (dq, shift, energy) buffer triple, except for
                       large structures (>= 32768 atoms with privatised spreading), whose
@@ -197,6 +200,20 @@ int ew_gradients(ew_handle *h, const double *dq, double *grad);
  *   5 inverse FFT y   6 FFT x (c2r)   7 pair sum       8 gather + combine + energy */
 #define EW_NUM_PHASES 9
 int ew_phase_times(ew_handle *h, double *ms, long long *calls, int reset);
+
+/* Device time per span of the evaluations run since the last reset, in milliseconds, measured
+ * by HIP events in the evaluation's own launch shape (opts.profile = 2; each evaluation waits
+ * for its last event).  The spans on the evaluation stream are the critical path; the charge
+ * gather and the pair sum run on the auxiliary stream beside it when the handle forks:
+ *   0 charge gather (+ charge sum)   1 pair sum            2 spread: region kernel
+ *   3 spread: assembly (or the whole spreading without privatised blocks)
+ *   4 FFT x / fused x-y (r2c)        5 FFT y               6 FFT z + influence + z^-1
+ *   7 inverse FFT y                  8 inverse FFT x / fused y-x (c2r)
+ *   9 wait for the pair sum          10 gather + combine + energy
+ *   11 whole evaluation (start to end on the evaluation stream)
+ * No counterpart in the reference (its per-phase timing is ew_phase_times'). */
+#define EW_NUM_TRACE 12
+int ew_trace_times(ew_handle *h, double *ms, long long *calls, int reset);
 
 /* TCoulomb%addStress (src/dftbp/dftb/coulomb.F90:721-757, invRStress 2777-2863): adds the Ewald
  * stress at charges dq into stress[3][3] (row-major, Hartree/Bohr^3): sigma = W/V with the
