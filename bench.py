@@ -50,22 +50,40 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False):
     }
 
 
-def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True):
+def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, tuning=None):
     """Algorithmic bytes of each span of ew_trace_times (the evaluation's launch shape).
 
-    The privatised spreading's region kernel reads the atoms' stencil origins, weights, charges
-    and sort permutation; its assembly writes the mesh (the block copies between them are not
-    algorithmic).  Without privatised blocks the whole spreading is the assembly span."""
+    The privatised spreading's region kernel reads the atoms' stencil origins, weights and
+    charges (and the sort permutation when it reads the caller's charges); its assembly writes
+    the mesh (the block copies between them are not algorithmic; with direct write-out the
+    region kernel writes the mesh).  Without privatised blocks the whole spreading is the
+    assembly span.  With the pair sum co-scheduled in the chain's launches its bytes go to
+    those spans in the proportions of its row split."""
     pb = phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes)
     G = K[0] * K[1] * K[2]
-    atoms = pb["spread"] - rs * G + (4 * n if region else 0)
-    return {"charges": pb["gather_charges"], "pair_sum": pb["pair_sum"],
-            "spread_region": atoms if region else 0,
-            "spread_assemble": rs * G if region else pb["spread"],
-            "fft_x_r2c": pb["fft_x_r2c"], "fft_y": pb["fft_y"],
-            "fft_z_influence": pb["fft_z_influence"], "fft_y_inverse": pb["fft_y_inverse"],
-            "fft_x_c2r": pb["fft_x_c2r"], "join_wait": 0,
-            "gather_combine": pb["gather_combine"]}
+    tuning = tuning or {}
+    co = bool(tuning.get("pair_cosched", 0))
+    direct = bool(tuning.get("spread_direct", 0))
+    atoms = pb["spread"] - rs * G + (4 * n if region and not co else 0)
+    out = {"charges": pb["gather_charges"], "pair_sum": pb["pair_sum"],
+           "spread_region": atoms if region else 0,
+           "spread_assemble": rs * G if region else pb["spread"],
+           "fft_x_r2c": pb["fft_x_r2c"], "fft_y": pb["fft_y"],
+           "fft_z_influence": pb["fft_z_influence"], "fft_y_inverse": pb["fft_y_inverse"],
+           "fft_x_c2r": pb["fft_x_c2r"], "join_wait": 0,
+           "gather_combine": pb["gather_combine"]}
+    if region and direct:
+        out["spread_region"] += rs * G
+        out["spread_assemble"] = 0
+    if co:
+        share = tuning.get("pair_spread_share", 0) / 100.0 if region else 0.0
+        ffts = ["fft_x_r2c", "fft_z_influence", "fft_x_c2r"] if planes else \
+            ["fft_x_r2c", "fft_y", "fft_z_influence", "fft_y_inverse", "fft_x_c2r"]
+        out["spread_region"] += share * pb["pair_sum"]
+        for k in ffts:
+            out[k] += (1.0 - share) * pb["pair_sum"] / len(ffts)
+        out["pair_sum"] = 0
+    return out
 
 
 #: spans of ew_trace_times on the evaluation stream (the chain), in launch order
@@ -561,6 +579,9 @@ def main():
                     help="mesh workload: mesh points per axis")
     ap.add_argument("--external", type=int, default=20000,
                     help="triclinic: external point charges timed once per precision (0: skip)")
+    ap.add_argument("--tune", default="auto", choices=["auto", "rules"],
+                    help="launch shapes of the timed engines: the autotuner (default) or the "
+                         "fixed rules")
     ap.add_argument("--headline-only", action="store_true",
                     help="triclinic: only the timed, latency and traced passes (every chain "
                          "kernel then runs in the timed launch shape, for kernel-trace "
@@ -610,13 +631,21 @@ def main():
     res = {}
     shifts = {}
     for prec in ("fp64", "mixed"):
+        # the fixed launch-shape rules, for comparison with the autotuned engine below
         with ew.EwaldEngine(precision=prec, **opts) as e:
+            e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+            rules_ms = timed_run(e, torch, dqs, out, en, args.steps, args.warmup)
+            rules = e.tuning()
+        with ew.EwaldEngine(precision=prec, tune=args.tune, **opts) as e:
             t = time.perf_counter()
             e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
             torch.cuda.synchronize()
             geom_s = time.perf_counter() - t
             prm = e.parameters()
+            tuning = e.tuning()
             ms = timed_run(e, torch, dqs, out, en, args.steps, args.warmup)
+            print(f"[bench] {prec}: rules {rules_ms * 1e3:.1f} us, {args.tune} {ms * 1e3:.1f} us "
+                  f"per evaluation; shapes {tuning}", file=sys.stderr, flush=True)
             p50, p99 = latency_run(e, torch, dqs, out, en, args.latency_steps)
             s, energy = e.shift(qs[0])
             shifts[prec] = (s, energy)
@@ -692,7 +721,7 @@ def main():
                       flush=True)
         sp = torch.cuda.current_stream().cuda_stream
         # the timed launch shape again, with HIP events between its kernels (ew_trace_times)
-        with ew.EwaldEngine(precision=prec, profile="trace", **opts) as e:
+        with ew.EwaldEngine(precision=prec, profile="trace", tuning=tuning, **opts) as e:
             e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
             for i in range(3):
                 e.shift_device(dqs[i % 4].data_ptr(), out.data_ptr(), en.data_ptr(), sp)
@@ -703,7 +732,7 @@ def main():
         rs = 4 if prec == "mixed" else 8
         planes = bool(prm.get("fft_planes", 0))
         tb = trace_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
-                         prm["index_bytes"], False, planes, trace["spread_region"] > 0)
+                         prm["index_bytes"], False, planes, trace["spread_region"] > 0, tuning)
         crit = critical_path(trace, tcalls, tb, prec, prm, ms)
         print(f"[bench] {prec} critical path: chain {crit['chain_us']:.1f} us "
               f"({crit['chain_frac']:.3f} of HBM peak), dominant {crit['dominant']['span']} "
@@ -711,7 +740,7 @@ def main():
         # every phase alone on the stream (ew_phase_times)
         phases, calls = {}, 0
         if not args.headline_only:
-            with ew.EwaldEngine(precision=prec, profile=True, **opts) as e:
+            with ew.EwaldEngine(precision=prec, profile=True, tuning=tuning, **opts) as e:
                 e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
                 for i in range(3):
                     e.shift_device(dqs[i % 4].data_ptr(), out.data_ptr(), en.data_ptr(), sp)
@@ -731,6 +760,7 @@ def main():
                 ph[k]["note"] = "fused into the x-y plane kernels (event overhead only)"
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
                      "multi": multi, "external": ext, "md_step": md, "critical_path": crit,
+                     "tuning": tuning, "rules_ms": rules_ms, "rules": rules,
                      "self_energy": self_energy,
                      "p50_ms": p50, "p99_ms": p99, "params": prm, "phases": ph,
                      "host_api_ms": host_ms}
@@ -773,7 +803,9 @@ def main():
         "config": {"workload": "triclinic 98,304 atoms, a=b=c=99.4 A, gamma=75 deg, "
                                "Poisson-disc d_min 1 A, q U[-1,1]; SPME 128^3 order 6, "
                                "r_cut 10 A, tol 1e-10 (alpha from the reference's real-space "
-                               "budget rule); geometry resident, charges in HBM",
+                               "budget rule); geometry resident, charges in HBM; launch "
+                               "shapes from the autotuner (tuned at ew_set_geometry, outside "
+                               "the timed region)",
                    "atoms": n, "grid": list(head["params"]["grid"]),
                    "order": head["params"]["order"], "alpha": head["params"]["alpha"],
                    "r_cut_bohr": head["params"]["r_cut"], "pairs": head["params"]["n_pairs"],
@@ -794,6 +826,10 @@ def main():
                                       "the reciprocal energy is amplified by that cancellation; "
                                       "the 1e-6 gate of the metric is on the shifts"},
         "critical_path": {k: v["critical_path"] for k, v in res.items()},
+        "launch_shapes": {k: {"tuned": v["tuning"], "rules": v["rules"],
+                              "rules_ms_per_eval": round(v["rules_ms"], 5),
+                              "tuned_ms_per_eval": round(v["ms_per_eval"], 5)}
+                          for k, v in res.items()},
         "multi_vector": {k: v["multi"] for k, v in res.items()},
         "best_mixed_multi_over_plain_fp64": round(
             max(m["evals_per_s"] for m in res["mixed"]["multi"].values()) /

@@ -5,11 +5,13 @@
 // orders the atoms by SPME tile, uploads them and builds the resident pair list.  ew_shift is
 // one SCC iteration (TCoulomb%updateShifts, coulomb.F90:786-880): gather charges, spread, FFT,
 // influence, inverse FFT, pair sum, gather and combine -- nine kernels on the handle's stream.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -39,11 +41,12 @@ int set_global_error(int code, const std::string& msg) {
 
 namespace {
 
-// FFT workgroup shape for lines of length n: as many lines as fit the LDS budget.
-FftPlan make_plan(int n, int want_lines) {
+// FFT workgroup shape for lines of length n: want_lines lines, at most as many as fit the LDS
+// budget; radix passes in the order of `variant` (fft_plan).
+FftPlan make_plan(int n, int want_lines, int variant = 0) {
   FftPlan pl{};
   pl.n = n;
-  pl.npass = fft_plan(n, pl.radix);
+  pl.npass = fft_plan(n, pl.radix, variant);
   int lines = std::max(1, std::min(kFftMaxElems / n, want_lines));
   pl.lines = lines;
   // one pad slot per eight elements (line_slot) and an odd pitch between lines
@@ -86,13 +89,13 @@ bool build_source_table(int K, int T, int nt, int p, int ntmax, int axis, std::v
 }
 
 template <typename R>
-void twiddles(int n, std::vector<R>& out) {
+void twiddles(int n, std::vector<R>& out, int variant) {
   // Per-pass tables, concatenated in pass order: a pass of radix r after a product Ns of the
   // earlier radices holds exp(-2 pi i t k / (Ns r)) at [(t - 1) * Ns + k], t = 1..r-1,
   // k = 0..Ns-1, so that the lanes of a butterfly read consecutive entries.  The sizes
   // telescope to n - 1 entries; one spare keeps the table n long.
   int radix[kMaxRadixPasses];
-  const int np = fft_plan(n, radix);
+  const int np = fft_plan(n, radix, variant);
   out.assign(2 * (size_t)n, R(0));
   size_t o = 0;
   int Ns = 1;
@@ -163,6 +166,7 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     return set_global_error(EW_ERR_ARGUMENT, "pair_storage must be 0 or 1");
   if (o.profile < 0 || o.profile > 2)
     return set_global_error(EW_ERR_ARGUMENT, "profile must be 0, 1 or 2");
+  if (o.tune < 0 || o.tune > 3) return set_global_error(EW_ERR_ARGUMENT, "tune must be 0..3");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     (void)hipGetLastError();
@@ -451,6 +455,383 @@ static void order_by_tile(const ew_handle* h, int n, const double* coords, TileO
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// Launch shapes (ew_tuning): fixed rules, their application, the tuning cache and the tuner
+// ------------------------------------------------------------------------------------------
+
+static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
+                        hipStream_t s);
+
+// Privatised spreading blocks of f[a] sort tiles per axis, if the mesh holds them (a copy may
+// cover no point twice: K >= S + p - 1) and their FP64 LDS copy fits max_bytes.
+static bool region_shape(const ew_handle* h, const int* f, size_t max_bytes, RegionArgs* out) {
+  const int* K = h->prm.K;
+  const int p = h->prm.order;
+  RegionArgs rg{};
+  for (int a = 0; a < 3; ++a) {
+    if (f[a] < 1) return false;
+    rg.K[a] = K[a];
+    rg.f[a] = f[a];
+    rg.S[a] = f[a] * h->T[a];
+    rg.nt[a] = h->nt[a];
+    rg.nb[a] = (h->nt[a] + f[a] - 1) / f[a];
+    if (rg.S[a] > K[a] || K[a] < rg.S[a] + p - 1) return false;
+  }
+  rg.order = p;
+  rg.region_stride = (long long)(rg.S[0] + p - 1) * (rg.S[1] + p - 1) * (rg.S[2] + p - 1);
+  if (sizeof(double) * (size_t)rg.region_stride > max_bytes) return false;
+  if (out) *out = rg;
+  return true;
+}
+
+// Direct write-out of the spreading blocks by the fixed rule: measured faster than copies plus
+// assembly on the 98,304-atom cell at 128^3 (and at 64^3 for 12,288 atoms).
+constexpr int kDirectRule = 1;
+// The pair sum beside the chain on the auxiliary stream by the fixed rule (co-scheduling it in
+// the chain's launches measured slower on the 98,304-atom cell: 0.226 against 0.189 ms mixed;
+// faster on the 12,288-atom water box: 0.096 against 0.097 ms with 20 % of the rows in the
+// spreading launch); the autotuner chooses per structure.
+constexpr int kCoschedRule = 0;
+constexpr int kCoSpreadShareRule = 0;
+
+// The fixed rules: the largest whole-sort-tile spreading blocks whose FP64 copy fits 60 KiB;
+// fused x-y plane transforms in mixed precision on planes that occupy a 1024-thread workgroup
+// (an FP64 plane needs up to 133 KiB of LDS, one workgroup per CU, and small planes leave
+// most of the workgroup idle); x passes of as many rows as fit, y and z passes of 16 lines;
+// radix 8 first.
+static ew_tuning rule_tuning(const ew_handle* h) {
+  ew_tuning t{};
+  const int* K = h->prm.K;
+  long long best_pts = 0;
+  for (int f3 = 1; f3 <= 4; ++f3)
+    for (int f2 = 1; f2 <= 4; ++f2)
+      for (int f1 = 1; f1 <= 2; ++f1) {
+        const int f[3] = {f1, f2, f3};
+        RegionArgs rg;
+        if (!region_shape(h, f, kRegionMaxBytes, &rg)) continue;
+        const long long pts = (long long)rg.S[0] * rg.S[1] * rg.S[2];
+        if (pts > best_pts) {
+          best_pts = pts;
+          for (int a = 0; a < 3; ++a) t.spread_block[a] = f[a];
+        }
+      }
+  t.spread_direct = best_pts > 0 && h->own_stream ? kDirectRule : 0;
+  const bool big_plane = (long long)(K[0] / 2 + 1) * K[1] >= kPlaneMinPoints;
+  t.fft_planes = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32 && big_plane)) ? 1 : 0;
+  t.fft_lines[0] = (K[1] * K[2] + 1) / 2;
+  t.fft_lines[1] = 16;
+  t.fft_lines[2] = 16;
+  t.pair_cosched = kCoschedRule;
+  t.pair_spread_share = kCoSpreadShareRule;
+  return t;
+}
+
+// Sets the region blocks, FFT plans, twiddle tables and plane flag of the current geometry to
+// the shapes t (what a shape cannot have is corrected: lines capped by the LDS, planes only
+// where a plane fits, direct write-out only on own-stream handles).  Drops captured graphs.
+static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
+  ew_tuning t = t_in;
+  const int* K = h->prm.K;
+  const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
+  RegionArgs rg{};
+  if (t.spread_block[0] > 0 || t.spread_block[1] > 0 || t.spread_block[2] > 0) {
+    if (!region_shape(h, t.spread_block, kRegionMaxLds, &rg))
+      return h->fail(EW_ERR_ARGUMENT, "spreading block does not fit the mesh or the LDS");
+    rg.direct = t.spread_direct && h->own_stream ? 1 : 0;
+  } else {
+    for (int a = 0; a < 3; ++a) t.spread_block[a] = 0;
+  }
+  t.spread_direct = rg.direct;
+  t.pair_cosched = t.pair_cosched ? 1 : 0;
+  t.pair_spread_share = std::max(0, std::min(100, t.pair_spread_share));
+  for (int a = 0; a < 3; ++a)
+    if (t.fft_radix[a] < 0 || t.fft_radix[a] >= kRadixVariants || t.fft_lines[a] < 1)
+      return h->fail(EW_ERR_ARGUMENT, "FFT radix order must be 0..3 and lines >= 1");
+  // nothing queued may still use the old shapes, copies or tables
+  h->quiesce();
+  h->drop_graphs();
+  if (h->region.scratch) h->free_buf(h->region.scratch);
+  h->region = rg;
+  if (rg.region_stride > 0 && !rg.direct) {
+    const size_t nblk = (size_t)rg.nb[0] * rg.nb[1] * rg.nb[2];
+    unsigned char* scr;
+    CHK(h->alloc(&scr, nblk * (size_t)rg.region_stride * rs));
+    h->region.scratch = scr;
+  }
+  if (sizeof(double) * (size_t)rg.region_stride > 65536) HIPCHK(h, prepare_region_kernels());
+  h->px = make_plan(K[0], t.fft_lines[0], t.fft_radix[0]);
+  h->py = make_plan(K[1], t.fft_lines[1], t.fft_radix[1]);
+  h->pz = make_plan(K[2], t.fft_lines[2], t.fft_radix[2]);
+  const FftPlan* pl[3] = {&h->px, &h->py, &h->pz};
+  for (int a = 0; a < 3; ++a) {
+    if (pl[a]->npass <= 0) return h->fail(EW_ERR_ARGUMENT, "grid not 2^a 3^b 5^c");
+    t.fft_lines[a] = pl[a]->lines;
+    if (h->fp32) {
+      std::vector<float>& tv = h->tab_tw32[a];
+      twiddles(K[a], tv, t.fft_radix[a]);
+      HIPCHK(h, hipMemcpyAsync(h->tw[a], tv.data(), sizeof(float) * tv.size(),
+                               hipMemcpyHostToDevice, h->stream));
+    } else {
+      std::vector<double>& tv = h->tab_tw64[a];
+      twiddles(K[a], tv, t.fft_radix[a]);
+      HIPCHK(h, hipMemcpyAsync(h->tw[a], tv.data(), sizeof(double) * tv.size(),
+                               hipMemcpyHostToDevice, h->stream));
+    }
+  }
+  h->plane = t.fft_planes && h->opt.fft_mode != 1 && plane_fits(h->px, h->py, 2 * rs);
+  if (h->plane) HIPCHK(h, prepare_plane_kernels());
+  t.fft_planes = h->plane ? 1 : 0;
+  h->tuning = t;
+  return EW_OK;
+}
+
+// Key of the tuning cache: device, mesh, order, precision, size class (log2 of the atoms),
+// pair storage and the handle's FFT mode option.
+static std::string tune_key(const ew_handle* h) {
+  hipDeviceProp_t prop;
+  std::string arch = "gpu";
+  if (hipGetDeviceProperties(&prop, h->opt.device) == hipSuccess) arch = prop.gcnArchName;
+  for (char& ch : arch)
+    if (ch == ' ' || ch == ':') ch = '_';
+  int nb = 0;
+  while ((1 << (nb + 1)) <= h->n) ++nb;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "v2:%s:%dx%dx%d:p%d:%s:n%d:ps%d:fm%d:own%d", arch.c_str(),
+                h->prm.K[0], h->prm.K[1], h->prm.K[2], h->prm.order, h->fp32 ? "mixed" : "fp64",
+                nb, h->opt.pair_storage, h->opt.fft_mode, h->own_stream ? 1 : 0);
+  return buf;
+}
+
+static std::mutex g_tune_mutex;
+static bool g_tune_loaded = false;
+static std::vector<std::pair<std::string, ew_tuning>> g_tune_entries;
+
+// $EW_TUNE_CACHE, else ewald_tune.cache in the directory that holds this library.
+static std::string tune_cache_path() {
+  const char* env = std::getenv("EW_TUNE_CACHE");
+  if (env && *env) return env;
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&tune_cache_path), &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    const size_t slash = p.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) +
+           "/ewald_tune.cache";
+  }
+  return "ewald_tune.cache";
+}
+
+// One line per key: key direct f0 f1 f2 planes l0 l1 l2 r0 r1 r2 cosched share eval_us.
+static bool parse_tuning(const char* line, std::string* key, ew_tuning* t) {
+  char k[200];
+  ew_tuning u{};
+  const int got = std::sscanf(line, "%199s %d %d %d %d %d %d %d %d %d %d %d %d %d %f", k,
+                              &u.spread_direct, &u.spread_block[0], &u.spread_block[1],
+                              &u.spread_block[2], &u.fft_planes, &u.fft_lines[0], &u.fft_lines[1],
+                              &u.fft_lines[2], &u.fft_radix[0], &u.fft_radix[1], &u.fft_radix[2],
+                              &u.pair_cosched, &u.pair_spread_share, &u.eval_us);
+  if (got != 15) return false;
+  *key = k;
+  *t = u;
+  return true;
+}
+
+static void tune_cache_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  FILE* f = std::fopen(tune_cache_path().c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (std::fgets(line, sizeof(line), f)) {
+    std::string k;
+    ew_tuning t;
+    if (parse_tuning(line, &k, &t)) g_tune_entries.emplace_back(k, t);  // later lines win
+  }
+  std::fclose(f);
+}
+
+static bool tune_cache_get(const std::string& key, ew_tuning* out) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  tune_cache_load_locked();
+  for (auto it = g_tune_entries.rbegin(); it != g_tune_entries.rend(); ++it)
+    if (it->first == key) {
+      *out = it->second;
+      return true;
+    }
+  return false;
+}
+
+static void tune_cache_put(const std::string& key, const ew_tuning& t) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  tune_cache_load_locked();
+  g_tune_entries.emplace_back(key, t);
+  FILE* f = std::fopen(tune_cache_path().c_str(), "a");
+  if (!f) return;  // a read-only location: the choice still holds for this process
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %.3f\n", key.c_str(),
+               t.spread_direct, t.spread_block[0], t.spread_block[1], t.spread_block[2],
+               t.fft_planes, t.fft_lines[0], t.fft_lines[1], t.fft_lines[2], t.fft_radix[0],
+               t.fft_radix[1], t.fft_radix[2], t.pair_cosched, t.pair_spread_share, t.eval_us);
+  std::fclose(f);
+}
+
+// Device time of one evaluation in microseconds with the shapes t applied: launched as an
+// ordinary evaluation on the handle's stream, on synthetic neutral charges; the smaller mean of
+// two runs of about a millisecond each, timed by events.
+static int time_tuning(ew_handle* h, const ew_tuning& t, const double* d_dq, double* d_out,
+                       double* d_en, hipEvent_t e0, hipEvent_t e1, float* us) {
+  CHK(apply_tuning(h, t));
+  hipStream_t s = h->stream;
+  for (int i = 0; i < 2; ++i) CHK(enqueue_eval(h, d_dq, d_out, d_en, s));
+  HIPCHK(h, hipEventRecord(e0, s));
+  CHK(enqueue_eval(h, d_dq, d_out, d_en, s));
+  HIPCHK(h, hipEventRecord(e1, s));
+  HIPCHK(h, hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+  const int reps = std::max(3, std::min(64, int(1.0f / std::max(ms, 1e-3f))));
+  float best = 1e30f;
+  for (int run = 0; run < 2; ++run) {
+    HIPCHK(h, hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) CHK(enqueue_eval(h, d_dq, d_out, d_en, s));
+    HIPCHK(h, hipEventRecord(e1, s));
+    HIPCHK(h, hipEventSynchronize(e1));
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    best = std::min(best, 1e3f * ms / reps);
+  }
+  *us = best;
+  return EW_OK;
+}
+
+// The autotuner: coordinate search over the spreading (write-out and block shape), the FFT
+// pass structure (fused planes, lines per workgroup) and the radix order of each axis, each
+// candidate timed on the device with this structure's data; the fastest shapes are applied and
+// stored in the tuning cache.
+static int autotune(ew_handle* h) {
+  const int n = h->n;
+  std::vector<double> q(n);
+  double mean = 0.0;
+  for (int i = 0; i < n; ++i) mean += (q[i] = std::sin(0.7 * i + 0.3));
+  mean /= n;
+  for (double& x : q) x -= mean;
+  double *d_dq = nullptr, *d_out = nullptr, *d_en = nullptr;
+  CHK(h->alloc(&d_dq, n));
+  CHK(h->alloc(&d_out, n));
+  CHK(h->alloc(&d_en, 1));
+  HIPCHK(h, hipMemcpyAsync(d_dq, q.data(), sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev_guard{e0, e1};
+  ew_tuning best = h->tuning;
+  float best_us = 0.0f;
+  CHK(time_tuning(h, best, d_dq, d_out, d_en, e0, e1, &best_us));
+  auto consider = [&](const ew_tuning& c) -> int {
+    float us = 0.0f;
+    const int st = time_tuning(h, c, d_dq, d_out, d_en, e0, e1, &us);
+    if (st == EW_ERR_ARGUMENT) return EW_OK;  // a shape this mesh cannot take
+    if (st) return st;
+    if (us < best_us) {
+      best_us = us;
+      best = h->tuning;
+    }
+    return EW_OK;
+  };
+  // the pair sum beside the chain (auxiliary stream) or co-scheduled in its launches, and the
+  // share of its rows in the spreading launch: tried first and again after the shapes
+  auto tune_pairs = [&]() -> int {
+    {
+      const ew_tuning base = best;
+      for (int co = 0; co <= 1; ++co)
+        for (int share : {0, 20, 35, 50}) {
+          if (!co && share) continue;
+          if (co == base.pair_cosched && share == base.pair_spread_share) continue;
+          ew_tuning c = base;
+          c.pair_cosched = co;
+          c.pair_spread_share = share;
+          CHK(consider(c));
+        }
+    }
+    return EW_OK;
+  };
+  CHK(tune_pairs());
+  // 1. spreading: write-out and block shape
+  {
+    const ew_tuning base = best;
+    for (int direct = 1; direct >= 0; --direct)
+      for (int f3 = 1; f3 <= 4; ++f3)
+        for (int f2 = 1; f2 <= 4; ++f2)
+          for (int f1 = 1; f1 <= 2; ++f1) {
+            const int f[3] = {f1, f2, f3};
+            if (!region_shape(h, f, kRegionMaxLds, nullptr)) continue;
+            if (direct && !h->own_stream) continue;
+            ew_tuning c = base;
+            c.spread_direct = direct;
+            for (int a = 0; a < 3; ++a) c.spread_block[a] = f[a];
+            CHK(consider(c));
+          }
+  }
+  // 2. FFT pass structure: fused planes or separate passes, lines per workgroup of y and z
+  {
+    const ew_tuning base = best;
+    const int ly_max = std::max(1, kFftMaxElems / h->prm.K[1]);
+    const int lz_max = std::max(1, kFftMaxElems / h->prm.K[2]);
+    for (int planes = 0; planes <= 1; ++planes) {
+      if (h->opt.fft_mode == 1 && planes) continue;
+      if (h->opt.fft_mode == 2 && !planes) continue;
+      for (int ly = 2; ly <= ly_max; ly *= 2) {
+        if (planes && ly > 2) break;  // the y lines are unused with fused planes
+        for (int lz = 2; lz <= lz_max; lz *= 2) {
+          ew_tuning c = base;
+          c.fft_planes = planes;
+          c.fft_lines[1] = ly;
+          c.fft_lines[2] = lz;
+          CHK(consider(c));
+        }
+      }
+    }
+  }
+  // 3. radix order per axis (orders that give the same passes are timed once)
+  for (int a = 0; a < 3; ++a) {
+    const ew_tuning base = best;
+    std::vector<std::vector<int>> seen;
+    for (int v = 0; v < kRadixVariants; ++v) {
+      int rad[kMaxRadixPasses];
+      const int np = fft_plan(h->prm.K[a], rad, v);
+      std::vector<int> key(rad, rad + std::max(np, 0));
+      bool dup = false;
+      for (auto& k : seen) dup = dup || k == key;
+      seen.push_back(key);
+      if (dup || v == base.fft_radix[a]) continue;
+      ew_tuning c = base;
+      c.fft_radix[a] = v;
+      CHK(consider(c));
+    }
+  }
+  CHK(tune_pairs());
+  best.eval_us = best_us;
+  CHK(apply_tuning(h, best));
+  h->tuning.eval_us = best_us;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->free_buf(d_dq);
+  h->free_buf(d_out);
+  h->free_buf(d_en);
+  tune_cache_put(tune_key(h), h->tuning);
+  return EW_OK;
+}
+
+extern "C" int ew_get_tuning(const ew_handle* h, ew_tuning* out) {
+  if (!h || !out) return set_global_error(EW_ERR_ARGUMENT, "ew_get_tuning: NULL argument");
+  if (!h->have_geom) return set_global_error(EW_ERR_NO_GEOMETRY, "no geometry set");
+  *out = h->tuning;
+  return EW_OK;
+}
+
+
 extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const double* latvecs,
                                const int* species, const double* hubbard_u, int n_species,
                                const double* charges) {
@@ -533,44 +914,8 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
 
   // device buffers
   const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
-  // privatised spreading blocks: the largest whole-sort-tile blocks whose copy fits the LDS
-  // budget, on meshes where a block copy covers no point twice (K >= S + p - 1)
+  // privatised spreading blocks and FFT plans are set by apply_tuning below
   h->region = RegionArgs{};
-  {
-    int best[3] = {0, 0, 0};
-    long long best_pts = 0;
-    for (int f3 = 1; f3 <= 4; ++f3)
-      for (int f2 = 1; f2 <= 4; ++f2)
-        for (int f1 = 1; f1 <= 2; ++f1) {
-          const int S[3] = {f1 * h->T[0], f2 * h->T[1], f3 * h->T[2]};
-          bool ok = true;
-          for (int a = 0; a < 3; ++a)
-            if (S[a] > K[a] || K[a] < S[a] + p - 1) ok = false;
-          // the copy accumulates in FP64 LDS in both precision modes
-          const size_t bytes = sizeof(double) * (size_t)(S[0] + p - 1) * (S[1] + p - 1) *
-                               (S[2] + p - 1);
-          const long long pts = (long long)S[0] * S[1] * S[2];
-          if (ok && bytes <= kRegionMaxBytes && pts > best_pts) {
-            best_pts = pts;
-            best[0] = f1;
-            best[1] = f2;
-            best[2] = f3;
-          }
-        }
-    if (best_pts > 0) {
-      RegionArgs& rg = h->region;
-      for (int a = 0; a < 3; ++a) {
-        rg.K[a] = K[a];
-        rg.f[a] = best[a];
-        rg.S[a] = best[a] * h->T[a];
-        rg.nt[a] = h->nt[a];
-        rg.nb[a] = (h->nt[a] + best[a] - 1) / best[a];
-      }
-      rg.order = p;
-      rg.region_stride =
-          (long long)(rg.S[0] + p - 1) * (rg.S[1] + p - 1) * (rg.S[2] + p - 1);
-    }
-  }
   const int Kh = K[0] / 2 + 1;
   const size_t ngrid = (size_t)K[0] * K[1] * K[2], ncgrid = (size_t)Kh * K[1] * K[2];
   CHK(h->alloc(&h->perm, n));
@@ -595,12 +940,6 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   CHK(h->alloc(&h->dq_io, n));
   CHK(h->alloc(&h->shift_io, n));
   CHK(h->alloc(&h->err_flag, 2));  // [0] coincident charges, [1] a partner offset beyond 16 bits
-  if (h->region.region_stride > 0) {
-    const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
-    unsigned char* scr;
-    CHK(h->alloc(&scr, nblk * (size_t)h->region.region_stride * rs));
-    h->region.scratch = scr;
-  }
   {
     unsigned char *g, *cgp;
     CHK(h->alloc(&g, ngrid * rs));
@@ -688,16 +1027,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     HIPCHK(h, hipMemcpyAsync(h->bsq[a], bq.data(), sizeof(double) * K[a], hipMemcpyHostToDevice, s));
     unsigned char* t;
     CHK(h->alloc(&t, 2 * (size_t)K[a] * rs));
-    h->tw[a] = t;
-    if (h->fp32) {
-      std::vector<float>& tv = h->tab_tw32[a];
-      twiddles(K[a], tv);
-      HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice, s));
-    } else {
-      std::vector<double>& tv = h->tab_tw64[a];
-      twiddles(K[a], tv);
-      HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice, s));
-    }
+    h->tw[a] = t;  // filled by apply_tuning (the table follows the radix order)
   }
   for (int a = 0; a < 3; ++a)
     if (fft_plan(K[a], h->px.radix) < 0) return h->fail(EW_ERR_ARGUMENT, "grid not 2^a 3^b 5^c");
@@ -708,16 +1038,22 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
       return h->fail(EW_ERR_ARGUMENT, "mesh dimension " + std::to_string(K[a]) +
                                           " exceeds the single-workgroup FFT line limit");
   }
-  h->px = make_plan(K[0], (K[1] * K[2] + 1) / 2);
-  h->py = make_plan(K[1], 16);
-  h->pz = make_plan(K[2], 16);
-  // fused plane transforms: by default in mixed precision on planes that occupy a
-  // 1024-thread workgroup (an FP64 plane needs up to 133 KiB of LDS, one workgroup per CU, and
-  // small planes leave most of the workgroup idle; the separate x and y passes are faster there)
-  const bool big_plane = (long long)(K[0] / 2 + 1) * K[1] >= kPlaneMinPoints;
-  h->plane = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32 && big_plane)) &&
-             plane_fits(h->px, h->py, 2 * rs);
-  if (h->plane) HIPCHK(h, prepare_plane_kernels());
+  // launch shapes: fixed rules, the tuning cache, or the caller's (the autotuner itself runs
+  // once the geometry is complete, below)
+  bool tune_now = false;
+  {
+    ew_tuning t = rule_tuning(h);
+    if (h->opt.tune == 3) {
+      t = h->opt.tuning;
+    } else if (h->opt.tune == 1 || h->opt.tune == 2) {
+      ew_tuning c;
+      if (tune_cache_get(tune_key(h), &c))
+        t = c;
+      else
+        tune_now = h->opt.tune == 1;
+    }
+    CHK(apply_tuning(h, t));
+  }
 
   // B-spline weights
   BsplineArgs ba{{K[0], K[1], K[2]}, p};
@@ -848,6 +1184,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     }
   }
   h->have_geom = true;
+  if (tune_now) CHK(autotune(h));
   return EW_OK;
 }
 
@@ -950,7 +1287,7 @@ static hipError_t launch_gather(const ew_handle* h, const GatherArgs& ga, const 
 // The root fork (see enqueue_eval): privatised spreading on an own-stream handle.  Phase
 // profiling (opt.profile = 1) runs every phase alone; tracing (2) keeps the launch shape.
 static bool root_fork(const ew_handle* h) {
-  return h->opt.profile != 1 && h->aux && h->region.scratch != nullptr;
+  return h->opt.profile != 1 && h->aux && h->region.region_stride > 0;
 }
 
 static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, double* d_energy,
@@ -964,8 +1301,10 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   // and the spreading reads the caller's charges through the sort permutation, so the chain
   // on stream s waits for nothing until the gather.  Otherwise the pair sum forks after the
   // charge gather.
-  const bool fork = !prof && h->aux;
-  const bool root = root_fork(h);
+  // co-scheduled pair sum: on extra workgroups of the chain's launches, one stream
+  const bool co = h->tuning.pair_cosched && !h->half && !prof;
+  const bool fork = !prof && h->aux && !co;
+  const bool root = root_fork(h) && !co;
   const void* col = h->col16 ? static_cast<const void*>(h->col16) : h->col;
   const int ntiles = h->nt[0] * h->nt[1] * h->nt[2];
   auto pairs = [&](hipStream_t st) {
@@ -1007,6 +1346,21 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     L.q = d_dq;
     L.qperm = h->perm;
   }
+  if (co) {
+    // the pair rows run inside the chain launches (their span is empty in the trace)
+    h->record_trace(ew_handle::kTrPair0, s);
+    h->record_trace(ew_handle::kTrPair1, s);
+    L.pair.n = n;
+    L.pair.r0 = 0;
+    L.pair.r1 = n;
+    L.pair.col16 = h->col16 != nullptr;
+    L.pair.row_ptr = h->row_ptr;
+    L.pair.col = col;
+    L.pair.coef = h->coef;
+    L.pair.q = h->qs;
+    L.pair.out = h->phi_real;
+    L.pair_spread_share = h->tuning.pair_spread_share;
+  }
   h->record_trace(ew_handle::kTrChain0, s);
   hipEvent_t* marks = prof ? &h->ev[2] : (trace ? &h->tev[ew_handle::kTrSpread] : nullptr);
   hipEvent_t region_mark = trace ? h->tev[ew_handle::kTrRegion] : nullptr;
@@ -1016,7 +1370,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     HIPCHK(h, launch_spme<double>(L, s, marks, region_mark));
   if (fork)
     HIPCHK(h, hipStreamWaitEvent(s, h->join_ev, 0));
-  else
+  else if (!co)
     HIPCHK(h, pairs(s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
   h->record_trace(ew_handle::kTrJoined, s);
@@ -1163,7 +1517,8 @@ static NoallocLayout noalloc_layout(const ew_handle* h) {
   L.grid = off;   off += up(K0 * K1 * K2 * rs);
   L.cgrid = off;  off += up((K0 / 2 + 1) * K1 * K2 * 2 * rs);
   L.region = off;
-  if (h->region.scratch) off += up(nblk * (size_t)h->region.region_stride * rs);
+  if (h->region.region_stride > 0 && !h->region.direct)
+    off += up(nblk * (size_t)h->region.region_stride * rs);
   L.total = off;
   return L;
 }
@@ -1208,7 +1563,7 @@ extern "C" int ew_shift_device_noalloc(ew_handle* h, const double* d_dq, double*
   if (h->qsum) h->qsum = reinterpret_cast<double*>(b + L.qsum);
   h->grid = b + L.grid;
   h->cgrid = b + L.cgrid;
-  if (h->region.scratch) h->region.scratch = b + L.region;
+  if (h->region.region_stride > 0 && !h->region.direct) h->region.scratch = b + L.region;
   double* energy = d_energy ? d_energy : reinterpret_cast<double*>(b + L.energy);
   HIPCHK(h, h->after_last(s));
   CHK(enqueue_eval(h, d_dq, d_shift, energy, s));
@@ -1271,7 +1626,8 @@ static int ensure_views(ew_handle* h, int nvec) {
   const size_t nparts = std::max<size_t>((size_t)(n + 255) / 256,
                                          (size_t)h->nt[0] * h->nt[1] * h->nt[2]);
   const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
-  const size_t rstride = h->region.scratch ? nblk * (size_t)h->region.region_stride * rs : 0;
+  const size_t rstride =
+      h->region.region_stride > 0 ? nblk * (size_t)h->region.region_stride * rs : 0;
   const size_t cap = (size_t)nvec;
   unsigned char *g, *cg, *rg = nullptr;
   CHK(h->alloc(&V.qs, cap * n));
@@ -1954,7 +2310,7 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->n = h->n;
   for (int a = 0; a < 3; ++a) d->K[a] = K[a];
   d->ntiles = h->nt[0] * h->nt[1] * h->nt[2];
-  d->spread_mode = h->region.scratch ? 0 : (h->spread_slots > 0 ? 1 : 2);
+  d->spread_mode = h->region.region_stride > 0 ? 0 : (h->spread_slots > 0 ? 1 : 2);
   d->gather_tile =
       gather_tile_smem(h->T, p, h->fp32 ? sizeof(float) : sizeof(double)) <= kGatherRegionMaxBytes;
   d->nparts = d->gather_tile ? d->ntiles : (h->n + 255) / 256;
@@ -1989,6 +2345,7 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->spread.slots = h->spread_slots;
   d->spread.src = h->src_tab;
   d->region = h->region;
+  d->region.direct = 0;  // the grouped kernels assemble block copies
   d->ga.order = p;
   d->ga.self_coef = h->self_coef;
   d->ga.bg = h->background;

@@ -52,6 +52,7 @@ struct ew_handle {
   ewamd::RegionArgs region{};   // privatised spreading (scratch == null: not used)
   ewamd::FftPlan px{}, py{}, pz{};
   bool plane = false;        // fused x-y plane transforms
+  ew_tuning tuning{};        // launch shapes in use (what px/py/pz, plane and region hold)
   double self_coef = 0.0;
   double background = 0.0;   // -pi / (V alpha^2), applied when opt.charged
   double* qsum = nullptr;    // total charge of the last evaluation (opt.charged)

@@ -247,6 +247,124 @@ __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R 
 // FFT kernels
 // ----------------------------------------------------------------------------------------
 
+// ----------------------------------------------------------------------------------------
+// real-space pair sum rows (shared by the pair-sum kernel and the chain kernels that co-run it)
+// ----------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Partner index of a stored pair: absolute (int) or a signed 16-bit offset from the row
+// (cyclic in n), which halves the index stream when rows are in spatial order.
+__device__ __forceinline__ int partner(int row, int c, int) { return c; }
+__device__ __forceinline__ int partner(int row, short c, int n) {
+  int j = row + int(c);
+  j += j < 0 ? n : 0;
+  j -= j >= n ? n : 0;
+  return j;
+}
+
+// Real-space pair sum out[i] = sum_j coef_ij q_j over rows r0 + wave, r0 + wave + nw, ...
+// below r1: one 64-lane wave per row.  A sweep covers 512 entries: lane l takes entries
+// l + 64u (u < 8), so that every load instruction of the wave reads consecutive entries and
+// consecutive partners, and all eight loads of a lane are issued before the first is used.
+// While a wave gathers one row's partner charges it already loads the next row's bounds,
+// offsets and coefficients, so a row costs one memory latency instead of three.
+template <typename CT, typename IT>
+__device__ __forceinline__ void pair_stream_rows(int n, int r0, int r1, int wave, int nw,
+                                                 const long long* __restrict__ row_ptr,
+                                                 const IT* __restrict__ col,
+                                                 const CT* __restrict__ coef,
+                                                 const double* __restrict__ q,
+                                                 double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  int row = __builtin_amdgcn_readfirstlane(r0 + wave);
+  if (row >= r1) return;  // uniform across the wave
+  long long a = row_ptr[row], b = row_ptr[row + 1];
+  IT c[8];
+  CT w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const long long k = a + lane + 64 * u;
+    c[u] = k < b ? col[k] : IT(0);
+    w[u] = k < b ? coef[k] : CT(0);
+  }
+  while (true) {
+    const int nrow = row + nw;
+    const bool more = nrow < r1;
+    // this row's partners' charges
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = a + lane + 64 * u < b ? q[partner(row, c[u], n)] : 0.0;
+    // the next row's bounds and first sweep, in flight during this row's sums
+    long long na = 0, nb = 0;
+    IT nc[8];
+    CT nwt[8];
+    if (more) {
+      na = row_ptr[nrow];
+      nb = row_ptr[nrow + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long k = na + lane + 64 * u;
+      nc[u] = k < nb ? col[k] : IT(0);
+      nwt[u] = k < nb ? coef[k] : CT(0);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += double(w[u]) * x[u];
+    // rows longer than one sweep (dense image sums, long short-range lists)
+    for (long long g = a + lane + 512; g < b; g += 512) {
+      IT cc[8];
+      CT ww[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long long k = g + 64 * u;
+        cc[u] = k < b ? col[k] : IT(0);
+        ww[u] = k < b ? coef[k] : CT(0);
+      }
+      double qv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) qv[u] = g + 64 * u < b ? q[partner(row, cc[u], n)] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += double(ww[u]) * qv[u];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[row] = acc;
+    if (!more) break;
+    row = nrow;
+    a = na;
+    b = nb;
+    for (int u = 0; u < 8; ++u) {
+      c[u] = nc[u];
+      w[u] = nwt[u];
+    }
+  }
+}
+
+
+// Blocks past co.first of a chain kernel launch run rows [co.r0, co.r1) of the pair sum
+// (returns true for them; the chain blocks return false and carry on).
+template <typename CT>
+__device__ __forceinline__ bool co_pairs(const PairCo& co) {
+  if ((int)blockIdx.x < co.first) return false;
+  const int wpb = blockDim.x >> 6;
+  const int wave = ((int)blockIdx.x - co.first) * wpb + (int)(threadIdx.x >> 6);
+  const int nw = co.nblk * wpb;
+  if (co.col16)
+    pair_stream_rows<CT, short>(co.n, co.r0, co.r1, wave, nw, co.row_ptr,
+                                static_cast<const short*>(co.col),
+                                static_cast<const CT*>(co.coef), co.q, co.out);
+  else
+    pair_stream_rows<CT, int>(co.n, co.r0, co.r1, wave, nw, co.row_ptr,
+                              static_cast<const int*>(co.col), static_cast<const CT*>(co.coef),
+                              co.q, co.out);
+  return true;
+}
+
 // Rows of K1 reals (row = z*K2 + y); workgroup takes pl.lines complex lines = 2*lines rows.
 template <typename R>
 __device__ __forceinline__ void k_fft_x_r2c_body(const R* __restrict__ grid,
@@ -296,7 +414,9 @@ template <typename R>
 __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ grid,
                                                          cx<R>* __restrict__ cg, int nrows,
                                                          int Kh, FftPlan pl,
-                                                         const cx<R>* __restrict__ tw) {
+                                                         const cx<R>* __restrict__ tw,
+                                                         PairCo co) {
+  if (co_pairs<R>(co)) return;
   k_fft_x_r2c_body<R>(grid, cg, nrows, Kh, pl, tw, blockIdx.x);
 }
 
@@ -354,7 +474,9 @@ template <typename R>
 __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict__ cg,
                                                          R* __restrict__ grid, int nrows, int Kh,
                                                          FftPlan pl,
-                                                         const cx<R>* __restrict__ tw) {
+                                                         const cx<R>* __restrict__ tw,
+                                                         PairCo co) {
+  if (co_pairs<R>(co)) return;
   k_fft_x_c2r_body<R>(cg, grid, nrows, Kh, pl, tw, blockIdx.x);
 }
 
@@ -441,7 +563,8 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg,
                                                          long long es,
                                                          FftPlan pl,
                                                          const cx<R>* __restrict__ tw,
-                                                         InfluenceArgs inf) {
+                                                         InfluenceArgs inf, PairCo co) {
+  if (co_pairs<R>(co)) return;
   k_fft_lines_body<R, MODE>(cg, nlines, A, ad, B, es, pl, tw, inf, blockIdx.x);
 }
 
@@ -598,7 +721,9 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_r2c(const R* __restrict_
                                                             cx<R>* __restrict__ cg, FftPlan px,
                                                             FftPlan py,
                                                             const cx<R>* __restrict__ twx,
-                                                            const cx<R>* __restrict__ twy) {
+                                                            const cx<R>* __restrict__ twy,
+                                                            PairCo co) {
+  if (co_pairs<R>(co)) return;
   k_fft_xy_r2c_body<R>(grid, cg, px, py, twx, twy, blockIdx.x);
 }
 
@@ -607,7 +732,9 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_c2r(const cx<R>* __restr
                                                             R* __restrict__ grid, FftPlan px,
                                                             FftPlan py,
                                                             const cx<R>* __restrict__ twx,
-                                                            const cx<R>* __restrict__ twy) {
+                                                            const cx<R>* __restrict__ twy,
+                                                            PairCo co) {
+  if (co_pairs<R>(co)) return;
   k_fft_xy_c2r_body<R>(cg, grid, px, py, twx, twy, blockIdx.x);
 }
 
@@ -875,7 +1002,8 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
                                                        const double* __restrict__ q,
                                                        const int* __restrict__ tile_start,
                                                        RegionArgs g, R* __restrict__ regions, int bid,
-                                                       const int* __restrict__ qperm = nullptr) {
+                                                       const int* __restrict__ qperm = nullptr,
+                                                       R* __restrict__ grid = nullptr) {
   // the private copy accumulates in FP64 in both precision modes (LDS FP64 adds are native
   // and fast; the FP32 ones are not), and is stored in the mesh precision R
   extern __shared__ unsigned char smem_raw[];
@@ -932,6 +1060,22 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
     }
   }
   __syncthreads();
+  if (g.direct) {
+    // the copy (block + halo, wrapped onto the periodic mesh) is added into the mesh, which was
+    // cleared before the launch; untouched halo points are skipped
+    for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
+      const double v = reg[i];
+      if (v == 0.0) continue;
+      const int ix = i % RX, t = i / RX;
+      const int iy = t % RY, iz = t / RY;
+      int gx = lo1 + ix, gy = lo2 + iy, gz = lo3 + iz;
+      while (gx >= g.K[0]) gx -= g.K[0];
+      while (gy >= g.K[1]) gy -= g.K[1];
+      while (gz >= g.K[2]) gz -= g.K[2];
+      unsafeAtomicAdd(grid + ((size_t)gz * g.K[1] + gy) * g.K[0] + gx, R(v));
+    }
+    return;
+  }
   R* out = regions + (size_t)bid * g.region_stride;
   for (int i = threadIdx.x; i < nreg; i += blockDim.x) out[i] = R(reg[i]);
 }
@@ -942,8 +1086,26 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
                                                        const double* __restrict__ q,
                                                        const int* __restrict__ tile_start,
                                                        RegionArgs g, R* __restrict__ regions,
-                                                       const int* __restrict__ qperm) {
-  k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x, qperm);
+                                                       const int* __restrict__ qperm,
+                                                       R* __restrict__ grid, PairCo co) {
+  if (co_pairs<R>(co)) return;
+  k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x, qperm, grid);
+}
+
+// Clears n elements (the mesh before a direct spreading): four per thread, grid-stride.
+template <typename R>
+__global__ __launch_bounds__(256) void k_clear(R* __restrict__ p, long long n) {
+  const long long stride = 4LL * blockDim.x * gridDim.x;
+  for (long long i = 4LL * (blockIdx.x * (long long)blockDim.x + threadIdx.x); i < n; i += stride) {
+    if (i + 3 < n) {
+      p[i] = R(0);
+      p[i + 1] = R(0);
+      p[i + 2] = R(0);
+      p[i + 3] = R(0);
+    } else {
+      for (long long j = i; j < n; ++j) p[j] = R(0);
+    }
+  }
 }
 
 // Assembly of the privatised copies, one workgroup per block of the mesh: a point at offset l
@@ -1064,21 +1226,6 @@ __global__ void k_gather_charges(int n, const int* __restrict__ perm,
   k_gather_charges_body(n, perm, dq, qs, zero, blockIdx.x);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Partner index of a stored pair: absolute (int) or a signed 16-bit offset from the row
-// (cyclic in n), which halves the index stream when rows are in spatial order.
-__device__ __forceinline__ int partner(int row, int c, int) { return c; }
-__device__ __forceinline__ int partner(int row, short c, int n) {
-  int j = row + int(c);
-  j += j < 0 ? n : 0;
-  j -= j >= n ? n : 0;
-  return j;
-}
 
 // Real-space pair sum: out[i] = sum_j coef_ij q_j, one 64-lane wave per atom (row).  A sweep
 // covers 512 entries: lane l takes entries l + 64u (u < 8), so that every load instruction of
@@ -1128,70 +1275,9 @@ __global__ __launch_bounds__(256) void k_pair_spmv_stream(int n,
                                                           const CT* __restrict__ coef,
                                                           const double* __restrict__ q,
                                                           double* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * (blockDim.x >> 6);
-  int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  if (row >= n) return;  // uniform across the wave
-  long long a = row_ptr[row], b = row_ptr[row + 1];
-  IT c[8];
-  CT w[8];
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const long long k = a + lane + 64 * u;
-    c[u] = k < b ? col[k] : IT(0);
-    w[u] = k < b ? coef[k] : CT(0);
-  }
-  while (true) {
-    const int nrow = row + nw;
-    const bool more = nrow < n;
-    // this row's partners' charges
-    double x[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = a + lane + 64 * u < b ? q[partner(row, c[u], n)] : 0.0;
-    // the next row's bounds and first sweep, in flight during this row's sums
-    long long na = 0, nb = 0;
-    IT nc[8];
-    CT nwt[8];
-    if (more) {
-      na = row_ptr[nrow];
-      nb = row_ptr[nrow + 1];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const long long k = na + lane + 64 * u;
-      nc[u] = k < nb ? col[k] : IT(0);
-      nwt[u] = k < nb ? coef[k] : CT(0);
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc += double(w[u]) * x[u];
-    // rows longer than one sweep (dense image sums, long short-range lists)
-    for (long long g = a + lane + 512; g < b; g += 512) {
-      IT cc[8];
-      CT ww[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const long long k = g + 64 * u;
-        cc[u] = k < b ? col[k] : IT(0);
-        ww[u] = k < b ? coef[k] : CT(0);
-      }
-      double qv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) qv[u] = g + 64 * u < b ? q[partner(row, cc[u], n)] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += double(ww[u]) * qv[u];
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) out[row] = acc;
-    if (!more) break;
-    row = nrow;
-    a = na;
-    b = nb;
-    for (int u = 0; u < 8; ++u) {
-      c[u] = nc[u];
-      w[u] = nwt[u];
-    }
-  }
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  pair_stream_rows<CT, IT>(n, 0, n, wave, nw, row_ptr, col, coef, q, out);
 }
 
 template <typename CT, typename IT>
@@ -2767,7 +2853,38 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
-  if (ev_region && !L.region.scratch) (void)hipEventRecord(ev_region, st);
+  if (ev_region && L.region.region_stride <= 0) (void)hipEventRecord(ev_region, st);
+  // co-scheduled pair sum: rows split over the spreading launch (a share in percent) and the
+  // FFT launches of the full transform (the rest, equally), each launch running its rows on
+  // pair blocks appended after its own blocks; waves of pair blocks per launch: kCoWaves
+  const bool co_on = L.pair.r1 > 0 && !L.forward_only;
+  const int nfft = L.plane ? 3 : 5;
+  const bool co_spread = co_on && L.region.region_stride > 0 && L.pair_spread_share > 0;
+  int co_next = 0;
+  int co_left = nfft + (co_spread ? 1 : 0);
+  auto co_for = [&](int chain_blocks, int threads, bool spread_launch, dim3* grid_out) {
+    PairCo c = L.pair;
+    c.first = 0x7fffffff;
+    *grid_out = dim3(chain_blocks);
+    if (!co_on || (spread_launch && !co_spread)) return c;
+    const int n = L.pair.r1;
+    int take;
+    if (spread_launch) {
+      take = (int)((long long)n * L.pair_spread_share / 100);
+    } else {
+      take = (n - co_next) / co_left;
+    }
+    --co_left;
+    if (co_left == 0) take = n - co_next;
+    c.r0 = co_next;
+    c.r1 = co_next + take;
+    co_next = c.r1;
+    c.first = chain_blocks;
+    c.nblk = std::max(1, kCoWaves / (threads / 64));
+    if (c.r1 > c.r0) *grid_out = dim3(chain_blocks + c.nblk);
+    else c.first = 0x7fffffff;
+    return c;
+  };
   const int K1 = L.K[0], K2 = L.K[1], K3 = L.K[2], Kh = K1 / 2 + 1;
   R* grid = static_cast<R*>(L.grid);
   cx<R>* cg = static_cast<cx<R>*>(L.cgrid);
@@ -2775,16 +2892,23 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
   const cx<R>* tw2 = static_cast<const cx<R>*>(L.tw[1]);
   const cx<R>* tw3 = static_cast<const cx<R>*>(L.tw[2]);
   const dim3 sgrid(L.spread.nt[0] * L.spread.nt[1] * L.spread.nt[2]);
-  if (L.region.scratch) {
+  if (L.region.region_stride > 0) {
     const RegionArgs& rg = L.region;
     const int nblk = rg.nb[0] * rg.nb[1] * rg.nb[2];
     const size_t smem = sizeof(double) * (size_t)(rg.S[0] + rg.order - 1) *
                         (rg.S[1] + rg.order - 1) * (rg.S[2] + rg.order - 1);
     const R* th = static_cast<const R*>(L.theta);
     R* scr = static_cast<R*>(rg.scratch);
+    if (rg.direct) {
+      const long long G = (long long)K1 * K2 * K3;
+      const int nb = (int)std::min<long long>(2048, (G / 4 + 255) / 256 + 1);
+      hipLaunchKernelGGL(k_clear<R>, dim3(nb), dim3(256), 0, st, grid, G);
+    }
+    dim3 rgrid;
+    const PairCo rco = co_for(nblk, 256, true, &rgrid);
 #define EW_REGION(PP)                                                                         \
-  hipLaunchKernelGGL((k_spread_region<R, PP>), dim3(nblk), dim3(256), smem, st, L.k0, th, L.q, \
-                     L.tile_start, rg, scr, L.qperm)
+  hipLaunchKernelGGL((k_spread_region<R, PP>), rgrid, dim3(256), smem, st, L.k0, th, L.q,      \
+                     L.tile_start, rg, scr, L.qperm, grid, rco)
     switch (rg.order) {
       case 4: EW_REGION(4); break;
       case 5: EW_REGION(5); break;
@@ -2796,8 +2920,9 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
     }
 #undef EW_REGION
     if (ev_region) (void)hipEventRecord(ev_region, st);
-    hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk * kAssembleSplit), dim3(256), 0, st,
-                       static_cast<const R*>(scr), rg, grid);
+    if (!rg.direct)
+      hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk * kAssembleSplit), dim3(256), 0, st,
+                         static_cast<const R*>(scr), rg, grid);
   } else if (L.spread.slots > 0) {
     const size_t smem = sizeof(unsigned long long) * kSpreadRows * 4 +
                         sizeof(int4) * (size_t)L.spread.slots +
@@ -2826,42 +2951,51 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
   const int gx = (nrows + 2 * L.px.lines - 1) / (2 * L.px.lines);
   const int ny = Kh * K3;
   const size_t psmem = plane_smem(L.px, L.py, sizeof(cx<R>));
+  dim3 g1;
   if (L.plane) {
     // x and y in one pass per plane (the y phase event follows immediately)
-    hipLaunchKernelGGL(k_fft_xy_r2c<R>, dim3(K3), dim3(kPlaneBlock), psmem, st, grid, cg, L.px,
-                       L.py, tw1, tw2);
+    const PairCo c1 = co_for(K3, kPlaneBlock, false, &g1);
+    hipLaunchKernelGGL(k_fft_xy_r2c<R>, g1, dim3(kPlaneBlock), psmem, st, grid, cg, L.px,
+                       L.py, tw1, tw2, c1);
     mark(1);
   } else {
-    hipLaunchKernelGGL(k_fft_x_r2c<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, grid,
-                       cg, nrows, Kh, L.px, tw1);
+    const PairCo c1 = co_for(gx, kFftBlock, false, &g1);
+    hipLaunchKernelGGL(k_fft_x_r2c<R>, g1, dim3(kFftBlock), fft_smem<R>(L.px), st, grid,
+                       cg, nrows, Kh, L.px, tw1, c1);
     mark(1);
-    hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((ny + L.py.lines - 1) / L.py.lines),
-                       dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh),
-                       (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf);
+    const PairCo c2 = co_for((ny + L.py.lines - 1) / L.py.lines, kFftBlock, false, &g1);
+    hipLaunchKernelGGL((k_fft_lines<R, 0>), g1, dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny,
+                       Kh, make_fastdiv(Kh), (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf,
+                       c2);
   }
   mark(2);
   const int nz = Kh * K2;
   if (L.forward_only) {
     hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
                        dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2,
-                       make_fastdiv(Kh * K2), 0LL, (long long)Kh * K2, L.pz, tw3, L.inf);
+                       make_fastdiv(Kh * K2), 0LL, (long long)Kh * K2, L.pz, tw3, L.inf,
+                       PairCo{});
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((k_fft_lines<R, 2>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
-                     dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2, make_fastdiv(Kh * K2), 0LL,
-                     (long long)Kh * K2, L.pz, tw3, L.inf);
+  const PairCo c3 = co_for((nz + L.pz.lines - 1) / L.pz.lines, kFftBlock, false, &g1);
+  hipLaunchKernelGGL((k_fft_lines<R, 2>), g1, dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz,
+                     Kh * K2, make_fastdiv(Kh * K2), 0LL, (long long)Kh * K2, L.pz, tw3, L.inf,
+                     c3);
   mark(3);
   if (L.plane) {
     mark(4);
-    hipLaunchKernelGGL(k_fft_xy_c2r<R>, dim3(K3), dim3(kPlaneBlock), psmem, st, cg, grid, L.px,
-                       L.py, tw1, tw2);
+    const PairCo c4 = co_for(K3, kPlaneBlock, false, &g1);
+    hipLaunchKernelGGL(k_fft_xy_c2r<R>, g1, dim3(kPlaneBlock), psmem, st, cg, grid, L.px,
+                       L.py, tw1, tw2, c4);
   } else {
-    hipLaunchKernelGGL((k_fft_lines<R, 1>), dim3((ny + L.py.lines - 1) / L.py.lines),
-                       dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny, Kh, make_fastdiv(Kh),
-                       (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf);
+    const PairCo c4 = co_for((ny + L.py.lines - 1) / L.py.lines, kFftBlock, false, &g1);
+    hipLaunchKernelGGL((k_fft_lines<R, 1>), g1, dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny,
+                       Kh, make_fastdiv(Kh), (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf,
+                       c4);
     mark(4);
-    hipLaunchKernelGGL(k_fft_x_c2r<R>, dim3(gx), dim3(kFftBlock), fft_smem<R>(L.px), st, cg,
-                       grid, nrows, Kh, L.px, tw1);
+    const PairCo c5 = co_for(gx, kFftBlock, false, &g1);
+    hipLaunchKernelGGL(k_fft_x_c2r<R>, g1, dim3(kFftBlock), fft_smem<R>(L.px), st, cg,
+                       grid, nrows, Kh, L.px, tw1, c5);
   }
   mark(5);
   return hipGetLastError();
@@ -2884,6 +3018,33 @@ hipError_t prepare_plane_kernels() {
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
+  }();
+  return done;
+}
+
+template <typename R, int P>
+static hipError_t region_attr(int bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_region<R, P>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_region<R, P>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+template <typename R>
+static hipError_t region_attrs(int b) {
+  const hipError_t es[] = {region_attr<R, 4>(b), region_attr<R, 5>(b), region_attr<R, 6>(b),
+                           region_attr<R, 7>(b), region_attr<R, 8>(b), region_attr<R, 10>(b),
+                           region_attr<R, 0>(b)};
+  for (hipError_t e : es)
+    if (e != hipSuccess) return e;
+  return hipSuccess;
+}
+
+hipError_t prepare_region_kernels() {
+  static hipError_t done = [] {
+    const hipError_t e = region_attrs<double>((int)kRegionMaxLds);
+    return e != hipSuccess ? e : region_attrs<float>((int)kRegionMaxLds);
   }();
   return done;
 }

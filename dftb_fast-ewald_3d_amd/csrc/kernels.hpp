@@ -152,10 +152,17 @@ struct RegionArgs {
   int order;
   long long region_stride;  // elements per block copy: (S0+p-1)(S1+p-1)(S2+p-1)
   void* scratch;   // nb0*nb1*nb2 copies; null: no privatised spreading
+  int direct;      // 1: each block adds its LDS copy into the (cleared) mesh with device-scope
+                   // atomics instead of writing it to scratch for the assembly (single-structure
+                   // evaluations; the grouped kernels keep the copies)
 };
 
-// LDS budget of a block copy
+// LDS budget of a block copy: the fixed rule keeps copies within 60 KiB (several workgroups per
+// CU); tuned shapes may use up to the gfx950 workgroup limit
 constexpr size_t kRegionMaxBytes = 60 * 1024;
+constexpr size_t kRegionMaxLds = 160 * 1024;
+// Raises the dynamic LDS limit of the region spreading kernels to kRegionMaxLds (once).
+hipError_t prepare_region_kernels();
 // Assembly of the block copies: workgroups per mesh block, mesh points in flight per thread
 #ifndef EW_ASSEMBLE_SPLIT
 #define EW_ASSEMBLE_SPLIT 1
@@ -183,6 +190,23 @@ inline bool plane_fits(const FftPlan& px, const FftPlan& py, size_t cs) {
 // Raises the dynamic LDS limit of the plane kernels to kPlaneMaxLds (once per process).
 hipError_t prepare_plane_kernels();
 
+// Rows [r0, r1) of the real-space pair sum run by the extra blocks of one chain kernel launch
+// (blocks >= first), so that the pair sum fills the CUs the latency-bound chain kernel leaves
+// idle, on the same stream (co-scheduling; no fork and join across streams).
+constexpr int kCoWaves = 2048;  // waves of pair blocks appended to one chain kernel launch
+struct PairCo {
+  int first = 0x7fffffff;  // chain blocks of the launch; the blocks after them run pair rows
+  int r0 = 0, r1 = 0;      // rows of this launch
+  int nblk = 0;            // pair blocks of this launch
+  int n = 0;
+  int col16 = 0;
+  const long long* row_ptr = nullptr;
+  const void* col = nullptr;
+  const void* coef = nullptr;
+  const double* q = nullptr;
+  double* out = nullptr;
+};
+
 struct SpmeLaunch {
   int K[3];
   int plane;  // 1: fused x-y plane transforms (plane_fits), 0: separate x and y passes
@@ -199,6 +223,10 @@ struct SpmeLaunch {
   InfluenceArgs inf;
   void* grid;
   void* cgrid;
+  // co-scheduled pair sum (pair.r1 > 0): its rows are split over the spreading launch
+  // (pair_spread_share percent) and the FFT launches (the rest, in equal parts)
+  PairCo pair;
+  int pair_spread_share;
 };
 
 // ---------------------------------------------------------------------------------------

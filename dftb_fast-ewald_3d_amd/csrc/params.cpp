@@ -285,14 +285,39 @@ FastDiv make_fastdiv(int d) {
   return FastDiv{unsigned(m), s};
 }
 
-int fft_plan(int n, int* radices) {
-  int np = 0, r = n;
-  while (r % 8 == 0 && np < kMaxRadixPasses) { radices[np++] = 8; r /= 8; }
-  while (r % 4 == 0 && np < kMaxRadixPasses) { radices[np++] = 4; r /= 4; }
-  while (r % 2 == 0 && np < kMaxRadixPasses) { radices[np++] = 2; r /= 2; }
-  while (r % 3 == 0 && np < kMaxRadixPasses) { radices[np++] = 3; r /= 3; }
-  while (r % 5 == 0 && np < kMaxRadixPasses) { radices[np++] = 5; r /= 5; }
-  return r == 1 ? np : -1;
+// Radix passes of a length n = 2^a 3^b 5^c in the order of `variant`:
+//   0: 8s, then a 4 or a 2, then 3s, 5s      1: 4s, then a 2, then 3s, 5s
+//   2: the leftover 2 or 4 first, then 8s, 3s, 5s      3: 3s and 5s first, then 8s, 4, 2
+// Every order is a valid Stockham plan (the twiddle tables follow the same order).
+int fft_plan(int n, int* radices, int variant) {
+  int a = 0, b = 0, c = 0, r = n;
+  while (r > 0 && r % 2 == 0) { ++a; r /= 2; }
+  while (r > 0 && r % 3 == 0) { ++b; r /= 3; }
+  while (r > 0 && r % 5 == 0) { ++c; r /= 5; }
+  if (r != 1) return -1;
+  std::vector<int> pw;  // powers of two
+  if (variant == 1) {
+    for (int i = 0; i < a / 2; ++i) pw.push_back(4);
+    if (a % 2) pw.push_back(2);
+  } else {
+    if (variant == 2 && a % 3) pw.push_back(a % 3 == 1 ? 2 : 4);
+    for (int i = 0; i < a / 3; ++i) pw.push_back(8);
+    if (variant != 2 && a % 3) pw.push_back(a % 3 == 1 ? 2 : 4);
+  }
+  std::vector<int> odd;
+  for (int i = 0; i < b; ++i) odd.push_back(3);
+  for (int i = 0; i < c; ++i) odd.push_back(5);
+  std::vector<int> all;
+  if (variant == 3) {
+    all = odd;
+    all.insert(all.end(), pw.begin(), pw.end());
+  } else {
+    all = pw;
+    all.insert(all.end(), odd.begin(), odd.end());
+  }
+  if ((int)all.size() > kMaxRadixPasses) return -1;
+  for (size_t i = 0; i < all.size(); ++i) radices[i] = all[i];
+  return (int)all.size();
 }
 
 }  // namespace ewamd

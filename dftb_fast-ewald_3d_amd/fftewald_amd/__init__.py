@@ -41,7 +41,7 @@ MAX_VECTORS = 8
 EXPORTED = ("ew_create", "ew_destroy", "ew_last_error", "ew_set_geometry", "ew_get_parameters",
             "ew_shift", "ew_shift_device", "ew_shift_device_noalloc", "ew_shift_multi", "ew_shift_multi_device",
             "ew_gradients", "ew_stress", "ew_external_potential", "ew_external_gradients",
-            "ew_phase_times", "ew_trace_times",
+            "ew_phase_times", "ew_trace_times", "ew_get_tuning",
             "ew_potential_force",
             "ew_parameters_for", "ew_grid_memory", "ew_pool_create", "ew_pool_destroy",
             "ew_pool_last_error", "ew_pool_add", "ew_pool_remove", "ew_pool_parameters",
@@ -68,12 +68,43 @@ class EwaldError(RuntimeError):
         self.code = code
 
 
+class Tuning(C.Structure):
+    """Mirrors struct ew_tuning: launch shapes of one structure's evaluation."""
+    _fields_ = [("spread_direct", C.c_int), ("spread_block", C.c_int * 3),
+                ("fft_planes", C.c_int), ("fft_lines", C.c_int * 3), ("fft_radix", C.c_int * 3),
+                ("pair_cosched", C.c_int), ("pair_spread_share", C.c_int),
+                ("eval_us", C.c_float)]
+
+    def as_dict(self):
+        return {"spread_direct": self.spread_direct, "spread_block": tuple(self.spread_block),
+                "fft_planes": self.fft_planes, "fft_lines": tuple(self.fft_lines),
+                "fft_radix": tuple(self.fft_radix), "pair_cosched": self.pair_cosched,
+                "pair_spread_share": self.pair_spread_share, "eval_us": self.eval_us}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        t.spread_direct = int(d["spread_direct"])
+        t.fft_planes = int(d["fft_planes"])
+        t.pair_cosched = int(d.get("pair_cosched", 0))
+        t.pair_spread_share = int(d.get("pair_spread_share", 0))
+        for a in range(3):
+            t.spread_block[a] = int(d["spread_block"][a])
+            t.fft_lines[a] = int(d["fft_lines"][a])
+            t.fft_radix[a] = int(d["fft_radix"][a])
+        return t
+
+
+TUNE = {"rules": 0, "auto": 1, "cached": 2, "given": 3}
+
+
 class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("alpha", C.c_double), ("r_cut", C.c_double),
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
                 ("eager", C.c_int), ("charged", C.c_int), ("fft_mode", C.c_int),
-                ("pair_storage", C.c_int), ("reserved", C.c_int), ("k_cut", C.c_double)]
+                ("pair_storage", C.c_int), ("reserved", C.c_int), ("k_cut", C.c_double),
+                ("tune", C.c_int), ("tuning", Tuning)]
 
 
 class ServerOptions(C.Structure):
@@ -171,6 +202,7 @@ def lib() -> C.CDLL:
         L.ew_phase_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
+        L.ew_get_tuning.argtypes = [H, C.POINTER(Tuning)]
         L.ew_trace_times.argtypes = [H, np.ctypeslib.ndpointer(dtype=np.float64, ndim=1,
                                                                 flags="C_CONTIGUOUS"),
                                      C.POINTER(C.c_longlong), C.c_int]
@@ -217,7 +249,8 @@ def lib() -> C.CDLL:
 
 def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
                  device=0, alpha_rule="fftewald", profile=False, eager=False,
-                 charged=False, fft_mode=0, pair_storage=0, k_cut=None) -> Options:
+                 charged=False, fft_mode=0, pair_storage=0, k_cut=None, tune="rules",
+                 tuning=None) -> Options:
     o = Options()
     o.tol = tol or 0.0
     o.alpha = alpha or 0.0
@@ -238,6 +271,11 @@ def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precis
     o.charged = 1 if charged else 0
     o.fft_mode = int(fft_mode)
     o.pair_storage = int(pair_storage)
+    o.tune = TUNE[tune] if isinstance(tune, str) else int(tune)
+    if tuning is not None:
+        o.tuning = tuning if isinstance(tuning, Tuning) else Tuning.from_dict(tuning)
+        if isinstance(tune, str) and tune == "rules":
+            o.tune = TUNE["given"]
     return o
 
 
@@ -403,6 +441,12 @@ class EwaldEngine:
         calls = C.c_longlong()
         _check(lib().ew_phase_times(self._h, ms, C.byref(calls), 1 if reset else 0), self._h)
         return dict(zip(PHASES, ms.tolist())), calls.value
+
+    def tuning(self) -> dict:
+        """Launch shapes of the current geometry (ew_get_tuning)."""
+        t = Tuning()
+        _check(lib().ew_get_tuning(self._h, C.byref(t)), self._h)
+        return t.as_dict()
 
     def trace_times(self, reset: bool = False):
         """({span: ms}, calls) accumulated since the last reset, timed in the evaluation's own

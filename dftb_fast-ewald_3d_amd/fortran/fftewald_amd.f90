@@ -11,8 +11,8 @@
 !> Errors: the one-shot routines keep the reference's convention (`error stop` with a message,
 !> fe/common/ewald_validation.f90:41-46); the engine routines return the C status instead.
 module fftewald_amd
-   use, intrinsic :: iso_c_binding, only: c_int, c_double, c_long_long, c_ptr, c_null_ptr, &
-                                          c_char, c_associated, c_f_pointer, c_loc
+   use, intrinsic :: iso_c_binding, only: c_int, c_double, c_float, c_long_long, c_ptr, &
+                                          c_null_ptr, c_char, c_associated, c_f_pointer, c_loc
    implicit none
 
    private
@@ -34,6 +34,18 @@ module fftewald_amd
    integer(c_int), parameter :: EW_MIXER_BROYDEN = 0
    integer(c_int), parameter :: EW_MIXER_SIMPLE = 1
 
+   !> Mirrors struct ew_tuning (launch shapes).
+   type, bind(C) :: ew_tuning
+      integer(c_int) :: spread_direct = 0
+      integer(c_int) :: spread_block(3) = 0
+      integer(c_int) :: fft_planes = 0
+      integer(c_int) :: fft_lines(3) = 0
+      integer(c_int) :: fft_radix(3) = 0
+      integer(c_int) :: pair_cosched = 0
+      integer(c_int) :: pair_spread_share = 0
+      real(c_float) :: eval_us = 0.0_c_float
+   end type ew_tuning
+
    !> Mirrors struct ew_options.
    type, bind(C) :: ew_options
       real(c_double) :: tol = 0.0_c_double
@@ -51,6 +63,8 @@ module fftewald_amd
       integer(c_int) :: pair_storage = 0
       integer(c_int) :: reserved = 0
       real(c_double) :: k_cut = 0.0_c_double
+      integer(c_int) :: tune = 0
+      type(ew_tuning) :: tuning = ew_tuning()
    end type ew_options
 
    !> Mirrors struct ew_parameters (the TEwaldParameters3d analogue).

@@ -50,6 +50,28 @@ extern "C" {
 
 typedef struct ew_handle ew_handle;
 
+/* Launch shapes of one structure's evaluation: how the spreading is blocked and how the 3D FFT
+ * is cut into passes.  Chosen by fixed rules, by the autotuner (ew_options.tune) or given.  No
+ * counterpart in the reference, whose analogue is FFTW's per-call planning
+ * (fe/nfft/fft_backend.f90:52-56). */
+typedef struct ew_tuning {
+  int spread_direct;    /* 1: each spreading block adds its LDS copy into the cleared mesh with
+                           device atomics; 0: copies to scratch, then a fixed-order assembly    */
+  int spread_block[3];  /* SPME sort tiles (16x8x8 mesh points) per spreading block per axis;
+                           0 0 0: no privatised blocks                                         */
+  int fft_planes;       /* 1: fused x-y plane transforms (a whole plane per workgroup)          */
+  int fft_lines[3];     /* lines per workgroup of the x (pairs of real rows), y and z passes   */
+  int fft_radix[3];     /* radix order per axis: 0 = 8s, 4, 2, then 3s, 5s; 1 = 4s, 2, 3s, 5s;
+                           2 = the leftover power of two first, then 8s, 3s, 5s;
+                           3 = 3s and 5s first, then 8s, 4, 2                                 */
+  int pair_cosched;     /* 1: the real-space pair sum runs on extra workgroups of the spreading
+                           and FFT launches, on the evaluation stream; 0: on an auxiliary
+                           stream beside them (fork and join by events)                       */
+  int pair_spread_share; /* pair_cosched: percent of the pair rows given to the spreading
+                           launch; the FFT launches share the rest equally                    */
+  float eval_us;        /* autotuned: the measured evaluation time of this choice; else 0       */
+} ew_tuning;
+
 /* Options; zero-initialise and set what you need.  Fields <= 0 mean "choose automatically",
  * mirroring the optional *_in overrides of ewald_potential_force
  * (fe/fft/ewald_fft_3d.f90:36-76). */
@@ -90,6 +112,13 @@ typedef struct ew_options {
                       follows from it as the modes follow from k_cut in modes_along_axis
                       (:159-167): K_i = smallest 2^a 3^b 5^c >= 2 sigma k_cut |a_i| / 2 pi;
                       an explicit grid still wins, as nModes_in does                          */
+  int tune;        /* launch shapes (ew_tuning): 0 (default): fixed rules; 1: autotune -- at
+                      ew_set_geometry, time candidate shapes on the device with the structure's
+                      own data and keep the fastest, once per (mesh, order, precision, size
+                      class, pair storage, device), persisted in a cache file
+                      ($EW_TUNE_CACHE, else ewald_tune.cache beside the library); 2: cached
+                      choices only (fixed rules on a miss); 3: `tuning` as given          */
+  ew_tuning tuning; /* tune = 3: the shapes to use                                              */
 } ew_options;
 
 /* Parameters an evaluation uses; mirrors TEwaldParameters3d
@@ -141,6 +170,9 @@ int ew_set_geometry(ew_handle *h, int n, const double *coords, const double *lat
 /* Parameters of the current geometry (ewald_parameters, fe/fft/ewald_fft_3d.f90:142-191). */
 int ew_get_parameters(const ew_handle *h, ew_parameters *out);
 
+/* The launch shapes the handle's evaluations use for its current geometry. */
+int ew_get_tuning(const ew_handle *h, ew_tuning *out);
+
 /* One SCC iteration, host arrays: shift = phi(dq) and energy = 1/2 sum shift*dq
  * (coulomb.F90:786-880, energy as printed at coulomb.F90:875).  Charges must be neutral
  * (canUseEwaldCode, coulomb.F90:255-265).  energy may be NULL. */
@@ -163,8 +195,8 @@ int ew_shift_device(ew_handle *h, const double *d_dq, double *d_shift, double *d
  * potentials, energy partials, mesh, spectrum and spreading copies) lives in one static device
  * array of 64 MiB per device, held until the process ends, instead of the handle's buffers; the
  * kernels are ew_shift_device's, launched directly (no graph is captured or instantiated).  A
- * geometry whose scratch exceeds 64 MiB gets EW_ERR_MEMORY (e.g. the 98,304-atom cell at 128^3
- * fits in mixed precision, not in FP64).  The caller passes the inputs and outputs as for
+ * geometry whose scratch exceeds 64 MiB gets EW_ERR_MEMORY (the 98,304-atom cell at 128^3 fits
+ * in both precisions with direct spreading; FP64 at 192^3 does not).  The caller passes the inputs and outputs as for
  * ew_shift_device (d_energy may be NULL).  Every call on one device shares that array, which
  * keeps the last call's contents: calls that overlap on one device race (order them on one
  * stream or synchronise between them). */

@@ -1,0 +1,112 @@
+"""Launch shapes (ew_tuning): every shape the autotuner may pick gives the same shifts and
+energy as the fixed rules, to the round-off of the spreading's accumulation order, and the
+CPU SPME at identical parameters; the autotuner's choice is cached and reused.
+
+Tolerances (max_i |x_i - ref_i| / max_i |ref_i|): FP64 1e-13 between shapes and 1e-11 against
+the CPU SPME; mixed 1e-6 against the FP64 rules.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+from conftest import rel_max
+
+import fftewald_amd as ew
+from fftewald_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _cases():
+    w = W.water(n_side=8, box=8 * 3.1)  # 1536 atoms with short gamma, 32^3
+    w["grid"] = (32, 32, 32)
+    t = W.triclinic(n=3000, side=31.0)  # sheared, 45 x 27 x 50 mesh (odd radices)
+    t["grid"] = (45, 27, 50)
+    return [("water 1536, 32^3", w), ("triclinic 3000, 45x27x50", t)]
+
+
+def _shapes(base):
+    """A spread of shapes around the rules: write-out, blocks, planes, lines, radix orders and
+    the pair sum's placement."""
+    out = []
+    for direct in (0, 1):
+        for blk in ((1, 1, 1), (2, 1, 1), (1, 2, 2), (2, 2, 2)):
+            t = dict(base, spread_direct=direct, spread_block=blk)
+            out.append(t)
+    for planes in (0, 1):
+        for lines in ((8, 2, 4), (16, 8, 16), (32, 16, 2)):
+            out.append(dict(base, fft_planes=planes, fft_lines=lines))
+    for r in range(4):
+        out.append(dict(base, fft_radix=(r, (r + 1) % 4, (r + 2) % 4)))
+    for co, share in ((1, 0), (1, 35), (1, 100), (0, 0)):
+        out.append(dict(base, pair_cosched=co, pair_spread_share=share))
+    return out
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_every_shape_matches_the_rules_and_the_oracle(case, oracle_mod):
+    name, w = _cases()[case]
+    opts = dict(tol=1e-10, grid=w["grid"], order=6, r_cut=8.0 * W.AA)
+    geo = dict(species=w.get("species"), hubbard_u=w.get("hubbard_u"), charges=w["charges"])
+    with ew.EwaldEngine(**opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], **geo)
+        base = e.tuning()
+        prm = e.parameters()
+        ref, eref = e.shift(w["charges"])
+    want = oracle_mod.spme_ewald(w["coords"], w["charges"], w["lattice"], prm["grid"],
+                                 prm["order"], prm["alpha"], prm["r_cut"])
+    if w.get("species") is not None:
+        want = want + oracle_mod.short_gamma_shift(w["coords"], w["charges"], w["lattice"],
+                                                   w["species"], w["hubbard_u"])
+    assert rel_max(ref, want) < 1e-11
+    tried = 0
+    for t in _shapes(base):
+        say(f"shapes {name}: {t}")
+        with ew.EwaldEngine(**opts, tuning=t) as e:
+            try:
+                e.set_geometry(w["coords"], w["lattice"], **geo)
+            except ew.EwaldError as err:
+                # a block larger than this mesh is refused, never run
+                assert err.code == ew.EW_ERR_ARGUMENT and "spreading block" in str(err)
+                continue
+            got = e.tuning()
+            s, en = e.shift(w["charges"])
+        assert rel_max(s, ref) < 1e-13, t
+        assert en == pytest.approx(eref, rel=1e-13)
+        tried += 1
+        assert got["fft_radix"] == tuple(t["fft_radix"])
+    assert tried >= 12
+
+
+def test_autotuned_shapes_agree_and_are_cached(tmp_path, monkeypatch):
+    monkeypatch.setenv("EW_TUNE_CACHE", str(tmp_path / "tune.cache"))
+    w = W.water(n_side=8, box=8 * 3.1)
+    opts = dict(tol=1e-10, grid=(32, 32, 32), order=6, r_cut=8.0 * W.AA)
+    geo = dict(species=w["species"], hubbard_u=w["hubbard_u"], charges=w["charges"])
+    for prec, tol in (("fp64", 1e-13), ("mixed", 1e-6)):
+        say(f"autotune water 1536 {prec}")
+        with ew.EwaldEngine(**opts, precision="fp64") as e:
+            e.set_geometry(w["coords"], w["lattice"], **geo)
+            ref, eref = e.shift(w["charges"])
+        with ew.EwaldEngine(**opts, precision=prec, tune="auto") as e:
+            e.set_geometry(w["coords"], w["lattice"], **geo)
+            tuned = e.tuning()
+            s, en = e.shift(w["charges"])
+        assert tuned["eval_us"] > 0.0
+        assert rel_max(s, ref) < tol
+        assert en == pytest.approx(eref, rel=10 * tol)
+        # the choice is in the cache: a cache-only engine takes it without timing
+        with ew.EwaldEngine(**opts, precision=prec, tune="cached") as e:
+            e.set_geometry(w["coords"], w["lattice"], **geo)
+            again = e.tuning()
+        for k in ("spread_direct", "spread_block", "fft_planes", "fft_lines", "fft_radix",
+                  "pair_cosched", "pair_spread_share"):
+            assert again[k] == tuned[k], k
+    lines = open(tmp_path / "tune.cache").read().splitlines()
+    assert len(lines) == 2 and all(l.startswith("v2:") for l in lines)

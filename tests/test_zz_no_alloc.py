@@ -67,16 +67,17 @@ def test_noalloc_matches_shift_device_large_then_smaller(precision):
 
 
 def test_noalloc_refuses_scratch_beyond_the_static_array():
-    """The 98,304-atom cell at 128^3 needs about 88 MB of scratch in FP64, above 64 MiB: refused
-    on the host before anything is launched; the handle still evaluates normally."""
+    """The 98,304-atom cell on a 192^3 mesh needs over 110 MB of scratch in FP64 (mesh and
+    spectrum), above 64 MiB: refused on the host before anything is launched; the handle still
+    evaluates normally."""
     import torch
     w = W.triclinic()
     dev = torch.device("cuda:0")
     n = len(w["charges"])
     dq = torch.from_numpy(w["charges"]).to(dev)
     out = torch.zeros(n, dtype=torch.float64, device=dev)
-    say("noalloc fp64 98,304 atoms: expect EW_ERR_MEMORY")
-    with ew.EwaldEngine(tol=1e-10, grid=128, order=6, r_cut=w["r_cut"]) as e:
+    say("noalloc fp64 98,304 atoms at 192^3: expect EW_ERR_MEMORY")
+    with ew.EwaldEngine(tol=1e-10, grid=192, order=6, r_cut=w["r_cut"]) as e:
         e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
         st = ew.lib().ew_shift_device_noalloc(e._h, C.c_void_p(dq.data_ptr()),
                                               C.c_void_p(out.data_ptr()), None, None)
