@@ -18,6 +18,9 @@ Fixtures:
   ref_small_cell.npz   40 charges in a cell too small for the linked cells (image-sum path).
   ref_alpha.npz        balanced_alpha on a table of (minG, minR, V, tol) and its status.
   ref_real_space.npz   the linked-cell real-space potential/force of 2000 charges.
+  ref_water_c2.npz     the 12,288-atom water workload (its coordinates and charges) and the
+                       reference direct-Ewald potential at tol 1e-10, alpha 0.2 Bohr^-1 (a
+                       split that keeps the O(N^2 images + modes N) direct sum to minutes).
   ref_truncation.npz   the fast path's parameter chain at tol 1e-6 ... 1e-12 on the cells of the
                        five workloads and on batch/serving-like cells: lambert_w0 on a table,
                        r_cut by the mean-spacing rule, alpha = splitting_from_real_budget and
@@ -26,7 +29,7 @@ Fixtures:
                        balanced_alpha(2 pi min|b_i|, min|a_i|, V, tol), the balance condition
                        getOptimalAlphaEwald bisects (src/dftbp/dftb/coulomb.F90:2097-2179).
 
-    python tests/golden/make_golden.py [--only truncation]
+    python tests/golden/make_golden.py [--only truncation|water]
 """
 import math
 import os
@@ -101,13 +104,26 @@ def truncation():
              budget_fraction=np.float64(fraction), lw_x=xs, lw_w=ws)
 
 
+def water_c2():
+    """The reference direct Ewald (direct/ewald_direct_3d.f90:37-182) on the 12,288-atom water
+    box, Coulomb only (no short gamma: the GPU test adds it from the oracle when needed)."""
+    w = W.water()
+    alpha = 0.2
+    pot = oracle.ref_direct_ewald(w["coords"], w["charges"], w["lattice"], 1e-10, alpha=alpha)
+    np.savez(os.path.join(HERE, "ref_water_c2.npz"), coords=w["coords"], lattice=w["lattice"],
+             charges=w["charges"], alpha=np.float64(alpha), tol=np.float64(1e-10), pot=pot)
+
+
 def main():
     oracle.build(with_reference=True)
     if not oracle.have_reference():
         raise SystemExit("oracle/_ref/libewald_ref.so is missing: /root/reference not present")
-    truncation()
     if "--only" in sys.argv:
+        what = sys.argv[sys.argv.index("--only") + 1]
+        {"truncation": truncation, "water": water_c2}[what]()
         return
+    truncation()
+    water_c2()
 
     # 1. the NaCl workload
     w = W.nacl()
