@@ -273,7 +273,7 @@ def run_batch(args, torch, ew, W):
     for prec in ("fp64", "mixed"):
         rs = 4 if prec == "mixed" else 8
         with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=ws[0]["order"],
-                          precision=prec) as pool:
+                          precision=prec, group_grid=args.group_grid) as pool:
             t = time.perf_counter()
             ids = [pool.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"])
                    for w in ws]
@@ -579,6 +579,8 @@ def main():
                     help="mesh workload: mesh points per axis")
     ap.add_argument("--external", type=int, default=20000,
                     help="triclinic: external point charges timed once per precision (0: skip)")
+    ap.add_argument("--group-grid", type=int, default=0, choices=[0, 1],
+                    help="batch: 0 one workgroup per work item (default), 1 persistent grids")
     ap.add_argument("--tune", default="auto", choices=["auto", "rules"],
                     help="launch shapes of the timed engines: the autotuner (default) or the "
                          "fixed rules")

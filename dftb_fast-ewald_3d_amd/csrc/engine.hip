@@ -167,6 +167,8 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
   if (o.profile < 0 || o.profile > 2)
     return set_global_error(EW_ERR_ARGUMENT, "profile must be 0, 1 or 2");
   if (o.tune < 0 || o.tune > 3) return set_global_error(EW_ERR_ARGUMENT, "tune must be 0..3");
+  if (o.group_grid < 0 || o.group_grid > 1)
+    return set_global_error(EW_ERR_ARGUMENT, "group_grid must be 0 or 1");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     (void)hipGetLastError();
@@ -185,8 +187,8 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     h->own_stream = false;
   } else if (hipStreamCreate(&h->stream) != hipSuccess ||
              hipStreamCreate(&h->aux) != hipSuccess ||
-             hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
-             hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming) != hipSuccess) {
+             hipEventCreateWithFlags(&h->fork_ev, kSyncEventFlags) != hipSuccess ||
+             hipEventCreateWithFlags(&h->join_ev, kSyncEventFlags) != hipSuccess) {
     (void)hipGetLastError();
     if (h->join_ev) (void)hipEventDestroy(h->join_ev);
     if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
@@ -195,7 +197,7 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
     delete h;
     return set_global_error(EW_ERR_DEVICE, "hipStreamCreate failed");
   }
-  if (hipEventCreateWithFlags(&h->last_ev, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreateWithFlags(&h->last_ev, kSyncEventFlags) != hipSuccess) {
     (void)hipGetLastError();
     if (h->join_ev) (void)hipEventDestroy(h->join_ev);
     if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
@@ -484,9 +486,10 @@ static bool region_shape(const ew_handle* h, const int* f, size_t max_bytes, Reg
   return true;
 }
 
-// Direct write-out of the spreading blocks by the fixed rule: measured faster than copies plus
-// assembly on the 98,304-atom cell at 128^3 (and at 64^3 for 12,288 atoms).
-constexpr int kDirectRule = 1;
+// Direct write-out of the spreading blocks (device atomics into the cleared mesh) is not the
+// fixed rule: on the 98,304-atom cell at 128^3 it measured 3 % (mixed) to 6 % (FP64) slower
+// than block copies plus assembly; the autotuner chooses per structure.
+constexpr int kDirectRule = 0;
 // The pair sum beside the chain on the auxiliary stream by the fixed rule (co-scheduling it in
 // the chain's launches measured slower on the 98,304-atom cell: 0.226 against 0.189 ms mixed;
 // faster on the 12,288-atom water box: 0.096 against 0.097 ms with 20 % of the rows in the
@@ -1668,7 +1671,7 @@ static int ensure_views(ew_handle* h, int nvec) {
 // list the pair sums of all vectors are one multi-vector sum; the symmetric storage keeps one
 // pair-sum block set per vector.
 static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, const void* dev) {
-  GroupLaunch G = group_launch_of(S, h->views.desc, dev, h->prm.order);
+  GroupLaunch G = group_launch_of(S, h->views.desc, dev, h->prm.order, h->opt.group_grid == 1);
   if (!h->half) {
     G.has_pm = 1;
     G.pm = PairMulti{h->n,

@@ -110,14 +110,16 @@ inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bo
 
 // The launch over a device copy (at dev) of S's words, with the descriptor table at desc_dev.
 inline GroupLaunch group_launch_of(const Staged& S, const StructDesc* desc_dev, const void* dev,
-                                   int order) {
+                                   int order, bool persistent) {
   GroupLaunch G{};
+  G.persistent = persistent ? 1 : 0;
   G.desc = desc_dev;
   const int* w = static_cast<const int*>(dev);
   G.io = reinterpret_cast<const IoSlot*>(static_cast<const char*>(dev) + S.io_offset);
   G.order = order;
   for (int k = 0; k < kGroupKinds; ++k) {
-    G.map[k] = GroupMap{w + S.offs[k], w + S.offs[k] + S.counts[k] + 1, S.counts[k]};
+    G.map[k] = GroupMap{w + S.offs[k], w + S.offs[k] + S.counts[k] + 1, S.counts[k],
+                        S.blocks[k]};
     G.blocks[k] = S.blocks[k];
     G.smem[k] = S.smem[k];
   }

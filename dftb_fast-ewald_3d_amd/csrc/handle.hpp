@@ -14,6 +14,12 @@
 
 namespace ewamd {
 
+// Events that order work between streams of one device (fork, join, the last evaluation): no
+// timing, and a device-scope release when recorded -- their consumers are kernels on the same
+// device or a host wait for completion, neither needs the default system-scope fence (a cache
+// write-back at every record).
+constexpr unsigned kSyncEventFlags = hipEventDisableTiming | hipEventReleaseToDevice;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

@@ -410,13 +410,15 @@ __device__ __forceinline__ void k_fft_x_r2c_body(const R* __restrict__ grid,
   }
 }
 
-template <typename R>
+template <typename R, bool CO>
 __global__ __launch_bounds__(kFftBlock) void k_fft_x_r2c(const R* __restrict__ grid,
                                                          cx<R>* __restrict__ cg, int nrows,
                                                          int Kh, FftPlan pl,
                                                          const cx<R>* __restrict__ tw,
                                                          PairCo co) {
-  if (co_pairs<R>(co)) return;
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
   k_fft_x_r2c_body<R>(grid, cg, nrows, Kh, pl, tw, blockIdx.x);
 }
 
@@ -470,13 +472,15 @@ __device__ __forceinline__ void k_fft_x_c2r_body(const cx<R>* __restrict__ cg,
   }
 }
 
-template <typename R>
+template <typename R, bool CO>
 __global__ __launch_bounds__(kFftBlock) void k_fft_x_c2r(const cx<R>* __restrict__ cg,
                                                          R* __restrict__ grid, int nrows, int Kh,
                                                          FftPlan pl,
                                                          const cx<R>* __restrict__ tw,
                                                          PairCo co) {
-  if (co_pairs<R>(co)) return;
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
   k_fft_x_c2r_body<R>(cg, grid, nrows, Kh, pl, tw, blockIdx.x);
 }
 
@@ -557,14 +561,16 @@ __device__ __forceinline__ void k_fft_lines_body(cx<R>* __restrict__ cg, int nli
   }
 }
 
-template <typename R, int MODE>
+template <typename R, int MODE, bool CO>
 __global__ __launch_bounds__(kFftBlock) void k_fft_lines(cx<R>* __restrict__ cg, int nlines,
                                                          int A, FastDiv ad, long long B,
                                                          long long es,
                                                          FftPlan pl,
                                                          const cx<R>* __restrict__ tw,
                                                          InfluenceArgs inf, PairCo co) {
-  if (co_pairs<R>(co)) return;
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
   k_fft_lines_body<R, MODE>(cg, nlines, A, ad, B, es, pl, tw, inf, blockIdx.x);
 }
 
@@ -716,25 +722,29 @@ __device__ __forceinline__ void k_fft_xy_c2r_body(const cx<R>* __restrict__ cg,
   }
 }
 
-template <typename R>
+template <typename R, bool CO>
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_r2c(const R* __restrict__ grid,
                                                             cx<R>* __restrict__ cg, FftPlan px,
                                                             FftPlan py,
                                                             const cx<R>* __restrict__ twx,
                                                             const cx<R>* __restrict__ twy,
                                                             PairCo co) {
-  if (co_pairs<R>(co)) return;
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
   k_fft_xy_r2c_body<R>(grid, cg, px, py, twx, twy, blockIdx.x);
 }
 
-template <typename R>
+template <typename R, bool CO>
 __global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_c2r(const cx<R>* __restrict__ cg,
                                                             R* __restrict__ grid, FftPlan px,
                                                             FftPlan py,
                                                             const cx<R>* __restrict__ twx,
                                                             const cx<R>* __restrict__ twy,
                                                             PairCo co) {
-  if (co_pairs<R>(co)) return;
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
   k_fft_xy_c2r_body<R>(cg, grid, px, py, twx, twy, blockIdx.x);
 }
 
@@ -1080,7 +1090,7 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
   for (int i = threadIdx.x; i < nreg; i += blockDim.x) out[i] = R(reg[i]);
 }
 
-template <typename R, int P>
+template <typename R, int P, bool CO>
 __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ k0,
                                                        const R* __restrict__ theta,
                                                        const double* __restrict__ q,
@@ -1088,7 +1098,9 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
                                                        RegionArgs g, R* __restrict__ regions,
                                                        const int* __restrict__ qperm,
                                                        R* __restrict__ grid, PairCo co) {
-  if (co_pairs<R>(co)) return;
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
   k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x, qperm, grid);
 }
 
@@ -1275,9 +1287,70 @@ __global__ __launch_bounds__(256) void k_pair_spmv_stream(int n,
                                                           const CT* __restrict__ coef,
                                                           const double* __restrict__ q,
                                                           double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * (blockDim.x >> 6);
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  pair_stream_rows<CT, IT>(n, 0, n, wave, nw, row_ptr, col, coef, q, out);
+  int row = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (row >= n) return;  // uniform across the wave
+  long long a = row_ptr[row], b = row_ptr[row + 1];
+  IT c[8];
+  CT w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const long long k = a + lane + 64 * u;
+    c[u] = k < b ? col[k] : IT(0);
+    w[u] = k < b ? coef[k] : CT(0);
+  }
+  while (true) {
+    const int nrow = row + nw;
+    const bool more = nrow < n;
+    // this row's partners' charges
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = a + lane + 64 * u < b ? q[partner(row, c[u], n)] : 0.0;
+    // the next row's bounds and first sweep, in flight during this row's sums
+    long long na = 0, nb = 0;
+    IT nc[8];
+    CT nwt[8];
+    if (more) {
+      na = row_ptr[nrow];
+      nb = row_ptr[nrow + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long k = na + lane + 64 * u;
+      nc[u] = k < nb ? col[k] : IT(0);
+      nwt[u] = k < nb ? coef[k] : CT(0);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += double(w[u]) * x[u];
+    // rows longer than one sweep (dense image sums, long short-range lists)
+    for (long long g = a + lane + 512; g < b; g += 512) {
+      IT cc[8];
+      CT ww[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long long k = g + 64 * u;
+        cc[u] = k < b ? col[k] : IT(0);
+        ww[u] = k < b ? coef[k] : CT(0);
+      }
+      double qv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) qv[u] = g + 64 * u < b ? q[partner(row, cc[u], n)] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += double(ww[u]) * qv[u];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[row] = acc;
+    if (!more) break;
+    row = nrow;
+    a = na;
+    b = nb;
+    for (int u = 0; u < 8; ++u) {
+      c[u] = nc[u];
+      w[u] = nwt[u];
+    }
+  }
 }
 
 template <typename CT, typename IT>
@@ -2326,184 +2399,245 @@ __global__ __launch_bounds__(256) void k_charge_sum(int n, const double* __restr
 
 // Evaluation slot and local block index of this workgroup (binary search over the block
 // prefix; the prefix is uniform per workgroup, so the loads go through the scalar cache).
-__device__ __forceinline__ int group_slot(const GroupMap& m, int& local) {
-  const int b = blockIdx.x;
+// Work item `item` of a grouped launch: its evaluation slot, and in `local` its block index
+// within that evaluation's structure (binary search over the per-kind block prefix).
+__device__ __forceinline__ int group_slot(const GroupMap& m, int item, int& local) {
   int lo = 0, hi = m.count;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (m.prefix[mid] <= b)
+    if (m.prefix[mid] <= item)
       lo = mid;
     else
       hi = mid;
   }
-  local = b - m.prefix[lo];
+  local = item - m.prefix[lo];
   return m.slot[lo];
 }
 
+// The work items of a grouped kernel.  LOOP (persistent grid): workgroup w takes items
+// w, w + gridDim.x, ..., a barrier between consecutive items (their bodies reuse the
+// workgroup's LDS); otherwise one workgroup per item, compiled without the loop.
+#define EW_GROUP_RUN(LOOP, m, item)                                     \
+  if constexpr (LOOP) {                                                 \
+    for (int gi_ = blockIdx.x; gi_ < (m).total; gi_ += gridDim.x) {     \
+      item(gi_);                                                        \
+      __syncthreads();                                                  \
+    }                                                                   \
+  } else {                                                              \
+    item((int)blockIdx.x);                                              \
+  }
+
+template <bool LOOP>
 __global__ __launch_bounds__(256) void kg_gather_charges(const StructDesc* __restrict__ D,
                                                          const IoSlot* __restrict__ io,
                                                          GroupMap m) {
-  int b;
-  const IoSlot& o = io[group_slot(m, b)];
-  const StructDesc& d = D[o.sid];
-  k_gather_charges_body(d.n, d.perm, o.dq, d.qs, d.half ? d.phi_real : nullptr, b);
+  auto item = [&](int gi) {
+    int b;
+    const IoSlot& o = io[group_slot(m, gi, b)];
+    const StructDesc& d = D[o.sid];
+    k_gather_charges_body(d.n, d.perm, o.dq, d.qs, d.half ? d.phi_real : nullptr, b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R, int P>
+template <typename R, int P, bool LOOP>
 __global__ __launch_bounds__(256) void kg_spread_region(const StructDesc* __restrict__ D,
                                                         const IoSlot* __restrict__ io,
                                                         GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  k_spread_region_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.region,
-                             static_cast<R*>(d.region.scratch), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    k_spread_region_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.region,
+                               static_cast<R*>(d.region.scratch), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R>
+template <typename R, bool LOOP>
 __global__ __launch_bounds__(256) void kg_spread_assemble(const StructDesc* __restrict__ D,
                                                           const IoSlot* __restrict__ io,
                                                           GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  k_spread_assemble_body<R>(static_cast<const R*>(d.region.scratch), d.region,
-                            static_cast<R*>(d.grid), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    k_spread_assemble_body<R>(static_cast<const R*>(d.region.scratch), d.region,
+                              static_cast<R*>(d.grid), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R, int P>
+template <typename R, int P, bool LOOP>
 __global__ __launch_bounds__(kSpreadThreads) void kg_spread(const StructDesc* __restrict__ D,
                                                             const IoSlot* __restrict__ io,
                                                             GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  k_spread_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.spread,
-                      static_cast<R*>(d.grid), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    k_spread_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.spread,
+                        static_cast<R*>(d.grid), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R>
+template <typename R, bool LOOP>
 __global__ __launch_bounds__(256) void kg_spread_checked(const StructDesc* __restrict__ D,
                                                          const IoSlot* __restrict__ io,
                                                          GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  k_spread_checked_body<R>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.spread,
-                           static_cast<R*>(d.grid), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    k_spread_checked_body<R>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.spread,
+                             static_cast<R*>(d.grid), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R, bool INVERSE>
+template <typename R, bool INVERSE, bool LOOP>
 __global__ __launch_bounds__(kFftBlock) void kg_fft_x(const StructDesc* __restrict__ D,
                                                       const IoSlot* __restrict__ io,
                                                       GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  const int nrows = d.K[1] * d.K[2], Kh = d.K[0] / 2 + 1;
-  if (INVERSE)
-    k_fft_x_c2r_body<R>(static_cast<const cx<R>*>(d.cgrid), static_cast<R*>(d.grid), nrows, Kh,
-                        d.px, static_cast<const cx<R>*>(d.tw[0]), b);
-  else
-    k_fft_x_r2c_body<R>(static_cast<const R*>(d.grid), static_cast<cx<R>*>(d.cgrid), nrows, Kh,
-                        d.px, static_cast<const cx<R>*>(d.tw[0]), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    const int nrows = d.K[1] * d.K[2], Kh = d.K[0] / 2 + 1;
+    if (INVERSE)
+      k_fft_x_c2r_body<R>(static_cast<const cx<R>*>(d.cgrid), static_cast<R*>(d.grid), nrows, Kh,
+                          d.px, static_cast<const cx<R>*>(d.tw[0]), b);
+    else
+      k_fft_x_r2c_body<R>(static_cast<const R*>(d.grid), static_cast<cx<R>*>(d.cgrid), nrows, Kh,
+                          d.px, static_cast<const cx<R>*>(d.tw[0]), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
 // AXIS 1: y lines (MODE 0 forward, 1 inverse); AXIS 2: z lines with the influence (MODE 2).
-template <typename R, int AXIS, int MODE>
+template <typename R, int AXIS, int MODE, bool LOOP>
 __global__ __launch_bounds__(kFftBlock) void kg_fft_lines(const StructDesc* __restrict__ D,
                                                           const IoSlot* __restrict__ io,
                                                           GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  const int Kh = d.K[0] / 2 + 1;
-  cx<R>* cg = static_cast<cx<R>*>(d.cgrid);
-  if (AXIS == 1)
-    k_fft_lines_body<R, MODE>(cg, Kh * d.K[2], Kh, d.yd, (long long)d.K[1] * Kh, (long long)Kh,
-                              d.py, static_cast<const cx<R>*>(d.tw[1]), d.inf, b);
-  else
-    k_fft_lines_body<R, MODE>(cg, Kh * d.K[1], Kh * d.K[1], d.zd, 0LL, (long long)Kh * d.K[1],
-                              d.pz, static_cast<const cx<R>*>(d.tw[2]), d.inf, b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    const int Kh = d.K[0] / 2 + 1;
+    cx<R>* cg = static_cast<cx<R>*>(d.cgrid);
+    if (AXIS == 1)
+      k_fft_lines_body<R, MODE>(cg, Kh * d.K[2], Kh, d.yd, (long long)d.K[1] * Kh, (long long)Kh,
+                                d.py, static_cast<const cx<R>*>(d.tw[1]), d.inf, b);
+    else
+      k_fft_lines_body<R, MODE>(cg, Kh * d.K[1], Kh * d.K[1], d.zd, 0LL, (long long)Kh * d.K[1],
+                                d.pz, static_cast<const cx<R>*>(d.tw[2]), d.inf, b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R, bool INVERSE>
+template <typename R, bool INVERSE, bool LOOP>
 __global__ __launch_bounds__(kPlaneBlock) void kg_fft_xy(const StructDesc* __restrict__ D,
                                                          const IoSlot* __restrict__ io,
                                                          GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  if (INVERSE)
-    k_fft_xy_c2r_body<R>(static_cast<const cx<R>*>(d.cgrid), static_cast<R*>(d.grid), d.px,
-                         d.py, static_cast<const cx<R>*>(d.tw[0]),
-                         static_cast<const cx<R>*>(d.tw[1]), b);
-  else
-    k_fft_xy_r2c_body<R>(static_cast<const R*>(d.grid), static_cast<cx<R>*>(d.cgrid), d.px,
-                         d.py, static_cast<const cx<R>*>(d.tw[0]),
-                         static_cast<const cx<R>*>(d.tw[1]), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    if (INVERSE)
+      k_fft_xy_c2r_body<R>(static_cast<const cx<R>*>(d.cgrid), static_cast<R*>(d.grid), d.px,
+                           d.py, static_cast<const cx<R>*>(d.tw[0]),
+                           static_cast<const cx<R>*>(d.tw[1]), b);
+    else
+      k_fft_xy_r2c_body<R>(static_cast<const R*>(d.grid), static_cast<cx<R>*>(d.cgrid), d.px,
+                           d.py, static_cast<const cx<R>*>(d.tw[0]),
+                           static_cast<const cx<R>*>(d.tw[1]), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename CT>
+template <typename CT, bool LOOP>
 __global__ __launch_bounds__(256) void kg_pair_spmv(const StructDesc* __restrict__ D,
                                                     const IoSlot* __restrict__ io, GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  if (d.half)
-    k_pair_half_body<CT>(d.tile_start, d.hp, d.qs, d.phi_real, b);
-  else if (d.col16)
-    k_pair_spmv_body<CT, short>(d.n, d.row_ptr, static_cast<const short*>(d.col),
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    if (d.half)
+      k_pair_half_body<CT>(d.tile_start, d.hp, d.qs, d.phi_real, b);
+    else if (d.col16)
+      k_pair_spmv_body<CT, short>(d.n, d.row_ptr, static_cast<const short*>(d.col),
+                                  static_cast<const CT*>(d.coef), d.qs, d.phi_real, b);
+    else
+      k_pair_spmv_body<CT, int>(d.n, d.row_ptr, static_cast<const int*>(d.col),
                                 static_cast<const CT*>(d.coef), d.qs, d.phi_real, b);
-  else
-    k_pair_spmv_body<CT, int>(d.n, d.row_ptr, static_cast<const int*>(d.col),
-                              static_cast<const CT*>(d.coef), d.qs, d.phi_real, b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R, int P>
+template <typename R, int P, bool LOOP>
 __global__ __launch_bounds__(256) void kg_gather_tile(const StructDesc* __restrict__ D,
                                                       const IoSlot* __restrict__ io,
                                                       GroupMap m) {
-  int b;
-  const IoSlot& o = io[group_slot(m, b)];
-  const StructDesc& d = D[o.sid];
-  k_gather_tile_body<R, P>(d.k0, static_cast<const R*>(d.theta), static_cast<const R*>(d.grid),
-                           d.qs, d.phi_real, d.onsite, d.perm, d.tile_start, d.ga, o.shift,
-                           d.epart, b);
+  auto item = [&](int gi) {
+    int b;
+    const IoSlot& o = io[group_slot(m, gi, b)];
+    const StructDesc& d = D[o.sid];
+    k_gather_tile_body<R, P>(d.k0, static_cast<const R*>(d.theta), static_cast<const R*>(d.grid),
+                             d.qs, d.phi_real, d.onsite, d.perm, d.tile_start, d.ga, o.shift,
+                             d.epart, b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R>
+template <typename R, bool LOOP>
 __global__ __launch_bounds__(256) void kg_gather_combine(const StructDesc* __restrict__ D,
                                                          const IoSlot* __restrict__ io,
                                                          GroupMap m) {
-  int b;
-  const IoSlot& o = io[group_slot(m, b)];
-  const StructDesc& d = D[o.sid];
-  k_gather_combine_body<R>(d.n, d.k0, static_cast<const R*>(d.theta),
-                           static_cast<const R*>(d.grid), d.qs, d.phi_real, d.onsite, d.perm,
-                           d.ga, o.shift, d.epart, b);
+  auto item = [&](int gi) {
+    int b;
+    const IoSlot& o = io[group_slot(m, gi, b)];
+    const StructDesc& d = D[o.sid];
+    k_gather_combine_body<R>(d.n, d.k0, static_cast<const R*>(d.theta),
+                             static_cast<const R*>(d.grid), d.qs, d.phi_real, d.onsite, d.perm,
+                             d.ga, o.shift, d.epart, b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
+template <bool LOOP>
 __global__ __launch_bounds__(256) void kg_charge_sum(const StructDesc* __restrict__ D,
                                                      const IoSlot* __restrict__ io,
                                                      GroupMap m) {
-  int b;
-  const StructDesc& d = D[io[group_slot(m, b)].sid];
-  k_charge_sum_body(d.n, d.qs, const_cast<double*>(d.ga.qsum), b);
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    k_charge_sum_body(d.n, d.qs, const_cast<double*>(d.ga.qsum), b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
+template <bool LOOP>
 __global__ __launch_bounds__(256) void kg_energy_final(const StructDesc* __restrict__ D,
                                                        const IoSlot* __restrict__ io,
                                                        GroupMap m) {
-  int b;
-  const IoSlot& o = io[group_slot(m, b)];
-  const StructDesc& d = D[o.sid];
-  k_energy_final_body(d.nparts, d.epart, o.energy, b);
+  auto item = [&](int gi) {
+    int b;
+    const IoSlot& o = io[group_slot(m, gi, b)];
+    const StructDesc& d = D[o.sid];
+    k_energy_final_body(d.nparts, d.epart, o.energy, b);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
 }
 
-template <typename R>
-hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
+template <typename R, bool LOOP>
+static hipError_t launch_group_t(const GroupLaunch& G, hipStream_t st) {
   const StructDesc* D = G.desc;
   const IoSlot* io = G.io;
   auto run = [&](int kind) { return G.blocks[kind] > 0; };
-  auto grid = [&](int kind) { return dim3(G.blocks[kind]); };
+  // persistent: at most kGroupWgPerCu workgroups per CU (of 256 threads; the 1024-thread plane
+  // kernels one per CU), each walking its share of the kind's work items
+  auto grid = [&](int kind) {
+    const int cap = kind == kGkPlane ? kNumCus : kGroupWgPerCu * kNumCus;
+    return dim3(LOOP ? std::min(G.blocks[kind], cap) : G.blocks[kind]);
+  };
   if (run(kGkCharges))
-    hipLaunchKernelGGL(kg_gather_charges, grid(kGkCharges), dim3(256), 0, st, D, io,
+    hipLaunchKernelGGL(kg_gather_charges<LOOP>, grid(kGkCharges), dim3(256), 0, st, D, io,
                        G.map[kGkCharges]);
   if (run(kGkChargeSum))
-    hipLaunchKernelGGL(kg_charge_sum, grid(kGkChargeSum), dim3(256), 0, st, D, io,
+    hipLaunchKernelGGL(kg_charge_sum<LOOP>, grid(kGkChargeSum), dim3(256), 0, st, D, io,
                        G.map[kGkChargeSum]);
   const bool fork = G.aux && (run(kGkPairs) || G.has_pm);
   if (fork) {
@@ -2514,7 +2648,7 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
       e = launch_pair_spmm(G.pm, sizeof(R) == sizeof(float), G.aux);
       if (e != hipSuccess) return e;
     } else {
-      hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, G.aux, D, io,
+      hipLaunchKernelGGL((kg_pair_spmv<R, LOOP>), grid(kGkPairs), dim3(256), 0, G.aux, D, io,
                          G.map[kGkPairs]);
     }
     e = hipEventRecord(G.join_ev, G.aux);
@@ -2532,42 +2666,42 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
   }
   if (run(kGkRegion)) {
 #define EW_G_REGION(PP)                                                                  \
-  hipLaunchKernelGGL((kg_spread_region<R, PP>), grid(kGkRegion), dim3(256),             \
+  hipLaunchKernelGGL((kg_spread_region<R, PP, LOOP>), grid(kGkRegion), dim3(256),             \
                      G.smem[kGkRegion], st, D, io, G.map[kGkRegion])
     EW_ORDER_SWITCH(EW_G_REGION)
 #undef EW_G_REGION
-    hipLaunchKernelGGL(kg_spread_assemble<R>, grid(kGkAssemble), dim3(256), 0, st, D, io,
+    hipLaunchKernelGGL((kg_spread_assemble<R, LOOP>), grid(kGkAssemble), dim3(256), 0, st, D, io,
                        G.map[kGkAssemble]);
   }
   if (run(kGkSlotted)) {
 #define EW_G_SLOTTED(PP)                                                                  \
-  hipLaunchKernelGGL((kg_spread<R, PP>), grid(kGkSlotted), dim3(kSpreadThreads),         \
+  hipLaunchKernelGGL((kg_spread<R, PP, LOOP>), grid(kGkSlotted), dim3(kSpreadThreads),         \
                      G.smem[kGkSlotted], st, D, io, G.map[kGkSlotted])
     EW_ORDER_SWITCH(EW_G_SLOTTED)
 #undef EW_G_SLOTTED
   }
   if (run(kGkChecked))
-    hipLaunchKernelGGL(kg_spread_checked<R>, grid(kGkChecked), dim3(256), 0, st, D, io,
+    hipLaunchKernelGGL((kg_spread_checked<R, LOOP>), grid(kGkChecked), dim3(256), 0, st, D, io,
                        G.map[kGkChecked]);
   if (run(kGkPlane))
-    hipLaunchKernelGGL((kg_fft_xy<R, false>), grid(kGkPlane), dim3(kPlaneBlock),
+    hipLaunchKernelGGL((kg_fft_xy<R, false, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
                        G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
   if (run(kGkFftX)) {
-    hipLaunchKernelGGL((kg_fft_x<R, false>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
+    hipLaunchKernelGGL((kg_fft_x<R, false, LOOP>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
                        st, D, io, G.map[kGkFftX]);
-    hipLaunchKernelGGL((kg_fft_lines<R, 1, 0>), grid(kGkFftY), dim3(kFftBlock),
+    hipLaunchKernelGGL((kg_fft_lines<R, 1, 0, LOOP>), grid(kGkFftY), dim3(kFftBlock),
                        G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
   }
-  hipLaunchKernelGGL((kg_fft_lines<R, 2, 2>), grid(kGkFftZ), dim3(kFftBlock), G.smem[kGkFftZ],
+  hipLaunchKernelGGL((kg_fft_lines<R, 2, 2, LOOP>), grid(kGkFftZ), dim3(kFftBlock), G.smem[kGkFftZ],
                      st, D, io, G.map[kGkFftZ]);
   if (run(kGkFftX)) {
-    hipLaunchKernelGGL((kg_fft_lines<R, 1, 1>), grid(kGkFftY), dim3(kFftBlock),
+    hipLaunchKernelGGL((kg_fft_lines<R, 1, 1, LOOP>), grid(kGkFftY), dim3(kFftBlock),
                        G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
-    hipLaunchKernelGGL((kg_fft_x<R, true>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
+    hipLaunchKernelGGL((kg_fft_x<R, true, LOOP>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
                        st, D, io, G.map[kGkFftX]);
   }
   if (run(kGkPlane))
-    hipLaunchKernelGGL((kg_fft_xy<R, true>), grid(kGkPlane), dim3(kPlaneBlock),
+    hipLaunchKernelGGL((kg_fft_xy<R, true, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
                        G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
   if (fork) {
     const hipError_t e = hipStreamWaitEvent(st, G.join_ev, 0);
@@ -2576,23 +2710,28 @@ hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
     const hipError_t e = launch_pair_spmm(G.pm, sizeof(R) == sizeof(float), st);
     if (e != hipSuccess) return e;
   } else if (run(kGkPairs)) {
-    hipLaunchKernelGGL(kg_pair_spmv<R>, grid(kGkPairs), dim3(256), 0, st, D, io,
+    hipLaunchKernelGGL((kg_pair_spmv<R, LOOP>), grid(kGkPairs), dim3(256), 0, st, D, io,
                        G.map[kGkPairs]);
   }
   if (run(kGkGatherTile)) {
 #define EW_G_GATHER(PP)                                                                   \
-  hipLaunchKernelGGL((kg_gather_tile<R, PP>), grid(kGkGatherTile), dim3(256),            \
+  hipLaunchKernelGGL((kg_gather_tile<R, PP, LOOP>), grid(kGkGatherTile), dim3(256),            \
                      G.smem[kGkGatherTile], st, D, io, G.map[kGkGatherTile])
     EW_ORDER_SWITCH(EW_G_GATHER)
 #undef EW_G_GATHER
   }
   if (run(kGkGatherAtom))
-    hipLaunchKernelGGL(kg_gather_combine<R>, grid(kGkGatherAtom), dim3(256), 0, st, D, io,
+    hipLaunchKernelGGL((kg_gather_combine<R, LOOP>), grid(kGkGatherAtom), dim3(256), 0, st, D, io,
                        G.map[kGkGatherAtom]);
-  hipLaunchKernelGGL(kg_energy_final, grid(kGkEnergy), dim3(256), 0, st, D, io,
+  hipLaunchKernelGGL(kg_energy_final<LOOP>, grid(kGkEnergy), dim3(256), 0, st, D, io,
                      G.map[kGkEnergy]);
 #undef EW_ORDER_SWITCH
   return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_group(const GroupLaunch& G, hipStream_t st) {
+  return G.persistent ? launch_group_t<R, true>(G, st) : launch_group_t<R, false>(G, st);
 }
 
 template hipError_t launch_group<double>(const GroupLaunch&, hipStream_t);
@@ -2847,9 +2986,9 @@ static size_t fft_smem(const FftPlan& pl) {
   return sizeof(cx<R>) * ((size_t)pl.lines * pl.pitch + pl.n);
 }
 
-template <typename R>
-hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
-                       hipEvent_t ev_region) {
+template <typename R, bool CO>
+static hipError_t launch_spme_t(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
+                                hipEvent_t ev_region) {
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], st);
   };
@@ -2857,7 +2996,7 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
   // co-scheduled pair sum: rows split over the spreading launch (a share in percent) and the
   // FFT launches of the full transform (the rest, equally), each launch running its rows on
   // pair blocks appended after its own blocks; waves of pair blocks per launch: kCoWaves
-  const bool co_on = L.pair.r1 > 0 && !L.forward_only;
+  const bool co_on = CO && L.pair.r1 > 0 && !L.forward_only;
   const int nfft = L.plane ? 3 : 5;
   const bool co_spread = co_on && L.region.region_stride > 0 && L.pair_spread_share > 0;
   int co_next = 0;
@@ -2907,7 +3046,7 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
     dim3 rgrid;
     const PairCo rco = co_for(nblk, 256, true, &rgrid);
 #define EW_REGION(PP)                                                                         \
-  hipLaunchKernelGGL((k_spread_region<R, PP>), rgrid, dim3(256), smem, st, L.k0, th, L.q,      \
+  hipLaunchKernelGGL((k_spread_region<R, PP, CO>), rgrid, dim3(256), smem, st, L.k0, th, L.q,      \
                      L.tile_start, rg, scr, L.qperm, grid, rco)
     switch (rg.order) {
       case 4: EW_REGION(4); break;
@@ -2955,64 +3094,78 @@ hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev,
   if (L.plane) {
     // x and y in one pass per plane (the y phase event follows immediately)
     const PairCo c1 = co_for(K3, kPlaneBlock, false, &g1);
-    hipLaunchKernelGGL(k_fft_xy_r2c<R>, g1, dim3(kPlaneBlock), psmem, st, grid, cg, L.px,
+    hipLaunchKernelGGL((k_fft_xy_r2c<R, CO>), g1, dim3(kPlaneBlock), psmem, st, grid, cg, L.px,
                        L.py, tw1, tw2, c1);
     mark(1);
   } else {
     const PairCo c1 = co_for(gx, kFftBlock, false, &g1);
-    hipLaunchKernelGGL(k_fft_x_r2c<R>, g1, dim3(kFftBlock), fft_smem<R>(L.px), st, grid,
+    hipLaunchKernelGGL((k_fft_x_r2c<R, CO>), g1, dim3(kFftBlock), fft_smem<R>(L.px), st, grid,
                        cg, nrows, Kh, L.px, tw1, c1);
     mark(1);
     const PairCo c2 = co_for((ny + L.py.lines - 1) / L.py.lines, kFftBlock, false, &g1);
-    hipLaunchKernelGGL((k_fft_lines<R, 0>), g1, dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny,
+    hipLaunchKernelGGL((k_fft_lines<R, 0, CO>), g1, dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny,
                        Kh, make_fastdiv(Kh), (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf,
                        c2);
   }
   mark(2);
   const int nz = Kh * K2;
   if (L.forward_only) {
-    hipLaunchKernelGGL((k_fft_lines<R, 0>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
+    hipLaunchKernelGGL((k_fft_lines<R, 0, CO>), dim3((nz + L.pz.lines - 1) / L.pz.lines),
                        dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz, Kh * K2,
                        make_fastdiv(Kh * K2), 0LL, (long long)Kh * K2, L.pz, tw3, L.inf,
                        PairCo{});
     return hipGetLastError();
   }
   const PairCo c3 = co_for((nz + L.pz.lines - 1) / L.pz.lines, kFftBlock, false, &g1);
-  hipLaunchKernelGGL((k_fft_lines<R, 2>), g1, dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz,
+  hipLaunchKernelGGL((k_fft_lines<R, 2, CO>), g1, dim3(kFftBlock), fft_smem<R>(L.pz), st, cg, nz,
                      Kh * K2, make_fastdiv(Kh * K2), 0LL, (long long)Kh * K2, L.pz, tw3, L.inf,
                      c3);
   mark(3);
   if (L.plane) {
     mark(4);
     const PairCo c4 = co_for(K3, kPlaneBlock, false, &g1);
-    hipLaunchKernelGGL(k_fft_xy_c2r<R>, g1, dim3(kPlaneBlock), psmem, st, cg, grid, L.px,
+    hipLaunchKernelGGL((k_fft_xy_c2r<R, CO>), g1, dim3(kPlaneBlock), psmem, st, cg, grid, L.px,
                        L.py, tw1, tw2, c4);
   } else {
     const PairCo c4 = co_for((ny + L.py.lines - 1) / L.py.lines, kFftBlock, false, &g1);
-    hipLaunchKernelGGL((k_fft_lines<R, 1>), g1, dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny,
+    hipLaunchKernelGGL((k_fft_lines<R, 1, CO>), g1, dim3(kFftBlock), fft_smem<R>(L.py), st, cg, ny,
                        Kh, make_fastdiv(Kh), (long long)K2 * Kh, (long long)Kh, L.py, tw2, L.inf,
                        c4);
     mark(4);
     const PairCo c5 = co_for(gx, kFftBlock, false, &g1);
-    hipLaunchKernelGGL(k_fft_x_c2r<R>, g1, dim3(kFftBlock), fft_smem<R>(L.px), st, cg,
+    hipLaunchKernelGGL((k_fft_x_c2r<R, CO>), g1, dim3(kFftBlock), fft_smem<R>(L.px), st, cg,
                        grid, nrows, Kh, L.px, tw1, c5);
   }
   mark(5);
   return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_spme(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev, hipEvent_t ev_region) {
+  return L.pair.r1 > 0 ? launch_spme_t<R, true>(L, st, ev, ev_region)
+                       : launch_spme_t<R, false>(L, st, ev, ev_region);
+}
+
 hipError_t prepare_plane_kernels() {
   static hipError_t done = [] {
     const int b = (int)kPlaneMaxLds;
     const void* fns[] = {
-        reinterpret_cast<const void*>(&k_fft_xy_r2c<double>),
-        reinterpret_cast<const void*>(&k_fft_xy_r2c<float>),
-        reinterpret_cast<const void*>(&k_fft_xy_c2r<double>),
-        reinterpret_cast<const void*>(&k_fft_xy_c2r<float>),
-        reinterpret_cast<const void*>(&kg_fft_xy<double, false>),
-        reinterpret_cast<const void*>(&kg_fft_xy<float, false>),
-        reinterpret_cast<const void*>(&kg_fft_xy<double, true>),
-        reinterpret_cast<const void*>(&kg_fft_xy<float, true>)};
+        reinterpret_cast<const void*>(&k_fft_xy_r2c<double, false>),
+        reinterpret_cast<const void*>(&k_fft_xy_r2c<float, false>),
+        reinterpret_cast<const void*>(&k_fft_xy_c2r<double, false>),
+        reinterpret_cast<const void*>(&k_fft_xy_c2r<float, false>),
+        reinterpret_cast<const void*>(&k_fft_xy_r2c<double, true>),
+        reinterpret_cast<const void*>(&k_fft_xy_r2c<float, true>),
+        reinterpret_cast<const void*>(&k_fft_xy_c2r<double, true>),
+        reinterpret_cast<const void*>(&k_fft_xy_c2r<float, true>),
+        reinterpret_cast<const void*>(&kg_fft_xy<double, false, false>),
+        reinterpret_cast<const void*>(&kg_fft_xy<float, false, false>),
+        reinterpret_cast<const void*>(&kg_fft_xy<double, true, false>),
+        reinterpret_cast<const void*>(&kg_fft_xy<float, true, false>),
+        reinterpret_cast<const void*>(&kg_fft_xy<double, false, true>),
+        reinterpret_cast<const void*>(&kg_fft_xy<float, false, true>),
+        reinterpret_cast<const void*>(&kg_fft_xy<double, true, true>),
+        reinterpret_cast<const void*>(&kg_fft_xy<float, true, true>)};
     for (const void* f : fns) {
       const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b);
       if (e != hipSuccess) return e;
@@ -3024,10 +3177,16 @@ hipError_t prepare_plane_kernels() {
 
 template <typename R, int P>
 static hipError_t region_attr(int bytes) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_region<R, P>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_region<R, P, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_region<R, P>),
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_region<R, P, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_region<R, P, false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_region<R, P, true>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 

@@ -283,7 +283,13 @@ struct GroupMap {
   const int* prefix;
   const int* slot;
   int count;
+  int total;  // work items (blocks) of the launch: prefix[count]
 };
+
+// Persistent grouped launches (GroupLaunch::persistent): a kind with more work items than
+// this many workgroups per CU (256 CUs) runs on a bounded grid whose workgroups walk the items.
+constexpr int kGroupWgPerCu = 4;
+constexpr int kNumCus = 256;
 
 enum GroupKind {
   kGkCharges = 0,
@@ -336,6 +342,7 @@ struct GroupLaunch {
   GroupMap map[kGroupKinds];
   int blocks[kGroupKinds];
   size_t smem[kGroupKinds];  // dynamic LDS: the largest of the structures in the kind
+  int persistent;            // 1: bounded grids walking the work items (kGroupWgPerCu per CU)
 };
 
 template <typename R>

@@ -156,7 +156,7 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
 }
 
 GroupLaunch launch_of(const ew_pool* P, const Staged& S, const void* dev) {
-  return group_launch_of(S, P->desc_dev, dev, P->order);
+  return group_launch_of(S, P->desc_dev, dev, P->order, P->opt.group_grid == 1);
 }
 
 int check_ids(ew_pool* P, int count, const int* ids) {
@@ -324,9 +324,9 @@ extern "C" int ew_pool_create(const ew_options* opts, long long capacity_bytes, 
   P->fp32 = o.precision == EW_PRECISION_MIXED;
   P->capacity = capacity_bytes;
   if (hipStreamCreate(&P->stream) != hipSuccess || hipStreamCreate(&P->aux) != hipSuccess ||
-      hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&P->join_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&P->last_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&P->fork_ev, kSyncEventFlags) != hipSuccess ||
+      hipEventCreateWithFlags(&P->join_ev, kSyncEventFlags) != hipSuccess ||
+      hipEventCreateWithFlags(&P->last_ev, kSyncEventFlags) != hipSuccess) {
     (void)hipGetLastError();
     if (P->last_ev) (void)hipEventDestroy(P->last_ev);
     if (P->join_ev) (void)hipEventDestroy(P->join_ev);

@@ -277,8 +277,8 @@ extern "C" int ew_server_create(ew_pool* pool, const ew_server_options* opts, ew
     b.server = S;
     const bool ok = hipStreamCreate(&b.stream) == hipSuccess &&
                     hipStreamCreate(&b.aux) == hipSuccess &&
-                    hipEventCreateWithFlags(&b.fork_ev, hipEventDisableTiming) == hipSuccess &&
-                    hipEventCreateWithFlags(&b.join_ev, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&b.fork_ev, kSyncEventFlags) == hipSuccess &&
+                    hipEventCreateWithFlags(&b.join_ev, kSyncEventFlags) == hipSuccess &&
                     hipHostMalloc(&b.h_dq, sizeof(double) * big) == hipSuccess &&
                     hipHostMalloc(&b.h_shift, sizeof(double) * big) == hipSuccess &&
                     hipHostMalloc(&b.h_energy, sizeof(double) * S->max_batch) == hipSuccess &&

@@ -103,7 +103,7 @@ class Options(C.Structure):
                 ("grid", C.c_int * 3), ("order", C.c_int), ("precision", C.c_int),
                 ("device", C.c_int), ("alpha_rule", C.c_int), ("profile", C.c_int),
                 ("eager", C.c_int), ("charged", C.c_int), ("fft_mode", C.c_int),
-                ("pair_storage", C.c_int), ("reserved", C.c_int), ("k_cut", C.c_double),
+                ("pair_storage", C.c_int), ("group_grid", C.c_int), ("k_cut", C.c_double),
                 ("tune", C.c_int), ("tuning", Tuning)]
 
 
@@ -250,7 +250,7 @@ def lib() -> C.CDLL:
 def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precision="fp64",
                  device=0, alpha_rule="fftewald", profile=False, eager=False,
                  charged=False, fft_mode=0, pair_storage=0, k_cut=None, tune="rules",
-                 tuning=None) -> Options:
+                 tuning=None, group_grid=0) -> Options:
     o = Options()
     o.tol = tol or 0.0
     o.alpha = alpha or 0.0
@@ -271,6 +271,7 @@ def make_options(tol=None, alpha=None, r_cut=None, grid=None, order=None, precis
     o.charged = 1 if charged else 0
     o.fft_mode = int(fft_mode)
     o.pair_storage = int(pair_storage)
+    o.group_grid = int(group_grid)
     o.tune = TUNE[tune] if isinstance(tune, str) else int(tune)
     if tuning is not None:
         o.tuning = tuning if isinstance(tuning, Tuning) else Tuning.from_dict(tuning)

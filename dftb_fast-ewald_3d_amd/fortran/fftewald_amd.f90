@@ -61,7 +61,7 @@ module fftewald_amd
       integer(c_int) :: charged = 0
       integer(c_int) :: fft_mode = 0
       integer(c_int) :: pair_storage = 0
-      integer(c_int) :: reserved = 0
+      integer(c_int) :: group_grid = 0
       real(c_double) :: k_cut = 0.0_c_double
       integer(c_int) :: tune = 0
       type(ew_tuning) :: tuning = ew_tuning()

@@ -106,7 +106,9 @@ typedef struct ew_options {
   int pair_storage; /* 0 (default): the row list (every linked pair in both rows, the fastest
                        pair sum); 1: each linked pair stored once (symmetric, tile-blocked):
                        half the pair memory, about twice the pair-sum time                    */
-  int reserved;
+  int group_grid;  /* grouped launches (pools, several charge vectors): 0 (default): one
+                      workgroup per work item (structure, block); 1: persistent grids of at
+                      most 4 workgroups per CU that walk the table of work items            */
   double k_cut;    /* Fourier cutoff override (k_cut_in), Bohr^-1: replaces the cutoff of the
                       Fourier budget (fe/fft/ewald_fft_3d_parameters.f90:127-130) and the mesh
                       follows from it as the modes follow from k_cut in modes_along_axis
