@@ -66,7 +66,7 @@ int smooth235(int n);
 void bspline_moduli(int K, int order, double* bsq);
 
 // radix plan for a length (radices 2, 3, 4, 5, 8) in the pass order of `variant` (0..3)
-constexpr int kRadixVariants = 4;
+constexpr int kRadixVariants = 5;
 int fft_plan(int n, int* radices, int variant = 0);
 
 // Division of a non-negative 31-bit integer by a fixed divisor d >= 1 as a multiply-high and a

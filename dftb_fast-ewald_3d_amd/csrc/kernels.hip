@@ -126,6 +126,41 @@ struct Dft<8, R> {
   }
 };
 
+template <typename R>
+struct Dft<16, R> {
+  // 4 x 4: DFT-4 over the stride-4 subsequences, twiddles W16^(n2 k1), DFT-4 across them
+  __device__ static void run(cx<R>* v, R s) {
+    const R c1 = R(0.923879532511286756128183189396788933);  // cos(pi/8)
+    const R s1 = R(0.382683432365089771728459984030398866);  // sin(pi/8)
+    const R c2 = R(0.707106781186547524400844362104849039);  // cos(pi/4)
+    cx<R> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cx<R> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Dft<4, R>::run(t, s);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+    // W16^m = cos(pi m / 8) + i s sin(pi m / 8) for m = n2 k1
+    a[1][1] = cmul(a[1][1], cx<R>{c1, s * s1});
+    a[1][2] = cmul(a[1][2], cx<R>{c2, s * c2});
+    a[1][3] = cmul(a[1][3], cx<R>{s1, s * c1});
+    a[2][1] = cmul(a[2][1], cx<R>{c2, s * c2});
+    a[2][2] = cmuli(a[2][2], s);
+    a[2][3] = cmul(a[2][3], cx<R>{-c2, s * c2});
+    a[3][1] = cmul(a[3][1], cx<R>{s1, s * c1});
+    a[3][2] = cmul(a[3][2], cx<R>{-c2, s * c2});
+    a[3][3] = cmul(a[3][3], cx<R>{-c1, -s * s1});
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cx<R> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Dft<4, R>::run(t, s);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
 // One in-place Stockham pass over `nl` lines of length n (LDS pitch P).
 // tw[m] = exp(-2 pi i m / n); the inverse uses its conjugate.
 // LDS slot of element i of a line: one pad slot after every eight elements, so that the
@@ -223,7 +258,10 @@ __device__ void fft_lds(cx<R>* s, const FftPlan& pl, int nl, const cx<R>* tw, R 
   const cx<R>* t = tw;
   for (int i = 0; i < pl.npass; ++i) {
     const int r = pl.radix[i];
-    if (r == 8)
+    if (r == 16)
+      stockham_pass<16, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
+                                           pl.nsd[i]);
+    else if (r == 8)
       stockham_pass<8, R, BLOCK, EPT>(s, pl.n, nl, pl.pitch, Ns, t, dir, pl.nbd[i],
                                           pl.nsd[i]);
     else if (r == 4)
@@ -1039,6 +1077,47 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
       const int ta = (u3 * g.nt[1] + u2) * g.nt[0];
       const int a0 = tile_start[ta + u1a], a1 = tile_start[ta + u1b];
       const int nitems = (a1 - a0) * p;  // one (atom, z plane) per item
+      if (P > 0) {
+        // kSpreadBatch items per thread at a time: all their loads (stencil origin, weights,
+        // charge) are issued before the first LDS atomic, so a batch costs one memory latency
+        constexpr int PP = P > 0 ? P : 1;
+        constexpr int NB = kSpreadBatch;
+        for (int base = threadIdx.x; base < nitems; base += NB * blockDim.x) {
+          int4 kk[NB];
+          double c3[NB];
+          R w1[NB][PP], w2[NB][PP];
+          int o3v[NB];
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            const int it = base + u * (int)blockDim.x;
+            const bool ok = it < nitems;
+            const int a = ok ? a0 + it / PP : a0, o3 = ok ? it % PP : 0;
+            o3v[u] = ok ? o3 : -1;
+            kk[u] = k0[a];
+            const R* th = theta + (size_t)a * 3 * PP;
+#pragma unroll
+            for (int o = 0; o < PP; ++o) {
+              w1[u][o] = th[o];
+              w2[u][o] = th[PP + o];
+            }
+            c3[u] = (qperm ? q[qperm[a]] : q[a]) * double(th[2 * PP + o3]);
+          }
+#pragma unroll
+          for (int u = 0; u < NB; ++u) {
+            if (o3v[u] < 0) continue;
+            double* plane = reg + ((size_t)(kk[u].z - lo3 + o3v[u]) * RY + (kk[u].y - lo2)) * RX +
+                            (kk[u].x - lo1);
+#pragma unroll
+            for (int o2 = 0; o2 < PP; ++o2) {
+              const double c = c3[u] * double(w2[u][o2]);
+#pragma unroll
+              for (int o1 = 0; o1 < PP; ++o1)
+                atomicAdd(plane + o2 * RX + o1, c * double(w1[u][o1]));
+            }
+          }
+        }
+        continue;
+      }
       for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
         const int a = a0 + it / p, o3 = it % p;
         const int4 kk = k0[a];

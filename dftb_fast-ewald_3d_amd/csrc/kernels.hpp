@@ -160,6 +160,7 @@ struct RegionArgs {
 // LDS budget of a block copy: the fixed rule keeps copies within 60 KiB (several workgroups per
 // CU); tuned shapes may use up to the gfx950 workgroup limit
 constexpr size_t kRegionMaxBytes = 60 * 1024;
+constexpr int kSpreadBatch = 3;  // (atom, z plane) items whose loads a thread issues together
 constexpr size_t kRegionMaxLds = 160 * 1024;
 // Raises the dynamic LDS limit of the region spreading kernels to kRegionMaxLds (once).
 hipError_t prepare_region_kernels();

@@ -288,6 +288,7 @@ FastDiv make_fastdiv(int d) {
 // Radix passes of a length n = 2^a 3^b 5^c in the order of `variant`:
 //   0: 8s, then a 4 or a 2, then 3s, 5s      1: 4s, then a 2, then 3s, 5s
 //   2: the leftover 2 or 4 first, then 8s, 3s, 5s      3: 3s and 5s first, then 8s, 4, 2
+//   4: 16s, then an 8, 4 or 2, then 3s, 5s
 // Every order is a valid Stockham plan (the twiddle tables follow the same order).
 int fft_plan(int n, int* radices, int variant) {
   int a = 0, b = 0, c = 0, r = n;
@@ -296,7 +297,10 @@ int fft_plan(int n, int* radices, int variant) {
   while (r > 0 && r % 5 == 0) { ++c; r /= 5; }
   if (r != 1) return -1;
   std::vector<int> pw;  // powers of two
-  if (variant == 1) {
+  if (variant == 4) {
+    for (int i = 0; i < a / 4; ++i) pw.push_back(16);
+    if (a % 4) pw.push_back(1 << (a % 4));
+  } else if (variant == 1) {
     for (int i = 0; i < a / 2; ++i) pw.push_back(4);
     if (a % 2) pw.push_back(2);
   } else {

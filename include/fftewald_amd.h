@@ -63,7 +63,8 @@ typedef struct ew_tuning {
   int fft_lines[3];     /* lines per workgroup of the x (pairs of real rows), y and z passes   */
   int fft_radix[3];     /* radix order per axis: 0 = 8s, 4, 2, then 3s, 5s; 1 = 4s, 2, 3s, 5s;
                            2 = the leftover power of two first, then 8s, 3s, 5s;
-                           3 = 3s and 5s first, then 8s, 4, 2                                 */
+                           3 = 3s and 5s first, then 8s, 4, 2; 4 = 16s, then 8, 4 or 2, then
+                           3s, 5s                                                           */
   int pair_cosched;     /* 1: the real-space pair sum runs on extra workgroups of the spreading
                            and FFT launches, on the evaluation stream; 0: on an auxiliary
                            stream beside them (fork and join by events)                       */

@@ -42,8 +42,8 @@ def _shapes(base):
     for planes in (0, 1):
         for lines in ((8, 2, 4), (16, 8, 16), (32, 16, 2)):
             out.append(dict(base, fft_planes=planes, fft_lines=lines))
-    for r in range(4):
-        out.append(dict(base, fft_radix=(r, (r + 1) % 4, (r + 2) % 4)))
+    for r in range(5):
+        out.append(dict(base, fft_radix=(r, (r + 1) % 5, (r + 2) % 5)))
     for co, share in ((1, 0), (1, 35), (1, 100), (0, 0)):
         out.append(dict(base, pair_cosched=co, pair_spread_share=share))
     return out
