@@ -253,11 +253,12 @@ def batch_bytes(prm, n, rs):
     return sum(pb.values())
 
 
-def run_batch(args, torch, ew, W):
+def batch_leg(args, torch, ew, W, steps, warmup, sequential=True):
     """Batched workload: 256 independent triclinic structures (N log-uniform in [64, 8192],
     angles U[70, 110] deg, density 0.1 A^-3, ~0.8 A grids, order 6) evaluated together by one
-    grouped launch per kernel kind; FP64 and mixed; sequential single-structure engines timed
-    beside it."""
+    grouped launch per kernel kind; FP64 and mixed; optionally the same structures through
+    single-structure engines one after another.  Returns (results per precision, structures,
+    atoms, worst mixed-vs-FP64 shift error)."""
     t = time.perf_counter()
     ws = W.batch()
     print(f"[bench] batch workload generated in {time.perf_counter() - t:.1f} s", file=sys.stderr,
@@ -283,18 +284,26 @@ def run_batch(args, torch, ew, W):
                       for i, w in zip(ids, ws))
             dptr = [d.data_ptr() for d in dq_t]
             sptr = [d.data_ptr() for d in sh_t]
-            for _ in range(args.warmup):
+            for _ in range(warmup):
                 pool.shift_device(ids, dptr, sptr, en_t.data_ptr(), sp)
             torch.cuda.synchronize()
             t0 = torch.cuda.Event(enable_timing=True)
             t1 = torch.cuda.Event(enable_timing=True)
             t0.record(stream)
-            for _ in range(args.steps):
+            for _ in range(steps):
                 pool.shift_device(ids, dptr, sptr, en_t.data_ptr(), sp)
             t1.record(stream)
             torch.cuda.synchronize()
-            ms = t0.elapsed_time(t1) / args.steps
+            ms = t0.elapsed_time(t1) / steps
             shifts[prec] = [s.cpu().numpy() for s in sh_t]
+        res[prec] = {"ms_per_step": ms, "evals_per_s": len(ws) * 1e3 / ms,
+                     "atoms_per_s": natoms * 1e3 / ms, "geometry_s": geom_s,
+                     "algorithmic_bytes": alg, "GBs": alg / (ms * 1e-3) / 1e9}
+        print(f"[bench] batch {prec}: {ms:.3f} ms/step ({len(ws)} structures, {natoms} atoms), "
+              f"{len(ws) * 1e3 / ms:.0f} evals/s; geometry {geom_s:.1f} s", file=sys.stderr,
+              flush=True)
+        if not sequential:
+            continue
         # the same structures one after another through single-structure engines
         engines = []
         for w in ws:
@@ -305,7 +314,7 @@ def run_batch(args, torch, ew, W):
         for e, d, o in zip(engines, dq_t, sh_t):
             e.shift_device(d.data_ptr(), o.data_ptr(), 0, sp)
         torch.cuda.synchronize()
-        steps_seq = max(1, args.steps // 4)
+        steps_seq = max(1, steps // 4)
         t0.record(stream)
         for _ in range(steps_seq):
             for e, d, o in zip(engines, dq_t, sh_t):
@@ -315,15 +324,18 @@ def run_batch(args, torch, ew, W):
         ms_seq = t0.elapsed_time(t1) / steps_seq
         for e in engines:
             e.close()
-        res[prec] = {"ms_per_step": ms, "evals_per_s": len(ws) * 1e3 / ms,
-                     "atoms_per_s": natoms * 1e3 / ms, "geometry_s": geom_s,
-                     "sequential_ms_per_step": ms_seq, "grouped_over_sequential": ms_seq / ms,
-                     "algorithmic_bytes": alg, "GBs": alg / (ms * 1e-3) / 1e9}
-        print(f"[bench] batch {prec}: {ms:.3f} ms/step ({len(ws)} structures, {natoms} atoms), "
-              f"{len(ws) * 1e3 / ms:.0f} evals/s; sequential engines {ms_seq:.3f} ms/step; "
-              f"geometry {geom_s:.1f} s", file=sys.stderr, flush=True)
+        res[prec]["sequential_ms_per_step"] = ms_seq
+        res[prec]["grouped_over_sequential"] = ms_seq / ms
+        print(f"[bench] batch {prec}: sequential engines {ms_seq:.3f} ms/step", file=sys.stderr,
+              flush=True)
     err = max(float(np.abs(a - b).max() / np.abs(b).max())
               for a, b in zip(shifts["mixed"], shifts["fp64"]))
+    return res, ws, natoms, err
+
+
+def run_batch(args, torch, ew, W):
+    """The batched workload as its own JSON line (see batch_leg)."""
+    res, ws, natoms, err = batch_leg(args, torch, ew, W, args.steps, args.warmup)
     head = res[args.precision]
     line = {
         "metric": "SPME shift+energy evals/s (batch of 256 structures)",
@@ -423,10 +435,11 @@ def poisson_load(srv, ids, bufs, qps, seconds, rng):
     return offered, done[0], time.perf_counter() - t0, dropped
 
 
-def run_serving(args, torch, ew, W):
+def serving_leg(args, ew, W, sweep, seconds, precisions=("mixed", "fp64")):
     """Serving workload: 1,000 resident structures (N log-uniform in [100, 20000], built like
-    the 98,304-atom cell); SCC charge updates arrive as a Poisson stream at swept rates and are
-    served by the continuous-batching loop; throughput and p50/p99 latency, mixed and FP64."""
+    the 98,304-atom cell); SCC charge updates arrive as a Poisson stream at the rates `sweep` and
+    are served by the continuous-batching loop; throughput and p50/p99 latency per precision.
+    Returns (results per precision, structures, atoms)."""
     t = time.perf_counter()
     workers = min(16, os.cpu_count() or 1)
     ws = W.serving(workers=workers)
@@ -434,9 +447,8 @@ def run_serving(args, torch, ew, W):
           f"({workers} processes)", file=sys.stderr, flush=True)
     rng = np.random.default_rng(2026)
     natoms = sum(len(w["charges"]) for w in ws)
-    sweep = [float(x) for x in args.qps.split(",")]
     res = {}
-    for prec in ("mixed", "fp64"):
+    for prec in precisions:
         with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=ws[0]["order"],
                           precision=prec) as pool:
             t = time.perf_counter()
@@ -454,7 +466,7 @@ def run_serving(args, torch, ew, W):
                 srv.stats(reset=True)
                 for qps in sweep:
                     offered, completed, el, dropped = poisson_load(srv, ids, bufs, qps,
-                                                                   args.seconds, rng)
+                                                                   seconds, rng)
                     st = srv.stats(reset=True)
                     row = {"offered_qps": qps, "achieved_qps": round(completed / el, 1),
                            "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4),
@@ -472,6 +484,13 @@ def run_serving(args, torch, ew, W):
             best = max(ok, key=lambda r: r["achieved_qps"]) if ok else None
             res[prec] = {"sweep": rows, "sustained": best, "setup_s": round(setup, 1),
                          "resident_bytes": used}
+    return res, ws, natoms
+
+
+def run_serving(args, torch, ew, W):
+    """The serving workload as its own JSON line (see serving_leg)."""
+    res, ws, natoms = serving_leg(args, ew, W, [float(x) for x in args.qps.split(",")],
+                                  args.seconds)
     head = res[args.precision]
     best = head["sustained"]
     line = {
@@ -553,6 +572,93 @@ def run_mesh(args, torch, ew, W):
     print(json.dumps(line))
 
 
+def replicas_leg(args, torch, ew, W, w, qs, fp64_plain_ms):
+    """Several geometries of the 98,304-atom cell (the cell and copies with every atom moved by
+    N(0, 0.05 A): SCC cycles of neighbouring molecular-dynamics snapshots) resident in one pool,
+    all evaluated in one grouped launch per kernel kind; evaluations per second against the
+    plain FP64 engine (one structure, one charge vector per call)."""
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    rng = np.random.default_rng(31415)
+    geos = [w["coords"]] + [w["coords"] + rng.normal(scale=0.05 * W.AA, size=w["coords"].shape)
+                            for _ in range(args.replicas - 1)]
+    dq_t = [torch.tensor(qs[k % len(qs)], dtype=torch.float64, device=dev)
+            for k in range(args.replicas)]
+    sh_t = [torch.empty_like(d) for d in dq_t]
+    en_t = torch.empty(args.replicas, dtype=torch.float64, device=dev)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with ew.EwaldPool(tol=1e-10, r_cut=w["r_cut"], order=w["order"], precision=prec) as pool:
+            ids = [pool.add(x, w["lattice"], charges=w["charges"], grid=w["grid"]) for x in geos]
+            dptr = [d.data_ptr() for d in dq_t]
+            sptr = [d.data_ptr() for d in sh_t]
+            for _ in range(3):
+                pool.shift_device(ids, dptr, sptr, en_t.data_ptr(), sp)
+            torch.cuda.synchronize()
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            steps = max(5, args.steps // 4)
+            t0.record(stream)
+            for _ in range(steps):
+                pool.shift_device(ids, dptr, sptr, en_t.data_ptr(), sp)
+            t1.record(stream)
+            torch.cuda.synchronize()
+            ms = t0.elapsed_time(t1) / steps
+        out[prec] = {"ms_per_launch": round(ms, 5),
+                     "evals_per_s": round(args.replicas * 1e3 / ms, 1),
+                     "over_plain_fp64": round(args.replicas * fp64_plain_ms / ms, 3)}
+        print(f"[bench] {args.replicas} resident 98,304-atom geometries, {prec}: {ms:.3f} ms per "
+              f"grouped evaluation, {args.replicas * 1e3 / ms:.0f} evals/s", file=sys.stderr,
+              flush=True)
+    return {"structures": args.replicas, "note": "one grouped launch per kernel kind over "
+            "several resident geometries of the 98,304-atom cell (atoms moved by N(0, 0.05 A)); "
+            "over_plain_fp64 divides by the plain FP64 engine's one-structure rate", **out}
+
+
+def cpu_legs(w_c1, w_c2, threads):
+    """Further CPU baselines on the host cores (reported only): the reference's direct Ewald
+    (restated in the oracle) on the 512-atom NaCl cell at tol 1e-10, and the stock DFTB+ path's
+    per-SCC-iteration cost, the product of the N x N Ewald matrix with the charges
+    (coulomb.F90:1253-1308, 868; a symmetric random matrix of that size, whose cost does not
+    depend on its entries) for the 512- and 12,288-atom cells."""
+    import oracle
+    legs = {}
+    a = oracle.optimal_alpha_dftb(w_c1["lattice"], 1e-10)
+    oracle.direct_ewald(w_c1["coords"], w_c1["charges"], w_c1["lattice"], tol=1e-10, alpha=a)
+    t = time.perf_counter()
+    reps = 0
+    while reps < 3 or time.perf_counter() - t < 2.0:
+        oracle.direct_ewald(w_c1["coords"], w_c1["charges"], w_c1["lattice"], tol=1e-10, alpha=a)
+        reps += 1
+        if reps >= 10:
+            break
+    legs["direct_ewald_nacl512"] = {"s_per_eval": round((time.perf_counter() - t) / reps, 4),
+                                    "cores": threads, "kind": "port",
+                                    "sample": f"{reps} evaluations, tol 1e-10, alpha {a:.4f}"}
+    for name, w in (("nacl512", w_c1), ("water12288", w_c2)):
+        n = len(w["charges"])
+        rng = np.random.default_rng(5)
+        m = rng.standard_normal((n, n))
+        m = m + m.T
+        q = np.ascontiguousarray(w["charges"])
+        m @ q
+        t = time.perf_counter()
+        reps = 0
+        while reps < 5 or time.perf_counter() - t < 1.0:
+            m @ q
+            reps += 1
+            if reps >= 200:
+                break
+        legs[f"stock_hemv_{name}"] = {
+            "s_per_scc_iteration": round((time.perf_counter() - t) / reps, 6), "cores": threads,
+            "kind": "port", "matrix_bytes": 8 * n * n,
+            "sample": f"{reps} products of the {n} x {n} FP64 matrix with the charges (numpy)"}
+        del m
+    print(f"[bench] cpu legs: {legs}", file=sys.stderr, flush=True)
+    return legs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -584,6 +690,11 @@ def main():
     ap.add_argument("--tune", default="auto", choices=["auto", "rules"],
                     help="launch shapes of the timed engines: the autotuner (default) or the "
                          "fixed rules")
+    ap.add_argument("--replicas", type=int, default=4,
+                    help="triclinic: resident geometries of the 98,304-atom cell evaluated in "
+                         "one grouped launch (0: skip)")
+    ap.add_argument("--no-workload-legs", action="store_true",
+                    help="triclinic: skip the compact batch and serving legs")
     ap.add_argument("--headline-only", action="store_true",
                     help="triclinic: only the timed, latency and traced passes (every chain "
                          "kernel then runs in the timed launch shape, for kernel-trace "
@@ -593,6 +704,8 @@ def main():
         args.vectors = ""
         args.external = 0
         args.no_cpu_baseline = True
+        args.replicas = 0
+        args.no_workload_legs = True
     args.vectors = [int(v) for v in args.vectors.split(",") if v.strip()]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
@@ -769,6 +882,33 @@ def main():
         print(f"[bench] {prec}: {ms:.4f} ms/eval, {1e3 / ms:.1f} evals/s, geometry {geom_s:.2f} s,"
               f" p50 {p50:.3f} ms p99 {p99:.3f} ms", file=sys.stderr, flush=True)
 
+    replicas = None
+    if args.replicas > 0:
+        replicas = replicas_leg(args, torch, ew, W, w, qs, res["fp64"]["ms_per_eval"])
+    workload_legs = None
+    if not args.no_workload_legs:
+        bres, bws, batoms, berr = batch_leg(args, torch, ew, W, max(5, args.steps // 5), 3,
+                                            sequential=False)
+        sres, sws, satoms = serving_leg(args, ew, W, [20000.0, 60000.0, 100000.0, 120000.0],
+                                        1.0, precisions=("mixed",))
+        sbest = sres["mixed"]["sustained"]
+        workload_legs = {
+            "batch_256": {"structures": len(bws), "atoms": batoms,
+                          "fp64_evals_per_s": round(bres["fp64"]["evals_per_s"], 1),
+                          "mixed_evals_per_s": round(bres["mixed"]["evals_per_s"], 1),
+                          "ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in bres.items()},
+                          "mixed_vs_fp64_shift_max": berr,
+                          "roofline_whole_evaluation": {
+                              k: {"achieved_GBs": round(v["GBs"], 1),
+                                  "frac": round(v["GBs"] / HBM_PEAK_GBS, 4),
+                                  "algorithmic_bytes": v["algorithmic_bytes"]}
+                              for k, v in bres.items()}},
+            "serving_1000": {"structures": len(sws), "atoms": satoms, "precision": "mixed",
+                             "sweep": sres["mixed"]["sweep"],
+                             "sustained_qps_at_p99_2ms": sbest["achieved_qps"] if sbest else None,
+                             "p50_ms": sbest["p50_ms"] if sbest else None,
+                             "p99_ms": sbest["p99_ms"] if sbest else None,
+                             "seconds_per_rate": 1.0}}
     s64, e64 = shifts["fp64"]
     s32, e32 = shifts["mixed"]
     err_shift = float(np.abs(s32 - s64).max() / np.abs(s64).max())
@@ -788,6 +928,7 @@ def main():
         cpu = cpu_baseline(w)
         print(f"[bench] cpu baseline {cpu['value']:.4f} evals/s on {cpu['cores']} threads",
               file=sys.stderr, flush=True)
+        cpu["other_legs"] = cpu_legs(W.nacl(), W.water(), cpu["cores"])
     line = {
         "metric": "SPME shift+energy evals/s (98k atoms), p99 latency; mixed vs FP64 @1e-6 rel",
         "value": round(head["evals_per_s"], 2),
@@ -828,6 +969,8 @@ def main():
                                       "the reciprocal energy is amplified by that cancellation; "
                                       "the 1e-6 gate of the metric is on the shifts"},
         "critical_path": {k: v["critical_path"] for k, v in res.items()},
+        "replicas_98k": replicas,
+        "workload_legs": workload_legs,
         "launch_shapes": {k: {"tuned": v["tuning"], "rules": v["rules"],
                               "rules_ms_per_eval": round(v["rules_ms"], 5),
                               "tuned_ms_per_eval": round(v["ms_per_eval"], 5)}

@@ -104,7 +104,10 @@ inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bo
   std::memcpy(reinterpret_cast<char*>(S->words.data()) + S->io_offset, io,
               sizeof(IoSlot) * (size_t)count);
   for (int k = 0; k < kGroupKinds; ++k)
-    if (S->smem[k] > (k == kGkPlane ? kPlaneMaxLds : 65536)) return false;
+    if (S->smem[k] > (k == kGkPlane    ? kPlaneMaxLds
+                      : k == kGkRegion ? kRegionMaxLds
+                                       : 65536))
+      return false;
   return true;
 }
 

@@ -2744,7 +2744,11 @@ static hipError_t launch_group_t(const GroupLaunch& G, hipStream_t st) {
     default: MACRO(0); break;  \
   }
   if (run(kGkRegion)) {
-#define EW_G_REGION(PP)                                                                  \
+    if (G.smem[kGkRegion] > 65536) {
+      const hipError_t e = prepare_region_kernels();
+      if (e != hipSuccess) return e;
+    }
+#define EW_G_REGION(PP)                                                                \
   hipLaunchKernelGGL((kg_spread_region<R, PP, LOOP>), grid(kGkRegion), dim3(256),             \
                      G.smem[kGkRegion], st, D, io, G.map[kGkRegion])
     EW_ORDER_SWITCH(EW_G_REGION)

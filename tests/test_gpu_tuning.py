@@ -77,7 +77,14 @@ def test_every_shape_matches_the_rules_and_the_oracle(case, oracle_mod):
                 continue
             got = e.tuning()
             s, en = e.shift(w["charges"])
+            # the multi-vector grouped launch over the same shapes (its spreading blocks may
+            # take more than 64 KiB of LDS, as the single-vector chain's may)
+            sm, em = e.shift_multi(np.stack([w["charges"], 0.5 * w["charges"]]))
         assert rel_max(s, ref) < 1e-13, t
+        assert rel_max(sm[0], ref) < 1e-13, t
+        assert rel_max(sm[1], 0.5 * ref) < 1e-13, t
+        assert em[0] == pytest.approx(eref, rel=1e-13)
+        assert em[1] == pytest.approx(0.25 * eref, rel=1e-13)
         assert en == pytest.approx(eref, rel=1e-13)
         tried += 1
         assert got["fft_radix"] == tuple(t["fft_radix"])
