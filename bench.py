@@ -91,26 +91,34 @@ CHAIN = ("spread_region", "spread_assemble", "fft_x_r2c", "fft_y", "fft_z_influe
          "fft_y_inverse", "fft_x_c2r", "join_wait", "gather_combine")
 
 
-def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0):
-    """The kernel(s) that carry a span of ew_trace_times, as rocprofv3 names them."""
+def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0, tuning=None):
+    """The kernel that carries a span of ew_trace_times, as rocprofv3 names it (template
+    arguments included: the chain kernels carry the pair sum's co-scheduling flag)."""
     r = "float" if prec == "mixed" else "double"
     i = "short" if index_bytes == 2 else "int"
+    tuning = tuning or {}
+    co = "true" if tuning.get("pair_cosched", 0) else "false"
+    direct = int(tuning.get("spread_direct", 0))
+    spread = {0: f"void ewamd::k_spread_region<{r}, {order}, {co}>",
+              1: f"void ewamd::k_spread_region<{r}, {order}, {co}>",
+              2: f"void ewamd::k_spread_owned<{r}, {order}, {co}>"}[direct]
     return {"charges": "ewamd::k_gather_charges",
             "pair_sum": f"void ewamd::k_pair_spmv_stream<{r}, {i}>" if pair_mode != 2 else
             f"void ewamd::k_pair_half<{r}>",
-            "spread_region": f"void ewamd::k_spread_region<{r}, {order}>",
-            "spread_assemble": f"void ewamd::k_spread_assemble<{r}>",
-            "fft_x_r2c": f"void ewamd::k_fft_xy_r2c<{r}>" if planes else
-            f"void ewamd::k_fft_x_r2c<{r}>",
-            "fft_y": None if planes else f"void ewamd::k_fft_lines<{r}, 0>",
-            "fft_z_influence": f"void ewamd::k_fft_lines<{r}, 2>",
-            "fft_y_inverse": None if planes else f"void ewamd::k_fft_lines<{r}, 1>",
-            "fft_x_c2r": f"void ewamd::k_fft_xy_c2r<{r}>" if planes else
-            f"void ewamd::k_fft_x_c2r<{r}>",
+            "spread_region": spread,
+            "spread_assemble": {0: f"void ewamd::k_spread_assemble<{r}>",
+                                1: f"void ewamd::k_clear<{r}>", 2: None}[direct],
+            "fft_x_r2c": f"void ewamd::k_fft_xy_r2c<{r}, {co}>" if planes else
+            f"void ewamd::k_fft_x_r2c<{r}, {co}>",
+            "fft_y": None if planes else f"void ewamd::k_fft_lines<{r}, 0, {co}>",
+            "fft_z_influence": f"void ewamd::k_fft_lines<{r}, 2, {co}>",
+            "fft_y_inverse": None if planes else f"void ewamd::k_fft_lines<{r}, 1, {co}>",
+            "fft_x_c2r": f"void ewamd::k_fft_xy_c2r<{r}, {co}>" if planes else
+            f"void ewamd::k_fft_x_c2r<{r}, {co}>",
             "gather_combine": f"void ewamd::k_gather_tile<{r}, {order}>"}.get(span)
 
 
-def critical_path(trace, calls, tb, prec, prm, step_ms):
+def critical_path(trace, calls, tb, prec, prm, step_ms, tuning=None):
     """The evaluation's critical path from its traced spans: the chain on the evaluation stream
     (spreading, FFTs, the wait for the pair sum, the gather) and, beside it, the charge gather
     and pair sum.  The pair-sum branch is on the critical path when the chain waits for it
@@ -125,7 +133,7 @@ def critical_path(trace, calls, tb, prec, prm, step_ms):
         b = tb[k]
         rows.append({"span": k, "beside_chain": k in ("charges", "pair_sum"),
                      "kernel": trace_symbol(k, prec, prm["index_bytes"], prm["order"], planes,
-                                            prm["pair_mode"]),
+                                            prm["pair_mode"], tuning),
                      "us": round(us[k], 2), "algorithmic_bytes": b,
                      "GBs": round(b / (us[k] * 1e-6) / 1e9, 1) if b and us[k] > 0 else None,
                      "frac": round(b / (us[k] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
@@ -848,7 +856,7 @@ def main():
         planes = bool(prm.get("fft_planes", 0))
         tb = trace_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
                          prm["index_bytes"], False, planes, trace["spread_region"] > 0, tuning)
-        crit = critical_path(trace, tcalls, tb, prec, prm, ms)
+        crit = critical_path(trace, tcalls, tb, prec, prm, ms, tuning)
         print(f"[bench] {prec} critical path: chain {crit['chain_us']:.1f} us "
               f"({crit['chain_frac']:.3f} of HBM peak), dominant {crit['dominant']['span']} "
               f"{crit['dominant']['us']:.1f} us", file=sys.stderr, flush=True)

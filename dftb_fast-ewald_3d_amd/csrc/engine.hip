@@ -486,10 +486,23 @@ static bool region_shape(const ew_handle* h, const int* f, size_t max_bytes, Reg
   return true;
 }
 
-// Direct write-out of the spreading blocks (device atomics into the cleared mesh) is not the
-// fixed rule: on the 98,304-atom cell at 128^3 it measured 3 % (mixed) to 6 % (FP64) slower
-// than block copies plus assembly; the autotuner chooses per structure.
-constexpr int kDirectRule = 0;
+// Owned blocks can take a block shape if the block copy and the item list fit the LDS, the tile
+// runs fit the kernel's tables and an item (atom << 4 | plane) fits 31 bits.
+static bool owned_fits(const ew_handle* h, const RegionArgs& rg) {
+  int tcm[3];
+  for (int a = 0; a < 3; ++a) {
+    const int T = h->T[a];
+    tcm[a] = std::min(h->nt[a], (rg.S[a] + h->prm.order - 1 + T - 1) / T + 1);
+  }
+  return owned_smem(rg.S) + kOwnedStatic <= kRegionMaxLds && 2 * tcm[1] * tcm[2] <= kOwnedMaxRuns &&
+         h->prm.order <= 15 && h->n < (1 << 27);
+}
+
+// The spreading's write-out by the fixed rule: owned blocks (on the 98,304-atom cell at 128^3,
+// 0.211 against 0.224 ms per FP64 evaluation with block copies plus assembly, and 0.160
+// against 0.166 ms mixed; direct write-out of the copies with device atomics measured between
+// the two); the autotuner chooses per structure.
+constexpr int kDirectRule = 2;
 // The pair sum beside the chain on the auxiliary stream by the fixed rule (co-scheduling it in
 // the chain's launches measured slower on the 98,304-atom cell: 0.226 against 0.189 ms mixed;
 // faster on the 12,288-atom water box: 0.096 against 0.097 ms with 20 % of the rows in the
@@ -507,6 +520,7 @@ static ew_tuning rule_tuning(const ew_handle* h) {
   ew_tuning t{};
   const int* K = h->prm.K;
   long long best_pts = 0;
+  RegionArgs best_rg{};
   for (int f3 = 1; f3 <= 4; ++f3)
     for (int f2 = 1; f2 <= 4; ++f2)
       for (int f1 = 1; f1 <= 2; ++f1) {
@@ -516,10 +530,15 @@ static ew_tuning rule_tuning(const ew_handle* h) {
         const long long pts = (long long)rg.S[0] * rg.S[1] * rg.S[2];
         if (pts > best_pts) {
           best_pts = pts;
+          best_rg = rg;
           for (int a = 0; a < 3; ++a) t.spread_block[a] = f[a];
         }
       }
-  t.spread_direct = best_pts > 0 && h->own_stream ? kDirectRule : 0;
+  t.spread_direct = 0;
+  if (best_pts > 0) {
+    if (kDirectRule == 2 && owned_fits(h, best_rg)) t.spread_direct = 2;
+    else if (kDirectRule == 1 && h->own_stream) t.spread_direct = 1;
+  }
   const bool big_plane = (long long)(K[0] / 2 + 1) * K[1] >= kPlaneMinPoints;
   t.fft_planes = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32 && big_plane)) ? 1 : 0;
   t.fft_lines[0] = (K[1] * K[2] + 1) / 2;
@@ -542,7 +561,10 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
   if (t.spread_block[0] > 0 || t.spread_block[1] > 0 || t.spread_block[2] > 0) {
     if (!region_shape(h, t.spread_block, kRegionMaxLds, &rg))
       return h->fail(EW_ERR_ARGUMENT, "spreading block does not fit the mesh or the LDS");
-    rg.direct = t.spread_direct && h->own_stream ? 1 : 0;
+    // 2: owned blocks (any handle); 1: direct write-out (own-stream handles); 0: copies
+    rg.direct = t.spread_direct == 2 ? 2 : (t.spread_direct && h->own_stream ? 1 : 0);
+    if (rg.direct == 2 && !owned_fits(h, rg))
+      return h->fail(EW_ERR_ARGUMENT, "spreading block does not fit the mesh or the LDS");
   } else {
     for (int a = 0; a < 3; ++a) t.spread_block[a] = 0;
   }
@@ -563,7 +585,9 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
     CHK(h->alloc(&scr, nblk * (size_t)rg.region_stride * rs));
     h->region.scratch = scr;
   }
-  if (sizeof(double) * (size_t)rg.region_stride > 65536) HIPCHK(h, prepare_region_kernels());
+  if (sizeof(double) * (size_t)rg.region_stride > 65536 ||
+      (rg.direct == 2 && owned_smem(rg.S) > 65536))
+    HIPCHK(h, prepare_region_kernels());
   h->px = make_plan(K[0], t.fft_lines[0], t.fft_radix[0]);
   h->py = make_plan(K[1], t.fft_lines[1], t.fft_radix[1]);
   h->pz = make_plan(K[2], t.fft_lines[2], t.fft_radix[2]);
@@ -767,13 +791,13 @@ static int autotune(ew_handle* h) {
   // 1. spreading: write-out and block shape
   {
     const ew_tuning base = best;
-    for (int direct = 1; direct >= 0; --direct)
+    for (int direct = 2; direct >= 0; --direct)
       for (int f3 = 1; f3 <= 4; ++f3)
         for (int f2 = 1; f2 <= 4; ++f2)
           for (int f1 = 1; f1 <= 2; ++f1) {
             const int f[3] = {f1, f2, f3};
             if (!region_shape(h, f, kRegionMaxLds, nullptr)) continue;
-            if (direct && !h->own_stream) continue;
+            if (direct == 1 && !h->own_stream) continue;
             ew_tuning c = base;
             c.spread_direct = direct;
             for (int a = 0; a < 3; ++a) c.spread_block[a] = f[a];
@@ -1631,8 +1655,9 @@ static int ensure_views(ew_handle* h, int nvec) {
   const size_t nparts = std::max<size_t>((size_t)(n + 255) / 256,
                                          (size_t)h->nt[0] * h->nt[1] * h->nt[2]);
   const size_t nblk = (size_t)h->region.nb[0] * h->region.nb[1] * h->region.nb[2];
-  const size_t rstride =
-      h->region.region_stride > 0 ? nblk * (size_t)h->region.region_stride * rs : 0;
+  const size_t rstride = h->region.region_stride > 0 && h->region.direct != 2
+                             ? nblk * (size_t)h->region.region_stride * rs
+                             : 0;
   const size_t cap = (size_t)nvec;
   unsigned char *g, *cg, *rg = nullptr;
   CHK(h->alloc(&V.qs, cap * n));
@@ -2315,7 +2340,8 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->n = h->n;
   for (int a = 0; a < 3; ++a) d->K[a] = K[a];
   d->ntiles = h->nt[0] * h->nt[1] * h->nt[2];
-  d->spread_mode = h->region.region_stride > 0 ? 0 : (h->spread_slots > 0 ? 1 : 2);
+  d->spread_mode = h->region.region_stride > 0 ? (h->region.direct == 2 ? 3 : 0)
+                                                : (h->spread_slots > 0 ? 1 : 2);
   d->gather_tile =
       gather_tile_smem(h->T, p, h->fp32 ? sizeof(float) : sizeof(double)) <= kGatherRegionMaxBytes;
   d->nparts = d->gather_tile ? d->ntiles : (h->n + 255) / 256;
@@ -2350,7 +2376,8 @@ void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   d->spread.slots = h->spread_slots;
   d->spread.src = h->src_tab;
   d->region = h->region;
-  d->region.direct = 0;  // the grouped kernels assemble block copies
+  // the grouped kernels take owned blocks as they are; any other blocks assemble copies
+  d->region.direct = h->region.direct == 2 ? 2 : 0;
   d->ga.order = p;
   d->ga.self_coef = h->self_coef;
   d->ga.bg = h->background;

@@ -32,6 +32,8 @@ inline size_t smem_of(const StructDesc& d, int kind, bool fp32) {
       return gather_tile_smem(d.spread.T, p, rs);
     case kGkPlane:
       return d.plane ? plane_smem(d.px, d.py, cs) : 0;
+    case kGkOwned:
+      return d.spread_mode == 3 ? owned_smem(d.region.S) : 0;
     default:
       return 0;
   }
@@ -46,6 +48,8 @@ inline int blocks_of(const StructDesc& d, int kind) {
       return d.spread_mode == 0
                  ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] * kAssembleSplit
                  : 0;
+    case kGkOwned:
+      return d.spread_mode == 3 ? d.region.nb[0] * d.region.nb[1] * d.region.nb[2] : 0;
     case kGkSlotted: return d.spread_mode == 1 ? d.ntiles : 0;
     case kGkChecked: return d.spread_mode == 2 ? d.ntiles : 0;
     case kGkFftX: return d.plane ? 0 : (d.K[1] * d.K[2] + 2 * d.px.lines - 1) / (2 * d.px.lines);
@@ -106,6 +110,7 @@ inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bo
   for (int k = 0; k < kGroupKinds; ++k)
     if (S->smem[k] > (k == kGkPlane    ? kPlaneMaxLds
                       : k == kGkRegion ? kRegionMaxLds
+                      : k == kGkOwned  ? kRegionMaxLds - kOwnedStatic
                                        : 65536))
       return false;
   return true;

@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void k_spread_region_body(const int4* __restrict__ k0
     }
   }
   __syncthreads();
-  if (g.direct) {
+  if (g.direct == 1) {
     // the copy (block + halo, wrapped onto the periodic mesh) is added into the mesh, which was
     // cleared before the launch; untouched halo points are skipped
     for (int i = threadIdx.x; i < nreg; i += blockDim.x) {
@@ -1181,6 +1181,210 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
     if (co_pairs<R>(co)) return;
   }
   k_spread_region_body<R, P>(k0, theta, q, tile_start, g, regions, blockIdx.x, qperm, grid);
+}
+
+// Owner-computes spreading (RegionArgs::direct == 2): one workgroup per block of whole sort
+// tiles owns the block's mesh points.  The atoms whose stencils can reach the block are those of
+// the sort tiles holding the cells lo-(p-1) .. lo+w-1 of each axis (cyclically); in each tile
+// row they form one or two contiguous runs of the sorted atoms.  Pass 1 walks those runs as one
+// flattened index and keeps, for each atom whose stencil reaches the block, its in-block z
+// planes as (atom, plane) items in an LDS list; pass 2 adds each item's in-block part of its
+// plane into an FP64 LDS copy of the block alone (no halo).  The copy is then stored into the
+// mesh: every mesh point is written once, with no copies, no assembly, no mesh clear and no
+// device atomics.  Atoms near a block face are read by the workgroups of both blocks.
+template <typename R, int P>
+__device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
+                                                    const R* __restrict__ theta,
+                                                    const double* __restrict__ q,
+                                                    const int* __restrict__ tile_start,
+                                                    RegionArgs g, R* __restrict__ grid, int bid,
+                                                    const int* __restrict__ qperm) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ int run_a0[kOwnedMaxRuns];       // first sorted atom of run r
+  __shared__ int run_off[kOwnedMaxRuns + 1];  // atoms in the runs before r
+  __shared__ int n_items;
+  const int p = P > 0 ? P : g.order;
+  const int S0 = g.S[0], S1 = g.S[1], S2 = g.S[2];
+  const int PX = owned_row_pitch(g.S), PZ = owned_plane_pitch(g.S);
+  const int nreg = PZ * S2;
+  double* reg = reinterpret_cast<double*>(smem_raw);
+  int* items = reinterpret_cast<int*>(smem_raw + sizeof(double) * (size_t)nreg);
+  for (int i = threadIdx.x; i < nreg; i += blockDim.x) reg[i] = 0.0;
+  int blk[3];
+  {
+    int b = bid;
+    blk[0] = b % g.nb[0];
+    b /= g.nb[0];
+    blk[1] = b % g.nb[1];
+    blk[2] = b / g.nb[1];
+  }
+  // per axis: the block's first point and width, and the cyclic run of sort tiles (first,
+  // count) that holds the stencil origins lo-(p-1) .. lo+w-1
+  int lo[3], w[3], tf[3], tc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int T = g.S[a] / g.f[a];
+    lo[a] = blk[a] * g.S[a];
+    w[a] = min(g.S[a], g.K[a] - lo[a]);
+    int c0 = lo[a] - (p - 1);
+    if (c0 < 0) c0 += g.K[a];
+    tf[a] = c0 / T;
+    int cnt = (lo[a] + w[a] - 1) / T - tf[a];
+    if (cnt < 0) cnt += g.nt[a];
+    tc[a] = min(cnt + 1, g.nt[a]);
+  }
+  // the runs: tile row (iz, iy), x tiles tf[0] .. tf[0]+tc[0]-1, split where they wrap
+  const int xe = tf[0] + tc[0];
+  const int pieces = xe > g.nt[0] ? 2 : 1;
+  const int nruns = tc[1] * tc[2] * pieces;  // the host keeps this within kOwnedMaxRuns
+  for (int r = threadIdx.x; r < nruns; r += blockDim.x) {
+    const int row = r / pieces, piece = r - row * pieces;
+    const int iz = row / tc[1], iy = row - iz * tc[1];
+    const int u3 = tf[2] + iz < g.nt[2] ? tf[2] + iz : tf[2] + iz - g.nt[2];
+    const int u2 = tf[1] + iy < g.nt[1] ? tf[1] + iy : tf[1] + iy - g.nt[1];
+    const int ta = (u3 * g.nt[1] + u2) * g.nt[0];
+    const int ua = piece == 0 ? tf[0] : 0;
+    const int ub = piece == 0 ? min(xe, g.nt[0]) : xe - g.nt[0];
+    const int a0 = tile_start[ta + ua];
+    run_a0[r] = a0;
+    run_off[r + 1] = tile_start[ta + ub] - a0;
+  }
+  if (threadIdx.x == 0) {
+    run_off[0] = 0;
+    n_items = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int r = 0; r < nruns; ++r) run_off[r + 1] += run_off[r];
+  __syncthreads();
+  const int natoms = run_off[nruns];
+  const int chunk = kOwnedItems / p;  // atoms per pass: each keeps at most p items
+  for (int j0 = 0; j0 < natoms; j0 += chunk) {
+    const int j1 = min(natoms, j0 + chunk);
+    // pass 1
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
+      int lr = 0, hr = nruns - 1;  // the last run that starts at or before j
+      while (lr < hr) {
+        const int mid = (lr + hr + 1) >> 1;
+        if (run_off[mid] <= j) lr = mid;
+        else hr = mid - 1;
+      }
+      const int a = run_a0[lr] + (j - run_off[lr]);
+      const int4 kk = k0[a];
+      int r[3] = {kk.x - lo[0], kk.y - lo[1], kk.z - lo[2]};
+      bool reach = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        // the representative in [-(p-1), K-(p-1)) of the origin relative to the block
+        if (r[d] >= g.K[d] - (p - 1)) r[d] -= g.K[d];
+        else if (r[d] < -(p - 1)) r[d] += g.K[d];
+        reach = reach && r[d] < w[d];
+      }
+      if (!reach) continue;
+      const int za = max(0, -r[2]), zb = min(p, w[2] - r[2]);
+      const int pos = atomicAdd(&n_items, zb - za);
+      for (int o3 = za; o3 < zb; ++o3) items[pos + o3 - za] = (a << 4) | o3;
+    }
+    __syncthreads();
+    const int m = n_items;
+    // pass 2
+    if (P > 0) {
+      constexpr int PP = P > 0 ? P : 1;
+      constexpr int NB = kSpreadBatch;
+      for (int base = threadIdx.x; base < m; base += NB * blockDim.x) {
+        int4 kk[NB];
+        double c3[NB];
+        R w1[NB][PP], w2[NB][PP];
+        int o3v[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const int it = base + u * (int)blockDim.x;
+          const int e = items[it < m ? it : base];
+          const int a = e >> 4, o3 = e & 15;
+          o3v[u] = it < m ? o3 : -1;
+          kk[u] = k0[a];
+          const R* th = theta + (size_t)a * 3 * PP;
+#pragma unroll
+          for (int o = 0; o < PP; ++o) {
+            w1[u][o] = th[o];
+            w2[u][o] = th[PP + o];
+          }
+          c3[u] = (qperm ? q[qperm[a]] : q[a]) * double(th[2 * PP + o3]);
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          if (o3v[u] < 0) continue;
+          int r[3] = {kk[u].x - lo[0], kk[u].y - lo[1], kk[u].z - lo[2]};
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            if (r[d] >= g.K[d] - (PP - 1)) r[d] -= g.K[d];
+            else if (r[d] < -(PP - 1)) r[d] += g.K[d];
+          }
+          const int z = r[2] + o3v[u];
+          bool vx[PP];
+#pragma unroll
+          for (int o1 = 0; o1 < PP; ++o1) vx[o1] = (unsigned)(r[0] + o1) < (unsigned)w[0];
+#pragma unroll
+          for (int o2 = 0; o2 < PP; ++o2) {
+            const int y = r[1] + o2;
+            if ((unsigned)y >= (unsigned)w[1]) continue;
+            const double c = c3[u] * double(w2[u][o2]);
+            const int row = z * PZ + y * PX + r[0];
+#pragma unroll
+            for (int o1 = 0; o1 < PP; ++o1)
+              if (vx[o1]) atomicAdd(reg + row + o1, c * double(w1[u][o1]));
+          }
+        }
+      }
+    } else {
+      for (int it = threadIdx.x; it < m; it += blockDim.x) {
+        const int e = items[it];
+        const int a = e >> 4, o3 = e & 15;
+        const int4 kk = k0[a];
+        const R* th = theta + (size_t)a * 3 * p;
+        int r[3] = {kk.x - lo[0], kk.y - lo[1], kk.z - lo[2]};
+        for (int d = 0; d < 3; ++d) {
+          if (r[d] >= g.K[d] - (p - 1)) r[d] -= g.K[d];
+          else if (r[d] < -(p - 1)) r[d] += g.K[d];
+        }
+        const int z = r[2] + o3;
+        const double c3 = (qperm ? q[qperm[a]] : q[a]) * double(th[2 * p + o3]);
+        for (int o2 = 0; o2 < p; ++o2) {
+          const int y = r[1] + o2;
+          if ((unsigned)y >= (unsigned)w[1]) continue;
+          const double c = c3 * double(th[p + o2]);
+          for (int o1 = 0; o1 < p; ++o1) {
+            const int x = r[0] + o1;
+            if ((unsigned)x < (unsigned)w[0])
+              atomicAdd(reg + z * PZ + y * PX + x, c * double(th[o1]));
+          }
+        }
+      }
+    }
+    __syncthreads();  // every item of this pass is added before the list is refilled
+    if (threadIdx.x == 0) n_items = 0;
+    __syncthreads();
+  }
+  BoxWalk wk((int)threadIdx.x, blockDim.x, S0, S1);
+  for (int i = threadIdx.x; i < S0 * S1 * S2; i += blockDim.x) {
+    if (wk.x < w[0] && wk.y < w[1] && wk.z < w[2])
+      grid[((size_t)(lo[2] + wk.z) * g.K[1] + lo[1] + wk.y) * g.K[0] + lo[0] + wk.x] =
+          R(reg[wk.z * PZ + wk.y * PX + wk.x]);
+    wk.next();
+  }
+}
+
+template <typename R, int P, bool CO>
+__global__ __launch_bounds__(256) void k_spread_owned(const int4* __restrict__ k0,
+                                                      const R* __restrict__ theta,
+                                                      const double* __restrict__ q,
+                                                      const int* __restrict__ tile_start,
+                                                      RegionArgs g, R* __restrict__ grid,
+                                                      const int* __restrict__ qperm, PairCo co) {
+  if constexpr (CO) {
+    if (co_pairs<R>(co)) return;
+  }
+  k_spread_owned_body<R, P>(k0, theta, q, tile_start, g, grid, blockIdx.x, qperm);
 }
 
 // Clears n elements (the mesh before a direct spreading): four per thread, grid-stride.
@@ -2532,6 +2736,19 @@ __global__ __launch_bounds__(256) void kg_spread_region(const StructDesc* __rest
   EW_GROUP_RUN(LOOP, m, item)
 }
 
+template <typename R, int P, bool LOOP>
+__global__ __launch_bounds__(256) void kg_spread_owned(const StructDesc* __restrict__ D,
+                                                       const IoSlot* __restrict__ io,
+                                                       GroupMap m) {
+  auto item = [&](int gi) {
+    int b;
+    const StructDesc& d = D[io[group_slot(m, gi, b)].sid];
+    k_spread_owned_body<R, P>(d.k0, static_cast<const R*>(d.theta), d.qs, d.tile_start, d.region,
+                              static_cast<R*>(d.grid), b, nullptr);
+  };
+  EW_GROUP_RUN(LOOP, m, item)
+}
+
 template <typename R, bool LOOP>
 __global__ __launch_bounds__(256) void kg_spread_assemble(const StructDesc* __restrict__ D,
                                                           const IoSlot* __restrict__ io,
@@ -2755,6 +2972,17 @@ static hipError_t launch_group_t(const GroupLaunch& G, hipStream_t st) {
 #undef EW_G_REGION
     hipLaunchKernelGGL((kg_spread_assemble<R, LOOP>), grid(kGkAssemble), dim3(256), 0, st, D, io,
                        G.map[kGkAssemble]);
+  }
+  if (run(kGkOwned)) {
+    if (G.smem[kGkOwned] > 65536) {
+      const hipError_t e = prepare_region_kernels();
+      if (e != hipSuccess) return e;
+    }
+#define EW_G_OWNED(PP)                                                                    \
+  hipLaunchKernelGGL((kg_spread_owned<R, PP, LOOP>), grid(kGkOwned), dim3(256),               \
+                     G.smem[kGkOwned], st, D, io, G.map[kGkOwned])
+    EW_ORDER_SWITCH(EW_G_OWNED)
+#undef EW_G_OWNED
   }
   if (run(kGkSlotted)) {
 #define EW_G_SLOTTED(PP)                                                                  \
@@ -3121,26 +3349,43 @@ static hipError_t launch_spme_t(const SpmeLaunch& L, hipStream_t st, hipEvent_t*
                         (rg.S[1] + rg.order - 1) * (rg.S[2] + rg.order - 1);
     const R* th = static_cast<const R*>(L.theta);
     R* scr = static_cast<R*>(rg.scratch);
-    if (rg.direct) {
+    if (rg.direct == 1) {
       const long long G = (long long)K1 * K2 * K3;
       const int nb = (int)std::min<long long>(2048, (G / 4 + 255) / 256 + 1);
       hipLaunchKernelGGL(k_clear<R>, dim3(nb), dim3(256), 0, st, grid, G);
     }
     dim3 rgrid;
     const PairCo rco = co_for(nblk, 256, true, &rgrid);
+    if (rg.direct == 2) {
+      const size_t osmem = owned_smem(rg.S);
+#define EW_OWNED(PP)                                                                          \
+  hipLaunchKernelGGL((k_spread_owned<R, PP, CO>), rgrid, dim3(256), osmem, st, L.k0, th, L.q,      \
+                     L.tile_start, rg, grid, L.qperm, rco)
+      switch (rg.order) {
+        case 4: EW_OWNED(4); break;
+        case 5: EW_OWNED(5); break;
+        case 6: EW_OWNED(6); break;
+        case 7: EW_OWNED(7); break;
+        case 8: EW_OWNED(8); break;
+        case 10: EW_OWNED(10); break;
+        default: EW_OWNED(0); break;
+      }
+#undef EW_OWNED
+    } else {
 #define EW_REGION(PP)                                                                         \
   hipLaunchKernelGGL((k_spread_region<R, PP, CO>), rgrid, dim3(256), smem, st, L.k0, th, L.q,      \
                      L.tile_start, rg, scr, L.qperm, grid, rco)
-    switch (rg.order) {
-      case 4: EW_REGION(4); break;
-      case 5: EW_REGION(5); break;
-      case 6: EW_REGION(6); break;
-      case 7: EW_REGION(7); break;
-      case 8: EW_REGION(8); break;
-      case 10: EW_REGION(10); break;
-      default: EW_REGION(0); break;
-    }
+      switch (rg.order) {
+        case 4: EW_REGION(4); break;
+        case 5: EW_REGION(5); break;
+        case 6: EW_REGION(6); break;
+        case 7: EW_REGION(7); break;
+        case 8: EW_REGION(8); break;
+        case 10: EW_REGION(10); break;
+        default: EW_REGION(0); break;
+      }
 #undef EW_REGION
+    }
     if (ev_region) (void)hipEventRecord(ev_region, st);
     if (!rg.direct)
       hipLaunchKernelGGL(k_spread_assemble<R>, dim3(nblk * kAssembleSplit), dim3(256), 0, st,
@@ -3265,6 +3510,18 @@ static hipError_t region_attr(int bytes) {
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_region<R, P, true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_owned<R, P, false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes - (int)kOwnedStatic);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spread_owned<R, P, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes - (int)kOwnedStatic);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_owned<R, P, false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes - (int)kOwnedStatic);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_owned<R, P, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes - (int)kOwnedStatic);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_spread_region<R, P, false>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

@@ -162,6 +162,21 @@ struct RegionArgs {
 constexpr size_t kRegionMaxBytes = 60 * 1024;
 constexpr int kSpreadBatch = 3;  // (atom, z plane) items whose loads a thread issues together
 constexpr size_t kRegionMaxLds = 160 * 1024;
+// Owned blocks: the (atom, z plane) items of one pass live in an LDS list of kOwnedItems
+// entries (4 bytes each) after the block copy; the tile runs (at most kOwnedMaxRuns) in LDS
+// tables.  A shape whose runs exceed the table is refused by the host.
+constexpr int kOwnedItems = 6144;
+constexpr int kOwnedMaxRuns = 256;
+constexpr size_t kOwnedStatic = 4096;  // LDS kept for the kernel's static tables
+// The block copy has odd row and plane pitches (in FP64 elements), so that the planes of one
+// atom, which neighbouring lanes add at once, fall in different LDS banks.
+inline __host__ __device__ int owned_row_pitch(const int* S) { return S[0] | 1; }
+inline __host__ __device__ int owned_plane_pitch(const int* S) {
+  return (owned_row_pitch(S) * S[1]) | 1;
+}
+inline size_t owned_smem(const int* S) {
+  return sizeof(double) * (size_t)owned_plane_pitch(S) * S[2] + sizeof(int) * (size_t)kOwnedItems;
+}
 // Raises the dynamic LDS limit of the region spreading kernels to kRegionMaxLds (once).
 hipError_t prepare_region_kernels();
 // Assembly of the block copies: workgroups per mesh block, mesh points in flight per thread
@@ -240,7 +255,8 @@ struct StructDesc {
   int K[3];
   int ntiles;
   int nparts;       // energy partials written by the gather
-  int spread_mode;  // 0: privatised blocks + assembly, 1: staged (slots), 2: checked
+  int spread_mode;  // 0: privatised blocks + assembly, 1: staged (slots), 2: checked,
+                    // 3: owned blocks
   int gather_tile;  // 1: tile gather, 0: per-atom gather
   int col16;        // 1: 16-bit partner offsets
   const int* perm;
@@ -307,6 +323,7 @@ enum GroupKind {
   kGkEnergy,
   kGkChargeSum,
   kGkPlane,
+  kGkOwned,
   kGroupKinds
 };
 

@@ -55,8 +55,11 @@ typedef struct ew_handle ew_handle;
  * counterpart in the reference, whose analogue is FFTW's per-call planning
  * (fe/nfft/fft_backend.f90:52-56). */
 typedef struct ew_tuning {
-  int spread_direct;    /* 1: each spreading block adds its LDS copy into the cleared mesh with
-                           device atomics; 0: copies to scratch, then a fixed-order assembly    */
+  int spread_direct;    /* 2: owned blocks -- each spreading block takes every atom whose stencil
+                           reaches it and stores its own mesh points once (no copies);
+                           1: each block adds its LDS copy (block + halo) into the cleared mesh
+                           with device atomics; 0: copies to scratch, then a fixed-order
+                           assembly                                                          */
   int spread_block[3];  /* SPME sort tiles (16x8x8 mesh points) per spreading block per axis;
                            0 0 0: no privatised blocks                                         */
   int fft_planes;       /* 1: fused x-y plane transforms (a whole plane per workgroup)          */

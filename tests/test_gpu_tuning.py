@@ -32,10 +32,10 @@ def _cases():
 
 
 def _shapes(base):
-    """A spread of shapes around the rules: write-out, blocks, planes, lines, radix orders and
-    the pair sum's placement."""
+    """A spread of shapes around the rules: write-out (copies, direct, owned blocks), blocks,
+    planes, lines, radix orders and the pair sum's placement."""
     out = []
-    for direct in (0, 1):
+    for direct in (0, 1, 2):
         for blk in ((1, 1, 1), (2, 1, 1), (1, 2, 2), (2, 2, 2)):
             t = dict(base, spread_direct=direct, spread_block=blk)
             out.append(t)
@@ -117,3 +117,33 @@ def test_autotuned_shapes_agree_and_are_cached(tmp_path, monkeypatch):
             assert again[k] == tuned[k], k
     lines = open(tmp_path / "tune.cache").read().splitlines()
     assert len(lines) == 2 and all(l.startswith("v2:") for l in lines)
+
+
+def test_owned_blocks_over_several_item_passes():
+    """Owned spreading blocks whose atoms overflow one pass of the LDS item list (a dense cell
+    and blocks of 2 x 2 x 2 sort tiles: over 2,000 candidate atoms per block, 1,024 per pass)
+    give the block copies' shifts and energy, in FP64 and, in mixed precision, the FP64
+    result to 1e-6; the same shapes in the multi-vector grouped launch."""
+    w = W.triclinic(n=20000, side=55.0)
+    opts = dict(tol=1e-10, grid=(64, 64, 64), order=6, r_cut=8.0 * W.AA)
+    geo = dict(charges=w["charges"])
+    with ew.EwaldEngine(**opts) as e:
+        e.set_geometry(w["coords"], w["lattice"], **geo)
+        base = e.tuning()
+    ref = None
+    for prec, tol in (("fp64", 1e-13), ("mixed", 1e-6)):
+        for direct in (0, 2):
+            t = dict(base, spread_direct=direct, spread_block=(2, 2, 2))
+            say(f"owned passes {prec} direct {direct}")
+            with ew.EwaldEngine(**opts, precision=prec, tuning=t) as e:
+                e.set_geometry(w["coords"], w["lattice"], **geo)
+                assert e.tuning()["spread_direct"] == direct
+                s, en = e.shift(w["charges"])
+                sm, em = e.shift_multi(np.stack([w["charges"], -w["charges"]]))
+            if ref is None:
+                ref, eref = s, en
+            assert rel_max(s, ref) < tol
+            assert en == pytest.approx(eref, rel=10 * tol)
+            assert rel_max(sm[0], ref) < tol
+            assert rel_max(sm[1], -ref) < tol
+            assert em[1] == pytest.approx(eref, rel=10 * tol)
