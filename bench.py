@@ -50,7 +50,8 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False):
     }
 
 
-def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, tuning=None):
+def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, tuning=None,
+                split_gather=False):
     """Algorithmic bytes of each span of ew_trace_times (the evaluation's launch shape).
 
     The privatised spreading's region kernel reads the atoms' stencil origins, weights and
@@ -70,8 +71,15 @@ def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, 
            "spread_assemble": rs * G if region else pb["spread"],
            "fft_x_r2c": pb["fft_x_r2c"], "fft_y": pb["fft_y"],
            "fft_z_influence": pb["fft_z_influence"], "fft_y_inverse": pb["fft_y_inverse"],
-           "fft_x_c2r": pb["fft_x_c2r"], "join_wait": 0,
+           "fft_x_c2r": pb["fft_x_c2r"], "gather_mesh": 0, "join_wait": 0,
            "gather_combine": pb["gather_combine"]}
+    if split_gather:
+        # the mesh potential at every atom (stencil origins, weights, the mesh; one FP64 out)
+        # before the join; after it the combine reads the mesh potential, pair sum, charge,
+        # on-site term and caller index and writes the shift
+        th = 3 * order * rs
+        out["gather_mesh"] = (16 + th + 8) * n + rs * G
+        out["gather_combine"] = (8 + 8 + 8 + 8 + 4 + 8) * n
     if region and direct:
         out["spread_region"] += rs * G
         out["spread_assemble"] = 0
@@ -88,15 +96,36 @@ def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, 
 
 #: spans of ew_trace_times on the evaluation stream (the chain), in launch order
 CHAIN = ("spread_region", "spread_assemble", "fft_x_r2c", "fft_y", "fft_z_influence",
-         "fft_y_inverse", "fft_x_c2r", "join_wait", "gather_combine")
+         "fft_y_inverse", "fft_x_c2r", "gather_mesh", "join_wait", "gather_combine")
 
 
-def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0, tuning=None):
+def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0, tuning=None,
+                 grid=None, split_gather=False):
     """The kernel that carries a span of ew_trace_times, as rocprofv3 names it (template
-    arguments included: the chain kernels carry the pair sum's co-scheduling flag)."""
+    arguments included: the Stockham chain kernels carry the pair sum's co-scheduling flag, the
+    register transforms their DFT length per lane and lanes per line)."""
     r = "float" if prec == "mixed" else "double"
     i = "short" if index_bytes == 2 else "int"
     tuning = tuning or {}
+    wave = int(tuning.get("fft_wave", 0))
+    gather = {"gather_mesh": f"void ewamd::k_gather_tile<{r}, {order}, true>" if split_gather
+              else None,
+              "gather_combine": "ewamd::k_shift_combine" if split_gather else
+              f"void ewamd::k_gather_tile<{r}, {order}, false>"}
+    if span in gather:
+        return gather[span]
+    if wave and grid is not None and span.startswith("fft"):
+        ln = tuning.get("fft_lanes", (0, 0, 0))
+        K = list(grid)
+        if wave == 2:
+            return {"fft_x_r2c": f"void ewamd::k_wplane_r2c<{r}, {K[0]}, {ln[0]}>",
+                    "fft_z_influence": f"void ewamd::k_wfft_z<{r}, {K[2] // ln[2]}, {ln[2]}>",
+                    "fft_x_c2r": f"void ewamd::k_wplane_c2r<{r}, {K[0]}, {ln[0]}>"}.get(span)
+        return {"fft_x_r2c": f"void ewamd::k_wfft_x_r2c<{r}, {K[0] // ln[0]}, {ln[0]}>",
+                "fft_y": f"void ewamd::k_wfft_lines<{r}, {K[1] // ln[1]}, {ln[1]}, -1>",
+                "fft_z_influence": f"void ewamd::k_wfft_z<{r}, {K[2] // ln[2]}, {ln[2]}>",
+                "fft_y_inverse": f"void ewamd::k_wfft_lines<{r}, {K[1] // ln[1]}, {ln[1]}, 1>",
+                "fft_x_c2r": f"void ewamd::k_wfft_x_c2r<{r}, {K[0] // ln[0]}, {ln[0]}>"}.get(span)
     co = "true" if tuning.get("pair_cosched", 0) else "false"
     direct = int(tuning.get("spread_direct", 0))
     spread = {0: f"void ewamd::k_spread_region<{r}, {order}, {co}>",
@@ -114,26 +143,27 @@ def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0, tuni
             "fft_z_influence": f"void ewamd::k_fft_lines<{r}, 2, {co}>",
             "fft_y_inverse": None if planes else f"void ewamd::k_fft_lines<{r}, 1, {co}>",
             "fft_x_c2r": f"void ewamd::k_fft_xy_c2r<{r}, {co}>" if planes else
-            f"void ewamd::k_fft_x_c2r<{r}, {co}>",
-            "gather_combine": f"void ewamd::k_gather_tile<{r}, {order}>"}.get(span)
+            f"void ewamd::k_fft_x_c2r<{r}, {co}>"}.get(span)
 
 
-def critical_path(trace, calls, tb, prec, prm, step_ms, tuning=None):
+def critical_path(trace, calls, tb, prec, prm, step_ms, tuning=None, split_gather=False):
     """The evaluation's critical path from its traced spans: the chain on the evaluation stream
     (spreading, FFTs, the wait for the pair sum, the gather) and, beside it, the charge gather
     and pair sum.  The pair-sum branch is on the critical path when the chain waits for it
     (join wait above 5 % of the evaluation).  The dominant kernel is the longest span on the
     critical path."""
     us = {k: trace[k] / calls * 1e3 for k in trace}
-    planes = bool(prm.get("fft_planes", 0))
+    planes = fused_planes(prm, tuning)
     rows = []
     for k in CHAIN + ("charges", "pair_sum"):
         if k in ("fft_y", "fft_y_inverse") and planes:
             continue
+        if k == "gather_mesh" and not split_gather:
+            continue
         b = tb[k]
         rows.append({"span": k, "beside_chain": k in ("charges", "pair_sum"),
                      "kernel": trace_symbol(k, prec, prm["index_bytes"], prm["order"], planes,
-                                            prm["pair_mode"], tuning),
+                                            prm["pair_mode"], tuning, prm["grid"], split_gather),
                      "us": round(us[k], 2), "algorithmic_bytes": b,
                      "GBs": round(b / (us[k] * 1e-6) / 1e9, 1) if b and us[k] > 0 else None,
                      "frac": round(b / (us[k] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
@@ -159,6 +189,13 @@ def critical_path(trace, calls, tb, prec, prm, step_ms, tuning=None):
         "spread_fft_frac": round(sf_b / (sf_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
         "pair_sum_on_critical_path": pair_on_path, "dominant": dom,
     }
+
+
+def fused_planes(prm, tuning):
+    """Whether the x and y passes of the launch shapes are one kernel per plane (the y phases
+    then move nothing): the register plane kernels, or the Stockham plane kernels."""
+    wave = int((tuning or {}).get("fft_wave", 0))
+    return wave == 2 or (wave == 0 and bool(prm.get("fft_planes", 0)))
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spme_triclinic98k_hbm_traffic.json")
@@ -853,10 +890,12 @@ def main():
                 e.shift_device(dqs[i % 4].data_ptr(), out.data_ptr(), en.data_ptr(), sp)
             trace, tcalls = e.trace_times()
         rs = 4 if prec == "mixed" else 8
-        planes = bool(prm.get("fft_planes", 0))
+        planes = fused_planes(prm, tuning)
+        split = bool(prm.get("gather_split", 0))
         tb = trace_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
-                         prm["index_bytes"], False, planes, trace["spread_region"] > 0, tuning)
-        crit = critical_path(trace, tcalls, tb, prec, prm, ms, tuning)
+                         prm["index_bytes"], False, planes, trace["spread_region"] > 0, tuning,
+                         split)
+        crit = critical_path(trace, tcalls, tb, prec, prm, ms, tuning, split)
         print(f"[bench] {prec} critical path: chain {crit['chain_us']:.1f} us "
               f"({crit['chain_frac']:.3f} of HBM peak), dominant {crit['dominant']['span']} "
               f"{crit['dominant']['us']:.1f} us", file=sys.stderr, flush=True)

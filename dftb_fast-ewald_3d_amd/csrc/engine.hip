@@ -112,6 +112,17 @@ void twiddles(int n, std::vector<R>& out, int variant) {
   }
 }
 
+// W_n^m = exp(-2 pi i m / n) for m < n, interleaved (re, im).
+template <typename R>
+void plain_roots(int n, std::vector<R>& out) {
+  out.assign(2 * (size_t)n, R(0));
+  for (int m = 0; m < n; ++m) {
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * m / (long double)n;
+    out[2 * (size_t)m] = R(cosl(a));
+    out[2 * (size_t)m + 1] = R(-sinl(a));
+  }
+}
+
 long long estimate_pairs(int n, const Cell& c, double r_list) {
   const double vol_sphere = 4.0 / 3.0 * kPi * r_list * r_list * r_list;
   double per_atom = double(n) / c.volume * vol_sphere;
@@ -539,6 +550,19 @@ static ew_tuning rule_tuning(const ew_handle* h) {
     if (kDirectRule == 2 && owned_fits(h, best_rg)) t.spread_direct = 2;
     else if (kDirectRule == 1 && h->own_stream) t.spread_direct = 1;
   }
+  if (t.spread_direct == 2) {
+    // owned blocks keep no halo in LDS: blocks of 4096 mesh points balance the atoms read twice
+    // near block faces against workgroups per CU (98,304-atom cell at 128^3, spreading alone:
+    // 36 us with 16 x 16 x 16 against 38-44 us for 1024-2048 points and 48-53 us for 8192)
+    static const int pref[][3] = {{1, 2, 2}, {2, 2, 1}, {1, 4, 1}, {1, 1, 4}};
+    for (const auto& f : pref) {
+      RegionArgs rg;
+      if (region_shape(h, f, kRegionMaxLds, &rg) && owned_fits(h, rg)) {
+        for (int a = 0; a < 3; ++a) t.spread_block[a] = f[a];
+        break;
+      }
+    }
+  }
   const bool big_plane = (long long)(K[0] / 2 + 1) * K[1] >= kPlaneMinPoints;
   t.fft_planes = (h->opt.fft_mode == 2 || (h->opt.fft_mode == 0 && h->fp32 && big_plane)) ? 1 : 0;
   t.fft_lines[0] = (K[1] * K[2] + 1) / 2;
@@ -547,6 +571,22 @@ static ew_tuning rule_tuning(const ew_handle* h) {
   for (int a = 0; a < 3; ++a) t.fft_radix[a] = 4;
   t.pair_cosched = kCoschedRule;
   t.pair_spread_share = kCoSpreadShareRule;
+  // register transforms: x-y plane kernels on square planes of a compiled side with a compiled z
+  // length, else line passes where every axis length has a compiled shape
+  t.fft_wave = 1;
+  for (int a = 0; a < 3; ++a) {
+    t.fft_lanes[a] = wave_fft_default_lanes(K[a], h->fp32);
+    if (!t.fft_lanes[a]) t.fft_wave = 0;
+  }
+  if (K[0] == K[1] && wave_plane_default_lanes(K[0]) && wave_fft_default_lanes(K[2], h->fp32)) {
+    t.fft_wave = 2;
+    t.fft_lanes[0] = t.fft_lanes[1] = wave_plane_default_lanes(K[0]);
+    t.fft_lanes[2] = wave_fft_default_lanes(K[2], h->fp32);
+  }
+  if (t.fft_wave) {
+    t.pair_cosched = 0;
+    t.pair_spread_share = 0;
+  }
   return t;
 }
 
@@ -610,6 +650,41 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
   h->plane = t.fft_planes && h->opt.fft_mode != 1 && plane_fits(h->px, h->py, 2 * rs);
   if (h->plane) HIPCHK(h, prepare_plane_kernels());
   t.fft_planes = h->plane ? 1 : 0;
+  // register transforms where every pass has a compiled shape (the co-scheduled pair sum runs
+  // only inside the Stockham launches): 2 = x-y plane kernels (square planes) + z lines, 1 =
+  // x, y and z line passes; otherwise the Stockham passes
+  // the handle's fft_mode holds for these too: 1 = separate passes (register lines), 2 = fused
+  // planes wherever a compiled plane kernel fits
+  const bool plane_ok = K[0] == K[1] && wave_plane_default_lanes(K[0]) > 0 &&
+                        wave_fft_default_lanes(K[2], h->fp32) > 0;
+  if (t.fft_wave == 2 && h->opt.fft_mode == 1) {
+    t.fft_wave = 1;
+    for (int a = 0; a < 3; ++a) t.fft_lanes[a] = wave_fft_default_lanes(K[a], h->fp32);
+  } else if (t.fft_wave == 1 && h->opt.fft_mode == 2 && plane_ok) {
+    t.fft_wave = 2;
+    t.fft_lanes[0] = t.fft_lanes[1] = wave_plane_default_lanes(K[0]);
+    t.fft_lanes[2] = wave_fft_default_lanes(K[2], h->fp32);
+  }
+  int wave = 0;
+  if (t.fft_wave == 2 && K[0] == K[1] && wave_plane_shape(K[0], t.fft_lanes[0]) &&
+      wave_fft_shape(K[2], t.fft_lanes[2])) {
+    wave = 2;
+    t.fft_lanes[1] = t.fft_lanes[0];
+  } else if (t.fft_wave != 0) {
+    wave = 1;
+    for (int a = 0; a < 3; ++a)
+      if (!wave_fft_shape(K[a], t.fft_lanes[a])) wave = 0;
+  }
+  h->wave = wave;
+  t.fft_wave = wave;
+  for (int a = 0; a < 3; ++a) {
+    if (!wave) t.fft_lanes[a] = 0;
+    h->lanes[a] = t.fft_lanes[a];
+  }
+  if (wave) {
+    t.pair_cosched = 0;
+    t.pair_spread_share = 0;
+  }
   h->tuning = t;
   return EW_OK;
 }
@@ -625,7 +700,7 @@ static std::string tune_key(const ew_handle* h) {
   int nb = 0;
   while ((1 << (nb + 1)) <= h->n) ++nb;
   char buf[256];
-  std::snprintf(buf, sizeof(buf), "v2:%s:%dx%dx%d:p%d:%s:n%d:ps%d:fm%d:own%d", arch.c_str(),
+  std::snprintf(buf, sizeof(buf), "v3:%s:%dx%dx%d:p%d:%s:n%d:ps%d:fm%d:own%d", arch.c_str(),
                 h->prm.K[0], h->prm.K[1], h->prm.K[2], h->prm.order, h->fp32 ? "mixed" : "fp64",
                 nb, h->opt.pair_storage, h->opt.fft_mode, h->own_stream ? 1 : 0);
   return buf;
@@ -649,16 +724,18 @@ static std::string tune_cache_path() {
   return "ewald_tune.cache";
 }
 
-// One line per key: key direct f0 f1 f2 planes l0 l1 l2 r0 r1 r2 cosched share eval_us.
+// One line per key: key direct f0 f1 f2 planes l0 l1 l2 r0 r1 r2 cosched share eval_us wave
+// lanes0 lanes1 lanes2.
 static bool parse_tuning(const char* line, std::string* key, ew_tuning* t) {
   char k[200];
   ew_tuning u{};
-  const int got = std::sscanf(line, "%199s %d %d %d %d %d %d %d %d %d %d %d %d %d %f", k,
-                              &u.spread_direct, &u.spread_block[0], &u.spread_block[1],
+  const int got = std::sscanf(line, "%199s %d %d %d %d %d %d %d %d %d %d %d %d %d %f %d %d %d %d",
+                              k, &u.spread_direct, &u.spread_block[0], &u.spread_block[1],
                               &u.spread_block[2], &u.fft_planes, &u.fft_lines[0], &u.fft_lines[1],
                               &u.fft_lines[2], &u.fft_radix[0], &u.fft_radix[1], &u.fft_radix[2],
-                              &u.pair_cosched, &u.pair_spread_share, &u.eval_us);
-  if (got != 15) return false;
+                              &u.pair_cosched, &u.pair_spread_share, &u.eval_us, &u.fft_wave,
+                              &u.fft_lanes[0], &u.fft_lanes[1], &u.fft_lanes[2]);
+  if (got != 19) return false;
   *key = k;
   *t = u;
   return true;
@@ -695,10 +772,11 @@ static void tune_cache_put(const std::string& key, const ew_tuning& t) {
   g_tune_entries.emplace_back(key, t);
   FILE* f = std::fopen(tune_cache_path().c_str(), "a");
   if (!f) return;  // a read-only location: the choice still holds for this process
-  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %.3f\n", key.c_str(),
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %.3f %d %d %d %d\n", key.c_str(),
                t.spread_direct, t.spread_block[0], t.spread_block[1], t.spread_block[2],
                t.fft_planes, t.fft_lines[0], t.fft_lines[1], t.fft_lines[2], t.fft_radix[0],
-               t.fft_radix[1], t.fft_radix[2], t.pair_cosched, t.pair_spread_share, t.eval_us);
+               t.fft_radix[1], t.fft_radix[2], t.pair_cosched, t.pair_spread_share, t.eval_us,
+               t.fft_wave, t.fft_lanes[0], t.fft_lanes[1], t.fft_lanes[2]);
   std::fclose(f);
 }
 
@@ -782,6 +860,7 @@ static int autotune(ew_handle* h) {
           ew_tuning c = base;
           c.pair_cosched = co;
           c.pair_spread_share = share;
+          if (co) c.fft_wave = 0;  // co-scheduling runs inside the Stockham launches
           CHK(consider(c));
         }
     }
@@ -804,8 +883,52 @@ static int autotune(ew_handle* h) {
             CHK(consider(c));
           }
   }
-  // 2. FFT pass structure: fused planes or separate passes, lines per workgroup of y and z
+  // 2. register transforms: x-y planes + z lines, x, y and z line passes (lanes per line of each
+  // axis), or the Stockham passes
   {
+    const int* K = h->prm.K;
+    bool lines = true;
+    for (int a = 0; a < 3; ++a) lines = lines && wave_fft_default_lanes(K[a], h->fp32) > 0;
+    const bool planes = K[0] == K[1] && wave_plane_default_lanes(K[0]) > 0 &&
+                        wave_fft_default_lanes(K[2], h->fp32) > 0;
+    const ew_tuning base = best;
+    if (planes && base.fft_wave != 2) {
+      ew_tuning c = base;
+      c.fft_wave = 2;
+      c.fft_lanes[0] = c.fft_lanes[1] = wave_plane_default_lanes(K[0]);
+      c.fft_lanes[2] = wave_fft_default_lanes(K[2], h->fp32);
+      CHK(consider(c));
+    }
+    if (lines && base.fft_wave != 1) {
+      ew_tuning c = base;
+      c.fft_wave = 1;
+      for (int a = 0; a < 3; ++a) c.fft_lanes[a] = wave_fft_default_lanes(K[a], h->fp32);
+      CHK(consider(c));
+    }
+    if (base.fft_wave != 0 && (planes || lines)) {
+      ew_tuning c = base;
+      c.fft_wave = 0;
+      CHK(consider(c));
+    }
+    // lanes per line of each pass of the winner
+    for (int a = 0; a < 3 && best.fft_wave; ++a) {
+      if (best.fft_wave == 2 && a == 1) continue;  // the plane kernels take the x lanes
+      const ew_tuning cur = best;
+      for (int l : {4, 8, 16}) {
+        if (l == cur.fft_lanes[a]) continue;
+        const bool ok = (cur.fft_wave == 2 && a == 0) ? wave_plane_shape(K[0], l)
+                                                       : wave_fft_shape(K[a], l);
+        if (!ok) continue;
+        ew_tuning c = cur;
+        c.fft_lanes[a] = l;
+        if (cur.fft_wave == 2 && a == 0) c.fft_lanes[1] = l;
+        CHK(consider(c));
+      }
+    }
+  }
+  // 3. Stockham passes (when they won): fused planes or separate passes, lines per workgroup of
+  // y and z
+  if (!best.fft_wave) {
     const ew_tuning base = best;
     const int ly_max = std::max(1, kFftMaxElems / h->prm.K[1]);
     const int lz_max = std::max(1, kFftMaxElems / h->prm.K[2]);
@@ -824,8 +947,9 @@ static int autotune(ew_handle* h) {
       }
     }
   }
-  // 3. radix order per axis (orders that give the same passes are timed once)
-  for (int a = 0; a < 3; ++a) {
+  // 4. radix order per axis of the Stockham passes (orders that give the same passes are timed
+  // once)
+  for (int a = 0; a < 3 && !best.fft_wave; ++a) {
     const ew_tuning base = best;
     std::vector<std::vector<int>> seen;
     for (int v = 0; v < kRadixVariants; ++v) {
@@ -851,6 +975,24 @@ static int autotune(ew_handle* h) {
   h->free_buf(d_en);
   tune_cache_put(tune_key(h), h->tuning);
   return EW_OK;
+}
+
+// The influence function's parameters for the current geometry (mesh, dual basis, volume, alpha,
+// B-spline moduli).
+static InfluenceArgs influence_args(const ew_handle* h) {
+  const int* K = h->prm.K;
+  InfluenceArgs inf{};
+  for (int a = 0; a < 3; ++a) inf.K[a] = K[a];
+  inf.Kh = K[0] / 2 + 1;
+  inf.khd = make_fastdiv(inf.Kh);
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) inf.rec[3 * a + d] = h->cell.rec[a][d];
+  inf.pref = 1.0 / (kPi * h->cell.volume);
+  inf.a2 = kPi * kPi / (h->prm.alpha * h->prm.alpha);
+  inf.bsq1 = h->bsq[0];
+  inf.bsq2 = h->bsq[1];
+  inf.bsq3 = h->bsq[2];
+  return inf;
 }
 
 extern "C" int ew_get_tuning(const ew_handle* h, ew_tuning* out) {
@@ -961,6 +1103,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   CHK(h->alloc(&h->row_ptr, (size_t)n + 1));
   CHK(h->alloc(&h->qs, n));
   CHK(h->alloc(&h->phi_real, n));
+  CHK(h->alloc(&h->gmesh, n));
   CHK(h->alloc(&h->epart, std::max<size_t>((size_t)(n + 255) / 256,
                                             (size_t)h->nt[0] * h->nt[1] * h->nt[2])));
   CHK(h->alloc(&h->energy, 1));
@@ -1057,6 +1200,28 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     unsigned char* t;
     CHK(h->alloc(&t, 2 * (size_t)K[a] * rs));
     h->tw[a] = t;  // filled by apply_tuning (the table follows the radix order)
+    // W_K^m = exp(-2 pi i m / K), m < K, for the register line transforms
+    CHK(h->alloc(&t, 2 * (size_t)K[a] * rs));
+    h->twp[a] = t;
+    if (h->fp32) {
+      std::vector<float>& tv = h->tab_twp32[a];
+      plain_roots(K[a], tv);
+      HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice, s));
+    } else {
+      std::vector<double>& tv = h->tab_twp64[a];
+      plain_roots(K[a], tv);
+      HIPCHK(h, hipMemcpyAsync(t, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice, s));
+    }
+  }
+  {
+    // the influence function on the half spectrum (read by the register z transform)
+    const size_t gh = (size_t)(K[0] / 2 + 1) * K[1] * K[2];
+    unsigned char* t;
+    CHK(h->alloc(&t, gh * rs));
+    h->inf_tab = t;
+    const InfluenceArgs inf = influence_args(h);
+    HIPCHK(h, h->fp32 ? launch_influence_table<float>(inf, t, s)
+                      : launch_influence_table<double>(inf, t, s));
   }
   for (int a = 0; a < 3; ++a)
     if (fft_plan(K[a], h->px.radix) < 0) return h->fail(EW_ERR_ARGUMENT, "grid not 2^a 3^b 5^c");
@@ -1217,6 +1382,14 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
   return EW_OK;
 }
 
+// The split gather (mesh potentials before the join, the rest after it): evaluations whose pair
+// sum runs beside the chain on the auxiliary stream, with the tile gather.
+static bool split_gather(const ew_handle* h) {
+  return h->aux && !h->tuning.pair_cosched && h->opt.profile != 1 &&
+         gather_tile_smem(h->T, h->prm.order, h->fp32 ? sizeof(float) : sizeof(double)) <=
+             kGatherRegionMaxBytes;
+}
+
 extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
   if (!h || !out) return set_global_error(EW_ERR_ARGUMENT, "ew_get_parameters: NULL argument");
   if (!h->have_geom) return set_global_error(EW_ERR_NO_GEOMETRY, "no geometry set");
@@ -1234,7 +1407,11 @@ extern "C" int ew_get_parameters(const ew_handle* h, ew_parameters* out) {
   out->precision = h->prm.precision;
   out->pair_mode = h->half ? 2 : h->pair_mode;
   out->index_bytes = (h->col16 || h->half) ? 2 : 4;
-  out->fft_planes = h->plane ? 1 : 0;
+  // the register transforms run in single-structure evaluations; a pool's grouped launches
+  // use the Stockham passes
+  out->fft_planes = h->own_stream ? ((h->wave == 2 || (h->wave == 0 && h->plane)) ? 1 : 0)
+                                  : (h->plane ? 1 : 0);
+  out->gather_split = split_gather(h) ? 1 : 0;
   out->n_pairs = h->half ? h->half_entries : h->nnz;
   out->device_bytes = (long long)h->device_bytes;
   return EW_OK;
@@ -1264,16 +1441,13 @@ static SpmeLaunch make_spme_launch(const ew_handle* h) {
   L.px = h->px;
   L.py = h->py;
   L.pz = h->pz;
-  for (int a = 0; a < 3; ++a) L.inf.K[a] = K[a];
-  L.inf.Kh = K[0] / 2 + 1;
-  L.inf.khd = make_fastdiv(L.inf.Kh);
-  for (int a = 0; a < 3; ++a)
-    for (int d = 0; d < 3; ++d) L.inf.rec[3 * a + d] = h->cell.rec[a][d];
-  L.inf.pref = 1.0 / (kPi * h->cell.volume);
-  L.inf.a2 = kPi * kPi / (h->prm.alpha * h->prm.alpha);
-  L.inf.bsq1 = h->bsq[0];
-  L.inf.bsq2 = h->bsq[1];
-  L.inf.bsq3 = h->bsq[2];
+  L.inf = influence_args(h);
+  L.wave = h->wave ? 1 : 0;
+  for (int a = 0; a < 3; ++a) {
+    L.lanes[a] = h->lanes[a];
+    L.twp[a] = h->twp[a];
+  }
+  L.inf_tab = h->inf_tab;
   L.grid = h->grid;
   L.cgrid = h->cgrid;
   return L;
@@ -1397,13 +1571,29 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     HIPCHK(h, launch_spme<float>(L, s, marks, region_mark));
   else
     HIPCHK(h, launch_spme<double>(L, s, marks, region_mark));
+  // With the pair sum beside the chain the gather splits: the mesh potential of every atom
+  // before the join (it needs nothing from the pair branch), the shift, energy and the
+  // charge-dependent terms after it, so the join is followed by one short kernel instead of the
+  // whole gather.
+  const GatherArgs ga = gather_args(h);
+  const bool split = fork && split_gather(h);
+  if (split)
+    HIPCHK(h, h->fp32 ? launch_gather_mesh<float>(h->k0, h->theta, h->grid, h->tile_start, ga,
+                                                  h->gmesh, s)
+                      : launch_gather_mesh<double>(h->k0, h->theta, h->grid, h->tile_start, ga,
+                                                   h->gmesh, s));
+  h->record_trace(ew_handle::kTrGathered, s);
   if (fork)
     HIPCHK(h, hipStreamWaitEvent(s, h->join_ev, 0));
   else if (!co)
     HIPCHK(h, pairs(s));
   if (prof) HIPCHK(h, hipEventRecord(h->ev[8], s));
   h->record_trace(ew_handle::kTrJoined, s);
-  HIPCHK(h, launch_gather(h, gather_args(h), h->qs, h->phi_real, h->onsite, d_shift, d_energy, s));
+  if (split)
+    HIPCHK(h, launch_shift_combine(n, h->gmesh, h->phi_real, h->qs, h->onsite, h->perm, ga,
+                                   d_shift, h->epart, d_energy, s));
+  else
+    HIPCHK(h, launch_gather(h, ga, h->qs, h->phi_real, h->onsite, d_shift, d_energy, s));
   if (trace) {
     h->record_trace(ew_handle::kTrEnd, s);
     HIPCHK(h, hipEventSynchronize(h->tev[ew_handle::kTrEnd]));
@@ -1412,7 +1602,8 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
         {T::kTrChg0, T::kTrChg1},    {T::kTrPair0, T::kTrPair1},  {T::kTrChain0, T::kTrRegion},
         {T::kTrRegion, T::kTrSpread}, {T::kTrSpread, T::kTrX},    {T::kTrX, T::kTrY},
         {T::kTrY, T::kTrZ},          {T::kTrZ, T::kTrYi},         {T::kTrYi, T::kTrXi},
-        {T::kTrXi, T::kTrJoined},    {T::kTrJoined, T::kTrEnd},   {T::kTrStart, T::kTrEnd}};
+        {T::kTrXi, T::kTrGathered},  {T::kTrGathered, T::kTrJoined},
+        {T::kTrJoined, T::kTrEnd},   {T::kTrStart, T::kTrEnd}};
     for (int i = 0; i < EW_NUM_TRACE; ++i) {
       float ms = 0.0f;
       HIPCHK(h, hipEventElapsedTime(&ms, h->tev[span[i][0]], h->tev[span[i][1]]));
@@ -1527,7 +1718,7 @@ extern "C" int ew_shift_device(ew_handle* h, const double* d_dq, double* d_shift
 // energy partials and total, charge sum, mesh, spectrum and spreading copies, each on a
 // 256-byte boundary.  With `base` set, points the handle's per-evaluation buffers there.
 struct NoallocLayout {
-  size_t qs, phi, epart, energy, qsum, grid, cgrid, region, total;
+  size_t qs, phi, gmesh, epart, energy, qsum, grid, cgrid, region, total;
 };
 static NoallocLayout noalloc_layout(const ew_handle* h) {
   const size_t rs = h->fp32 ? sizeof(float) : sizeof(double);
@@ -1540,6 +1731,7 @@ static NoallocLayout noalloc_layout(const ew_handle* h) {
   size_t off = 0;
   L.qs = off;     off += up(8 * n);
   L.phi = off;    off += up(8 * n);
+  L.gmesh = off;  off += up(8 * n);
   L.epart = off;  off += up(8 * nparts);
   L.energy = off; off += up(8);
   L.qsum = off;   off += up(8);
@@ -1574,20 +1766,23 @@ extern "C" int ew_shift_device_noalloc(ew_handle* h, const double* d_dq, double*
   // captured graph would keep the handle's own buffers)
   struct Swap {
     ew_handle* h;
-    double *qs, *phi, *epart, *qsum;
+    double *qs, *phi, *gmesh, *epart, *qsum;
     void *grid, *cgrid, *region;
     ~Swap() {
       h->qs = qs;
       h->phi_real = phi;
+      h->gmesh = gmesh;
       h->epart = epart;
       h->qsum = qsum;
       h->grid = grid;
       h->cgrid = cgrid;
       h->region.scratch = region;
     }
-  } keep{h, h->qs, h->phi_real, h->epart, h->qsum, h->grid, h->cgrid, h->region.scratch};
+  } keep{h,       h->qs,   h->phi_real, h->gmesh,           h->epart,
+         h->qsum, h->grid, h->cgrid,    h->region.scratch};
   h->qs = reinterpret_cast<double*>(b + L.qs);
   h->phi_real = reinterpret_cast<double*>(b + L.phi);
+  h->gmesh = reinterpret_cast<double*>(b + L.gmesh);
   h->epart = reinterpret_cast<double*>(b + L.epart);
   if (h->qsum) h->qsum = reinterpret_cast<double*>(b + L.qsum);
   h->grid = b + L.grid;

@@ -73,6 +73,8 @@ struct ew_handle {
   std::vector<double> tab_bsq[3];
   std::vector<float> tab_tw32[3];
   std::vector<double> tab_tw64[3];
+  std::vector<float> tab_twp32[3];
+  std::vector<double> tab_twp64[3];
   int* perm = nullptr;
   int4* k0 = nullptr;
   double* frac = nullptr;
@@ -95,8 +97,14 @@ struct ew_handle {
   void* cgrid = nullptr;
   void* tw[3] = {nullptr, nullptr, nullptr};
   double* bsq[3] = {nullptr, nullptr, nullptr};
+  // register line transforms: W_K^m per axis and the influence function on the half spectrum
+  void* twp[3] = {nullptr, nullptr, nullptr};
+  void* inf_tab = nullptr;
+  int wave = 0;  // 0: Stockham passes, 1: register line passes, 2: register x-y planes + z lines
+  int lanes[3] = {0, 0, 0};
   double* qs = nullptr;
   double* phi_real = nullptr;
+  double* gmesh = nullptr;  // mesh potential per sorted atom (split gather)
   double* epart = nullptr;
   double* energy = nullptr;
   double* dq_io = nullptr;
@@ -196,6 +204,7 @@ struct ew_handle {
     kTrChain0,     // stream s: before the spreading
     kTrRegion,     //           after the region kernel
     kTrSpread, kTrX, kTrY, kTrZ, kTrYi, kTrXi,  // after each chain phase
+    kTrGathered,   //           after the mesh gather (split gather; else right after kTrXi)
     kTrJoined,     //           after the wait for the pair sum
     kTrEnd,        //           after the gather and energy
     kTrEvents
@@ -237,8 +246,12 @@ struct ew_handle {
     for (int a = 0; a < 3; ++a) {
       tw[a] = nullptr;
       bsq[a] = nullptr;
+      twp[a] = nullptr;
+      lanes[a] = 0;
     }
-    qs = qsum = phi_real = epart = energy = dq_io = shift_io = nullptr;
+    inf_tab = nullptr;
+    wave = 0;
+    qs = qsum = phi_real = gmesh = epart = energy = dq_io = shift_io = nullptr;
     err_flag = nullptr;
     force = nullptr;
     region = ewamd::RegionArgs{};

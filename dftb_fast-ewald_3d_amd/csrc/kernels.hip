@@ -18,148 +18,15 @@
 // k_force_real).  External point charges: k_ext_real (with the spreading, FFTs
 // and gather above run on the sources and the resident atoms).
 //
-// All FFTs are Stockham autosort transforms of one or more lines staged in LDS: every pass
-// loads its butterflies into registers, synchronises, and stores them back in place.
+// The generic FFTs are Stockham autosort transforms of one or more lines staged in LDS: every
+// pass loads its butterflies into registers, synchronises, and stores them back in place; the
+// register line transforms of wfft.hip replace them for the axis lengths they cover.
 #include <hip/hip_runtime.h>
 
+#include "dft.hpp"
 #include "kernels.hpp"
 
 namespace ewamd {
-
-// ----------------------------------------------------------------------------------------
-// complex helpers
-// ----------------------------------------------------------------------------------------
-template <typename R>
-struct cx {
-  R x, y;
-};
-template <typename R>
-__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
-template <typename R>
-__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
-template <typename R>
-__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
-  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-template <typename R>
-__device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return {a.x * s, a.y * s}; }
-// multiply by s*i, s = +-1
-template <typename R>
-__device__ __forceinline__ cx<R> cmuli(cx<R> a, R s) { return {-s * a.y, s * a.x}; }
-
-// Small DFTs in registers; dir = -1 forward (exp(-2 pi i tu/R)), +1 inverse.
-template <int RAD, typename R>
-struct Dft;
-
-template <typename R>
-struct Dft<2, R> {
-  __device__ static void run(cx<R>* v, R) {
-    cx<R> a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  }
-};
-template <typename R>
-struct Dft<4, R> {
-  __device__ static void run(cx<R>* v, R s) {
-    cx<R> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-    cx<R> t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), s);
-    v[0] = cadd(t0, t2);
-    v[2] = csub(t0, t2);
-    v[1] = cadd(t1, t3);
-    v[3] = csub(t1, t3);
-  }
-};
-template <typename R>
-struct Dft<3, R> {
-  __device__ static void run(cx<R>* v, R s) {
-    const R c = R(-0.5), sn = R(0.866025403784438646763723170752936183);
-    cx<R> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    cx<R> b = cadd(v[0], cscale(t1, c));
-    cx<R> d = cmuli(cscale(t2, sn), s);
-    v[0] = cadd(v[0], t1);
-    v[1] = cadd(b, d);
-    v[2] = csub(b, d);
-  }
-};
-template <typename R>
-struct Dft<5, R> {
-  __device__ static void run(cx<R>* v, R s) {
-    const R c1 = R(0.309016994374947424102293417182819059);
-    const R c2 = R(-0.809016994374947424102293417182819059);
-    const R s1 = R(0.951056516295153572116439333379382143);
-    const R s2 = R(0.587785252292473129168705954639072769);
-    cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-    cx<R> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    cx<R> b1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
-    cx<R> b2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
-    cx<R> d1 = cmuli(cadd(cscale(t3, s1), cscale(t4, s2)), s);
-    cx<R> d2 = cmuli(csub(cscale(t3, s2), cscale(t4, s1)), s);
-    v[0] = cadd(v[0], cadd(t1, t2));
-    v[1] = cadd(b1, d1);
-    v[4] = csub(b1, d1);
-    v[2] = cadd(b2, d2);
-    v[3] = csub(b2, d2);
-  }
-};
-
-template <typename R>
-struct Dft<8, R> {
-  __device__ static void run(cx<R>* v, R s) {
-    // two radix-4 halves (even and odd inputs), then the eighth-root twiddles
-    cx<R> e[4] = {v[0], v[2], v[4], v[6]};
-    cx<R> o[4] = {v[1], v[3], v[5], v[7]};
-    Dft<4, R>::run(e, s);
-    Dft<4, R>::run(o, s);
-    const R c = R(0.707106781186547524400844362104849039);
-    const cx<R> w1 = {c * (o[1].x - s * o[1].y), c * (o[1].y + s * o[1].x)};
-    const cx<R> w2 = cmuli(o[2], s);
-    const cx<R> w3 = {c * (-o[3].x - s * o[3].y), c * (-o[3].y + s * o[3].x)};
-    v[0] = cadd(e[0], o[0]);
-    v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], w1);
-    v[5] = csub(e[1], w1);
-    v[2] = cadd(e[2], w2);
-    v[6] = csub(e[2], w2);
-    v[3] = cadd(e[3], w3);
-    v[7] = csub(e[3], w3);
-  }
-};
-
-template <typename R>
-struct Dft<16, R> {
-  // 4 x 4: DFT-4 over the stride-4 subsequences, twiddles W16^(n2 k1), DFT-4 across them
-  __device__ static void run(cx<R>* v, R s) {
-    const R c1 = R(0.923879532511286756128183189396788933);  // cos(pi/8)
-    const R s1 = R(0.382683432365089771728459984030398866);  // sin(pi/8)
-    const R c2 = R(0.707106781186547524400844362104849039);  // cos(pi/4)
-    cx<R> a[4][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) {
-      cx<R> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
-      Dft<4, R>::run(t, s);
-#pragma unroll
-      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
-    }
-    // W16^m = cos(pi m / 8) + i s sin(pi m / 8) for m = n2 k1
-    a[1][1] = cmul(a[1][1], cx<R>{c1, s * s1});
-    a[1][2] = cmul(a[1][2], cx<R>{c2, s * c2});
-    a[1][3] = cmul(a[1][3], cx<R>{s1, s * c1});
-    a[2][1] = cmul(a[2][1], cx<R>{c2, s * c2});
-    a[2][2] = cmuli(a[2][2], s);
-    a[2][3] = cmul(a[2][3], cx<R>{-c2, s * c2});
-    a[3][1] = cmul(a[3][1], cx<R>{s1, s * c1});
-    a[3][2] = cmul(a[3][2], cx<R>{-c2, s * c2});
-    a[3][3] = cmul(a[3][3], cx<R>{-c1, -s * s1});
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-      cx<R> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
-      Dft<4, R>::run(t, s);
-#pragma unroll
-      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
-    }
-  }
-};
 
 // One in-place Stockham pass over `nl` lines of length n (LDS pitch P).
 // tw[m] = exp(-2 pi i m / n); the inverse uses its conjugate.
@@ -167,9 +34,6 @@ struct Dft<16, R> {
 // stride-radix stores of the first pass and the strided column reads fall in distinct banks.
 __device__ __forceinline__ int line_slot(int i) { return i + (i >> 3); }
 
-__device__ __forceinline__ int fdiv(int x, FastDiv f) {
-  return int((__umulhi(unsigned(x), f.m) + unsigned(x)) >> f.s);
-}
 
 // Coordinates (x, y, z) of the flat index i0, i0 + S, i0 + 2S, ... of an E1 x E2 x E3 box (x
 // fastest), without a division per step: S is split once into box coordinates and each step
@@ -1997,7 +1861,9 @@ __global__ __launch_bounds__(256) void k_gather_combine(
 // tile plus a (p-1)-point halo, staged once in LDS; four lanes share an atom's p^2 rows and are
 // reduced by shuffles.  Then the pair sum, on-site and self terms are added and the shift is
 // written in caller order; one energy partial per tile.
-template <typename R, int P>
+// MO (mesh only): the mesh potential of each atom goes to shift[a] in sorted order (no other
+// term, no energy); k_shift_combine adds the rest once the pair sum is done.
+template <typename R, int P, bool MO = false>
 __device__ __forceinline__ void k_gather_tile_body(
     const int4* __restrict__ k0, const R* __restrict__ theta, const R* __restrict__ grid,
     const double* __restrict__ q, const double* __restrict__ phi_real,
@@ -2090,6 +1956,10 @@ __device__ __forceinline__ void k_gather_tile_body(
     }
     part += __shfl_xor(part, 1, 64);
     part += __shfl_xor(part, 2, 64);
+    if (MO) {
+      if (live && sub == 0) shift[a] = part;
+      continue;
+    }
     if (live && sub == 0) {
       const double qa = q[a];
       const double phi =
@@ -2099,20 +1969,48 @@ __device__ __forceinline__ void k_gather_tile_body(
       e += 0.5 * phi * qa;
     }
   }
+  if (MO) return;
   e = wave_sum(e);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
   __syncthreads();
   if (threadIdx.x == 0) epart[bid] = red[0] + red[1] + red[2] + red[3];
 }
 
-template <typename R, int P>
+template <typename R, int P, bool MO>
 __global__ __launch_bounds__(256) void k_gather_tile(
     const int4* __restrict__ k0, const R* __restrict__ theta, const R* __restrict__ grid,
     const double* __restrict__ q, const double* __restrict__ phi_real,
     const double* __restrict__ onsite, const int* __restrict__ perm,
     const int* __restrict__ tile_start, GatherArgs g, double* __restrict__ shift,
     double* __restrict__ epart) {
-  k_gather_tile_body<R, P>(k0, theta, grid, q, phi_real, onsite, perm, tile_start, g, shift, epart, blockIdx.x);
+  k_gather_tile_body<R, P, MO>(k0, theta, grid, q, phi_real, onsite, perm, tile_start, g, shift,
+                               epart, blockIdx.x);
+}
+
+// The shift of each atom from its mesh potential (k_gather_tile<.., true>, sorted order) and
+// the real-space pair sum, plus the on-site, self and background terms, written in caller order;
+// one energy partial 1/2 sum shift q per workgroup, in a fixed order.
+__global__ __launch_bounds__(256) void k_shift_combine(int n, const double* __restrict__ mesh,
+                                                       const double* __restrict__ phi_real,
+                                                       const double* __restrict__ q,
+                                                       const double* __restrict__ onsite,
+                                                       const int* __restrict__ perm, GatherArgs g,
+                                                       double* __restrict__ shift,
+                                                       double* __restrict__ epart) {
+  __shared__ double red[4];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  double e = 0.0;
+  if (k < n) {
+    const double qk = q[k];
+    const double phi = phi_real[k] + mesh[k] + (onsite ? onsite[k] : 0.0) * qk +
+                       g.self_coef * qk + (g.qsum ? g.bg * g.qsum[0] : 0.0);
+    shift[perm[k]] = phi;
+    e = 0.5 * phi * qk;
+  }
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) epart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 __device__ __forceinline__ void k_energy_final_body(int nparts, const double* __restrict__ epart,
@@ -3414,6 +3312,7 @@ static hipError_t launch_spme_t(const SpmeLaunch& L, hipStream_t st, hipEvent_t*
                        static_cast<const R*>(L.theta), L.q, L.tile_start, L.spread, grid);
   }
   mark(0);
+  if (L.wave && !L.forward_only && !co_on) return launch_wave_fft<R>(L, st, ev);
   const int nrows = K2 * K3;
   const int gx = (nrows + 2 * L.px.lines - 1) / (2 * L.px.lines);
   const int ny = Kh * K3;
@@ -3656,8 +3555,8 @@ hipError_t launch_gather_tile(const int4* k0, const void* theta, const void* gri
   const R* th = static_cast<const R*>(theta);
   const R* gr = static_cast<const R*>(grid);
 #define EW_GATHER(PP)                                                                            \
-  hipLaunchKernelGGL((k_gather_tile<R, PP>), dim3(ntiles), dim3(256), smem, st, k0, th, gr, q,    \
-                     phi_real, onsite, perm, tile_start, g, shift, epart)
+  hipLaunchKernelGGL((k_gather_tile<R, PP, false>), dim3(ntiles), dim3(256), smem, st, k0, th, gr,\
+                     q, phi_real, onsite, perm, tile_start, g, shift, epart)
   switch (g.order) {
     case 4: EW_GATHER(4); break;
     case 5: EW_GATHER(5); break;
@@ -3671,6 +3570,45 @@ hipError_t launch_gather_tile(const int4* k0, const void* theta, const void* gri
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, ntiles, epart, energy);
   return hipGetLastError();
 }
+template <typename R>
+hipError_t launch_gather_mesh(const int4* k0, const void* theta, const void* grid,
+                              const int* tile_start, const GatherArgs& g, double* mesh,
+                              hipStream_t st) {
+  const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+  const size_t smem = gather_tile_smem(g.T, g.order, sizeof(R));
+  const R* th = static_cast<const R*>(theta);
+  const R* gr = static_cast<const R*>(grid);
+#define EW_GATHER(PP)                                                                            \
+  hipLaunchKernelGGL((k_gather_tile<R, PP, true>), dim3(ntiles), dim3(256), smem, st, k0, th, gr, \
+                     nullptr, nullptr, nullptr, nullptr, tile_start, g, mesh, nullptr)
+  switch (g.order) {
+    case 4: EW_GATHER(4); break;
+    case 5: EW_GATHER(5); break;
+    case 6: EW_GATHER(6); break;
+    case 7: EW_GATHER(7); break;
+    case 8: EW_GATHER(8); break;
+    case 10: EW_GATHER(10); break;
+    default: EW_GATHER(0); break;
+  }
+#undef EW_GATHER
+  return hipGetLastError();
+}
+template hipError_t launch_gather_mesh<double>(const int4*, const void*, const void*, const int*,
+                                               const GatherArgs&, double*, hipStream_t);
+template hipError_t launch_gather_mesh<float>(const int4*, const void*, const void*, const int*,
+                                              const GatherArgs&, double*, hipStream_t);
+
+hipError_t launch_shift_combine(int n, const double* mesh, const double* phi_real,
+                                const double* q, const double* onsite, const int* perm,
+                                const GatherArgs& g, double* shift, double* epart,
+                                double* energy, hipStream_t st) {
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_shift_combine, dim3(nb), dim3(256), 0, st, n, mesh, phi_real, q, onsite,
+                     perm, g, shift, epart);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, nb, epart, energy);
+  return hipGetLastError();
+}
+
 template hipError_t launch_gather_tile<double>(const int4*, const void*, const void*,
                                                const double*, const double*, const double*,
                                                const int*, const int*, const GatherArgs&, double*,

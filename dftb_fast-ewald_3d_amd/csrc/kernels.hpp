@@ -243,7 +243,30 @@ struct SpmeLaunch {
   // (pair_spread_share percent) and the FFT launches (the rest, in equal parts)
   PairCo pair;
   int pair_spread_share;
+  // register transforms (wfft.hip) in place of the Stockham passes (1: x, y, z line passes; 2:
+  // x-y plane kernels and the z line pass): lanes per line of each axis (planes: lanes[0]), the
+  // tables W_K^m of each axis and the influence function on the half spectrum
+  int wave;
+  int lanes[3];
+  const void* twp[3];
+  const void* inf_tab;
 };
+
+// Register line transforms (wfft.hip): whether an axis of length n has a compiled transform
+// with `lanes` lanes per line; the default lanes for a length (0: none).
+bool wave_fft_shape(int n, int lanes);
+int wave_fft_default_lanes(int n, bool fp32);
+// Register x-y plane transforms (square planes): whether a plane side has a compiled kernel with
+// `lanes` lanes per line; the default lanes for a side (0: none).
+bool wave_plane_shape(int n, int lanes);
+int wave_plane_default_lanes(int n);
+// x, y, z (with the influence table), inverse y and inverse x passes of a full evaluation
+// (L.wave, every pass a compiled shape); ev (optional): events 1..5 after each pass.
+template <typename R>
+hipError_t launch_wave_fft(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
+// The influence function on the half spectrum ((kz K2 + ky) Kh + kx), as R, once per geometry.
+template <typename R>
+hipError_t launch_influence_table(const InfluenceArgs& inf, void* tab, hipStream_t st);
 
 // ---------------------------------------------------------------------------------------
 // Grouped evaluation: many resident structures in one launch per kernel.
@@ -406,6 +429,17 @@ hipError_t launch_gather_combine(int n, const int4* k0, const void* theta, const
                                  double* epart, double* energy, hipStream_t st);
 // Per-tile gather with the mesh region staged in LDS; needs
 // (T1+p-1)(T2+p-1)(T3+p-1) * sizeof(R) <= kGatherRegionMaxBytes.  epart: one per tile.
+// The split gather of an evaluation whose pair sum runs beside the chain: the mesh potential of
+// every atom (sorted order) before the join, then the shift in caller order with the pair sum,
+// on-site, self and background terms and the energy after it.
+template <typename R>
+hipError_t launch_gather_mesh(const int4* k0, const void* theta, const void* grid,
+                              const int* tile_start, const GatherArgs& g, double* mesh,
+                              hipStream_t st);
+hipError_t launch_shift_combine(int n, const double* mesh, const double* phi_real,
+                                const double* q, const double* onsite, const int* perm,
+                                const GatherArgs& g, double* shift, double* epart,
+                                double* energy, hipStream_t st);
 template <typename R>
 hipError_t launch_gather_tile(const int4* k0, const void* theta, const void* grid,
                               const double* q, const double* phi_real, const double* onsite,

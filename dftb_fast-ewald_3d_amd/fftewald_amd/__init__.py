@@ -56,8 +56,8 @@ PHASES = ("gather_charges", "spread", "fft_x_r2c", "fft_y", "fft_z_influence", "
           "fft_x_c2r", "pair_sum", "gather_combine")
 #: spans reported by ew_trace_times (the evaluation's own launch shape), in order
 TRACE = ("charges", "pair_sum", "spread_region", "spread_assemble", "fft_x_r2c", "fft_y",
-         "fft_z_influence", "fft_y_inverse", "fft_x_c2r", "join_wait", "gather_combine",
-         "evaluation")
+         "fft_z_influence", "fft_y_inverse", "fft_x_c2r", "gather_mesh", "join_wait",
+         "gather_combine", "evaluation")
 
 
 class EwaldError(RuntimeError):
@@ -73,13 +73,14 @@ class Tuning(C.Structure):
     _fields_ = [("spread_direct", C.c_int), ("spread_block", C.c_int * 3),
                 ("fft_planes", C.c_int), ("fft_lines", C.c_int * 3), ("fft_radix", C.c_int * 3),
                 ("pair_cosched", C.c_int), ("pair_spread_share", C.c_int),
-                ("eval_us", C.c_float)]
+                ("eval_us", C.c_float), ("fft_wave", C.c_int), ("fft_lanes", C.c_int * 3)]
 
     def as_dict(self):
         return {"spread_direct": self.spread_direct, "spread_block": tuple(self.spread_block),
                 "fft_planes": self.fft_planes, "fft_lines": tuple(self.fft_lines),
                 "fft_radix": tuple(self.fft_radix), "pair_cosched": self.pair_cosched,
-                "pair_spread_share": self.pair_spread_share, "eval_us": self.eval_us}
+                "pair_spread_share": self.pair_spread_share, "eval_us": self.eval_us,
+                "fft_wave": self.fft_wave, "fft_lanes": tuple(self.fft_lanes)}
 
     @classmethod
     def from_dict(cls, d):
@@ -88,7 +89,9 @@ class Tuning(C.Structure):
         t.fft_planes = int(d["fft_planes"])
         t.pair_cosched = int(d.get("pair_cosched", 0))
         t.pair_spread_share = int(d.get("pair_spread_share", 0))
+        t.fft_wave = int(d.get("fft_wave", 0))
         for a in range(3):
+            t.fft_lanes[a] = int(d.get("fft_lanes", (0, 0, 0))[a])
             t.spread_block[a] = int(d["spread_block"][a])
             t.fft_lines[a] = int(d["fft_lines"][a])
             t.fft_radix[a] = int(d["fft_radix"][a])
@@ -134,7 +137,7 @@ class Parameters(C.Structure):
     _fields_ = [("alpha", C.c_double), ("r_cut", C.c_double), ("k_cut", C.c_double),
                 ("r_list", C.c_double), ("grid", C.c_int * 3), ("order", C.c_int),
                 ("precision", C.c_int), ("pair_mode", C.c_int), ("index_bytes", C.c_int),
-                ("fft_planes", C.c_int), ("n_pairs", C.c_longlong),
+                ("fft_planes", C.c_int), ("gather_split", C.c_int), ("n_pairs", C.c_longlong),
                 ("device_bytes", C.c_longlong)]
 
     def as_dict(self) -> dict:
@@ -142,6 +145,7 @@ class Parameters(C.Structure):
                 "r_list": self.r_list, "grid": tuple(self.grid), "order": self.order,
                 "precision": self.precision, "pair_mode": self.pair_mode,
                 "index_bytes": self.index_bytes, "fft_planes": self.fft_planes,
+                "gather_split": self.gather_split,
                 "n_pairs": self.n_pairs, "device_bytes": self.device_bytes}
 
 

@@ -44,6 +44,8 @@ module fftewald_amd
       integer(c_int) :: pair_cosched = 0
       integer(c_int) :: pair_spread_share = 0
       real(c_float) :: eval_us = 0.0_c_float
+      integer(c_int) :: fft_wave = 0
+      integer(c_int) :: fft_lanes(3) = 0
    end type ew_tuning
 
    !> Mirrors struct ew_options.
@@ -79,6 +81,7 @@ module fftewald_amd
       integer(c_int) :: pair_mode = 0
       integer(c_int) :: index_bytes = 0
       integer(c_int) :: fft_planes = 0
+      integer(c_int) :: gather_split = 0
       integer(c_long_long) :: n_pairs = 0
       integer(c_long_long) :: device_bytes = 0
    end type ew_parameters

@@ -74,6 +74,15 @@ typedef struct ew_tuning {
   int pair_spread_share; /* pair_cosched: percent of the pair rows given to the spreading
                            launch; the FFT launches share the rest equally                    */
   float eval_us;        /* autotuned: the measured evaluation time of this choice; else 0       */
+  int fft_wave;         /* register transforms in place of the Stockham passes above (a DFT of
+                           length K/lanes per lane, one LDS exchange per line, compile-time index
+                           arithmetic): 2 = x-y plane kernels (square planes of side 32, 64, 96
+                           or 128, one workgroup per z plane) and a z line pass; 1 = x, y and z
+                           line passes (one wave per workgroup; axis lengths 32, 48, 64, 96, 128,
+                           256); a shape without a compiled kernel falls back to 1, then 0;
+                           0 = the Stockham passes                                          */
+  int fft_lanes[3];     /* fft_wave: lanes per line of the x, y and z transforms (4, 8 or 16;
+                           the plane kernels use fft_lanes[0] for x and y)                   */
 } ew_tuning;
 
 /* Options; zero-initialise and set what you need.  Fields <= 0 mean "choose automatically",
@@ -103,10 +112,11 @@ typedef struct ew_options {
                       -pi/(V alpha^2) * sum(q) to every shift, as DFTB+'s stock Ewald does
                       (src/dftbp/dftb/coulomb.F90:1292-1296); 0 (default): reject them, as the
                       fast path does (coulomb.F90:255-265, fe/common/ewald_validation.f90:26-48) */
-  int fft_mode;    /* 0 (default): fused x-y plane transforms in mixed precision where a plane
-                      fits one workgroup's LDS, separate x and y passes otherwise (FP64 planes
-                      hold one workgroup per CU and are slower); 1: always separate passes;
-                      2: fused planes wherever a plane fits                                   */
+  int fft_mode;    /* 0 (default): the register x-y plane kernels where the plane is square with
+                      a compiled side, else register line passes where every axis has a
+                      compiled length, else the Stockham passes (fused planes in mixed
+                      precision where a plane fits one workgroup's LDS); 1: always separate x
+                      and y passes; 2: fused planes wherever a plane fits                  */
   int pair_storage; /* 0 (default): the row list (every linked pair in both rows, the fastest
                        pair sum); 1: each linked pair stored once (symmetric, tile-blocked):
                        half the pair memory, about twice the pair-sum time                    */
@@ -141,8 +151,10 @@ typedef struct ew_parameters {
                         1 = dense image sum (small cells); 2 = linked tiles, each pair stored
                         once (symmetric, tile-blocked)                                         */
   int index_bytes;   /* bytes per stored partner index: 2 (offsets from the row) or 4          */
-  int fft_planes;    /* 1: fused x-y plane transforms (a whole plane in one workgroup's LDS);
-                        0: separate x and y passes                                             */
+  int fft_planes;    /* 1: fused x-y plane transforms (a whole plane in one workgroup's LDS,
+                        register or Stockham transforms); 0: separate x and y passes           */
+  int gather_split;  /* 1: the pair sum runs beside the chain and the gather splits around the
+                        join (mesh potentials before it, the shift and energy after it)       */
   long long n_pairs; /* stored real-space pair coefficients                                      */
   long long device_bytes; /* device memory held by the handle for this geometry                 */
 } ew_parameters;
@@ -247,10 +259,12 @@ int ew_phase_times(ew_handle *h, double *ms, long long *calls, int reset);
  *   3 spread: assembly (or the whole spreading without privatised blocks)
  *   4 FFT x / fused x-y (r2c)        5 FFT y               6 FFT z + influence + z^-1
  *   7 inverse FFT y                  8 inverse FFT x / fused y-x (c2r)
- *   9 wait for the pair sum          10 gather + combine + energy
- *   11 whole evaluation (start to end on the evaluation stream)
+ *   9 mesh gather (when the pair sum runs beside the chain the gather splits: the mesh
+ *     potential at every atom before the join, the rest after it; else empty)
+ *   10 wait for the pair sum         11 gather + combine + energy (split: combine + energy)
+ *   12 whole evaluation (start to end on the evaluation stream)
  * No counterpart in the reference (its per-phase timing is ew_phase_times'). */
-#define EW_NUM_TRACE 12
+#define EW_NUM_TRACE 13
 int ew_trace_times(ew_handle *h, double *ms, long long *calls, int reset);
 
 /* TCoulomb%addStress (src/dftbp/dftb/coulomb.F90:721-757, invRStress 2777-2863): adds the Ewald

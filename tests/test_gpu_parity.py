@@ -154,8 +154,9 @@ def test_c3_fp64_and_mixed(oracle_mod):
     say("triclinic 98304 atoms, mixed precision")
     s32, e32, prm32 = engine_shift(w, tol=1e-10, r_cut=w["r_cut"], grid=w["grid"],
                                    order=w["order"], precision="mixed")
-    # default FFT forms on the 128^3 mesh: fused planes in mixed precision, passes in FP64
-    assert prm32["fft_planes"] == 1 and prm["fft_planes"] == 0
+    # default FFT form on the 128^3 mesh: the fused x-y plane kernels (register transforms) in
+    # both precisions
+    assert prm32["fft_planes"] == 1 and prm["fft_planes"] == 1
     err = rel_max(s32, s64)
     print(f"mixed vs FP64: shift {err:.3e}  energy {abs(e32 - e64) / abs(e64):.3e}")
     assert err < 1e-6
@@ -445,9 +446,9 @@ def test_plane_and_separate_fft_passes_agree(precision, tol):
             e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
             out[mode] = e.shift(w["charges"]) + (e.parameters()["fft_planes"],)
     assert out[2][2] == 1 and out[1][2] == 0
-    # the default fuses planes only in mixed precision and only for planes of >= 4096
-    # half-spectrum points; this 64^3 mesh has 33 x 64 = 2112, so both precisions use passes
-    assert out[0][2] == 0
+    # the default on this 64^3 mesh: the register x-y plane kernels (a compiled plane side)
+    assert out[0][2] == 1
+    assert rel_max(out[0][0], out[1][0]) < tol
     assert rel_max(out[2][0], out[1][0]) < tol
     assert out[2][1] == pytest.approx(out[1][1], rel=tol)
     with pytest.raises(ew.EwaldError) as exc:

@@ -33,19 +33,24 @@ def _cases():
 
 def _shapes(base):
     """A spread of shapes around the rules: write-out (copies, direct, owned blocks), blocks,
-    planes, lines, radix orders and the pair sum's placement."""
+    register line transforms with each compiled lane count, the Stockham passes' planes, lines
+    and radix orders, and the pair sum's placement (co-scheduling runs in Stockham launches)."""
     out = []
     for direct in (0, 1, 2):
         for blk in ((1, 1, 1), (2, 1, 1), (1, 2, 2), (2, 2, 2)):
             t = dict(base, spread_direct=direct, spread_block=blk)
             out.append(t)
+    for lanes in ((4, 4, 4), (8, 8, 8), (4, 8, 16), (16, 16, 16)):
+        out.append(dict(base, fft_wave=1, fft_lanes=lanes))
+    for lanes in ((4, 4, 4), (4, 4, 8), (8, 8, 8)):
+        out.append(dict(base, fft_wave=2, fft_lanes=lanes))
     for planes in (0, 1):
         for lines in ((8, 2, 4), (16, 8, 16), (32, 16, 2)):
-            out.append(dict(base, fft_planes=planes, fft_lines=lines))
+            out.append(dict(base, fft_wave=0, fft_planes=planes, fft_lines=lines))
     for r in range(5):
-        out.append(dict(base, fft_radix=(r, (r + 1) % 5, (r + 2) % 5)))
+        out.append(dict(base, fft_wave=0, fft_radix=(r, (r + 1) % 5, (r + 2) % 5)))
     for co, share in ((1, 0), (1, 35), (1, 100), (0, 0)):
-        out.append(dict(base, pair_cosched=co, pair_spread_share=share))
+        out.append(dict(base, fft_wave=0, pair_cosched=co, pair_spread_share=share))
     return out
 
 
@@ -113,10 +118,66 @@ def test_autotuned_shapes_agree_and_are_cached(tmp_path, monkeypatch):
             e.set_geometry(w["coords"], w["lattice"], **geo)
             again = e.tuning()
         for k in ("spread_direct", "spread_block", "fft_planes", "fft_lines", "fft_radix",
-                  "pair_cosched", "pair_spread_share"):
+                  "pair_cosched", "pair_spread_share", "fft_wave", "fft_lanes"):
             assert again[k] == tuned[k], k
     lines = open(tmp_path / "tune.cache").read().splitlines()
-    assert len(lines) == 2 and all(l.startswith("v2:") for l in lines)
+    assert len(lines) == 2 and all(l.startswith("v3:") for l in lines)
+    assert all(len(l.split()) == 19 for l in lines)
+
+
+# every compiled (axis length, lanes per line) of the register line transforms appears on
+# some axis of these meshes
+_WAVE_MESHES = [(32, 48, 64), (64, 32, 96), (96, 64, 32), (128, 96, 48), (48, 128, 128),
+                (256, 32, 32), (32, 256, 64), (64, 64, 256), (32, 32, 48), (96, 96, 64),
+                (128, 128, 96)]
+_LANES = {32: (4, 8), 48: (4,), 64: (4, 8, 16), 96: (4, 8), 128: (4, 8, 16), 256: (8, 16)}
+# square x-y planes with a compiled plane kernel
+_PLANE_LANES = {32: (4,), 64: (8,), 96: (4,), 128: (8,)}
+
+
+@pytest.mark.parametrize("mesh", _WAVE_MESHES)
+def test_register_transforms_match_stockham(mesh, oracle_mod):
+    """The register line transforms give the Stockham passes' shifts and energy (FP64 1e-13,
+    mixed 2e-6 against mixed Stockham and 1e-6 against FP64) for every compiled lane count of
+    every axis, in a sheared cell (gamma 75 deg); one FP64 case also against the CPU SPME
+    (1e-11)."""
+    n = 1500
+    w = W.triclinic(n=n, side=26.0)
+    opts = dict(tol=1e-10, grid=mesh, order=6, r_cut=7.0 * W.AA)
+    geo = dict(charges=w["charges"])
+    ref = {}
+    for prec in ("fp64", "mixed"):
+        with ew.EwaldEngine(**opts, precision=prec) as e:
+            e.set_geometry(w["coords"], w["lattice"], **geo)
+            base = e.tuning()
+            stock = dict(base, fft_wave=0)
+        with ew.EwaldEngine(**opts, precision=prec, tuning=stock) as e:
+            e.set_geometry(w["coords"], w["lattice"], **geo)
+            assert e.tuning()["fft_wave"] == 0
+            ref[prec] = e.shift(w["charges"])
+            prm = e.parameters()
+        if prec == "fp64" and mesh == _WAVE_MESHES[0]:
+            want = oracle_mod.spme_ewald(w["coords"], w["charges"], w["lattice"], prm["grid"],
+                                         prm["order"], prm["alpha"], prm["r_cut"])
+            assert rel_max(ref[prec][0], want) < 1e-11
+        combos = max(len(_LANES[k]) for k in mesh)
+        cases = [(1, tuple(_LANES[k][c % len(_LANES[k])] for k in mesh)) for c in range(combos)]
+        if mesh[0] == mesh[1]:
+            cases += [(2, (pl, pl, _LANES[mesh[2]][0])) for pl in _PLANE_LANES[mesh[0]]]
+        for wave, lanes in cases:
+            say(f"register transforms {mesh} wave {wave} lanes {lanes} {prec}")
+            with ew.EwaldEngine(**opts, precision=prec,
+                                tuning=dict(base, fft_wave=wave, fft_lanes=lanes)) as e:
+                e.set_geometry(w["coords"], w["lattice"], **geo)
+                got = e.tuning()
+                assert got["fft_wave"] == wave and got["fft_lanes"] == lanes
+                s, en = e.shift(w["charges"])
+            r, er = ref[prec]
+            tol = 1e-13 if prec == "fp64" else 2e-6
+            assert rel_max(s, r) < tol, (mesh, lanes, prec)
+            assert en == pytest.approx(er, rel=10 * tol)
+            if prec == "mixed":
+                assert rel_max(s, ref["fp64"][0]) < 1e-6
 
 
 def test_owned_blocks_over_several_item_passes():
