@@ -199,8 +199,10 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
   } else if (hipStreamCreate(&h->stream) != hipSuccess ||
              hipStreamCreate(&h->aux) != hipSuccess ||
              hipEventCreateWithFlags(&h->fork_ev, kSyncEventFlags) != hipSuccess ||
-             hipEventCreateWithFlags(&h->join_ev, kSyncEventFlags) != hipSuccess) {
+             hipEventCreateWithFlags(&h->join_ev, kSyncEventFlags) != hipSuccess ||
+             hipEventCreateWithFlags(&h->spread_ev, kSyncEventFlags) != hipSuccess) {
     (void)hipGetLastError();
+    if (h->spread_ev) (void)hipEventDestroy(h->spread_ev);
     if (h->join_ev) (void)hipEventDestroy(h->join_ev);
     if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
     if (h->aux) (void)hipStreamDestroy(h->aux);
@@ -210,6 +212,7 @@ int ewamd::handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** 
   }
   if (hipEventCreateWithFlags(&h->last_ev, kSyncEventFlags) != hipSuccess) {
     (void)hipGetLastError();
+    if (h->spread_ev) (void)hipEventDestroy(h->spread_ev);
     if (h->join_ev) (void)hipEventDestroy(h->join_ev);
     if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
     if (h->aux) (void)hipStreamDestroy(h->aux);
@@ -258,6 +261,7 @@ extern "C" int ew_destroy(ew_handle* h) {
     if (e) (void)hipEventDestroy(e);
   if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
   if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+  if (h->spread_ev) (void)hipEventDestroy(h->spread_ev);
   if (h->last_ev) (void)hipEventDestroy(h->last_ev);
   if (h->aux) (void)hipStreamDestroy(h->aux);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -610,6 +614,8 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
   }
   t.spread_direct = rg.direct;
   t.pair_cosched = t.pair_cosched ? 1 : 0;
+  t.pair_blocks = std::max(0, std::min(1 << 16, t.pair_blocks));
+  t.pair_fork = t.pair_fork ? 1 : 0;
   t.pair_spread_share = std::max(0, std::min(100, t.pair_spread_share));
   for (int a = 0; a < 3; ++a)
     if (t.fft_radix[a] < 0 || t.fft_radix[a] >= kRadixVariants || t.fft_lines[a] < 1)
@@ -700,7 +706,7 @@ static std::string tune_key(const ew_handle* h) {
   int nb = 0;
   while ((1 << (nb + 1)) <= h->n) ++nb;
   char buf[256];
-  std::snprintf(buf, sizeof(buf), "v3:%s:%dx%dx%d:p%d:%s:n%d:ps%d:fm%d:own%d", arch.c_str(),
+  std::snprintf(buf, sizeof(buf), "v5:%s:%dx%dx%d:p%d:%s:n%d:ps%d:fm%d:own%d", arch.c_str(),
                 h->prm.K[0], h->prm.K[1], h->prm.K[2], h->prm.order, h->fp32 ? "mixed" : "fp64",
                 nb, h->opt.pair_storage, h->opt.fft_mode, h->own_stream ? 1 : 0);
   return buf;
@@ -725,17 +731,19 @@ static std::string tune_cache_path() {
 }
 
 // One line per key: key direct f0 f1 f2 planes l0 l1 l2 r0 r1 r2 cosched share eval_us wave
-// lanes0 lanes1 lanes2.
+// lanes0 lanes1 lanes2 pair_blocks pair_fork.
 static bool parse_tuning(const char* line, std::string* key, ew_tuning* t) {
   char k[200];
   ew_tuning u{};
-  const int got = std::sscanf(line, "%199s %d %d %d %d %d %d %d %d %d %d %d %d %d %f %d %d %d %d",
+  const int got = std::sscanf(line,
+                              "%199s %d %d %d %d %d %d %d %d %d %d %d %d %d %f %d %d %d %d %d %d",
                               k, &u.spread_direct, &u.spread_block[0], &u.spread_block[1],
                               &u.spread_block[2], &u.fft_planes, &u.fft_lines[0], &u.fft_lines[1],
                               &u.fft_lines[2], &u.fft_radix[0], &u.fft_radix[1], &u.fft_radix[2],
                               &u.pair_cosched, &u.pair_spread_share, &u.eval_us, &u.fft_wave,
-                              &u.fft_lanes[0], &u.fft_lanes[1], &u.fft_lanes[2]);
-  if (got != 19) return false;
+                              &u.fft_lanes[0], &u.fft_lanes[1], &u.fft_lanes[2], &u.pair_blocks,
+                              &u.pair_fork);
+  if (got != 21) return false;
   *key = k;
   *t = u;
   return true;
@@ -772,11 +780,13 @@ static void tune_cache_put(const std::string& key, const ew_tuning& t) {
   g_tune_entries.emplace_back(key, t);
   FILE* f = std::fopen(tune_cache_path().c_str(), "a");
   if (!f) return;  // a read-only location: the choice still holds for this process
-  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %.3f %d %d %d %d\n", key.c_str(),
+  std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %.3f %d %d %d %d %d %d\n",
+               key.c_str(),
                t.spread_direct, t.spread_block[0], t.spread_block[1], t.spread_block[2],
                t.fft_planes, t.fft_lines[0], t.fft_lines[1], t.fft_lines[2], t.fft_radix[0],
                t.fft_radix[1], t.fft_radix[2], t.pair_cosched, t.pair_spread_share, t.eval_us,
-               t.fft_wave, t.fft_lanes[0], t.fft_lanes[1], t.fft_lanes[2]);
+               t.fft_wave, t.fft_lanes[0], t.fft_lanes[1], t.fft_lanes[2], t.pair_blocks,
+               t.pair_fork);
   std::fclose(f);
 }
 
@@ -863,6 +873,22 @@ static int autotune(ew_handle* h) {
           if (co) c.fft_wave = 0;  // co-scheduling runs inside the Stockham launches
           CHK(consider(c));
         }
+    }
+    if (!best.pair_cosched) {
+      // where the pair sum forks: beside the spreading or after it
+      {
+        ew_tuning c = best;
+        c.pair_fork = best.pair_fork ? 0 : 1;
+        CHK(consider(c));
+      }
+      // the pair sum's bounded grid beside the chain
+      const ew_tuning base = best;
+      for (int nb : {256, 384, 512, 768, 1024}) {
+        if (nb == (base.pair_blocks > 0 ? base.pair_blocks : 512)) continue;
+        ew_tuning c = base;
+        c.pair_blocks = nb;
+        CHK(consider(c));
+      }
     }
     return EW_OK;
   };
@@ -1515,7 +1541,7 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
     const hipError_t e =
         h->half ? launch_pair_half(ntiles, h->tile_start, h->hp, h->fp32, h->qs, h->phi_real, st)
                 : launch_pair_spmv(n, h->row_ptr, col, h->col16 != nullptr, h->coef, h->fp32,
-                                   h->qs, h->phi_real, fork, st);
+                                   h->qs, h->phi_real, fork, st, h->tuning.pair_blocks);
     h->record_trace(ew_handle::kTrPair1, st);
     return e;
   };
@@ -1528,12 +1554,17 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   };
   if (prof) HIPCHK(h, hipEventRecord(h->ev[0], s));
   h->record_trace(ew_handle::kTrStart, s);
+  // pair_fork = 1: the pair sum waits for the spreading (it then runs beside the FFTs and the
+  // mesh gather, which leave most CUs idle, instead of competing with the spreading for them)
+  const bool late = root && h->tuning.pair_fork == 1;
   if (root) {
     HIPCHK(h, hipEventRecord(h->fork_ev, s));
     HIPCHK(h, hipStreamWaitEvent(h->aux, h->fork_ev, 0));
     CHK(charges(h->aux));
-    HIPCHK(h, pairs(h->aux));
-    HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+    if (!late) {
+      HIPCHK(h, pairs(h->aux));
+      HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+    }
   } else {
     CHK(charges(s));
     if (prof) HIPCHK(h, hipEventRecord(h->ev[1], s));
@@ -1567,10 +1598,16 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
   h->record_trace(ew_handle::kTrChain0, s);
   hipEvent_t* marks = prof ? &h->ev[2] : (trace ? &h->tev[ew_handle::kTrSpread] : nullptr);
   hipEvent_t region_mark = trace ? h->tev[ew_handle::kTrRegion] : nullptr;
+  if (late) L.spread_done = h->spread_ev;
   if (h->fp32)
     HIPCHK(h, launch_spme<float>(L, s, marks, region_mark));
   else
     HIPCHK(h, launch_spme<double>(L, s, marks, region_mark));
+  if (late) {
+    HIPCHK(h, hipStreamWaitEvent(h->aux, h->spread_ev, 0));
+    HIPCHK(h, pairs(h->aux));
+    HIPCHK(h, hipEventRecord(h->join_ev, h->aux));
+  }
   // With the pair sum beside the chain the gather splits: the mesh potential of every atom
   // before the join (it needs nothing from the pair branch), the shift, energy and the
   // charge-dependent terms after it, so the join is followed by one short kernel instead of the
@@ -1836,7 +1873,8 @@ static int ensure_views(ew_handle* h, int nvec) {
   }
   h->mgraphs.clear();
   for (void* p : {(void*)V.qs, (void*)V.phi, (void*)V.qil, (void*)V.epart, (void*)V.qsum, V.grid, V.cgrid,
-                  V.region, (void*)V.dq, (void*)V.shift, (void*)V.energy, (void*)V.desc})
+                  V.region, (void*)V.dq, (void*)V.shift, (void*)V.energy, (void*)V.desc,
+                  (void*)V.wio})
     if (p) h->free_buf(p);
   const void* staging = V.staging;
   V = ew_handle::Views{};
@@ -1883,6 +1921,12 @@ static int ensure_views(ew_handle* h, int nvec) {
   }
   HIPCHK(h, hipMemcpyAsync(V.desc, d.data(), sizeof(StructDesc) * cap, hipMemcpyHostToDevice,
                            h->stream));
+  // the vectors' meshes, spectra and the shared influence table for the batched register FFTs
+  std::vector<WaveIo> wio(cap);
+  for (size_t v = 0; v < cap; ++v) wio[v] = WaveIo{d[v].grid, d[v].cgrid, h->inf_tab};
+  CHK(h->alloc(&V.wio, cap));
+  HIPCHK(h, hipMemcpyAsync(V.wio, wio.data(), sizeof(WaveIo) * cap, hipMemcpyHostToDevice,
+                           h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->views_host = std::move(d);
   V.cap = nvec;
@@ -1912,6 +1956,12 @@ static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, c
   G.pm.beside = G.aux != nullptr;
   G.fork_ev = h->fork_ev;
   G.join_ev = h->join_ev;
+  if (h->wave) {
+    // every vector has this geometry's mesh: one batched register-FFT sequence
+    G.wave_n = nvec;
+    G.wave_io = h->views.wio;
+    G.wave_l = make_spme_launch(h);
+  }
   return G;
 }
 

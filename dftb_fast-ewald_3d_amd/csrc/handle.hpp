@@ -37,6 +37,7 @@ struct ew_handle {
   bool own_stream = true;  // false: the stream belongs to the pool that holds this handle
   hipStream_t aux = nullptr;      // runs the pair sum beside the SPME chain (own-stream handles)
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipEvent_t spread_ev = nullptr;  // after the spreading (the pair sum's late fork)
   // Completion of the last evaluation, on whichever stream it ran (ew_shift_device and
   // ew_shift_multi_device take a caller's stream): the geometry's buffers, the multi-vector
   // views and the captured graphs are released, rewritten or destroyed only after it.
@@ -139,6 +140,7 @@ struct ew_handle {
     double* shift = nullptr;     // [cap][n]
     double* energy = nullptr;    // [cap]
     ewamd::StructDesc* desc = nullptr;  // [cap] device descriptors, one per vector
+    ewamd::WaveIo* wio = nullptr;       // [cap] meshes and spectra for the batched FFTs
     void* staging = nullptr;     // maps and IoSlots of eager launches
   } views;
   std::vector<ewamd::StructDesc> views_host;  // host image of views.desc

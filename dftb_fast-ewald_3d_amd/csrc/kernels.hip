@@ -2892,26 +2892,31 @@ static hipError_t launch_group_t(const GroupLaunch& G, hipStream_t st) {
   if (run(kGkChecked))
     hipLaunchKernelGGL((kg_spread_checked<R, LOOP>), grid(kGkChecked), dim3(256), 0, st, D, io,
                        G.map[kGkChecked]);
-  if (run(kGkPlane))
-    hipLaunchKernelGGL((kg_fft_xy<R, false, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
-                       G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
-  if (run(kGkFftX)) {
-    hipLaunchKernelGGL((kg_fft_x<R, false, LOOP>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
-                       st, D, io, G.map[kGkFftX]);
-    hipLaunchKernelGGL((kg_fft_lines<R, 1, 0, LOOP>), grid(kGkFftY), dim3(kFftBlock),
-                       G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
+  if (G.wave_n > 0) {
+    const hipError_t e = launch_wave_fft_batch<R>(G.wave_l, G.wave_io, G.wave_n, st);
+    if (e != hipSuccess) return e;
+  } else {
+    if (run(kGkPlane))
+      hipLaunchKernelGGL((kg_fft_xy<R, false, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
+                         G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
+    if (run(kGkFftX)) {
+      hipLaunchKernelGGL((kg_fft_x<R, false, LOOP>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
+                         st, D, io, G.map[kGkFftX]);
+      hipLaunchKernelGGL((kg_fft_lines<R, 1, 0, LOOP>), grid(kGkFftY), dim3(kFftBlock),
+                         G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
+    }
+    hipLaunchKernelGGL((kg_fft_lines<R, 2, 2, LOOP>), grid(kGkFftZ), dim3(kFftBlock), G.smem[kGkFftZ],
+                       st, D, io, G.map[kGkFftZ]);
+    if (run(kGkFftX)) {
+      hipLaunchKernelGGL((kg_fft_lines<R, 1, 1, LOOP>), grid(kGkFftY), dim3(kFftBlock),
+                         G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
+      hipLaunchKernelGGL((kg_fft_x<R, true, LOOP>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
+                         st, D, io, G.map[kGkFftX]);
+    }
+    if (run(kGkPlane))
+      hipLaunchKernelGGL((kg_fft_xy<R, true, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
+                         G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
   }
-  hipLaunchKernelGGL((kg_fft_lines<R, 2, 2, LOOP>), grid(kGkFftZ), dim3(kFftBlock), G.smem[kGkFftZ],
-                     st, D, io, G.map[kGkFftZ]);
-  if (run(kGkFftX)) {
-    hipLaunchKernelGGL((kg_fft_lines<R, 1, 1, LOOP>), grid(kGkFftY), dim3(kFftBlock),
-                       G.smem[kGkFftY], st, D, io, G.map[kGkFftY]);
-    hipLaunchKernelGGL((kg_fft_x<R, true, LOOP>), grid(kGkFftX), dim3(kFftBlock), G.smem[kGkFftX],
-                       st, D, io, G.map[kGkFftX]);
-  }
-  if (run(kGkPlane))
-    hipLaunchKernelGGL((kg_fft_xy<R, true, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
-                       G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
   if (fork) {
     const hipError_t e = hipStreamWaitEvent(st, G.join_ev, 0);
     if (e != hipSuccess) return e;
@@ -3311,6 +3316,7 @@ static hipError_t launch_spme_t(const SpmeLaunch& L, hipStream_t st, hipEvent_t*
     hipLaunchKernelGGL(k_spread_checked<R>, sgrid, dim3(256), 0, st, L.k0,
                        static_cast<const R*>(L.theta), L.q, L.tile_start, L.spread, grid);
   }
+  if (L.spread_done) (void)hipEventRecord(L.spread_done, st);
   mark(0);
   if (L.wave && !L.forward_only && !co_on) return launch_wave_fft<R>(L, st, ev);
   const int nrows = K2 * K3;
@@ -3463,15 +3469,16 @@ hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t 
 }
 
 #ifndef EW_PAIR_STREAM_BLOCKS
-#define EW_PAIR_STREAM_BLOCKS 512  // 2 workgroups (8 waves) per CU on 256 CUs
+#define EW_PAIR_STREAM_BLOCKS 512  // default beside the chain: 2 workgroups (8 waves) per CU
 #endif
 
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
                             const void* coef, bool fp32, const double* q, double* out,
-                            bool beside, hipStream_t st) {
+                            bool beside, hipStream_t st, int beside_blocks) {
   const int rows_blocks = (n + 3) / 4;
   const dim3 block(256);
-  const dim3 grid(beside ? std::min(rows_blocks, EW_PAIR_STREAM_BLOCKS) : rows_blocks);
+  const int bounded = beside_blocks > 0 ? beside_blocks : EW_PAIR_STREAM_BLOCKS;
+  const dim3 grid(beside ? std::min(rows_blocks, bounded) : rows_blocks);
 #define EW_SPMV(CT, IT)                                                                      \
   hipLaunchKernelGGL((k_pair_spmv_stream<CT, IT>), grid, block, 0, st, n, row_ptr,           \
                      static_cast<const IT*>(col), static_cast<const CT*>(coef), q, out)

@@ -223,6 +223,13 @@ struct PairCo {
   double* out = nullptr;
 };
 
+// One structure of a batched register-transform launch (launch_wave_fft_batch).
+struct WaveIo {
+  void* grid;
+  void* cgrid;
+  const void* inf;
+};
+
 struct SpmeLaunch {
   int K[3];
   int plane;  // 1: fused x-y plane transforms (plane_fits), 0: separate x and y passes
@@ -250,6 +257,7 @@ struct SpmeLaunch {
   int lanes[3];
   const void* twp[3];
   const void* inf_tab;
+  hipEvent_t spread_done;  // optional: recorded after the spreading, before the FFTs
 };
 
 // Register line transforms (wfft.hip): whether an axis of length n has a compiled transform
@@ -264,6 +272,12 @@ int wave_plane_default_lanes(int n);
 // (L.wave, every pass a compiled shape); ev (optional): events 1..5 after each pass.
 template <typename R>
 hipError_t launch_wave_fft(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev);
+// The same passes over `count` structures of one mesh shape and lanes at once (blockIdx.y picks
+// the structure): bt[count] in device memory holds their meshes, spectra and influence tables;
+// L gives the mesh, mode, lanes and twiddle tables.
+template <typename R>
+hipError_t launch_wave_fft_batch(const SpmeLaunch& L, const WaveIo* bt, int count,
+                                 hipStream_t st);
 // The influence function on the half spectrum ((kz K2 + ky) Kh + kx), as R, once per geometry.
 template <typename R>
 hipError_t launch_influence_table(const InfluenceArgs& inf, void* tab, hipStream_t st);
@@ -384,6 +398,12 @@ struct GroupLaunch {
   int blocks[kGroupKinds];
   size_t smem[kGroupKinds];  // dynamic LDS: the largest of the structures in the kind
   int persistent;            // 1: bounded grids walking the work items (kGroupWgPerCu per CU)
+  // optional: the FFTs of all evaluations as one batched register-transform sequence (every
+  // evaluation has the same mesh and lanes) in place of the FFT kinds: wave_n evaluations, their
+  // meshes, spectra and influence tables at wave_io (device); wave_l: mesh, mode, lanes, tables
+  int wave_n;
+  const WaveIo* wave_io;
+  SpmeLaunch wave_l;
 };
 
 template <typename R>
@@ -402,11 +422,12 @@ hipError_t launch_gather_charges(int n, const int* perm, const double* dq, doubl
                                  double* zero, hipStream_t st);
 // Total charge of qs[0..n) in a fixed order (one workgroup) into *qsum.
 hipError_t launch_charge_sum(int n, const double* qs, double* qsum, hipStream_t st);
-// beside: the sum runs next to other kernels (a bounded grid leaves them CU room); otherwise
-// it has the device to itself and takes a workgroup per four rows.
+// beside: the sum runs next to other kernels on a bounded grid of beside_blocks workgroups
+// (0: 512), which leaves them CU room and bandwidth; otherwise it has the device to itself and
+// takes a workgroup per four rows.
 hipError_t launch_pair_spmv(int n, const long long* row_ptr, const void* col, bool col16,
                             const void* coef, bool fp32, const double* q, double* out,
-                            bool beside, hipStream_t st);
+                            bool beside, hipStream_t st, int beside_blocks = 0);
 hipError_t launch_compress_cols(int n, const long long* row_ptr, const int* col, short* off,
                                 int* too_far, hipStream_t st);
 

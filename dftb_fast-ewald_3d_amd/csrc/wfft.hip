@@ -108,11 +108,14 @@ __device__ __forceinline__ void line_backward(cx<R> (*w)[N2], cx<R>* v, cx<R>* x
 
 // z lines (line L = ky Kh + kx, element kz at L + kz es): forward transform, the influence
 // function (same layout as the spectrum), inverse transform, in place.
-template <typename R, int N1, int N2>
-__global__ __launch_bounds__(64) void k_wfft_z(cx<R>* __restrict__ cg, int nlines, int es,
+template <typename R, int N1, int N2, bool BATCH>
+__global__ __launch_bounds__(64) void k_wfft_z(cx<R>* __restrict__ cg_a, int nlines, int es,
                                                const cx<R>* __restrict__ tw,
-                                               const R* __restrict__ inf) {
+                                               const R* __restrict__ inf_a,
+                                               const WaveIo* __restrict__ bt) {
   using W = WLine<R, N1, N2>;
+  cx<R>* __restrict__ cg = BATCH ? static_cast<cx<R>*>(bt[blockIdx.y].cgrid) : cg_a;
+  const R* __restrict__ inf = BATCH ? static_cast<const R*>(bt[blockIdx.y].inf) : inf_a;
   __shared__ cx<R> xs[W::kExch];
   const int lane = threadIdx.x, l = lane % W::LW, t = lane / W::LW;
   const int L = blockIdx.x * W::LW + l;
@@ -144,11 +147,13 @@ __global__ __launch_bounds__(64) void k_wfft_z(cx<R>* __restrict__ cg, int nline
 
 // y lines (line L = kz Kh + kx, element ky at kz K2 Kh + kx + ky Kh), forward (DIR = -1) or
 // inverse (+1), in place.
-template <typename R, int N1, int N2, int DIR>
-__global__ __launch_bounds__(64) void k_wfft_lines(cx<R>* __restrict__ cg, int nlines, int Kh,
+template <typename R, int N1, int N2, int DIR, bool BATCH>
+__global__ __launch_bounds__(64) void k_wfft_lines(cx<R>* __restrict__ cg_a, int nlines, int Kh,
                                                    FastDiv khd, int plane,
-                                                   const cx<R>* __restrict__ tw) {
+                                                   const cx<R>* __restrict__ tw,
+                                                   const WaveIo* __restrict__ bt) {
   using W = WLine<R, N1, N2>;
+  cx<R>* __restrict__ cg = BATCH ? static_cast<cx<R>*>(bt[blockIdx.y].cgrid) : cg_a;
   __shared__ cx<R> xs[W::kExch];
   const int lane = threadIdx.x, l = lane % W::LW, t = lane / W::LW;
   const int L = blockIdx.x * W::LW + l;
@@ -185,11 +190,14 @@ struct WRow {
 // x pass, forward: rows r0 .. r0 + 2 LW - 1 of K1 = N reals (row 2l real part, row 2l + 1
 // imaginary part of line l) -> their half spectra (Kh = N/2 + 1 complex per row).  Rows are
 // loaded and stored contiguously through LDS.
-template <typename R, int N1, int N2>
-__global__ __launch_bounds__(64) void k_wfft_x_r2c(const R* __restrict__ grid,
-                                                   cx<R>* __restrict__ cg, int nrows,
-                                                   const cx<R>* __restrict__ tw) {
+template <typename R, int N1, int N2, bool BATCH>
+__global__ __launch_bounds__(64) void k_wfft_x_r2c(const R* __restrict__ grid_a,
+                                                   cx<R>* __restrict__ cg_a, int nrows,
+                                                   const cx<R>* __restrict__ tw,
+                                                   const WaveIo* __restrict__ bt) {
   using W = WLine<R, N1, N2>;
+  const R* __restrict__ grid = BATCH ? static_cast<const R*>(bt[blockIdx.y].grid) : grid_a;
+  cx<R>* __restrict__ cg = BATCH ? static_cast<cx<R>*>(bt[blockIdx.y].cgrid) : cg_a;
   using X = WRow<R, N1, N2>;
   constexpr int N = W::N, LW = W::LW, LP = X::LP, Kh = X::Kh;
   __shared__ cx<R> buf[X::kBuf];
@@ -241,11 +249,14 @@ __global__ __launch_bounds__(64) void k_wfft_x_r2c(const R* __restrict__ grid,
 
 // x pass, inverse: half-spectrum rows r0 .. r0 + 2 LW - 1 -> rows of N reals.  Rows 2l and
 // 2l + 1 are extended to full spectra (X[N - k] = conj X[k]) and packed as A + iB into line l.
-template <typename R, int N1, int N2>
-__global__ __launch_bounds__(64) void k_wfft_x_c2r(const cx<R>* __restrict__ cg,
-                                                   R* __restrict__ grid, int nrows,
-                                                   const cx<R>* __restrict__ tw) {
+template <typename R, int N1, int N2, bool BATCH>
+__global__ __launch_bounds__(64) void k_wfft_x_c2r(const cx<R>* __restrict__ cg_a,
+                                                   R* __restrict__ grid_a, int nrows,
+                                                   const cx<R>* __restrict__ tw,
+                                                   const WaveIo* __restrict__ bt) {
   using W = WLine<R, N1, N2>;
+  const cx<R>* __restrict__ cg = BATCH ? static_cast<const cx<R>*>(bt[blockIdx.y].cgrid) : cg_a;
+  R* __restrict__ grid = BATCH ? static_cast<R*>(bt[blockIdx.y].grid) : grid_a;
   using X = WRow<R, N1, N2>;
   constexpr int N = W::N, LW = W::LW, LP = X::LP, Kh = X::Kh, RP = 2 * LW + 1;
   __shared__ cx<R> buf[X::kBuf];
@@ -332,11 +343,14 @@ size_t wplane_smem() {
 // spectra of the two rows are separated (Z[k], conj Z[N - k]); the real columns kx = 0 and
 // kx = N/2 are packed as column 0 (X[.][0] + i X[.][N/2]) so the y transform runs over N/2
 // complex columns, and are separated again on the way out.
-template <typename R, int N, int N2>
-__global__ __launch_bounds__(N / 2 * N2) void k_wplane_r2c(const R* __restrict__ grid,
-                                                                     cx<R>* __restrict__ cg,
-                                                                     const cx<R>* __restrict__ tw) {
+template <typename R, int N, int N2, bool BATCH>
+__global__ __launch_bounds__(N / 2 * N2) void k_wplane_r2c(const R* __restrict__ grid_a,
+                                                           cx<R>* __restrict__ cg_a,
+                                                           const cx<R>* __restrict__ tw,
+                                                           const WaveIo* __restrict__ bt) {
   using P = WPlane<R, N, N2>;
+  const R* __restrict__ grid = BATCH ? static_cast<const R*>(bt[blockIdx.y].grid) : grid_a;
+  cx<R>* __restrict__ cg = BATCH ? static_cast<cx<R>*>(bt[blockIdx.y].cgrid) : cg_a;
   constexpr int N1 = P::N1, LW = P::LW, PA = P::PA, PO = P::PO, Kh = P::Kh, KS = N1 / N2;
   extern __shared__ unsigned char smem_raw[];
   cx<R>* sh = reinterpret_cast<cx<R>*>(smem_raw);
@@ -410,11 +424,14 @@ __global__ __launch_bounds__(N / 2 * N2) void k_wplane_r2c(const R* __restrict__
 }
 
 // Inverse of k_wplane_r2c: plane z = bid of the half spectrum -> N rows of N reals.
-template <typename R, int N, int N2>
-__global__ __launch_bounds__(N / 2 * N2) void k_wplane_c2r(const cx<R>* __restrict__ cg,
-                                                                     R* __restrict__ grid,
-                                                                     const cx<R>* __restrict__ tw) {
+template <typename R, int N, int N2, bool BATCH>
+__global__ __launch_bounds__(N / 2 * N2) void k_wplane_c2r(const cx<R>* __restrict__ cg_a,
+                                                           R* __restrict__ grid_a,
+                                                           const cx<R>* __restrict__ tw,
+                                                           const WaveIo* __restrict__ bt) {
   using P = WPlane<R, N, N2>;
+  const cx<R>* __restrict__ cg = BATCH ? static_cast<const cx<R>*>(bt[blockIdx.y].cgrid) : cg_a;
+  R* __restrict__ grid = BATCH ? static_cast<R*>(bt[blockIdx.y].grid) : grid_a;
   constexpr int N1 = P::N1, LW = P::LW, PA = P::PA, PO = P::PO, Kh = P::Kh, KS = N1 / N2;
   extern __shared__ unsigned char smem_raw[];
   cx<R>* sh = reinterpret_cast<cx<R>*>(smem_raw);
@@ -581,73 +598,127 @@ template hipError_t launch_influence_table<float>(const InfluenceArgs&, void*, h
 
 namespace {
 
+// A launch of the passes: one structure (bt null: the pointers of L) or a batch of count
+// structures of the same mesh and lanes (bt: their meshes, spectra and influence tables, one per
+// blockIdx.y).
+struct WaveCall {
+  const SpmeLaunch* L;
+  const WaveIo* bt;
+  int count;
+};
+
 template <typename R, int N1, int N2>
-void wave_x(bool forward, const SpmeLaunch& L, hipStream_t st) {
+void wave_x(bool forward, const WaveCall& c, hipStream_t st) {
+  const SpmeLaunch& L = *c.L;
   constexpr int LW = 64 / N2;
   const int nrows = L.K[1] * L.K[2];
-  const dim3 g((nrows + 2 * LW - 1) / (2 * LW));
+  const dim3 g((nrows + 2 * LW - 1) / (2 * LW), c.count);
   const cx<R>* tw = static_cast<const cx<R>*>(L.twp[0]);
-  if (forward)
-    hipLaunchKernelGGL((k_wfft_x_r2c<R, N1, N2>), g, dim3(64), 0, st,
-                       static_cast<const R*>(L.grid), static_cast<cx<R>*>(L.cgrid), nrows, tw);
-  else
-    hipLaunchKernelGGL((k_wfft_x_c2r<R, N1, N2>), g, dim3(64), 0, st,
-                       static_cast<const cx<R>*>(L.cgrid), static_cast<R*>(L.grid), nrows, tw);
+  if (forward) {
+    if (c.bt)
+      hipLaunchKernelGGL((k_wfft_x_r2c<R, N1, N2, true>), g, dim3(64), 0, st, nullptr, nullptr,
+                         nrows, tw, c.bt);
+    else
+      hipLaunchKernelGGL((k_wfft_x_r2c<R, N1, N2, false>), g, dim3(64), 0, st,
+                         static_cast<const R*>(L.grid), static_cast<cx<R>*>(L.cgrid), nrows, tw,
+                         nullptr);
+  } else {
+    if (c.bt)
+      hipLaunchKernelGGL((k_wfft_x_c2r<R, N1, N2, true>), g, dim3(64), 0, st, nullptr, nullptr,
+                         nrows, tw, c.bt);
+    else
+      hipLaunchKernelGGL((k_wfft_x_c2r<R, N1, N2, false>), g, dim3(64), 0, st,
+                         static_cast<const cx<R>*>(L.cgrid), static_cast<R*>(L.grid), nrows, tw,
+                         nullptr);
+  }
 }
 
 template <typename R, int N1, int N2>
-void wave_y(bool forward, const SpmeLaunch& L, hipStream_t st) {
+void wave_y(bool forward, const WaveCall& c, hipStream_t st) {
+  const SpmeLaunch& L = *c.L;
   constexpr int LW = 64 / N2;
   const int Kh = L.K[0] / 2 + 1, nlines = Kh * L.K[2];
-  const dim3 g((nlines + LW - 1) / LW);
+  const dim3 g((nlines + LW - 1) / LW, c.count);
   const cx<R>* tw = static_cast<const cx<R>*>(L.twp[1]);
-  cx<R>* cg = static_cast<cx<R>*>(L.cgrid);
-  if (forward)
-    hipLaunchKernelGGL((k_wfft_lines<R, N1, N2, -1>), g, dim3(64), 0, st, cg, nlines, Kh,
-                       make_fastdiv(Kh), Kh * L.K[1], tw);
-  else
-    hipLaunchKernelGGL((k_wfft_lines<R, N1, N2, 1>), g, dim3(64), 0, st, cg, nlines, Kh,
-                       make_fastdiv(Kh), Kh * L.K[1], tw);
+  cx<R>* cg = c.bt ? nullptr : static_cast<cx<R>*>(L.cgrid);
+  const FastDiv khd = make_fastdiv(Kh);
+  if (forward) {
+    if (c.bt)
+      hipLaunchKernelGGL((k_wfft_lines<R, N1, N2, -1, true>), g, dim3(64), 0, st, cg, nlines, Kh,
+                         khd, Kh * L.K[1], tw, c.bt);
+    else
+      hipLaunchKernelGGL((k_wfft_lines<R, N1, N2, -1, false>), g, dim3(64), 0, st, cg, nlines,
+                         Kh, khd, Kh * L.K[1], tw, nullptr);
+  } else {
+    if (c.bt)
+      hipLaunchKernelGGL((k_wfft_lines<R, N1, N2, 1, true>), g, dim3(64), 0, st, cg, nlines, Kh,
+                         khd, Kh * L.K[1], tw, c.bt);
+    else
+      hipLaunchKernelGGL((k_wfft_lines<R, N1, N2, 1, false>), g, dim3(64), 0, st, cg, nlines, Kh,
+                         khd, Kh * L.K[1], tw, nullptr);
+  }
 }
 
 template <typename R, int N1, int N2>
-void wave_z(const SpmeLaunch& L, hipStream_t st) {
+void wave_z(const WaveCall& c, hipStream_t st) {
+  const SpmeLaunch& L = *c.L;
   constexpr int LW = 64 / N2;
   const int Kh = L.K[0] / 2 + 1, nlines = Kh * L.K[1];
-  hipLaunchKernelGGL((k_wfft_z<R, N1, N2>), dim3((nlines + LW - 1) / LW), dim3(64), 0, st,
-                     static_cast<cx<R>*>(L.cgrid), nlines, nlines,
-                     static_cast<const cx<R>*>(L.twp[2]), static_cast<const R*>(L.inf_tab));
+  const dim3 g((nlines + LW - 1) / LW, c.count);
+  const cx<R>* tw = static_cast<const cx<R>*>(L.twp[2]);
+  if (c.bt)
+    hipLaunchKernelGGL((k_wfft_z<R, N1, N2, true>), g, dim3(64), 0, st, nullptr, nlines, nlines,
+                       tw, nullptr, c.bt);
+  else
+    hipLaunchKernelGGL((k_wfft_z<R, N1, N2, false>), g, dim3(64), 0, st,
+                       static_cast<cx<R>*>(L.cgrid), nlines, nlines, tw,
+                       static_cast<const R*>(L.inf_tab), nullptr);
 }
 
 template <typename R, int N, int N2>
-hipError_t wave_plane(bool forward, const SpmeLaunch& L, hipStream_t st) {
+hipError_t wave_plane(bool forward, const WaveCall& c, hipStream_t st) {
+  const SpmeLaunch& L = *c.L;
   const size_t smem = wplane_smem<R, N, N2>();
   // more than 64 KiB of dynamic LDS needs the kernel attribute (set once per kernel)
   static const hipError_t attr = [&] {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wplane_r2c<R, N, N2>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wplane_c2r<R, N, N2>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    const void* ks[] = {reinterpret_cast<const void*>(&k_wplane_r2c<R, N, N2, false>),
+                        reinterpret_cast<const void*>(&k_wplane_c2r<R, N, N2, false>),
+                        reinterpret_cast<const void*>(&k_wplane_r2c<R, N, N2, true>),
+                        reinterpret_cast<const void*>(&k_wplane_c2r<R, N, N2, true>)};
+    hipError_t e = hipSuccess;
+    for (const void* k : ks)
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     return e;
   }();
   if (attr != hipSuccess) return attr;
-  const dim3 g(L.K[2]);
+  const dim3 g(L.K[2], c.count);
+  const dim3 b(WPlane<R, N, N2>::NT);
   const cx<R>* tw = static_cast<const cx<R>*>(L.twp[0]);
-  if (forward)
-    hipLaunchKernelGGL((k_wplane_r2c<R, N, N2>), g, dim3(WPlane<R, N, N2>::NT), smem, st,
-                       static_cast<const R*>(L.grid), static_cast<cx<R>*>(L.cgrid), tw);
-  else
-    hipLaunchKernelGGL((k_wplane_c2r<R, N, N2>), g, dim3(WPlane<R, N, N2>::NT), smem, st,
-                       static_cast<const cx<R>*>(L.cgrid), static_cast<R*>(L.grid), tw);
+  if (forward) {
+    if (c.bt)
+      hipLaunchKernelGGL((k_wplane_r2c<R, N, N2, true>), g, b, smem, st, nullptr, nullptr, tw,
+                         c.bt);
+    else
+      hipLaunchKernelGGL((k_wplane_r2c<R, N, N2, false>), g, b, smem, st,
+                         static_cast<const R*>(L.grid), static_cast<cx<R>*>(L.cgrid), tw, nullptr);
+  } else {
+    if (c.bt)
+      hipLaunchKernelGGL((k_wplane_c2r<R, N, N2, true>), g, b, smem, st, nullptr, nullptr, tw,
+                         c.bt);
+    else
+      hipLaunchKernelGGL((k_wplane_c2r<R, N, N2, false>), g, b, smem, st,
+                         static_cast<const cx<R>*>(L.cgrid), static_cast<R*>(L.grid), tw, nullptr);
+  }
   return hipGetLastError();
 }
 
 template <typename R>
-bool dispatch_plane(bool forward, const SpmeLaunch& L, hipStream_t st, hipError_t* err) {
+bool dispatch_plane(bool forward, const WaveCall& c, hipStream_t st, hipError_t* err) {
+  const SpmeLaunch& L = *c.L;
 #define EW_CASE(NN, LL)                                        \
   if (L.K[0] == NN && L.lanes[0] == LL) {                      \
-    *err = wave_plane<R, NN, LL>(forward, L, st);              \
+    *err = wave_plane<R, NN, LL>(forward, c, st);              \
     return true;                                               \
   }
   if (L.K[0] == L.K[1]) {
@@ -658,10 +729,10 @@ bool dispatch_plane(bool forward, const SpmeLaunch& L, hipStream_t st, hipError_
 }
 
 template <typename R>
-bool dispatch_x(bool forward, const SpmeLaunch& L, hipStream_t st) {
+bool dispatch_x(bool forward, const WaveCall& c, hipStream_t st) {
 #define EW_CASE(NN, LL)                                        \
-  if (L.K[0] == NN && L.lanes[0] == LL) {                      \
-    wave_x<R, NN / LL, LL>(forward, L, st);                    \
+  if (c.L->K[0] == NN && c.L->lanes[0] == LL) {                \
+    wave_x<R, NN / LL, LL>(forward, c, st);                    \
     return true;                                               \
   }
   EW_WAVE_SHAPES(EW_CASE)
@@ -669,10 +740,10 @@ bool dispatch_x(bool forward, const SpmeLaunch& L, hipStream_t st) {
   return false;
 }
 template <typename R>
-bool dispatch_y(bool forward, const SpmeLaunch& L, hipStream_t st) {
+bool dispatch_y(bool forward, const WaveCall& c, hipStream_t st) {
 #define EW_CASE(NN, LL)                                        \
-  if (L.K[1] == NN && L.lanes[1] == LL) {                      \
-    wave_y<R, NN / LL, LL>(forward, L, st);                    \
+  if (c.L->K[1] == NN && c.L->lanes[1] == LL) {                \
+    wave_y<R, NN / LL, LL>(forward, c, st);                    \
     return true;                                               \
   }
   EW_WAVE_SHAPES(EW_CASE)
@@ -680,51 +751,67 @@ bool dispatch_y(bool forward, const SpmeLaunch& L, hipStream_t st) {
   return false;
 }
 template <typename R>
-bool dispatch_z(const SpmeLaunch& L, hipStream_t st) {
+bool dispatch_z(const WaveCall& c, hipStream_t st) {
 #define EW_CASE(NN, LL)                                        \
-  if (L.K[2] == NN && L.lanes[2] == LL) {                      \
-    wave_z<R, NN / LL, LL>(L, st);                             \
+  if (c.L->K[2] == NN && c.L->lanes[2] == LL) {                \
+    wave_z<R, NN / LL, LL>(c, st);                             \
     return true;                                               \
   }
   EW_WAVE_SHAPES(EW_CASE)
 #undef EW_CASE
   return false;
+}
+
+template <typename R>
+hipError_t run_wave_fft(const WaveCall& c, hipStream_t st, hipEvent_t* ev) {
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
+  if (c.L->wave == 2) {
+    // x and y of each plane in one launch (no y phase: its events follow at once)
+    hipError_t e = hipSuccess;
+    if (!dispatch_plane<R>(true, c, st, &e)) return hipErrorInvalidValue;
+    if (e != hipSuccess) return e;
+    mark(1);
+    mark(2);
+    if (!dispatch_z<R>(c, st)) return hipErrorInvalidValue;
+    mark(3);
+    mark(4);
+    dispatch_plane<R>(false, c, st, &e);
+    mark(5);
+    return e != hipSuccess ? e : hipGetLastError();
+  }
+  if (!dispatch_x<R>(true, c, st)) return hipErrorInvalidValue;
+  mark(1);
+  if (!dispatch_y<R>(true, c, st)) return hipErrorInvalidValue;
+  mark(2);
+  if (!dispatch_z<R>(c, st)) return hipErrorInvalidValue;
+  mark(3);
+  dispatch_y<R>(false, c, st);
+  mark(4);
+  dispatch_x<R>(false, c, st);
+  mark(5);
+  return hipGetLastError();
 }
 
 }  // namespace
 
 template <typename R>
 hipError_t launch_wave_fft(const SpmeLaunch& L, hipStream_t st, hipEvent_t* ev) {
-  auto mark = [&](int i) {
-    if (ev) (void)hipEventRecord(ev[i], st);
-  };
-  if (L.wave == 2) {
-    // x and y of each plane in one launch (no y phase: its events follow at once)
-    hipError_t e = hipSuccess;
-    if (!dispatch_plane<R>(true, L, st, &e)) return hipErrorInvalidValue;
-    if (e != hipSuccess) return e;
-    mark(1);
-    mark(2);
-    if (!dispatch_z<R>(L, st)) return hipErrorInvalidValue;
-    mark(3);
-    mark(4);
-    dispatch_plane<R>(false, L, st, &e);
-    mark(5);
-    return e != hipSuccess ? e : hipGetLastError();
-  }
-  if (!dispatch_x<R>(true, L, st)) return hipErrorInvalidValue;
-  mark(1);
-  if (!dispatch_y<R>(true, L, st)) return hipErrorInvalidValue;
-  mark(2);
-  if (!dispatch_z<R>(L, st)) return hipErrorInvalidValue;
-  mark(3);
-  dispatch_y<R>(false, L, st);
-  mark(4);
-  dispatch_x<R>(false, L, st);
-  mark(5);
-  return hipGetLastError();
+  return run_wave_fft<R>(WaveCall{&L, nullptr, 1}, st, ev);
 }
 template hipError_t launch_wave_fft<double>(const SpmeLaunch&, hipStream_t, hipEvent_t*);
 template hipError_t launch_wave_fft<float>(const SpmeLaunch&, hipStream_t, hipEvent_t*);
+
+template <typename R>
+hipError_t launch_wave_fft_batch(const SpmeLaunch& L, const WaveIo* bt, int count,
+                                 hipStream_t st) {
+  if (count < 1) return hipSuccess;
+  return run_wave_fft<R>(WaveCall{&L, bt, count}, st, nullptr);
+}
+template hipError_t launch_wave_fft_batch<double>(const SpmeLaunch&, const WaveIo*, int,
+                                                  hipStream_t);
+template hipError_t launch_wave_fft_batch<float>(const SpmeLaunch&, const WaveIo*, int,
+                                                 hipStream_t);
 
 }  // namespace ewamd

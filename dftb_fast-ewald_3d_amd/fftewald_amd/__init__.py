@@ -73,14 +73,16 @@ class Tuning(C.Structure):
     _fields_ = [("spread_direct", C.c_int), ("spread_block", C.c_int * 3),
                 ("fft_planes", C.c_int), ("fft_lines", C.c_int * 3), ("fft_radix", C.c_int * 3),
                 ("pair_cosched", C.c_int), ("pair_spread_share", C.c_int),
-                ("eval_us", C.c_float), ("fft_wave", C.c_int), ("fft_lanes", C.c_int * 3)]
+                ("eval_us", C.c_float), ("fft_wave", C.c_int), ("fft_lanes", C.c_int * 3),
+                ("pair_blocks", C.c_int), ("pair_fork", C.c_int)]
 
     def as_dict(self):
         return {"spread_direct": self.spread_direct, "spread_block": tuple(self.spread_block),
                 "fft_planes": self.fft_planes, "fft_lines": tuple(self.fft_lines),
                 "fft_radix": tuple(self.fft_radix), "pair_cosched": self.pair_cosched,
                 "pair_spread_share": self.pair_spread_share, "eval_us": self.eval_us,
-                "fft_wave": self.fft_wave, "fft_lanes": tuple(self.fft_lanes)}
+                "fft_wave": self.fft_wave, "fft_lanes": tuple(self.fft_lanes),
+                "pair_blocks": self.pair_blocks, "pair_fork": self.pair_fork}
 
     @classmethod
     def from_dict(cls, d):
@@ -90,6 +92,8 @@ class Tuning(C.Structure):
         t.pair_cosched = int(d.get("pair_cosched", 0))
         t.pair_spread_share = int(d.get("pair_spread_share", 0))
         t.fft_wave = int(d.get("fft_wave", 0))
+        t.pair_blocks = int(d.get("pair_blocks", 0))
+        t.pair_fork = int(d.get("pair_fork", 0))
         for a in range(3):
             t.fft_lanes[a] = int(d.get("fft_lanes", (0, 0, 0))[a])
             t.spread_block[a] = int(d["spread_block"][a])

@@ -46,6 +46,8 @@ module fftewald_amd
       real(c_float) :: eval_us = 0.0_c_float
       integer(c_int) :: fft_wave = 0
       integer(c_int) :: fft_lanes(3) = 0
+      integer(c_int) :: pair_blocks = 0
+      integer(c_int) :: pair_fork = 0
    end type ew_tuning
 
    !> Mirrors struct ew_options.

@@ -83,6 +83,11 @@ typedef struct ew_tuning {
                            0 = the Stockham passes                                          */
   int fft_lanes[3];     /* fft_wave: lanes per line of the x, y and z transforms (4, 8 or 16;
                            the plane kernels use fft_lanes[0] for x and y)                   */
+  int pair_blocks;      /* workgroups of the real-space pair sum when it runs beside the chain
+                           on the auxiliary stream (fewer leave the chain more CUs and HBM
+                           bandwidth); 0: 512                                                 */
+  int pair_fork;        /* 0: the pair sum starts with the evaluation, beside the spreading;
+                           1: after the spreading, beside the FFTs and the mesh gather        */
 } ew_tuning;
 
 /* Options; zero-initialise and set what you need.  Fields <= 0 mean "choose automatically",

@@ -51,6 +51,9 @@ def _shapes(base):
         out.append(dict(base, fft_wave=0, fft_radix=(r, (r + 1) % 5, (r + 2) % 5)))
     for co, share in ((1, 0), (1, 35), (1, 100), (0, 0)):
         out.append(dict(base, fft_wave=0, pair_cosched=co, pair_spread_share=share))
+    for nb in (64, 256, 1024):
+        out.append(dict(base, pair_blocks=nb))
+        out.append(dict(base, pair_blocks=nb, pair_fork=1))
     return out
 
 
@@ -118,11 +121,12 @@ def test_autotuned_shapes_agree_and_are_cached(tmp_path, monkeypatch):
             e.set_geometry(w["coords"], w["lattice"], **geo)
             again = e.tuning()
         for k in ("spread_direct", "spread_block", "fft_planes", "fft_lines", "fft_radix",
-                  "pair_cosched", "pair_spread_share", "fft_wave", "fft_lanes"):
+                  "pair_cosched", "pair_spread_share", "fft_wave", "fft_lanes", "pair_blocks",
+                  "pair_fork"):
             assert again[k] == tuned[k], k
     lines = open(tmp_path / "tune.cache").read().splitlines()
-    assert len(lines) == 2 and all(l.startswith("v3:") for l in lines)
-    assert all(len(l.split()) == 19 for l in lines)
+    assert len(lines) == 2 and all(l.startswith("v5:") for l in lines)
+    assert all(len(l.split()) == 21 for l in lines)
 
 
 # every compiled (axis length, lanes per line) of the register line transforms appears on
