@@ -2578,6 +2578,8 @@ extern "C" int ew_grid_memory(int n, const double* charges, const double* latvec
 // grouped-kernel view of a handle
 // ------------------------------------------------------------------------------------------
 
+SpmeLaunch ewamd::spme_launch_of(const ew_handle* h) { return make_spme_launch(h); }
+
 void ewamd::fill_desc(const ew_handle* h, StructDesc* d) {
   *d = StructDesc{};
   const int* K = h->prm.K;

@@ -70,6 +70,9 @@ inline int blocks_of(const StructDesc& d, int kind) {
 struct Staged {
   std::vector<int> words;
   size_t io_offset = 0;  // in bytes
+  size_t wave_offset = 0;  // in bytes: the batched register FFTs' table (wave_n entries)
+  int wave_n = 0;
+  int wave_sid = 0;  // a structure of the batch (its mesh, lanes and twiddle tables)
   int offs[kGroupKinds] = {}, counts[kGroupKinds] = {}, blocks[kGroupKinds] = {};
   size_t smem[kGroupKinds] = {};
 };
@@ -116,6 +119,18 @@ inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bo
   return true;
 }
 
+// Appends the meshes, spectra and influence tables of the staged evaluations (in their order)
+// for one batched register-FFT sequence.
+inline void stage_wave_io(const std::vector<WaveIo>& w, Staged* S) {
+  while ((S->words.size() * sizeof(int)) % alignof(WaveIo)) S->words.push_back(0);
+  S->wave_offset = S->words.size() * sizeof(int);
+  S->wave_n = (int)w.size();
+  const size_t nw = (S->wave_offset + sizeof(WaveIo) * w.size() + sizeof(int) - 1) / sizeof(int);
+  S->words.resize(nw, 0);
+  std::memcpy(reinterpret_cast<char*>(S->words.data()) + S->wave_offset, w.data(),
+              sizeof(WaveIo) * w.size());
+}
+
 // The launch over a device copy (at dev) of S's words, with the descriptor table at desc_dev.
 inline GroupLaunch group_launch_of(const Staged& S, const StructDesc* desc_dev, const void* dev,
                                    int order, bool persistent) {
@@ -130,6 +145,10 @@ inline GroupLaunch group_launch_of(const Staged& S, const StructDesc* desc_dev, 
                         S.blocks[k]};
     G.blocks[k] = S.blocks[k];
     G.smem[k] = S.smem[k];
+  }
+  if (S.wave_n > 0) {
+    G.wave_n = S.wave_n;
+    G.wave_io = reinterpret_cast<const WaveIo*>(static_cast<const char*>(dev) + S.wave_offset);
   }
   return G;
 }

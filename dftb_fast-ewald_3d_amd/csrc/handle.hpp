@@ -326,5 +326,7 @@ namespace ewamd {
 int handle_create(const ew_options& o, hipStream_t borrowed, ew_handle** out);
 // The grouped kernels' view of a handle with a geometry.
 void fill_desc(const ew_handle* h, StructDesc* d);
+// The reciprocal chain's launch description of a handle with a geometry.
+SpmeLaunch spme_launch_of(const ew_handle* h);
 
 }  // namespace ewamd

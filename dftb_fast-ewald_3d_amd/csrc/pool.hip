@@ -152,11 +152,32 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
   for (int i = 0; i < count; ++i) io[i] = IoSlot{dq[i], shift[i], energies + i, ids[i], 0};
   if (!stage_group(P->desc.data(), io.data(), count, P->fp32, false, S))
     return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
+  // members of one mesh shape, register-transform mode and lanes (replicas of one cell) run
+  // their FFTs as one batched register-transform sequence
+  const ew_handle* h0 = P->members[ids[0]];
+  bool same = h0->wave != 0;
+  for (int i = 1; i < count && same; ++i) {
+    const ew_handle* h = P->members[ids[i]];
+    same = h->wave == h0->wave;
+    for (int a = 0; a < 3; ++a)
+      same = same && h->prm.K[a] == h0->prm.K[a] && h->lanes[a] == h0->lanes[a];
+  }
+  if (same) {
+    std::vector<WaveIo> w((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      const ew_handle* h = P->members[ids[i]];
+      w[i] = WaveIo{h->grid, h->cgrid, h->inf_tab};
+    }
+    stage_wave_io(w, S);
+    S->wave_sid = ids[0];
+  }
   return EW_OK;
 }
 
 GroupLaunch launch_of(const ew_pool* P, const Staged& S, const void* dev) {
-  return group_launch_of(S, P->desc_dev, dev, P->order, P->opt.group_grid == 1);
+  GroupLaunch G = group_launch_of(S, P->desc_dev, dev, P->order, P->opt.group_grid == 1);
+  if (G.wave_n > 0) G.wave_l = spme_launch_of(P->members[S.wave_sid]);
+  return G;
 }
 
 int check_ids(ew_pool* P, int count, const int* ids) {
