@@ -118,14 +118,14 @@ def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0, tuni
         ln = tuning.get("fft_lanes", (0, 0, 0))
         K = list(grid)
         if wave == 2:
-            return {"fft_x_r2c": f"void ewamd::k_wplane_r2c<{r}, {K[0]}, {ln[0]}>",
-                    "fft_z_influence": f"void ewamd::k_wfft_z<{r}, {K[2] // ln[2]}, {ln[2]}>",
-                    "fft_x_c2r": f"void ewamd::k_wplane_c2r<{r}, {K[0]}, {ln[0]}>"}.get(span)
-        return {"fft_x_r2c": f"void ewamd::k_wfft_x_r2c<{r}, {K[0] // ln[0]}, {ln[0]}>",
-                "fft_y": f"void ewamd::k_wfft_lines<{r}, {K[1] // ln[1]}, {ln[1]}, -1>",
-                "fft_z_influence": f"void ewamd::k_wfft_z<{r}, {K[2] // ln[2]}, {ln[2]}>",
-                "fft_y_inverse": f"void ewamd::k_wfft_lines<{r}, {K[1] // ln[1]}, {ln[1]}, 1>",
-                "fft_x_c2r": f"void ewamd::k_wfft_x_c2r<{r}, {K[0] // ln[0]}, {ln[0]}>"}.get(span)
+            return {"fft_x_r2c": f"void ewamd::k_wplane_r2c<{r}, {K[0]}, {ln[0]}, false>",
+                    "fft_z_influence": f"void ewamd::k_wfft_z<{r}, {K[2] // ln[2]}, {ln[2]}, false>",
+                    "fft_x_c2r": f"void ewamd::k_wplane_c2r<{r}, {K[0]}, {ln[0]}, false>"}.get(span)
+        return {"fft_x_r2c": f"void ewamd::k_wfft_x_r2c<{r}, {K[0] // ln[0]}, {ln[0]}, false>",
+                "fft_y": f"void ewamd::k_wfft_lines<{r}, {K[1] // ln[1]}, {ln[1]}, -1, false>",
+                "fft_z_influence": f"void ewamd::k_wfft_z<{r}, {K[2] // ln[2]}, {ln[2]}, false>",
+                "fft_y_inverse": f"void ewamd::k_wfft_lines<{r}, {K[1] // ln[1]}, {ln[1]}, 1, false>",
+                "fft_x_c2r": f"void ewamd::k_wfft_x_c2r<{r}, {K[0] // ln[0]}, {ln[0]}, false>"}.get(span)
     co = "true" if tuning.get("pair_cosched", 0) else "false"
     direct = int(tuning.get("spread_direct", 0))
     spread = {0: f"void ewamd::k_spread_region<{r}, {order}, {co}>",

@@ -1468,7 +1468,7 @@ static SpmeLaunch make_spme_launch(const ew_handle* h) {
   L.py = h->py;
   L.pz = h->pz;
   L.inf = influence_args(h);
-  L.wave = h->wave ? 1 : 0;
+  L.wave = h->wave;
   for (int a = 0; a < 3; ++a) {
     L.lanes[a] = h->lanes[a];
     L.twp[a] = h->twp[a];
