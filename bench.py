@@ -29,11 +29,13 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False):
+def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, inf_table=False):
     """Algorithmic (compulsory) bytes of one launch of each phase.
 
     rs: mesh real size, cs: pair coefficient size, ib: partner index size (2 or 4 bytes);
-    planes: the x and y passes are one fused kernel per direction (the y phases move nothing)."""
+    planes: the x and y passes are one fused kernel per direction (the y phases move nothing);
+    inf_table: the z pass reads the influence function from a per-geometry table (one real per
+    half-spectrum point) instead of evaluating it."""
     G = K[0] * K[1] * K[2]
     Gh = (K[0] // 2 + 1) * K[1] * K[2]
     th = 3 * order * rs
@@ -42,7 +44,7 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False):
         "spread": (16 + th + 8) * n + rs * G,
         "fft_x_r2c": rs * G + 2 * rs * Gh,
         "fft_y": 0 if planes else 4 * rs * Gh,
-        "fft_z_influence": 4 * rs * Gh,
+        "fft_z_influence": (5 if inf_table else 4) * rs * Gh,
         "fft_y_inverse": 0 if planes else 4 * rs * Gh,
         "fft_x_c2r": 2 * rs * Gh + rs * G,
         "pair_sum": nnz * (ib + cs) + n * (8 + 8 + 8),
@@ -51,7 +53,7 @@ def phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False):
 
 
 def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, tuning=None,
-                split_gather=False):
+                split_gather=False, inf_table=False):
     """Algorithmic bytes of each span of ew_trace_times (the evaluation's launch shape).
 
     The privatised spreading's region kernel reads the atoms' stencil origins, weights and
@@ -60,7 +62,7 @@ def trace_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes=False, region=True, 
     region kernel writes the mesh).  Without privatised blocks the whole spreading is the
     assembly span.  With the pair sum co-scheduled in the chain's launches its bytes go to
     those spans in the proportions of its row split."""
-    pb = phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes)
+    pb = phase_bytes(n, K, order, nnz, rs, cs, ib, gamma, planes, inf_table)
     G = K[0] * K[1] * K[2]
     tuning = tuning or {}
     co = bool(tuning.get("pair_cosched", 0))
@@ -578,6 +580,8 @@ def run_mesh(args, torch, ew, W):
                                 precision=prec, profile=True) as e:
                 e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
                 prm = e.parameters()
+                tun = e.tuning()
+                wave = int(tun.get("fft_wave", 0))
                 for _ in range(3):
                     e.shift_device(dq.data_ptr(), out.data_ptr(), en.data_ptr())
                 e.phase_times(reset=True)
@@ -585,10 +589,11 @@ def run_mesh(args, torch, ew, W):
                     e.shift_device(dq.data_ptr(), out.data_ptr(), en.data_ptr())
                 ph, calls = e.phase_times()
             rs = 4 if prec == "mixed" else 8
-            planes = bool(prm.get("fft_planes", 0))
+            planes = fused_planes(prm, tun)
             pb = phase_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
-                             prm["index_bytes"], False, planes)
-            row = {"mesh": list(prm["grid"]), "precision": prec, "fft_planes": int(planes)}
+                             prm["index_bytes"], False, planes, wave > 0)
+            row = {"mesh": list(prm["grid"]), "precision": prec, "fft_planes": int(planes),
+                   "fft_wave": wave}
             tot_b, tot_ms = 0, 0.0
             for k in keys:
                 ms = ph[k] / calls
@@ -894,7 +899,7 @@ def main():
         split = bool(prm.get("gather_split", 0))
         tb = trace_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
                          prm["index_bytes"], False, planes, trace["spread_region"] > 0, tuning,
-                         split)
+                         split, int(tuning.get("fft_wave", 0)) > 0)
         crit = critical_path(trace, tcalls, tb, prec, prm, ms, tuning, split)
         print(f"[bench] {prec} critical path: chain {crit['chain_us']:.1f} us "
               f"({crit['chain_frac']:.3f} of HBM peak), dominant {crit['dominant']['span']} "
@@ -911,7 +916,7 @@ def main():
                     e.shift_device(dqs[i % 4].data_ptr(), out.data_ptr(), en.data_ptr(), sp)
                 phases, calls = e.phase_times()
         pb = phase_bytes(n, prm["grid"], prm["order"], prm["n_pairs"], rs, rs,
-                         prm["index_bytes"], False, planes)
+                         prm["index_bytes"], False, planes, int(tuning.get("fft_wave", 0)) > 0)
         ph = {}
         for k, v in phases.items():
             avg_ms = v / calls
