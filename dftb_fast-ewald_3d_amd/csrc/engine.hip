@@ -501,16 +501,38 @@ static bool region_shape(const ew_handle* h, const int* f, size_t max_bytes, Reg
   return true;
 }
 
+// Sort tiles per axis whose atoms an owned block of rg's shape may have to read: a block starts
+// on a tile boundary, so the stencil origins lo-(p-1) .. lo+S-1 it gathers lie in its own f
+// tiles and the ceil((p-1)/T) tiles before them, one more when they wrap through a partial
+// last tile (K not a multiple of T).
+static void owned_tile_span(const ew_handle* h, const RegionArgs& rg, int* tcm) {
+  for (int a = 0; a < 3; ++a) {
+    const int T = h->T[a];
+    const int partial = h->prm.K[a] % T ? 1 : 0;
+    tcm[a] = std::min(h->nt[a], rg.f[a] + (h->prm.order - 1 + T - 1) / T + partial);
+  }
+}
+
+// Entries of an owned block's item list: p planes for every atom the block may read (the tiles
+// of owned_tile_span at the geometry's fullest tile), so one pass takes them all; at most
+// kOwnedItems (a block with more atoms takes several passes).
+static int owned_item_count(const ew_handle* h, const RegionArgs& rg) {
+  int tcm[3];
+  owned_tile_span(h, rg, tcm);
+  const long long atoms = (long long)std::max(1, h->max_tile_atoms) * tcm[0] * tcm[1] * tcm[2];
+  const long long want = (atoms * h->prm.order + 63) / 64 * 64;
+  return (int)std::max<long long>(h->prm.order, std::min<long long>(kOwnedItems, want));
+}
+
 // Owned blocks can take a block shape if the block copy and the item list fit the LDS, the tile
 // runs fit the kernel's tables and an item (atom << 4 | plane) fits 31 bits.
 static bool owned_fits(const ew_handle* h, const RegionArgs& rg) {
   int tcm[3];
-  for (int a = 0; a < 3; ++a) {
-    const int T = h->T[a];
-    tcm[a] = std::min(h->nt[a], (rg.S[a] + h->prm.order - 1 + T - 1) / T + 1);
-  }
-  return owned_smem(rg.S) + kOwnedStatic <= kRegionMaxLds && 2 * tcm[1] * tcm[2] <= kOwnedMaxRuns &&
-         h->prm.order <= 15 && h->n < (1 << 27);
+  owned_tile_span(h, rg, tcm);
+  RegionArgs sized = rg;
+  sized.items = owned_item_count(h, rg);
+  return owned_smem(sized) + kOwnedStatic <= kRegionMaxLds &&
+         2 * tcm[1] * tcm[2] <= kOwnedMaxRuns && h->prm.order <= 15 && h->n < (1 << 27);
 }
 
 // The spreading's write-out by the fixed rule: owned blocks (on the 98,304-atom cell at 128^3,
@@ -609,6 +631,7 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
     rg.direct = t.spread_direct == 2 ? 2 : (t.spread_direct && h->own_stream ? 1 : 0);
     if (rg.direct == 2 && !owned_fits(h, rg))
       return h->fail(EW_ERR_ARGUMENT, "spreading block does not fit the mesh or the LDS");
+    if (rg.direct == 2) rg.items = owned_item_count(h, rg);
   } else {
     for (int a = 0; a < 3; ++a) t.spread_block[a] = 0;
   }
@@ -632,7 +655,7 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
     h->region.scratch = scr;
   }
   if (sizeof(double) * (size_t)rg.region_stride > 65536 ||
-      (rg.direct == 2 && owned_smem(rg.S) > 65536))
+      (rg.direct == 2 && owned_smem(rg) > 65536))
     HIPCHK(h, prepare_region_kernels());
   h->px = make_plan(K[0], t.fft_lines[0], t.fft_radix[0]);
   h->py = make_plan(K[1], t.fft_lines[1], t.fft_radix[1]);
@@ -1215,6 +1238,7 @@ extern "C" int ew_set_geometry(ew_handle* h, int n, const double* coords, const 
     HIPCHK(h, launch_order(og, ob, s));
     HIPCHK(h, hipMemcpyAsync(&max_tile_atoms, d_max, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
+    h->max_tile_atoms = max_tile_atoms;
   }
   // FFT tables, from host copies that outlive their asynchronous uploads
   for (int a = 0; a < 3; ++a) {

@@ -33,7 +33,7 @@ inline size_t smem_of(const StructDesc& d, int kind, bool fp32) {
     case kGkPlane:
       return d.plane ? plane_smem(d.px, d.py, cs) : 0;
     case kGkOwned:
-      return d.spread_mode == 3 ? owned_smem(d.region.S) : 0;
+      return d.spread_mode == 3 ? owned_smem(d.region) : 0;
     default:
       return 0;
   }

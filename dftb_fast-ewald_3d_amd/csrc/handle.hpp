@@ -55,6 +55,7 @@ struct ew_handle {
   int pair_mode = 0;
   long long nnz = 0;
   int T[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, ntmax = 0;
+  int max_tile_atoms = 0;  // atoms of the fullest sort tile
   int spread_slots = 0;  // atoms staged per chunk by the interval spread (0: checked spread)
   ewamd::RegionArgs region{};   // privatised spreading (scratch == null: not used)
   ewamd::FftPlan px{}, py{}, pz{};

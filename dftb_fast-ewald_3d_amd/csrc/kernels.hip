@@ -1122,7 +1122,7 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
     for (int r = 0; r < nruns; ++r) run_off[r + 1] += run_off[r];
   __syncthreads();
   const int natoms = run_off[nruns];
-  const int chunk = kOwnedItems / p;  // atoms per pass: each keeps at most p items
+  const int chunk = owned_items(g) / p;  // atoms per pass: each keeps at most p items
   for (int j0 = 0; j0 < natoms; j0 += chunk) {
     const int j1 = min(natoms, j0 + chunk);
     // pass 1
@@ -1154,7 +1154,8 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
     // pass 2
     if (P > 0) {
       constexpr int PP = P > 0 ? P : 1;
-      constexpr int NB = kSpreadBatch;
+      // FP32 weights: two items in flight per thread, so that eight waves per SIMD fit
+      constexpr int NB = sizeof(R) == 4 ? 2 : kSpreadBatch;
       for (int base = threadIdx.x; base < m; base += NB * blockDim.x) {
         int4 kk[NB];
         double c3[NB];
@@ -1238,8 +1239,11 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
   }
 }
 
+// FP32 weights leave room for eight waves per SIMD (at most 64 VGPRs): with item lists sized
+// to the block, eight 256-thread blocks then share a CU.
 template <typename R, int P, bool CO>
-__global__ __launch_bounds__(256) void k_spread_owned(const int4* __restrict__ k0,
+__global__ __launch_bounds__(256, (sizeof(R) == 4 && !CO) ? 8 : 1) void k_spread_owned(
+    const int4* __restrict__ k0,
                                                       const R* __restrict__ theta,
                                                       const double* __restrict__ q,
                                                       const int* __restrict__ tile_start,
@@ -3260,7 +3264,7 @@ static hipError_t launch_spme_t(const SpmeLaunch& L, hipStream_t st, hipEvent_t*
     dim3 rgrid;
     const PairCo rco = co_for(nblk, 256, true, &rgrid);
     if (rg.direct == 2) {
-      const size_t osmem = owned_smem(rg.S);
+      const size_t osmem = owned_smem(rg);
 #define EW_OWNED(PP)                                                                          \
   hipLaunchKernelGGL((k_spread_owned<R, PP, CO>), rgrid, dim3(256), osmem, st, L.k0, th, L.q,      \
                      L.tile_start, rg, grid, L.qperm, rco)

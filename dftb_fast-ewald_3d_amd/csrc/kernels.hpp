@@ -155,6 +155,8 @@ struct RegionArgs {
   int direct;      // 1: each block adds its LDS copy into the (cleared) mesh with device-scope
                    // atomics instead of writing it to scratch for the assembly (single-structure
                    // evaluations; the grouped kernels keep the copies)
+  int items;       // owned blocks (direct 2): entries of the (atom, plane) item list, 0 for
+                   // kOwnedItems
 };
 
 // LDS budget of a block copy: the fixed rule keeps copies within 60 KiB (several workgroups per
@@ -162,11 +164,13 @@ struct RegionArgs {
 constexpr size_t kRegionMaxBytes = 60 * 1024;
 constexpr int kSpreadBatch = 3;  // (atom, z plane) items whose loads a thread issues together
 constexpr size_t kRegionMaxLds = 160 * 1024;
-// Owned blocks: the (atom, z plane) items of one pass live in an LDS list of kOwnedItems
-// entries (4 bytes each) after the block copy; the tile runs (at most kOwnedMaxRuns) in LDS
-// tables.  A shape whose runs exceed the table is refused by the host.
+// Owned blocks: the (atom, z plane) items of one pass live in an LDS list after the block copy
+// (4 bytes an entry; at most kOwnedItems, sized by the host to the atoms a block can reach so
+// that more workgroups share a CU; a block with more atoms takes several passes); the tile runs
+// (at most kOwnedMaxRuns) in LDS tables.  A shape whose runs exceed the table is refused by the
+// host.
 constexpr int kOwnedItems = 6144;
-constexpr int kOwnedMaxRuns = 256;
+constexpr int kOwnedMaxRuns = 128;
 constexpr size_t kOwnedStatic = 4096;  // LDS kept for the kernel's static tables
 // The block copy has odd row and plane pitches (in FP64 elements), so that the planes of one
 // atom, which neighbouring lanes add at once, fall in different LDS banks.
@@ -174,8 +178,12 @@ inline __host__ __device__ int owned_row_pitch(const int* S) { return S[0] | 1; 
 inline __host__ __device__ int owned_plane_pitch(const int* S) {
   return (owned_row_pitch(S) * S[1]) | 1;
 }
-inline size_t owned_smem(const int* S) {
-  return sizeof(double) * (size_t)owned_plane_pitch(S) * S[2] + sizeof(int) * (size_t)kOwnedItems;
+inline __host__ __device__ int owned_items(const RegionArgs& g) {
+  return g.items > 0 ? g.items : kOwnedItems;
+}
+inline size_t owned_smem(const RegionArgs& g) {
+  return sizeof(double) * (size_t)owned_plane_pitch(g.S) * g.S[2] +
+         sizeof(int) * (size_t)owned_items(g);
 }
 // Raises the dynamic LDS limit of the region spreading kernels to kRegionMaxLds (once).
 hipError_t prepare_region_kernels();
