@@ -531,7 +531,7 @@ static bool owned_fits(const ew_handle* h, const RegionArgs& rg) {
   owned_tile_span(h, rg, tcm);
   RegionArgs sized = rg;
   sized.items = owned_item_count(h, rg);
-  return owned_smem(sized) + kOwnedStatic <= kRegionMaxLds &&
+  return owned_smem(sized, h->fp32) + kOwnedStatic <= kRegionMaxLds &&
          2 * tcm[1] * tcm[2] <= kOwnedMaxRuns && h->prm.order <= 15 && h->n < (1 << 27);
 }
 
@@ -655,7 +655,7 @@ static int apply_tuning(ew_handle* h, const ew_tuning& t_in) {
     h->region.scratch = scr;
   }
   if (sizeof(double) * (size_t)rg.region_stride > 65536 ||
-      (rg.direct == 2 && owned_smem(rg) > 65536))
+      (rg.direct == 2 && owned_smem(rg, h->fp32) > 65536))
     HIPCHK(h, prepare_region_kernels());
   h->px = make_plan(K[0], t.fft_lines[0], t.fft_radix[0]);
   h->py = make_plan(K[1], t.fft_lines[1], t.fft_radix[1]);
@@ -1982,9 +1982,7 @@ static GroupLaunch multi_launch(const ew_handle* h, int nvec, const Staged& S, c
   G.join_ev = h->join_ev;
   if (h->wave) {
     // every vector has this geometry's mesh: one batched register-FFT sequence
-    G.wave_n = nvec;
-    G.wave_io = h->views.wio;
-    G.wave_l = make_spme_launch(h);
+    G.wave.assign(1, WaveGroup{nvec, h->views.wio, make_spme_launch(h)});
   }
   return G;
 }
@@ -1995,7 +1993,9 @@ static int stage_multi(ew_handle* h, int nvec, const double* d_dq, double* d_shi
   std::vector<IoSlot> io((size_t)nvec);
   for (int v = 0; v < nvec; ++v)
     io[v] = IoSlot{d_dq + v * n, d_shift + v * n, d_energy + v, v, 0};
-  if (!stage_group(h->views_host.data(), io.data(), nvec, h->fp32, !h->half, S))
+  // with register transforms the views' FFTs are one batched sequence (multi_launch)
+  const std::vector<unsigned char> by_wave((size_t)nvec, h->wave ? 1 : 0);
+  if (!stage_group(h->views_host.data(), io.data(), nvec, h->fp32, !h->half, S, by_wave.data()))
     return h->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
   return EW_OK;
 }

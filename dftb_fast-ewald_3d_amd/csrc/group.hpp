@@ -33,7 +33,7 @@ inline size_t smem_of(const StructDesc& d, int kind, bool fp32) {
     case kGkPlane:
       return d.plane ? plane_smem(d.px, d.py, cs) : 0;
     case kGkOwned:
-      return d.spread_mode == 3 ? owned_smem(d.region) : 0;
+      return d.spread_mode == 3 ? owned_smem(d.region, fp32) : 0;
     default:
       return 0;
   }
@@ -70,18 +70,24 @@ inline int blocks_of(const StructDesc& d, int kind) {
 struct Staged {
   std::vector<int> words;
   size_t io_offset = 0;  // in bytes
-  size_t wave_offset = 0;  // in bytes: the batched register FFTs' table (wave_n entries)
-  int wave_n = 0;
-  int wave_sid = 0;  // a structure of the batch (its mesh, lanes and twiddle tables)
+  // batched register-transform FFT groups: each group's table (in bytes), its evaluations and a
+  // structure of the group (its mesh, lanes and twiddle tables)
+  std::vector<size_t> wave_offset;
+  std::vector<int> wave_n, wave_sid;
   int offs[kGroupKinds] = {}, counts[kGroupKinds] = {}, blocks[kGroupKinds] = {};
   size_t smem[kGroupKinds] = {};
 };
 
+inline bool fft_kind(int k) {
+  return k == kGkFftX || k == kGkFftY || k == kGkFftZ || k == kGkPlane;
+}
+
 // Stages `count` evaluations; descs is indexed by IoSlot::sid.  With skip_pairs the pair kind
-// gets no blocks (the launch's multi-vector pair sum covers it).  False when a kind needs more
-// LDS than a workgroup has.
+// gets no blocks (the launch's multi-vector pair sum covers it); evaluations with wave_fft[i] set
+// get no blocks in the FFT kinds (a batched register-transform group covers them).  False when a
+// kind needs more LDS than a workgroup has.
 inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bool fp32,
-                        bool skip_pairs, Staged* S) {
+                        bool skip_pairs, Staged* S, const unsigned char* wave_fft = nullptr) {
   S->words.clear();
   for (int k = 0; k < kGroupKinds; ++k) {
     std::vector<int> prefix, slot;
@@ -89,7 +95,8 @@ inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bo
     size_t sm = 0;
     for (int i = 0; i < count; ++i) {
       const StructDesc& d = descs[io[i].sid];
-      const int b = (skip_pairs && k == kGkPairs) ? 0 : blocks_of(d, k);
+      const bool skip = (skip_pairs && k == kGkPairs) || (wave_fft && wave_fft[i] && fft_kind(k));
+      const int b = skip ? 0 : blocks_of(d, k);
       if (b <= 0) continue;
       prefix.push_back(run);
       slot.push_back(i);
@@ -119,16 +126,17 @@ inline bool stage_group(const StructDesc* descs, const IoSlot* io, int count, bo
   return true;
 }
 
-// Appends the meshes, spectra and influence tables of the staged evaluations (in their order)
-// for one batched register-FFT sequence.
-inline void stage_wave_io(const std::vector<WaveIo>& w, Staged* S) {
+// Appends one batched register-FFT group: the meshes, spectra and influence tables of its
+// evaluations; sid: a structure of the group.
+inline void stage_wave_io(const std::vector<WaveIo>& w, int sid, Staged* S) {
   while ((S->words.size() * sizeof(int)) % alignof(WaveIo)) S->words.push_back(0);
-  S->wave_offset = S->words.size() * sizeof(int);
-  S->wave_n = (int)w.size();
-  const size_t nw = (S->wave_offset + sizeof(WaveIo) * w.size() + sizeof(int) - 1) / sizeof(int);
+  const size_t off = S->words.size() * sizeof(int);
+  S->wave_offset.push_back(off);
+  S->wave_n.push_back((int)w.size());
+  S->wave_sid.push_back(sid);
+  const size_t nw = (off + sizeof(WaveIo) * w.size() + sizeof(int) - 1) / sizeof(int);
   S->words.resize(nw, 0);
-  std::memcpy(reinterpret_cast<char*>(S->words.data()) + S->wave_offset, w.data(),
-              sizeof(WaveIo) * w.size());
+  std::memcpy(reinterpret_cast<char*>(S->words.data()) + off, w.data(), sizeof(WaveIo) * w.size());
 }
 
 // The launch over a device copy (at dev) of S's words, with the descriptor table at desc_dev.
@@ -146,9 +154,12 @@ inline GroupLaunch group_launch_of(const Staged& S, const StructDesc* desc_dev, 
     G.blocks[k] = S.blocks[k];
     G.smem[k] = S.smem[k];
   }
-  if (S.wave_n > 0) {
-    G.wave_n = S.wave_n;
-    G.wave_io = reinterpret_cast<const WaveIo*>(static_cast<const char*>(dev) + S.wave_offset);
+  // (the caller fills each group's launch parameters, wave[g].l)
+  G.wave.resize(S.wave_n.size());
+  for (size_t g = 0; g < S.wave_n.size(); ++g) {
+    G.wave[g].n = S.wave_n[g];
+    G.wave[g].io =
+        reinterpret_cast<const WaveIo*>(static_cast<const char*>(dev) + S.wave_offset[g]);
   }
   return G;
 }

@@ -654,10 +654,57 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fft_xy_c2r(const cx<R>* __restr
 // charge spreading
 // ----------------------------------------------------------------------------------------
 
+// round(x) as an int64, for |x| < 2^51: x + 1.5 * 2^52 keeps the exponent of 2^52, so the
+// rounding to an integer happens in that addition and the bit patterns differ by round(x).
+__device__ __forceinline__ long long fix_round(double x) {
+  constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+  return __double_as_longlong(x + kMagic) - __double_as_longlong(kMagic);
+}
+
+// Fixed-point accumulator of a spreading workgroup's mesh points.  Integer adds are exact and commute, so
+// a point's value does not depend on the order in which the lanes add its contributions: the
+// spreading is bitwise reproducible.  FP32 mesh: one int64 at 2^-36 (contributions below 2^15 in
+// magnitude, sums below 1.3e8).  FP64 mesh: an int32 at 2^-4, added only when not zero (most
+// contributions are below 1/32), and an int64 remainder at 2^-55 (contributions and sums below
+// 1.3e8; the remainder of each contribution, |r| <= 2^-5, is exact by Sterbenz's lemma and
+// rounded at 2^-56).
+template <typename R>
+struct OwnedAcc {
+  static constexpr bool F32 = sizeof(R) == 4;
+  unsigned long long* lo;  // FP32: the value; FP64: the remainder
+  int* hi;                 // FP64: the leading part
+  __device__ __forceinline__ OwnedAcc(unsigned long long* l, int* h) : lo(l), hi(h) {}
+  // the remainders, then (FP64) the leading parts of npts points from base
+  __device__ __forceinline__ OwnedAcc(unsigned char* base, int npts)
+      : lo(reinterpret_cast<unsigned long long*>(base)),
+        hi(reinterpret_cast<int*>(base + sizeof(unsigned long long) * (size_t)npts)) {}
+  __device__ __forceinline__ void clear(int i) {
+    lo[i] = 0ull;
+    if constexpr (!F32) hi[i] = 0;
+  }
+  __device__ __forceinline__ void add(int i, double v) {
+    if constexpr (F32) {
+      atomicAdd(lo + i, (unsigned long long)fix_round(v * 0x1p36));
+    } else {
+      const long long h = fix_round(v * 0x1p4);
+      const double r = v - (double)h * 0x1p-4;
+      if (h != 0) atomicAdd(hi + i, (int)h);
+      atomicAdd(lo + i, (unsigned long long)fix_round(r * 0x1p55));
+    }
+  }
+  __device__ __forceinline__ double value(int i) const {
+    const double l = (double)(long long)lo[i];
+    if constexpr (F32) return l * 0x1p-36;
+    else return (double)hi[i] * 0x1p-4 + l * 0x1p-55;
+  }
+};
+
+
 // One workgroup per output tile; atoms are listed by the tile of their stencil origin, and the
 // workgroup visits the source tiles whose stencils reach into it.  Each contribution lands in
-// exactly one output tile, so the mesh is written once, without global atomics or a clear.
-// Work items are (candidate atom, z offset) pairs, so every thread gets a similar share.
+// exactly one output tile, so the mesh is written once, without global atomics or a clear; the
+// tile accumulates in fixed point (OwnedAcc), so the result does not depend on the order of the
+// adds.  Work items are (candidate atom, z offset) pairs, so every thread gets a similar share.
 template <typename R>
 __device__ __forceinline__ void k_spread_checked_body(const int4* __restrict__ k0,
                                                 const R* __restrict__ theta,
@@ -665,7 +712,9 @@ __device__ __forceinline__ void k_spread_checked_body(const int4* __restrict__ k
                                                 const int* __restrict__ tile_start,
                                                 SpreadArgs g, R* __restrict__ grid, int bid) {
   constexpr int kMaxVisit = kMaxSrcTiles * kMaxSrcTiles * kMaxSrcTiles;
-  __shared__ R acc[kSpreadTileMax];
+  __shared__ unsigned long long acc_lo[kSpreadTileMax];
+  __shared__ int acc_hi[sizeof(R) == 4 ? 1 : kSpreadTileMax];
+  OwnedAcc<R> acc(acc_lo, acc_hi);
   __shared__ int src_count[kMaxVisit];
   __shared__ int src_first[kMaxVisit + 1];
   __shared__ int src_begin[kMaxVisit];
@@ -679,7 +728,7 @@ __device__ __forceinline__ void k_spread_checked_body(const int4* __restrict__ k
   const int w1 = min(g.T[0], g.K[0] - lo1), w2 = min(g.T[1], g.K[1] - lo2),
             w3 = min(g.T[2], g.K[2] - lo3);
   const int npts = w1 * w2 * w3;
-  for (int i = threadIdx.x; i < npts; i += blockDim.x) acc[i] = R(0);
+  for (int i = threadIdx.x; i < npts; i += blockDim.x) acc.clear(i);
   const int* e1 = g.src + ((size_t)0 * g.ntmax + t1) * (kMaxSrcTiles + 1);
   const int* e2 = g.src + ((size_t)1 * g.ntmax + t2) * (kMaxSrcTiles + 1);
   const int* e3 = g.src + ((size_t)2 * g.ntmax + t3) * (kMaxSrcTiles + 1);
@@ -717,25 +766,25 @@ __device__ __forceinline__ void k_spread_checked_body(const int4* __restrict__ k
     if (r1 < 0) r1 += g.K[0];
     if (r2 < 0) r2 += g.K[1];
     const R* th = theta + (size_t)a * 3 * p;
-    const R c3 = R(q[a]) * th[2 * p + o3];
+    const double c3 = q[a] * double(th[2 * p + o3]);
     for (int o2 = 0; o2 < p; ++o2) {
       int y = r2 + o2;
       if (y >= g.K[1]) y -= g.K[1];
       if (y >= w2) continue;
-      const R c23 = c3 * th[p + o2];
-      R* row = acc + (z * w2 + y) * w1;
+      const double c23 = c3 * double(th[p + o2]);
+      const int row = (z * w2 + y) * w1;
       for (int o1 = 0; o1 < p; ++o1) {
         int x = r1 + o1;
         if (x >= g.K[0]) x -= g.K[0];
         if (x >= w1) continue;
-        atomicAdd(row + x, c23 * th[o1]);
+        acc.add(row + x, c23 * double(th[o1]));
       }
     }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < npts; i += blockDim.x) {
     const int x = i % w1, y = (i / w1) % w2, z = i / (w1 * w2);
-    grid[((size_t)(lo3 + z) * g.K[1] + (lo2 + y)) * g.K[0] + lo1 + x] = acc[i];
+    grid[((size_t)(lo3 + z) * g.K[1] + (lo2 + y)) * g.K[0] + lo1 + x] = R(acc.value(i));
   }
 }
 
@@ -1053,7 +1102,8 @@ __global__ __launch_bounds__(256) void k_spread_region(const int4* __restrict__ 
 // row they form one or two contiguous runs of the sorted atoms.  Pass 1 walks those runs as one
 // flattened index and keeps, for each atom whose stencil reaches the block, its in-block z
 // planes as (atom, plane) items in an LDS list; pass 2 adds each item's in-block part of its
-// plane into an FP64 LDS copy of the block alone (no halo).  The copy is then stored into the
+// plane into a fixed-point LDS copy of the block alone (no halo; OwnedAcc).  The copy is then
+// stored into the
 // mesh: every mesh point is written once, with no copies, no assembly, no mesh clear and no
 // device atomics.  Atoms near a block face are read by the workgroups of both blocks.
 template <typename R, int P>
@@ -1071,9 +1121,9 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
   const int S0 = g.S[0], S1 = g.S[1], S2 = g.S[2];
   const int PX = owned_row_pitch(g.S), PZ = owned_plane_pitch(g.S);
   const int nreg = PZ * S2;
-  double* reg = reinterpret_cast<double*>(smem_raw);
-  int* items = reinterpret_cast<int*>(smem_raw + sizeof(double) * (size_t)nreg);
-  for (int i = threadIdx.x; i < nreg; i += blockDim.x) reg[i] = 0.0;
+  OwnedAcc<R> reg(smem_raw, nreg);
+  int* items = reinterpret_cast<int*>(smem_raw + owned_point_bytes(sizeof(R) == 4) * (size_t)nreg);
+  for (int i = threadIdx.x; i < nreg; i += blockDim.x) reg.clear(i);
   int blk[3];
   {
     int b = bid;
@@ -1197,7 +1247,7 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
             const int row = z * PZ + y * PX + r[0];
 #pragma unroll
             for (int o1 = 0; o1 < PP; ++o1)
-              if (vx[o1]) atomicAdd(reg + row + o1, c * double(w1[u][o1]));
+              if (vx[o1]) reg.add(row + o1, c * double(w1[u][o1]));
           }
         }
       }
@@ -1220,8 +1270,7 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
           const double c = c3 * double(th[p + o2]);
           for (int o1 = 0; o1 < p; ++o1) {
             const int x = r[0] + o1;
-            if ((unsigned)x < (unsigned)w[0])
-              atomicAdd(reg + z * PZ + y * PX + x, c * double(th[o1]));
+            if ((unsigned)x < (unsigned)w[0]) reg.add(z * PZ + y * PX + x, c * double(th[o1]));
           }
         }
       }
@@ -1234,7 +1283,7 @@ __device__ __forceinline__ void k_spread_owned_body(const int4* __restrict__ k0,
   for (int i = threadIdx.x; i < S0 * S1 * S2; i += blockDim.x) {
     if (wk.x < w[0] && wk.y < w[1] && wk.z < w[2])
       grid[((size_t)(lo[2] + wk.z) * g.K[1] + lo[1] + wk.y) * g.K[0] + lo[0] + wk.x] =
-          R(reg[wk.z * PZ + wk.y * PX + wk.x]);
+          R(reg.value(wk.z * PZ + wk.y * PX + wk.x));
     wk.next();
   }
 }
@@ -2896,10 +2945,11 @@ static hipError_t launch_group_t(const GroupLaunch& G, hipStream_t st) {
   if (run(kGkChecked))
     hipLaunchKernelGGL((kg_spread_checked<R, LOOP>), grid(kGkChecked), dim3(256), 0, st, D, io,
                        G.map[kGkChecked]);
-  if (G.wave_n > 0) {
-    const hipError_t e = launch_wave_fft_batch<R>(G.wave_l, G.wave_io, G.wave_n, st);
+  for (const WaveGroup& w : G.wave) {
+    const hipError_t e = launch_wave_fft_batch<R>(w.l, w.io, w.n, st);
     if (e != hipSuccess) return e;
-  } else {
+  }
+  if (run(kGkFftZ)) {
     if (run(kGkPlane))
       hipLaunchKernelGGL((kg_fft_xy<R, false, LOOP>), grid(kGkPlane), dim3(kPlaneBlock),
                          G.smem[kGkPlane], st, D, io, G.map[kGkPlane]);
@@ -3264,7 +3314,7 @@ static hipError_t launch_spme_t(const SpmeLaunch& L, hipStream_t st, hipEvent_t*
     dim3 rgrid;
     const PairCo rco = co_for(nblk, 256, true, &rgrid);
     if (rg.direct == 2) {
-      const size_t osmem = owned_smem(rg);
+      const size_t osmem = owned_smem(rg, sizeof(R) == 4);
 #define EW_OWNED(PP)                                                                          \
   hipLaunchKernelGGL((k_spread_owned<R, PP, CO>), rgrid, dim3(256), osmem, st, L.k0, th, L.q,      \
                      L.tile_start, rg, grid, L.qperm, rco)

@@ -1,6 +1,7 @@
 // kernels.hpp -- kernel argument blocks and launchers (host <-> device contract).
 #pragma once
 
+#include <vector>
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
@@ -181,8 +182,11 @@ inline __host__ __device__ int owned_plane_pitch(const int* S) {
 inline __host__ __device__ int owned_items(const RegionArgs& g) {
   return g.items > 0 ? g.items : kOwnedItems;
 }
-inline size_t owned_smem(const RegionArgs& g) {
-  return sizeof(double) * (size_t)owned_plane_pitch(g.S) * g.S[2] +
+// Bytes of an owned block's fixed-point accumulator per mesh point: an int64 (FP32 mesh), or
+// an int64 remainder and an int32 leading part (FP64 mesh).
+constexpr size_t owned_point_bytes(bool fp32) { return fp32 ? 8 : 12; }
+inline size_t owned_smem(const RegionArgs& g, bool fp32) {
+  return owned_point_bytes(fp32) * (size_t)owned_plane_pitch(g.S) * g.S[2] +
          sizeof(int) * (size_t)owned_items(g);
 }
 // Raises the dynamic LDS limit of the region spreading kernels to kRegionMaxLds (once).
@@ -390,6 +394,15 @@ struct PairMulti {
 // Row length of the interleaved charges: nvec rounded up to an even count (16-byte rows).
 constexpr int spmm_stride(int nvec) { return nvec == 1 ? 1 : (nvec + 1) & ~1; }
 
+// One batched register-transform FFT sequence of a grouped launch: n evaluations of one mesh
+// shape, mode and lanes, their meshes, spectra and influence tables at io (device); l: the mesh,
+// mode, lanes and twiddle tables.
+struct WaveGroup {
+  int n;
+  const WaveIo* io;
+  SpmeLaunch l;
+};
+
 struct GroupLaunch {
   const StructDesc* desc;
   const IoSlot* io;
@@ -406,12 +419,10 @@ struct GroupLaunch {
   int blocks[kGroupKinds];
   size_t smem[kGroupKinds];  // dynamic LDS: the largest of the structures in the kind
   int persistent;            // 1: bounded grids walking the work items (kGroupWgPerCu per CU)
-  // optional: the FFTs of all evaluations as one batched register-transform sequence (every
-  // evaluation has the same mesh and lanes) in place of the FFT kinds: wave_n evaluations, their
-  // meshes, spectra and influence tables at wave_io (device); wave_l: mesh, mode, lanes, tables
-  int wave_n;
-  const WaveIo* wave_io;
-  SpmeLaunch wave_l;
+  // optional: the FFTs of the evaluations whose structures take register transforms, as one
+  // batched sequence per (mesh, mode, lanes); the FFT kinds cover the other evaluations, so an
+  // evaluation's transforms do not depend on what else is in the launch
+  std::vector<WaveGroup> wave;
 };
 
 template <typename R>

@@ -150,33 +150,40 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
           double* energies, Staged* S) {
   std::vector<IoSlot> io((size_t)count);
   for (int i = 0; i < count; ++i) io[i] = IoSlot{dq[i], shift[i], energies + i, ids[i], 0};
-  if (!stage_group(P->desc.data(), io.data(), count, P->fp32, false, S))
-    return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
-  // members of one mesh shape, register-transform mode and lanes (replicas of one cell) run
-  // their FFTs as one batched register-transform sequence
-  const ew_handle* h0 = P->members[ids[0]];
-  bool same = h0->wave != 0;
-  for (int i = 1; i < count && same; ++i) {
+  // members that take register transforms run their FFTs as one batched sequence per (mode,
+  // mesh shape, lanes), whatever else is in the launch
+  std::vector<unsigned char> by_wave((size_t)count, 0);
+  std::vector<std::vector<int>> groups;
+  auto key_eq = [&](const ew_handle* a, const ew_handle* b) {
+    bool eq = a->wave == b->wave;
+    for (int d = 0; d < 3; ++d) eq = eq && a->prm.K[d] == b->prm.K[d] && a->lanes[d] == b->lanes[d];
+    return eq;
+  };
+  for (int i = 0; i < count; ++i) {
     const ew_handle* h = P->members[ids[i]];
-    same = h->wave == h0->wave;
-    for (int a = 0; a < 3; ++a)
-      same = same && h->prm.K[a] == h0->prm.K[a] && h->lanes[a] == h0->lanes[a];
+    if (!h->wave) continue;
+    int g = 0;
+    while (g < (int)groups.size() && !key_eq(P->members[ids[groups[g][0]]], h)) ++g;
+    if (g == (int)groups.size()) groups.emplace_back();
+    groups[g].push_back(i);
+    by_wave[i] = 1;
   }
-  if (same) {
-    std::vector<WaveIo> w((size_t)count);
-    for (int i = 0; i < count; ++i) {
+  if (!stage_group(P->desc.data(), io.data(), count, P->fp32, false, S, by_wave.data()))
+    return P->fail(EW_ERR_ARGUMENT, "structure exceeds a workgroup's LDS");
+  for (const std::vector<int>& g : groups) {
+    std::vector<WaveIo> w;
+    for (int i : g) {
       const ew_handle* h = P->members[ids[i]];
-      w[i] = WaveIo{h->grid, h->cgrid, h->inf_tab};
+      w.push_back(WaveIo{h->grid, h->cgrid, h->inf_tab});
     }
-    stage_wave_io(w, S);
-    S->wave_sid = ids[0];
+    stage_wave_io(w, ids[g[0]], S);
   }
   return EW_OK;
 }
 
 GroupLaunch launch_of(const ew_pool* P, const Staged& S, const void* dev) {
   GroupLaunch G = group_launch_of(S, P->desc_dev, dev, P->order, P->opt.group_grid == 1);
-  if (G.wave_n > 0) G.wave_l = spme_launch_of(P->members[S.wave_sid]);
+  for (size_t g = 0; g < G.wave.size(); ++g) G.wave[g].l = spme_launch_of(P->members[S.wave_sid[g]]);
   return G;
 }
 

@@ -231,8 +231,9 @@ int ew_shift_device_noalloc(ew_handle *h, const double *d_dq, double *d_shift, d
  * vectors of independent SCC cycles that share a geometry (spin or k-point replicas, trial
  * charges of a mixer).  The pair coefficients are read once for all vectors.  dq, shift:
  * [nvec][n] row-major; energy: nvec doubles (may be NULL).  1 <= nvec <= EW_MAX_VECTORS.  Each
- * vector's result is ew_shift's for that vector alone (to the round-off of the spreading's
- * accumulation order).  Extra device memory per vector: about one mesh, one spreading scratch
+ * vector's result is ew_shift's for that vector alone (to round-off: the grouped launch adds
+ * the mesh and pair parts in one kernel).  Extra device memory per vector: about one mesh, one
+ * spreading scratch
  * and four atom arrays, held until the next ew_set_geometry. */
 #define EW_MAX_VECTORS 8
 int ew_shift_multi(ew_handle *h, int nvec, const double *dq, double *shift, double *energy);

@@ -1,8 +1,9 @@
 """GPU tests of the multi-vector evaluation (several charge vectors through one geometry in one
 grouped launch, ew_shift_multi / ew_shift_multi_device).
 
-Each vector's shift and energy must equal ew_shift of that vector alone, within the round-off
-of the spreading's LDS accumulation order (FP64 1e-13, mixed 1e-6 relative); one vector is also
+Each vector's shift and energy must equal ew_shift of that vector alone to round-off (the
+grouped launch gathers the mesh potential and adds the pair sum in one kernel, the
+single-vector evaluation in two: FP64 1e-13, mixed 1e-6 relative); one vector is also
 checked against the CPU SPME oracle at the engine's parameters (1e-11, FP64).
 """
 import sys

@@ -1,22 +1,21 @@
 """GPU tests of the serving loop (continuous batching over a structure pool).
 
-Each request's result must equal the same structure evaluated alone by the pool, within the
-round-off of the spreading's LDS accumulation order (FP64 1e-13, mixed 1e-6 relative), whatever
-batch the server put it in; requests for one structure are evaluated in submission order.
+Each request's result must equal the same structure evaluated alone by the pool bit for bit,
+whatever batch the server put it in (a structure's kernels do not depend on the rest of the
+launch, and every sum has a fixed order or is exact); requests for one structure are evaluated
+in submission order.
 """
 import sys
 import threading
 
 import numpy as np
 import pytest
-from conftest import rel_max
 
 import fftewald_amd as ew
 from fftewald_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-13, "mixed": 1e-6}
 
 
 def say(msg):
@@ -57,8 +56,8 @@ def test_results_alone_equal_results_in_the_mix(structures, precision):
                 for k, i in enumerate(ids):
                     shift, energy = srv.result(tickets[r][k])
                     s0, e0 = alone[r][k]
-                    assert rel_max(shift, s0[0]) < TOL[precision]
-                    assert energy == pytest.approx(float(e0[0]), rel=TOL[precision])
+                    assert np.array_equal(shift, s0[0])
+                    assert energy == float(e0[0])
             st = srv.stats()
             assert st["completed"] == rounds * len(ids) and st["failed"] == 0
             # requests that arrived together were batched (at most 5 per launch)
@@ -91,7 +90,7 @@ def test_two_submitting_threads(structures):
         for i in ids:
             assert len(results[i]) == 4
             for shift, energy in results[i]:
-                assert rel_max(shift, alone[i][0][0]) < TOL["fp64"]
+                assert np.array_equal(shift, alone[i][0][0])
 
 
 def test_close_finishes_submitted_requests(structures):

@@ -7,8 +7,8 @@ gives at that geometry, through the host, device-pointer and multi-vector entry 
 the gradients and the stress.
 
 Tolerances (max_i |x_i - ref_i| / max_i |ref_i|): FP64 1e-12, mixed 1e-6 (shifts) and 1e-5
-(energies, gradients, stress); the spreading's LDS atomics add in any order, so two engines agree
-to round-off, not bit for bit.
+(energies, gradients, stress): the fresh engine may take other launch shapes (its buffers and
+captured graphs are new), whose sums round differently.
 """
 import sys
 

@@ -1,5 +1,5 @@
 """Launch shapes (ew_tuning): every shape the autotuner may pick gives the same shifts and
-energy as the fixed rules, to the round-off of the spreading's accumulation order, and the
+energy as the fixed rules, to round-off (the shapes sum in different orders), and the
 CPU SPME at identical parameters; the autotuner's choice is cached and reused.
 
 Tolerances (max_i |x_i - ref_i| / max_i |ref_i|): FP64 1e-13 between shapes and 1e-11 against
