@@ -1864,6 +1864,7 @@ extern "C" int ew_shift(ew_handle* h, const double* dq, double* shift, double* e
   double net = 0.0, absq = 0.0;
   for (int i = 0; i < n; ++i) {
     if (!std::isfinite(dq[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+    if (std::fabs(dq[i]) > kMaxCharge) return h->fail(EW_ERR_ARGUMENT, "|charge| above 1e4");
     net += dq[i];
     absq += std::fabs(dq[i]);
   }
@@ -2116,6 +2117,7 @@ extern "C" int ew_shift_multi(ew_handle* h, int nvec, const double* dq, double* 
     for (size_t i = 0; i < n; ++i) {
       const double q = dq[v * n + i];
       if (!std::isfinite(q)) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+      if (std::fabs(q) > kMaxCharge) return h->fail(EW_ERR_ARGUMENT, "|charge| above 1e4");
       net += q;
       absq += std::fabs(q);
     }
@@ -2200,6 +2202,7 @@ extern "C" int ew_stress(ew_handle* h, const double* dq, double* stress) {
   double net = 0.0, absq = 0.0;
   for (int i = 0; i < n; ++i) {
     if (!std::isfinite(dq[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+    if (std::fabs(dq[i]) > kMaxCharge) return h->fail(EW_ERR_ARGUMENT, "|charge| above 1e4");
     net += dq[i];
     absq += std::fabs(dq[i]);
   }
@@ -2294,6 +2297,8 @@ static int stage_sources(ew_handle* h, int m, const double* coords, const double
     if (!std::isfinite(charges[s]) || !std::isfinite(coords[3 * s]) ||
         !std::isfinite(coords[3 * s + 1]) || !std::isfinite(coords[3 * s + 2]))
       return h->fail(EW_ERR_NOT_FINITE, "non-finite external charge or coordinate");
+    if (std::fabs(charges[s]) > kMaxCharge)
+      return h->fail(EW_ERR_ARGUMENT, "|external charge| above 1e4");
     if (widths) {
       if (!(widths[s] > 0.0) || !std::isfinite(widths[s]))
         return h->fail(EW_ERR_ARGUMENT, "blur widths must be finite and > 0");
@@ -2464,6 +2469,8 @@ extern "C" int ew_external_gradients(ew_handle* h, int m, const double* ext_coor
   if (dq)
     for (int i = 0; i < n; ++i)
       if (!std::isfinite(dq[i])) return h->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+      else if (std::fabs(dq[i]) > kMaxCharge)
+        return h->fail(EW_ERR_ARGUMENT, "|charge| above 1e4");
   if (m == 0) return EW_OK;
   const bool want_ext = dq && ext_grad;
   auto up = [](size_t b) { return (b + 255) & ~size_t(255); };

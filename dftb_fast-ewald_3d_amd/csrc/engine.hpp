@@ -26,6 +26,10 @@ constexpr double kSqrtPi = 1.77245385090551602729816748334114518;
 // fe/common/ewald_validation.f90:13-20
 constexpr double kMinSeparationSq = 1.0e-24;
 constexpr double kNeutralityTol = 1.0e-8;
+// Largest charge magnitude the entry points accept: the spreading accumulates in fixed point,
+// exactly for contributions below 2^15 (FP32 meshes) and sums below 1.3e8 per mesh point;
+// atomic and point charges are many orders of magnitude smaller.
+constexpr double kMaxCharge = 1.0e4;
 
 constexpr int kMaxOrder = 16;
 constexpr int kMaxRadixPasses = 12;

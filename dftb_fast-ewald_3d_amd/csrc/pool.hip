@@ -507,6 +507,7 @@ extern "C" int ew_pool_shift(ew_pool* P, int count, const int* ids, const double
     for (int k = 0; k < n; ++k) {
       const double v = dq[off[i] + k];
       if (!std::isfinite(v)) return P->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+      if (std::fabs(v) > kMaxCharge) return P->fail(EW_ERR_ARGUMENT, "|charge| above 1e4");
       net += v;
       absq += std::fabs(v);
     }

@@ -333,6 +333,7 @@ extern "C" int ew_server_submit(ew_server* S, int id, const double* dq, double* 
   double net = 0.0, absq = 0.0;
   for (int i = 0; i < n; ++i) {
     if (!std::isfinite(dq[i])) return S->fail(EW_ERR_NOT_FINITE, "non-finite charge");
+    if (std::fabs(dq[i]) > kMaxCharge) return S->fail(EW_ERR_ARGUMENT, "|charge| above 1e4");
     net += dq[i];
     absq += std::fabs(dq[i]);
   }

@@ -198,12 +198,14 @@ int ew_get_tuning(const ew_handle *h, ew_tuning *out);
 
 /* One SCC iteration, host arrays: shift = phi(dq) and energy = 1/2 sum shift*dq
  * (coulomb.F90:786-880, energy as printed at coulomb.F90:875).  Charges must be neutral
- * (canUseEwaldCode, coulomb.F90:255-265).  energy may be NULL. */
+ * (canUseEwaldCode, coulomb.F90:255-265) and at most 1e4 in magnitude (EW_ERR_ARGUMENT: the
+ * spreading accumulates each mesh point in fixed point, so an evaluation is bitwise
+ * reproducible).  energy may be NULL. */
 int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
 
 /* Same with device pointers (dq, shift: n doubles; energy: 1 double), enqueued on `stream`
  * (a hipStream_t; NULL = the handle's stream).  Asynchronous: no host synchronisation, no
- * neutrality check (validate with ew_shift or on the host).  The handle's buffers serve one
+ * neutrality or magnitude check (validate with ew_shift or on the host).  The handle's buffers serve one
  * evaluation at a time: an evaluation on another stream than the previous one is ordered after
  * it on the device, and every later call that rewrites or frees the handle's buffers
  * (ew_set_geometry, ew_gradients, ew_stress, ew_external_*, ew_destroy, evicting a captured

@@ -226,6 +226,12 @@ def test_error_paths():
         with pytest.raises(ew.EwaldError) as err:
             e.shift(q)
         assert err.value.code == ew.EW_ERR_NOT_FINITE
+        # charges beyond the fixed-point spreading's range are refused, not wrapped
+        q = w["charges"].copy()
+        q[3], q[4] = 2e4, -2e4
+        with pytest.raises(ew.EwaldError) as err:
+            e.shift(q)
+        assert err.value.code == ew.EW_ERR_ARGUMENT
         # arrays that do not match the geometry are refused before the C side reads them
         with pytest.raises(ValueError):
             e.shift(w["charges"][:-1])
