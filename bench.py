@@ -941,7 +941,8 @@ def main():
     if not args.no_workload_legs:
         bres, bws, batoms, berr = batch_leg(args, torch, ew, W, max(5, args.steps // 5), 3,
                                             sequential=False)
-        sres, sws, satoms = serving_leg(args, ew, W, [20000.0, 60000.0, 100000.0, 120000.0],
+        sres, sws, satoms = serving_leg(args, ew, W, [20000.0, 60000.0, 80000.0, 100000.0,
+                                                      120000.0],
                                         1.0, precisions=("mixed",))
         sbest = sres["mixed"]["sustained"]
         workload_legs = {

@@ -146,12 +146,18 @@ int sync_desc(ew_pool* P) {
   return EW_OK;
 }
 
+// Mesh points from which a pool member's register transforms run as a batched sequence.
+constexpr long long kPoolWaveMinPoints = 96LL * 96 * 96;
+
 int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double* const* shift,
           double* energies, Staged* S) {
   std::vector<IoSlot> io((size_t)count);
   for (int i = 0; i < count; ++i) io[i] = IoSlot{dq[i], shift[i], energies + i, ids[i], 0};
-  // members that take register transforms run their FFTs as one batched sequence per (mode,
-  // mesh shape, lanes), whatever else is in the launch
+  // members that take register transforms on a large mesh run their FFTs as one batched
+  // sequence per (mode, mesh shape, lanes), whatever else is in the launch; members with small
+  // meshes take the grouped Stockham kinds, one launch per kind for all of them (a sequence per
+  // small mesh shape would multiply the launches of a mixed batch).  Either way the choice is
+  // the member's own, so its kernels do not depend on the rest of the launch.
   std::vector<unsigned char> by_wave((size_t)count, 0);
   std::vector<std::vector<int>> groups;
   auto key_eq = [&](const ew_handle* a, const ew_handle* b) {
@@ -161,7 +167,8 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
   };
   for (int i = 0; i < count; ++i) {
     const ew_handle* h = P->members[ids[i]];
-    if (!h->wave) continue;
+    if (!h->wave || (long long)h->prm.K[0] * h->prm.K[1] * h->prm.K[2] < kPoolWaveMinPoints)
+      continue;
     int g = 0;
     while (g < (int)groups.size() && !key_eq(P->members[ids[groups[g][0]]], h)) ++g;
     if (g == (int)groups.size()) groups.emplace_back();
