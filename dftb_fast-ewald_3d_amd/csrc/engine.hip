@@ -609,6 +609,9 @@ static ew_tuning rule_tuning(const ew_handle* h) {
     t.fft_lanes[0] = t.fft_lanes[1] = wave_plane_default_lanes(K[0]);
     t.fft_lanes[2] = wave_fft_default_lanes(K[2], h->fp32);
   }
+  // FP64 beyond 128 points per axis: the Stockham passes (256^3, 98,304 atoms, each pass alone:
+  // 414 against 445 us for the register line passes, whose x passes are the slower ones)
+  if (!h->fp32 && std::max(K[0], std::max(K[1], K[2])) > 128) t.fft_wave = 0;
   if (t.fft_wave) {
     t.pair_cosched = 0;
     t.pair_spread_share = 0;
