@@ -300,12 +300,13 @@ def batch_bytes(prm, n, rs):
     return sum(pb.values())
 
 
-def batch_leg(args, torch, ew, W, steps, warmup, sequential=True):
+def batch_leg(args, torch, ew, W, steps, warmup, sequential=True, tune="rules"):
     """Batched workload: 256 independent triclinic structures (N log-uniform in [64, 8192],
     angles U[70, 110] deg, density 0.1 A^-3, ~0.8 A grids, order 6) evaluated together by one
     grouped launch per kernel kind; FP64 and mixed; optionally the same structures through
-    single-structure engines one after another.  Returns (results per precision, structures,
-    atoms, worst mixed-vs-FP64 shift error)."""
+    single-structure engines one after another.  tune: the members' launch shapes ("rules", or
+    "auto": each member autotuned, or read from the tuning cache, as it is added).  Returns
+    (results per precision, structures, atoms, worst mixed-vs-FP64 shift error)."""
     t = time.perf_counter()
     ws = W.batch()
     print(f"[bench] batch workload generated in {time.perf_counter() - t:.1f} s", file=sys.stderr,
@@ -321,7 +322,7 @@ def batch_leg(args, torch, ew, W, steps, warmup, sequential=True):
     for prec in ("fp64", "mixed"):
         rs = 4 if prec == "mixed" else 8
         with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=ws[0]["order"],
-                          precision=prec, group_grid=args.group_grid) as pool:
+                          precision=prec, group_grid=args.group_grid, tune=tune) as pool:
             t = time.perf_counter()
             ids = [pool.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"])
                    for w in ws]
@@ -382,7 +383,8 @@ def batch_leg(args, torch, ew, W, steps, warmup, sequential=True):
 
 def run_batch(args, torch, ew, W):
     """The batched workload as its own JSON line (see batch_leg)."""
-    res, ws, natoms, err = batch_leg(args, torch, ew, W, args.steps, args.warmup)
+    res, ws, natoms, err = batch_leg(args, torch, ew, W, args.steps, args.warmup,
+                                     tune=args.tune)
     head = res[args.precision]
     line = {
         "metric": "SPME shift+energy evals/s (batch of 256 structures)",
@@ -397,7 +399,7 @@ def run_batch(args, torch, ew, W):
                                "U[70,110] deg, density 0.1 A^-3, grids ~0.8 A (2^a3^b5^c), "
                                "order 6, r_cut 10 A, tol 1e-10; one grouped launch per kernel "
                                "kind, graph replay", "structures": len(ws), "atoms": natoms,
-                   "precision": args.precision},
+                   "precision": args.precision, "launch_shapes": args.tune},
         "fp64_evals_per_s": round(res["fp64"]["evals_per_s"], 2),
         "mixed_evals_per_s": round(res["mixed"]["evals_per_s"], 2),
         "atoms_per_s": {k: round(v["atoms_per_s"]) for k, v in res.items()},
@@ -482,7 +484,7 @@ def poisson_load(srv, ids, bufs, qps, seconds, rng):
     return offered, done[0], time.perf_counter() - t0, dropped
 
 
-def serving_leg(args, ew, W, sweep, seconds, precisions=("mixed", "fp64")):
+def serving_leg(args, ew, W, sweep, seconds, precisions=("mixed", "fp64"), tune="rules"):
     """Serving workload: 1,000 resident structures (N log-uniform in [100, 20000], built like
     the 98,304-atom cell); SCC charge updates arrive as a Poisson stream at the rates `sweep` and
     are served by the continuous-batching loop; throughput and p50/p99 latency per precision.
@@ -497,7 +499,7 @@ def serving_leg(args, ew, W, sweep, seconds, precisions=("mixed", "fp64")):
     res = {}
     for prec in precisions:
         with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=ws[0]["order"],
-                          precision=prec) as pool:
+                          precision=prec, tune=tune) as pool:
             t = time.perf_counter()
             ids = [pool.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"])
                    for w in ws]
@@ -537,7 +539,7 @@ def serving_leg(args, ew, W, sweep, seconds, precisions=("mixed", "fp64")):
 def run_serving(args, torch, ew, W):
     """The serving workload as its own JSON line (see serving_leg)."""
     res, ws, natoms = serving_leg(args, ew, W, [float(x) for x in args.qps.split(",")],
-                                  args.seconds)
+                                  args.seconds, tune=args.tune)
     head = res[args.precision]
     best = head["sustained"]
     line = {
@@ -554,7 +556,7 @@ def run_serving(args, torch, ew, W):
                                "arrivals of dq + N(0, 0.01) updates; continuous batching, "
                                f"max_batch {args.max_batch}, 2 batches in flight",
                    "structures": len(ws), "atoms": natoms, "precision": args.precision,
-                   "seconds_per_rate": args.seconds},
+                   "seconds_per_rate": args.seconds, "launch_shapes": args.tune},
         "p50_ms": best["p50_ms"] if best else None, "p99_ms": best["p99_ms"] if best else None,
         "serving": res, "roofline": None, "cpu_baseline": None,
     }

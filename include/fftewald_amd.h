@@ -336,12 +336,15 @@ int ew_grid_memory(int n, const double *charges, const double *latvecs, const ew
  * structures resident (geometry, pair lists, B-spline weights, FFT plans) and evaluates any
  * subset of them in one grouped launch per kernel; DFTB+ drivers that run many independent
  * SCC cycles (replicas, trajectories, screening) call it once per SCC iteration of the whole
- * set.  Each structure's result is the single-structure engine's, alone or in any batch. */
+ * set.  Each structure's result is the single-structure engine's to round-off, and bitwise the
+ * same alone and in any batch. */
 typedef struct ew_pool ew_pool;
 
 /* capacity_bytes: device memory the pool may hold (0: no limit); ew_pool_add refuses a
  * structure that does not fit (EW_ERR_MEMORY), which is the admission rule.  Every structure
- * of a pool shares opts (precision, tol, r_cut, order); grids may differ. */
+ * of a pool shares opts (precision, tol, r_cut, order); grids may differ.  Leave opts.tune at 0
+ * (the fixed rules) for pools of many structures: the autotuner times each structure alone on
+ * the whole device, and its choices can be slower in a launch shared with others. */
 int ew_pool_create(const ew_options *opts, long long capacity_bytes, ew_pool **out);
 int ew_pool_destroy(ew_pool *pool);
 const char *ew_pool_last_error(const ew_pool *pool);
