@@ -151,9 +151,10 @@ def trace_symbol(span, prec, index_bytes, order, planes=False, pair_mode=0, tuni
 def critical_path(trace, calls, tb, prec, prm, step_ms, tuning=None, split_gather=False):
     """The evaluation's critical path from its traced spans: the chain on the evaluation stream
     (spreading, FFTs, the wait for the pair sum, the gather) and, beside it, the charge gather
-    and pair sum.  The pair-sum branch is on the critical path when the chain waits for it
-    (join wait above 5 % of the evaluation).  The dominant kernel is the longest span on the
-    critical path."""
+    and pair sum.  The pair-sum branch is on the critical path when its spans (charge gather and
+    pair sum) are at least as long as the chain's spans before the join (the join wait itself
+    holds a cross-queue latency of about 9 us whichever branch ends first, so it does not
+    decide).  The dominant kernel is the longest span on the critical path."""
     us = {k: trace[k] / calls * 1e3 for k in trace}
     planes = fused_planes(prm, tuning)
     rows = []
@@ -173,7 +174,9 @@ def critical_path(trace, calls, tb, prec, prm, step_ms, tuning=None, split_gathe
     chain = [r for r in rows if not r["beside_chain"]]
     chain_us = sum(r["us"] for r in chain)
     chain_b = sum(r["algorithmic_bytes"] for r in chain)
-    pair_on_path = us["join_wait"] > 0.05 * us["evaluation"]
+    before_join = CHAIN[:CHAIN.index("join_wait")]
+    chain_to_join = sum(us[k] for k in before_join if k in us)
+    pair_on_path = us["charges"] + us["pair_sum"] >= chain_to_join
     on_path = [r for r in rows if r["kernel"] and (not r["beside_chain"] or pair_on_path)]
     dom = max(on_path, key=lambda r: r["us"])
     spread_fft = [r for r in chain if r["span"].startswith(("spread", "fft"))]
