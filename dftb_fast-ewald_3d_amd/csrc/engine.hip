@@ -1690,8 +1690,9 @@ static int enqueue_eval(ew_handle* h, const double* d_dq, double* d_shift, doubl
 
 // One evaluation on stream s.  Large evaluations that fork at their start (privatised
 // spreading) are launched directly: their two branches and the join replay slower from a
-// captured graph than from direct launches on this runtime (98,304 atoms: 0.183 against
-// 0.176 ms), and their kernels are long enough to hide the launch cost.  Other evaluations are
+// captured graph than from direct launches on this runtime (98,304 atoms: 0.151 against
+// 0.141 ms mixed, 0.209 against 0.200 ms FP64), and their kernels are long enough to hide the
+// launch cost.  Other evaluations are
 // captured once per buffer triple on the handle's own stream and replayed as a graph (the
 // sequence has no memset or host work, so every replay is the same work).
 constexpr int kDirectLaunchAtoms = 32768;
