@@ -930,6 +930,9 @@ def main():
                      else None}
             if planes and k in ("fft_y", "fft_y_inverse"):
                 ph[k]["note"] = "fused into the x-y plane kernels (event overhead only)"
+            if k == "pair_sum" and avg_ms > 0:
+                # in-cutoff pairs of the resident list summed per second (each linked pair twice)
+                ph[k]["pairs_per_s"] = round(prm["n_pairs"] / (avg_ms * 1e-3), 1)
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "geometry_s": geom_s,
                      "multi": multi, "external": ext, "md_step": md, "critical_path": crit,
                      "tuning": tuning, "rules_ms": rules_ms, "rules": rules,
