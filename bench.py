@@ -742,9 +742,10 @@ def main():
                     help="triclinic: external point charges timed once per precision (0: skip)")
     ap.add_argument("--group-grid", type=int, default=0, choices=[0, 1],
                     help="batch: 0 one workgroup per work item (default), 1 persistent grids")
-    ap.add_argument("--tune", default="auto", choices=["auto", "rules"],
-                    help="launch shapes of the timed engines: the autotuner (default) or the "
-                         "fixed rules")
+    ap.add_argument("--tune", default=None, choices=["auto", "rules"],
+                    help="launch shapes of the timed engines: the autotuner (the default for "
+                         "the 98,304-atom cell) or the fixed rules (the default for the batch "
+                         "and serving pools, where per-structure tuning is slower)")
     ap.add_argument("--replicas", type=int, default=4,
                     help="triclinic: resident geometries of the 98,304-atom cell evaluated in "
                          "one grouped launch (0: skip)")
@@ -762,6 +763,8 @@ def main():
         args.replicas = 0
         args.no_workload_legs = True
     args.vectors = [int(v) for v in args.vectors.split(",") if v.strip()]
+    if args.tune is None:
+        args.tune = "rules" if args.workload in ("batch", "serving") else "auto"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
               "(structures shard across GPUs by footprint)", file=sys.stderr)
