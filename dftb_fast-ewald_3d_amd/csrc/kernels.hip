@@ -661,13 +661,13 @@ __device__ __forceinline__ long long fix_round(double x) {
   return __double_as_longlong(x + kMagic) - __double_as_longlong(kMagic);
 }
 
-// Fixed-point accumulator of a spreading workgroup's mesh points.  Integer adds are exact and commute, so
-// a point's value does not depend on the order in which the lanes add its contributions: the
-// spreading is bitwise reproducible.  FP32 mesh: one int64 at 2^-36 (contributions below 2^15 in
-// magnitude, sums below 1.3e8).  FP64 mesh: an int32 at 2^-4, added only when not zero (most
-// contributions are below 1/32), and an int64 remainder at 2^-55 (contributions and sums below
-// 1.3e8; the remainder of each contribution, |r| <= 2^-5, is exact by Sterbenz's lemma and
-// rounded at 2^-56).
+// Fixed-point accumulator of a spreading workgroup's mesh points.  Integer adds are exact and
+// commute, so a point's value does not depend on the order in which the lanes add its
+// contributions: the spreading is bitwise reproducible.  FP32 mesh: one int64 at 2^-36
+// (contributions below 2^15 in magnitude, sums below 1.3e8).  FP64 mesh: an int32 at 2^-4, added
+// only when not zero (most contributions are below 1/32), and an int64 remainder at 2^-55
+// (contributions and sums below 1.3e8; the remainder of each contribution, |r| <= 2^-5, is exact
+// by Sterbenz's lemma and rounded at 2^-56).
 template <typename R>
 struct OwnedAcc {
   static constexpr bool F32 = sizeof(R) == 4;

@@ -162,7 +162,8 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
   std::vector<std::vector<int>> groups;
   auto key_eq = [&](const ew_handle* a, const ew_handle* b) {
     bool eq = a->wave == b->wave;
-    for (int d = 0; d < 3; ++d) eq = eq && a->prm.K[d] == b->prm.K[d] && a->lanes[d] == b->lanes[d];
+    for (int d = 0; d < 3; ++d)
+      eq = eq && a->prm.K[d] == b->prm.K[d] && a->lanes[d] == b->lanes[d];
     return eq;
   };
   for (int i = 0; i < count; ++i) {
@@ -190,7 +191,8 @@ int stage(ew_pool* P, int count, const int* ids, const double* const* dq, double
 
 GroupLaunch launch_of(const ew_pool* P, const Staged& S, const void* dev) {
   GroupLaunch G = group_launch_of(S, P->desc_dev, dev, P->order, P->opt.group_grid == 1);
-  for (size_t g = 0; g < G.wave.size(); ++g) G.wave[g].l = spme_launch_of(P->members[S.wave_sid[g]]);
+  for (size_t g = 0; g < G.wave.size(); ++g)
+    G.wave[g].l = spme_launch_of(P->members[S.wave_sid[g]]);
   return G;
 }
 

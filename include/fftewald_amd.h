@@ -205,9 +205,9 @@ int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
 
 /* Same with device pointers (dq, shift: n doubles; energy: 1 double), enqueued on `stream`
  * (a hipStream_t; NULL = the handle's stream).  Asynchronous: no host synchronisation, no
- * neutrality or magnitude check (validate with ew_shift or on the host).  The handle's buffers serve one
- * evaluation at a time: an evaluation on another stream than the previous one is ordered after
- * it on the device, and every later call that rewrites or frees the handle's buffers
+ * neutrality or magnitude check (validate with ew_shift or on the host).  The handle's buffers
+ * serve one evaluation at a time: an evaluation on another stream than the previous one is
+ * ordered after it on the device, and every later call that rewrites or frees the handle's buffers
  * (ew_set_geometry, ew_gradients, ew_stress, ew_external_*, ew_destroy, evicting a captured
  * graph) waits for the last evaluation first.  The caller still owns its own buffers: d_dq
  * must stay valid and d_shift/d_energy unread until `stream` has passed the evaluation. */
