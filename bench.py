@@ -644,7 +644,8 @@ def replicas_leg(args, torch, ew, W, w, qs, fp64_plain_ms):
     en_t = torch.empty(args.replicas, dtype=torch.float64, device=dev)
     out = {}
     for prec in ("fp64", "mixed"):
-        with ew.EwaldPool(tol=1e-10, r_cut=w["r_cut"], order=w["order"], precision=prec) as pool:
+        with ew.EwaldPool(tol=1e-10, r_cut=w["r_cut"], order=w["order"], precision=prec,
+                          group_grid=args.group_grid) as pool:
             ids = [pool.add(x, w["lattice"], charges=w["charges"], grid=w["grid"]) for x in geos]
             dptr = [d.data_ptr() for d in dq_t]
             sptr = [d.data_ptr() for d in sh_t]
