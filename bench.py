@@ -742,7 +742,8 @@ def main():
     ap.add_argument("--external", type=int, default=20000,
                     help="triclinic: external point charges timed once per precision (0: skip)")
     ap.add_argument("--group-grid", type=int, default=0, choices=[0, 1],
-                    help="batch: 0 one workgroup per work item (default), 1 persistent grids")
+                    help="grouped launches (batch pools, the triclinic replicas): 0 one "
+                         "workgroup per work item (default), 1 persistent grids")
     ap.add_argument("--tune", default=None, choices=["auto", "rules"],
                     help="launch shapes of the timed engines: the autotuner (the default for "
                          "the 98,304-atom cell) or the fixed rules (the default for the batch "
