@@ -36,12 +36,14 @@ extern "C" {
 #define EW_ERR_NOT_NEUTRAL 2  /* |sum q| > 1e-8 sum |q|  (fe/common/ewald_validation.f90:41-46) */
 #define EW_ERR_CELL 3         /* cell volume <= 1e-18    (fe/common/ewald_geometry.f90:78)      */
 #define EW_ERR_TOLERANCE 4    /* tol outside (0,1)       (fe/fft/ewald_fft_3d.f90:97-98)        */
-#define EW_ERR_COINCIDENT 5   /* two charges closer than 1e-12 Bohr (fe/fft/ewald_fft_3d_real.f90:159-160) */
+#define EW_ERR_COINCIDENT 5   /* two charges closer than 1e-12 Bohr
+                                 (fe/fft/ewald_fft_3d_real.f90:159-160)                     */
 #define EW_ERR_NO_GEOMETRY 6  /* ew_shift before ew_set_geometry                                 */
 #define EW_ERR_DEVICE 7       /* HIP runtime failure                                              */
 #define EW_ERR_MEMORY 8       /* device memory allocation failed / over the admission budget     */
 #define EW_ERR_NOT_FINITE 9   /* NaN or Inf in coordinates or charges                             */
-#define EW_ERR_PENDING 10     /* ew_server_wait: the request is not finished within the timeout    */
+#define EW_ERR_PENDING 10     /* ew_server_wait: the request is not finished within the
+                                 timeout                                                    */
 
 /* precision modes */
 #define EW_PRECISION_FP64 0   /* every step in FP64                                               */
@@ -124,7 +126,8 @@ typedef struct ew_options {
                       and y passes; 2: fused planes wherever a plane fits                  */
   int pair_storage; /* 0 (default): the row list (every linked pair in both rows, the fastest
                        pair sum); 1: each linked pair stored once (symmetric, tile-blocked):
-                       half the pair memory, about twice the pair-sum time                    */
+                       half the pair memory, about twice the pair-sum time; its transposed
+                       half adds through atomics, so results repeat to round-off only         */
   int group_grid;  /* grouped launches (pools, several charge vectors): 0 (default): one
                       workgroup per work item (structure, block); 1: persistent grids of at
                       most 4 workgroups per CU that walk the table of work items            */
@@ -138,7 +141,11 @@ typedef struct ew_options {
                       own data and keep the fastest, once per (mesh, order, precision, size
                       class, pair storage, device), persisted in a cache file
                       ($EW_TUNE_CACHE, else ewald_tune.cache beside the library); 2: cached
-                      choices only (fixed rules on a miss); 3: `tuning` as given          */
+                      choices only (fixed rules on a miss); 3: `tuning` as given.  Results
+                      repeat bit for bit with owned spreading blocks (spread_direct 2, which
+                      the rules choose wherever a block fits the LDS); the block-copy and
+                      direct-atomic write-outs add floating-point values through atomics
+                      and repeat to round-off only                                        */
   ew_tuning tuning; /* tune = 3: the shapes to use                                              */
 } ew_options;
 
@@ -199,8 +206,9 @@ int ew_get_tuning(const ew_handle *h, ew_tuning *out);
 /* One SCC iteration, host arrays: shift = phi(dq) and energy = 1/2 sum shift*dq
  * (coulomb.F90:786-880, energy as printed at coulomb.F90:875).  Charges must be neutral
  * (canUseEwaldCode, coulomb.F90:255-265) and at most 1e4 in magnitude (EW_ERR_ARGUMENT: the
- * spreading accumulates each mesh point in fixed point, so an evaluation is bitwise
- * reproducible).  energy may be NULL. */
+ * owned-block spreading accumulates each mesh point in fixed point, so an evaluation is bitwise
+ * reproducible; see ew_options.tune and pair_storage for the shapes that are not).  energy may
+ * be NULL. */
 int ew_shift(ew_handle *h, const double *dq, double *shift, double *energy);
 
 /* Same with device pointers (dq, shift: n doubles; energy: 1 double), enqueued on `stream`
@@ -221,10 +229,10 @@ int ew_shift_device(ew_handle *h, const double *d_dq, double *d_shift, double *d
  * array of 64 MiB per device, held until the process ends, instead of the handle's buffers; the
  * kernels are ew_shift_device's, launched directly (no graph is captured or instantiated).  A
  * geometry whose scratch exceeds 64 MiB gets EW_ERR_MEMORY (the 98,304-atom cell at 128^3 fits
- * in both precisions with direct spreading; FP64 at 192^3 does not).  The caller passes the inputs and outputs as for
- * ew_shift_device (d_energy may be NULL).  Every call on one device shares that array, which
- * keeps the last call's contents: calls that overlap on one device race (order them on one
- * stream or synchronise between them). */
+ * in both precisions with direct spreading; FP64 at 192^3 does not).  The caller passes the
+ * inputs and outputs as for ew_shift_device (d_energy may be NULL).  Every call on one device
+ * shares that array, which keeps the last call's contents: calls that overlap on one device
+ * race (order them on one stream or synchronise between them). */
 int ew_shift_device_noalloc(ew_handle *h, const double *d_dq, double *d_shift, double *d_energy,
                             void *stream);
 
