@@ -65,6 +65,26 @@ def test_pool_matches_single_engine(structures, precision):
         assert modes == {0, 1}
 
 
+@pytest.mark.parametrize("order,prec", [(4, "fp64"), (9, "fp64"), (9, "mixed")])
+def test_pool_other_orders(structures, order, prec):
+    """The grouped kernels at a compiled order other than 6 (4) and at an order they take at run
+    time (9), against the single engine at the same order and mesh.  The parity tests pin the
+    single engine to the CPU SPME at order 4 and, for the run-time-order kernels, at order 12."""
+    ws = structures[:5]
+    say(f"pool of {len(ws)} structures at order {order}, {prec}")
+    with ew.EwaldPool(tol=1e-10, r_cut=ws[0]["r_cut"], order=order, precision=prec) as p:
+        ids = [p.add(w["coords"], w["lattice"], charges=w["charges"], grid=w["grid"])
+               for w in ws]
+        shifts, energies = p.shift(ids, [w["charges"] for w in ws])
+    for k, w in enumerate(ws):
+        with ew.EwaldEngine(tol=1e-10, r_cut=w["r_cut"], grid=w["grid"], order=order,
+                            precision=prec) as e:
+            e.set_geometry(w["coords"], w["lattice"], charges=w["charges"])
+            s1, e1 = e.shift(w["charges"])
+        assert rel_max(shifts[k], s1) < TOL[prec], (order, k)
+        assert energies[k] == pytest.approx(e1, rel=TOL[prec]), (order, k)
+
+
 def test_pool_symmetric_pair_storage(structures):
     ws = structures[:5]
     say("pool with symmetric pair storage against single engines with the row list")
